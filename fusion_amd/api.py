@@ -1,0 +1,413 @@
+"""Python host mirror of the reference's host interface over the C ABI.
+
+Names follow the reference: the free functions of Tracking/Reduction.h (FilterDepth,
+PyrDownGauss, ComputeVMap, ComputeNMap, ResizeMap, ICPStep) and Mapping/RenderScene.h
+(ResetMap, FuseMapColor, CheckBlockVisibility, CreateRenderingBlocks, Raycast, MeshScene),
+the Mapping object (Mapping/Mapping.h) and System::GrabImage (Core/System.h).  Device
+memory and streams come from PyTorch (ROCm); every call runs the HIP kernels of
+libfusion_amd.so on the current stream.  Images are torch tensors on the GPU: depth u16 as
+int16 (H, W), ranges float32 (H, W), vertex/normal maps float32 (H, W, 4), colour uint8
+(H, W, 3).  Poses are 4x4 camera-to-world numpy float64 arrays.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import native as N
+from .native import check
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _s():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise N.FusionError("fusion_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda")
+
+
+def pose_f(pose):
+    """Frame::GpuRotation/GpuInvRotation/GpuTranslation (Core/Frame.cc:275-294)."""
+    pose = np.asarray(pose, dtype=np.float64).reshape(4, 4)
+    p = N.PoseF()
+    R = pose[:3, :3]
+    for i in range(9):
+        p.R[i] = float(np.float32(R.flat[i]))
+        p.Rinv[i] = float(np.float32(R.T.flat[i]))
+    for i in range(3):
+        p.t[i] = float(np.float32(pose[i, 3]))
+    return p
+
+
+def level_k(K, level):
+    """Intrinsics::operator()(scale): (fx, fy, cx, cy) / 2^level in float32."""
+    s = np.float32(1 << level)
+    return tuple(float(np.float32(k) / s) for k in K)
+
+
+# --------------------------------------------------------------------- Reduction.h
+def FilterDepth(depth_u16, depth_scale, depth_cutoff):
+    rows, cols = depth_u16.shape
+    raw = torch.empty((rows, cols), dtype=torch.float32, device=depth_u16.device)
+    filt = torch.empty_like(raw)
+    check(N.lib().fu_filter_depth(_p(depth_u16), _p(raw), _p(filt), cols, rows, depth_scale, depth_cutoff, _s()),
+          "fu_filter_depth")
+    return raw, filt
+
+
+def PyrDownGauss(src):
+    rows, cols = src.shape
+    dst = torch.empty((rows // 2, cols // 2), dtype=torch.float32, device=src.device)
+    check(N.lib().fu_pyrdown_gauss(_p(src), cols, rows, _p(dst), _s()), "fu_pyrdown_gauss")
+    return dst
+
+
+def ComputeVMap(depth, fx, fy, cx, cy, depth_cutoff):
+    rows, cols = depth.shape
+    vmap = torch.empty((rows, cols, 4), dtype=torch.float32, device=depth.device)
+    check(N.lib().fu_compute_vmap(_p(depth), _p(vmap), cols, rows, fx, fy, cx, cy, depth_cutoff, _s()),
+          "fu_compute_vmap")
+    return vmap
+
+
+def ComputeNMap(vmap):
+    rows, cols, _ = vmap.shape
+    nmap = torch.empty_like(vmap)
+    check(N.lib().fu_compute_nmap(_p(vmap), _p(nmap), cols, rows, _s()), "fu_compute_nmap")
+    return nmap
+
+
+def ResizeMap(vsrc, nsrc):
+    rows, cols, _ = vsrc.shape
+    vdst = torch.empty((rows // 2, cols // 2, 4), dtype=torch.float32, device=vsrc.device)
+    ndst = torch.empty_like(vdst)
+    check(N.lib().fu_resize_map(_p(vsrc), _p(nsrc), cols, rows, _p(vdst), _p(ndst), _s()), "fu_resize_map")
+    return vdst, ndst
+
+
+def ICPStep(next_vmap, last_vmap, next_nmap, last_nmap, pose_curr, pose_last, K):
+    """One ICP reduction: returns (A 6x6, b 6, residual[2]) as the reference's ICPStep."""
+    rows, cols, _ = next_vmap.shape
+    pc, pl = pose_f(pose_curr), pose_f(pose_last)
+    res = (C.c_float * 2)()
+    A = (C.c_double * 36)()
+    b = (C.c_double * 6)()
+    check(N.lib().fu_icp_step(_p(next_vmap), _p(last_vmap), _p(next_nmap), _p(last_nmap), cols, rows,
+                              pc.R, pc.t, pl.R, pl.Rinv, pl.t, K[0], K[1], K[2], K[3], res, A, b, _s()),
+          "fu_icp_step")
+    return np.array(A[:]).reshape(6, 6), np.array(b[:]), np.array(res[:], dtype=np.float32)
+
+
+# --------------------------------------------------------------------- map (Mapping)
+def map_params(**kw):
+    p = N.MapParams()
+    N.lib().fu_map_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError("unknown map parameter %r" % k)
+        setattr(p, k, v)
+    return p
+
+
+class Mapping:
+    """Voxel-hash map: Mapping (Mapping/Mapping.h) over DeviceMap (Mapping/DeviceMap.h)."""
+
+    def __init__(self, **params):
+        _dev()
+        self.params = map_params(**params)
+        h = C.c_void_p()
+        check(N.lib().fu_map_create(C.byref(self.params), C.byref(h)), "fu_map_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            N.lib().fu_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Reset(self):
+        check(N.lib().fu_reset_map(self.handle, _s()), "fu_reset_map")
+
+    def stats(self):
+        st = N.MapStats()
+        check(N.lib().fu_map_stats_get(self.handle, C.byref(st), _s()), "fu_map_stats_get")
+        return {f: getattr(st, f) for f, _ in N.MapStats._fields_}
+
+    def FuseMapColor(self, depth, rgb, nmap, pose, K):
+        rows, cols = depth.shape
+        n = C.c_uint32()
+        check(N.lib().fu_fuse_map_color(self.handle, _p(depth), _p(rgb), _p(nmap), cols, rows,
+                                        C.byref(pose_f(pose)), K[0], K[1], K[2], K[3], self.params.depth_max,
+                                        self.params.depth_min, C.byref(n), _s()), "fu_fuse_map_color")
+        return n.value
+
+    def CheckBlockVisibility(self, pose, cols, rows, K, depth_max, depth_min):
+        n = C.c_uint32()
+        check(N.lib().fu_check_block_visibility(self.handle, C.byref(pose_f(pose)), cols, rows, K[0], K[1], K[2],
+                                                K[3], depth_max, depth_min, C.byref(n), _s()),
+              "fu_check_block_visibility")
+        return n.value
+
+    def CreateRenderingBlocks(self, zcols, zrows, pose, n_visible, K, depth_max=None, depth_min=None):
+        dmax = self.params.depth_max if depth_max is None else depth_max
+        dmin = self.params.depth_min if depth_min is None else depth_min
+        zmin = torch.empty((zrows, zcols), dtype=torch.float32, device="cuda")
+        zmax = torch.empty_like(zmin)
+        nf = C.c_uint32()
+        rendered = C.c_int32()
+        check(N.lib().fu_create_rendering_blocks(self.handle, _p(zmin), _p(zmax), zcols, zrows, dmax, dmin,
+                                                 C.byref(pose_f(pose)), n_visible, K[0], K[1], K[2], K[3],
+                                                 C.byref(nf), C.byref(rendered), _s()), "fu_create_rendering_blocks")
+        return zmin, zmax, nf.value, bool(rendered.value)
+
+    def Raycast(self, cols, rows, zmin, zmax, pose, K):
+        vmap = torch.empty((rows, cols, 4), dtype=torch.float32, device="cuda")
+        nmap = torch.empty_like(vmap)
+        zrows, zcols = zmin.shape
+        invfx = float(np.float32(1.0 / np.float64(np.float32(K[0]))))
+        invfy = float(np.float32(1.0 / np.float64(np.float32(K[1]))))
+        check(N.lib().fu_raycast(self.handle, _p(vmap), _p(nmap), cols, rows, _p(zmin), _p(zmax), zcols, zrows,
+                                 C.byref(pose_f(pose)), invfx, invfy, K[2], K[3], _s()), "fu_raycast")
+        return vmap, nmap
+
+    def RayTrace(self, n_visible, pose, cols, rows, K):
+        """Mapping::RayTrace (Mapping.cc:98-109): z-range tiles then Raycast (None if nothing to render)."""
+        zmin, zmax, nf, rendered = self.CreateRenderingBlocks((cols + 7) // 8, (rows + 7) // 8, pose, n_visible, K)
+        if not rendered:
+            return None
+        return self.Raycast(cols, rows, zmin, zmax, pose, K)
+
+    def MeshScene(self, max_triangles=2_000_000, with_meta=False):
+        """MeshScene + Mapping::CreateModel: (vertex (n,3,3), normal (n,3,3), color (n,3,3), meta)."""
+        vert = torch.empty((max_triangles, 9), dtype=torch.float32, device="cuda")
+        norm = torch.empty_like(vert)
+        col = torch.empty((max_triangles, 9), dtype=torch.uint8, device="cuda")
+        meta = torch.empty((max_triangles, 4), dtype=torch.int32, device="cuda") if with_meta else None
+        n = C.c_uint32()
+        occ = C.c_uint32()
+        check(N.lib().fu_mesh_scene(self.handle, _p(vert), _p(norm), _p(col), _p(meta), max_triangles, C.byref(n),
+                                    C.byref(occ), _s()), "fu_mesh_scene")
+        k = n.value
+        out = (vert[:k].reshape(k, 3, 3), norm[:k].reshape(k, 3, 3), col[:k].reshape(k, 3, 3))
+        return out + ((meta[:k] if meta is not None else None),)
+
+    def export_blocks(self):
+        n = C.c_uint32()
+        check(N.lib().fu_map_export_blocks(self.handle, None, 0, C.byref(n), _s()), "fu_map_export_blocks")
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        check(N.lib().fu_map_export_blocks(self.handle, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n), _s()),
+              "fu_map_export_blocks")
+        return out
+
+    def export_visible(self):
+        n = C.c_uint32()
+        check(N.lib().fu_map_export_visible(self.handle, None, 0, C.byref(n), _s()), "fu_map_export_visible")
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        check(N.lib().fu_map_export_visible(self.handle, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n), _s()),
+              "fu_map_export_visible")
+        return out
+
+    def read_blocks(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        n = xyz.shape[0]
+        sdf = np.zeros((n, 512), dtype=np.float32)
+        w = np.zeros((n, 512), dtype=np.uint16)
+        rgb = np.zeros((n, 512, 3), dtype=np.uint8)
+        check(N.lib().fu_map_read_blocks(self.handle, xyz.ctypes.data_as(C.c_void_p), n,
+                                         sdf.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                                         rgb.ctypes.data_as(C.c_void_p), _s()), "fu_map_read_blocks")
+        return sdf, w, rgb
+
+
+# --------------------------------------------------------------------- tracker
+class Tracker:
+    """Tracker::ComputeSE3 in ICP-only mode, run on the device (Tracking/Tracking.cc:296-409)."""
+
+    def __init__(self, cols, rows):
+        _dev()
+        h = C.c_void_p()
+        check(N.lib().fu_tracker_create(cols, rows, C.byref(h)), "fu_tracker_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                N.lib().fu_tracker_destroy(self.handle)
+        except Exception:
+            pass
+
+    def set_last_pose(self, pose):
+        arr = (C.c_double * 16)(*np.asarray(pose, dtype=np.float64).ravel())
+        check(N.lib().fu_tracker_set_last_pose(self.handle, arr, _s()), "fu_tracker_set_last_pose")
+
+    def ComputeSE3(self, next_vmaps, next_nmaps, last_vmaps, last_nmaps, K, iters=(10, 5, 4),
+                   min_count=(2000, 1000, 100), thresh=1e-4):
+        arr = lambda ts: (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+        check(N.lib().fu_compute_se3(self.handle, arr(next_vmaps), arr(next_nmaps), arr(last_vmaps),
+                                     arr(last_nmaps), (C.c_float * 4)(*K), (C.c_int32 * 3)(*iters),
+                                     (C.c_int32 * 3)(*min_count), thresh, _s()), "fu_compute_se3")
+        return self.result()
+
+    def result(self):
+        r = N.TrackResult()
+        check(N.lib().fu_tracker_result(self.handle, C.byref(r), _s()), "fu_tracker_result")
+        return {"pose": np.array(r.pose[:]).reshape(4, 4), "last_pose": np.array(r.last_pose[:]).reshape(4, 4),
+                "x": np.array(r.x[:]), "A": np.array(r.A[:]).reshape(6, 6), "b": np.array(r.b[:]),
+                "icp_error": r.icp_error, "count": r.count, "status": r.status, "iterations": r.iterations}
+
+
+# --------------------------------------------------------------------- system
+def system_params(cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), plain=False, use_graph=False, map=None,
+                  **kw):
+    p = N.SystemParams()
+    N.lib().fu_system_params_default(C.byref(p))
+    p.cols, p.rows = cols, rows
+    p.fx, p.fy, p.cx, p.cy = K
+    p.plain = int(plain)
+    p.use_graph = int(use_graph)
+    for k, v in kw.items():
+        if k in ("iters", "min_count"):
+            for i in range(3):
+                getattr(p, k)[i] = v[i]
+        else:
+            setattr(p, k, v)
+    if map:
+        for k, v in map.items():
+            setattr(p.map, k, v)
+    return p
+
+
+class System:
+    """System::GrabImage (Core/System.cc:92-138) in ICP-only mode."""
+
+    def __init__(self, params):
+        _dev()
+        self.params = params
+        h = C.c_void_p()
+        check(N.lib().fu_system_create(C.byref(params), C.byref(h)), "fu_system_create")
+        self.handle = h
+        b = N.SystemBuffers()
+        check(N.lib().fu_system_get_buffers(h, C.byref(b)), "fu_system_get_buffers")
+        self.buffers = b
+
+    def close(self):
+        if self.handle:
+            N.lib().fu_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(N.lib().fu_system_reset(self.handle, _s()), "fu_system_reset")
+
+    def map_handle(self):
+        return N.lib().fu_system_map(self.handle)
+
+    def GrabImage(self, depth, rgb):
+        """depth: device int16 (H, W) holding u16 values; rgb: device uint8 (H, W, 3)."""
+        check(N.lib().fu_system_grab_image_from(self.handle, _p(depth), _p(rgb), _s()), "fu_system_grab_image")
+
+    def result(self):
+        r = N.FrameResult()
+        check(N.lib().fu_system_result(self.handle, C.byref(r), _s()), "fu_system_result")
+        d = {f: getattr(r, f) for f, _ in N.FrameResult._fields_ if f != "pose"}
+        d["pose"] = np.array(r.pose[:]).reshape(4, 4)
+        return d
+
+    def view(self, name, level=0):
+        """a torch view of one of the system's device maps"""
+        b = self.buffers
+        rows, cols = self.params.rows >> level, self.params.cols >> level
+        ptr = getattr(b, name)
+        if isinstance(ptr, int) or ptr is None:
+            addr = ptr
+        else:
+            addr = ptr[level]
+        shape = {"raw": (rows, cols), "depth": (rows, cols)}.get(name, (rows, cols, 4))
+        n = int(np.prod(shape))
+        return _wrap(addr, n, shape)
+
+    def mapping(self):
+        """a Mapping view over the system's map (not owning)"""
+        m = Mapping.__new__(Mapping)
+        m.handle = C.c_void_p(self.map_handle())
+        m.params = self.params.map
+        m.close = lambda: None
+        return m
+
+
+class _DevBuf:
+    """__cuda_array_interface__ over a device buffer owned by the library (zero-copy view)"""
+
+    def __init__(self, addr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (int(addr), False),
+                                         "version": 2, "strides": None}
+
+
+def _wrap(addr, n, shape):
+    del n
+    return torch.as_tensor(_DevBuf(addr, shape, "<f4"), device="cuda")
+
+
+# --------------------------------------------------------------------- synthetic scenes
+class Scene:
+    """Seeded synthetic scene (include/fusion_scene.h)."""
+
+    def __init__(self, kind=0, size=(4.0, 4.0, 3.0), seed=0):
+        h = C.c_void_p()
+        if N.scene_lib().fu_scene_create(kind, size[0], size[1], size[2], seed, C.byref(h)) != 0:
+            raise N.FusionError("fu_scene_create failed")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            N.scene_lib().fu_scene_destroy(self.handle)
+        except Exception:
+            pass
+
+    def pose(self, frame):
+        p = (C.c_double * 16)()
+        N.scene_lib().fu_scene_pose(self.handle, frame, p)
+        return np.array(p[:]).reshape(4, 4)
+
+    def render(self, frame, cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), depth_scale=5000.0, noise=True):
+        depth = np.zeros((rows, cols), dtype=np.uint16)
+        rgb = np.zeros((rows, cols, 3), dtype=np.uint8)
+        N.scene_lib().fu_scene_render(self.handle, frame, cols, rows, K[0], K[1], K[2], K[3], depth_scale,
+                                      int(noise), depth.ctypes.data_as(C.c_void_p), rgb.ctypes.data_as(C.c_void_p))
+        return depth, rgb
+
+    def render_pose(self, pose, cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), depth_scale=5000.0,
+                    noise=False, noise_seed=0):
+        depth = np.zeros((rows, cols), dtype=np.uint16)
+        rgb = np.zeros((rows, cols, 3), dtype=np.uint8)
+        arr = (C.c_double * 16)(*np.asarray(pose, dtype=np.float64).ravel())
+        N.scene_lib().fu_scene_render_pose(self.handle, arr, cols, rows, K[0], K[1], K[2], K[3], depth_scale,
+                                           int(noise), noise_seed, depth.ctypes.data_as(C.c_void_p),
+                                           rgb.ctypes.data_as(C.c_void_p))
+        return depth, rgb
+
+    def sdf(self, x, y, z):
+        return N.scene_lib().fu_scene_sdf(self.handle, x, y, z)
+
+
+def to_device(depth_u16, rgb_u8):
+    """host numpy frame -> device tensors (depth as int16 holding the u16 bits)"""
+    d = torch.from_numpy(np.ascontiguousarray(depth_u16).view(np.int16)).to("cuda")
+    c = torch.from_numpy(np.ascontiguousarray(rgb_u8)).to("cuda")
+    return d, c

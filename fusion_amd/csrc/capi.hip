@@ -1,0 +1,1118 @@
+// C ABI of the library (include/fusion_amd.h): object lifetime, argument checking, launch
+// sequencing and the two per-frame drivers (reference-style plain control and the
+// device-resident sequence, optionally replayed as a hipGraph).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/fusion_amd.h"
+#include "fu_device.h"
+#include "fu_kernels.h"
+#include "fu_track.h"
+#include "mc_tables.h"
+
+using namespace fu;
+
+// ------------------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+static fu_status set_err(fu_status code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return set_err(FU_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                            __FILE__, __LINE__);                                  \
+    } while (0)
+
+#define LAUNCHCHK()                                                                               \
+    do {                                                                                          \
+        hipError_t e_ = hipGetLastError();                                                        \
+        if (e_ != hipSuccess) return set_err(FU_ERR_HIP, "kernel launch: %s (%s:%d)", hipGetErrorString(e_), \
+                                            __FILE__, __LINE__);                                  \
+    } while (0)
+
+static hipStream_t S(fu_stream s) { return (hipStream_t)s; }
+
+extern "C" const char* fu_last_error(void) { return g_err; }
+extern "C" const char* fu_version(void) { return "fusion_amd 0.1 (gfx950)"; }
+
+static bool g_tables_uploaded = false;
+static fu_status ensure_tables() {
+    if (!g_tables_uploaded) {
+        upload_mc_tables();
+        HIPCHK(hipGetLastError());
+        g_tables_uploaded = true;
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_mc_tables(int32_t* edge, int32_t* tri, int32_t* vert) {
+    if (!edge || !tri || !vert) return set_err(FU_ERR_ARG, "fu_mc_tables: null output");
+    for (int c = 0; c < 256; ++c) {
+        edge[c] = mc_edge_mask(c);
+        vert[c] = mc_vertex_count(c);
+        int i = 0;
+        for (const char* p = kMcTriHex[c]; *p; ++p, ++i) tri[c * 16 + i] = mc_hex_digit(*p);
+        for (; i < 16; ++i) tri[c * 16 + i] = -1;
+    }
+    return FU_OK;
+}
+
+// ------------------------------------------------------------------------------ poses
+static m3 m3_from(const float* r) { return m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])}; }
+static PoseF posef_from(const fu_pose_f* p) {
+    PoseF q;
+    q.R = m3_from(p->R);
+    q.Rinv = m3_from(p->Rinv);
+    q.t = mk3(p->t[0], p->t[1], p->t[2]);
+    return q;
+}
+
+// ------------------------------------------------------------------------------ map
+struct fu_map {
+    fu_map_params p;
+    MapView v;
+    unsigned* mesh_counts = nullptr;
+    unsigned* mesh_chunk = nullptr;
+    std::vector<void*> allocs;
+};
+
+extern "C" void fu_map_params_default(fu_map_params* p) {
+    // Mapping/DeviceMap.h:135-151 with the voxel size and truncation of the 5 mm configuration
+    p->num_buckets = 1u << 21;
+    p->num_blocks = 1u << 20;
+    p->num_excess = p->num_blocks;
+    p->voxel_size = 0.005f;
+    p->trunc_dist = 0.005f * 3.0f;
+    p->depth_min = 0.1f;
+    p->depth_max = 3.0f;
+    p->voxel_format = FU_VOXEL_F32_W8;
+    p->max_weight = 255;
+    p->max_render_blocks = 260000;
+    p->max_new_blocks = 1u << 18;
+}
+
+static unsigned newset_size(const fu_map_params* p) {
+    unsigned n = 1024;
+    while (n < 2u * p->max_new_blocks) n <<= 1;
+    return n;
+}
+
+extern "C" size_t fu_map_bytes(const fu_map_params* p) {
+    const size_t ne = (size_t)p->num_buckets + p->num_excess;
+    const size_t chunks = (p->num_blocks + 2047) / 2048;
+    return ne * sizeof(HEntry) + 64 + (size_t)p->num_blocks * (8 + 4 + 4) + newset_size(p) * 8ull +
+           (size_t)p->max_new_blocks * 12 + chunks * 4 + (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format);
+}
+
+static fu_status check_params(const fu_map_params* p) {
+    if (!p) return set_err(FU_ERR_ARG, "null map params");
+    if (p->num_buckets == 0 || p->num_blocks == 0 || p->max_new_blocks == 0)
+        return set_err(FU_ERR_ARG, "map capacities must be positive");
+    if ((size_t)p->num_buckets + p->num_excess > 0x7fffffffu) return set_err(FU_ERR_ARG, "too many hash entries");
+    if (p->num_blocks > (0x7fffffffu >> 9)) return set_err(FU_ERR_ARG, "num_blocks too large for 32-bit voxel offsets");
+    if (p->voxel_format > FU_VOXEL_F16_W8) return set_err(FU_ERR_ARG, "unknown voxel format %u", p->voxel_format);
+    const unsigned wmax = p->voxel_format == FU_VOXEL_F32_W16 ? 65535u : 255u;
+    if (p->max_weight == 0 || p->max_weight > wmax)
+        return set_err(FU_ERR_ARG, "max_weight %u outside [1, %u] for this format", p->max_weight, wmax);
+    if (!(p->voxel_size > 0) || !(p->trunc_dist > 0)) return set_err(FU_ERR_ARG, "voxel_size and trunc_dist must be > 0");
+    if (!(p->depth_min < p->depth_max)) return set_err(FU_ERR_ARG, "depth_min must be < depth_max");
+    return FU_OK;
+}
+
+template <typename T>
+static fu_status dalloc(fu_map* m, T** out, size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(FU_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    }
+    m->allocs.push_back(p);
+    *out = (T*)p;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_destroy(fu_map* m) {
+    if (!m) return FU_OK;
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_create(const fu_map_params* p, fu_map** out) {
+    if (!out) return set_err(FU_ERR_ARG, "null output");
+    *out = nullptr;
+    fu_status st = check_params(p);
+    if (st) return st;
+    if ((st = ensure_tables())) return st;
+    fu_map* m = new fu_map();
+    m->p = *p;
+    MapView& v = m->v;
+    const unsigned ns = newset_size(p);
+    const size_t chunks = (p->num_blocks + 2047) / 2048;
+    if ((st = dalloc(m, &v.entries, (size_t)p->num_buckets + p->num_excess)) ||
+        (st = dalloc(m, &v.ctr, (size_t)C_COUNT)) || (st = dalloc(m, &v.block_key, p->num_blocks)) ||
+        (st = dalloc(m, &v.visible, p->num_blocks)) || (st = dalloc(m, &v.newset, ns)) ||
+        (st = dalloc(m, &v.newkeys, p->max_new_blocks)) || (st = dalloc(m, &v.newslot, p->max_new_blocks)) ||
+        (st = dalloc(m, &m->mesh_counts, p->num_blocks)) || (st = dalloc(m, &m->mesh_chunk, chunks)) ||
+        (st = dalloc(m, &v.pool, (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format)))) {
+        fu_map_destroy(m);
+        return st;
+    }
+    v.num_buckets = p->num_buckets;
+    v.num_excess = p->num_excess;
+    v.num_blocks = p->num_blocks;
+    v.newset_mask = ns - 1;
+    v.max_new = p->max_new_blocks;
+    v.max_weight = p->max_weight;
+    v.fmt = (int)p->voxel_format;
+    v.voxel_size = p->voxel_size;
+    v.voxel_inv = (float)(1.0 / (double)p->voxel_size);  // DeviceMap.h:148
+    v.trunc = p->trunc_dist;
+    v.depth_min = p->depth_min;
+    v.depth_max = p->depth_max;
+    launch_reset_map(v, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        fu_map_destroy(m);
+        return set_err(FU_ERR_HIP, "map reset failed");
+    }
+    *out = m;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_get_params(const fu_map* m, fu_map_params* out) {
+    if (!m || !out) return set_err(FU_ERR_ARG, "null argument");
+    *out = m->p;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_reset_map(fu_map* m, fu_stream s) {
+    if (!m) return set_err(FU_ERR_ARG, "null map");
+    launch_reset_map(m->v, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_stats_get(fu_map* m, fu_map_stats* out, fu_stream s) {
+    if (!m || !out) return set_err(FU_ERR_ARG, "null argument");
+    unsigned c[C_COUNT];
+    HIPCHK(hipMemcpyAsync(c, m->v.ctr, sizeof(c), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    out->n_allocated = c[C_ALLOC] < m->p.num_blocks ? c[C_ALLOC] : m->p.num_blocks;
+    out->n_excess = c[C_EXCESS] < m->p.num_excess ? c[C_EXCESS] : m->p.num_excess;
+    out->n_visible = c[C_VISIBLE];
+    out->n_new = c[C_NEW];
+    out->n_fragments = c[C_RENDER];
+    out->n_triangles = c[C_TRI];
+    out->flags = c[C_FLAGS];
+    out->capacity = m->p.num_blocks;
+    return FU_OK;
+}
+
+static unsigned read_ctr(fu_map* m, int which, hipStream_t s, fu_status* st) {
+    unsigned v = 0;
+    if (hipMemcpyAsync(&v, m->v.ctr + which, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        *st = set_err(FU_ERR_HIP, "counter read failed");
+    return v;
+}
+
+static FuseArgs make_fuse(const fu_map* m, const float* depth, const uint8_t* rgb, const float* nmap, int cols,
+                          int rows, const PoseF& pose, const TrackState* st, float fx, float fy, float cx, float cy) {
+    FuseArgs a;
+    a.pose = pose;
+    a.st = st;
+    a.fx = fx;
+    a.fy = fy;
+    a.cx = cx;
+    a.cy = cy;
+    a.invfx = (float)(1.0 / (double)fx);  // FuseMap.cu:280-281
+    a.invfy = (float)(1.0 / (double)fy);
+    a.cols = cols;
+    a.rows = rows;
+    a.depth = depth;
+    a.nmap = (const f4*)nmap;
+    a.rgb = rgb;
+    (void)m;
+    return a;
+}
+
+static VisArgs make_vis(const fu_map* m, const PoseF& pose, const TrackState* st, int cols, int rows, float fx,
+                        float fy, float cx, float cy, float dmin, float dmax) {
+    VisArgs v;
+    v.pose = pose;
+    v.st = st;
+    v.fx = fx;
+    v.fy = fy;
+    v.cx = cx;
+    v.cy = cy;
+    v.cols = cols;
+    v.rows = rows;
+    v.depth_min = dmin;
+    v.depth_max = dmax;
+    (void)m;
+    return v;
+}
+
+static constexpr unsigned kIntegrateGrid = 8192;
+
+extern "C" fu_status fu_fuse_map_color(fu_map* m, const float* depth, const uint8_t* rgb, const float* nmap, int cols,
+                                       int rows, const fu_pose_f* pose, float fx, float fy, float cx, float cy,
+                                       float depth_max, float depth_min, uint32_t* n_visible, fu_stream s) {
+    (void)depth_max;
+    (void)depth_min;
+    if (!m || !depth || !rgb || !nmap || !pose) return set_err(FU_ERR_ARG, "fu_fuse_map_color: null argument");
+    if (cols <= 0 || rows <= 0) return set_err(FU_ERR_ARG, "fu_fuse_map_color: bad size %dx%d", cols, rows);
+    const PoseF p = posef_from(pose);
+    const FuseArgs a = make_fuse(m, depth, rgb, nmap, cols, rows, p, nullptr, fx, fy, cx, cy);
+    // the fusion visibility test uses DeviceMap::DepthMin/DepthMax (FuseMap.cu:288-289)
+    const VisArgs v = make_vis(m, p, nullptr, cols, rows, fx, fy, cx, cy, m->p.depth_min, m->p.depth_max);
+    launch_begin_pass(m->v, 3, S(s));
+    launch_alloc(m->v, a, S(s));
+    launch_visible(m->v, v, S(s));
+    launch_integrate(m->v, a, kIntegrateGrid, S(s));
+    LAUNCHCHK();
+    if (n_visible) {
+        fu_status st = FU_OK;
+        *n_visible = read_ctr(m, C_VISIBLE, S(s), &st);
+        return st;
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_check_block_visibility(fu_map* m, const fu_pose_f* pose, int cols, int rows, float fx,
+                                               float fy, float cx, float cy, float depth_max, float depth_min,
+                                               uint32_t* n_visible, fu_stream s) {
+    if (!m || !pose) return set_err(FU_ERR_ARG, "fu_check_block_visibility: null argument");
+    const VisArgs v = make_vis(m, posef_from(pose), nullptr, cols, rows, fx, fy, cx, cy, depth_min, depth_max);
+    launch_begin_pass(m->v, 2, S(s));
+    launch_visible(m->v, v, S(s));
+    LAUNCHCHK();
+    if (n_visible) {
+        fu_status st = FU_OK;
+        *n_visible = read_ctr(m, C_VISIBLE, S(s), &st);
+        return st;
+    }
+    return FU_OK;
+}
+
+static ZRangeArgs make_zr(const fu_map* m, float* zmin, float* zmax, int zc, int zr, float dmax, float dmin,
+                          const PoseF& p, const TrackState* st, int nvis_override, float fx, float fy, float cx,
+                          float cy) {
+    ZRangeArgs a;
+    a.pose = p;
+    a.st = st;
+    a.fx = fx;
+    a.fy = fy;
+    a.cx = cx;
+    a.cy = cy;
+    a.depth_min = dmin;
+    a.depth_max = dmax;
+    a.zmin = (unsigned*)zmin;
+    a.zmax = (unsigned*)zmax;
+    a.zcols = zc;
+    a.zrows = zr;
+    a.max_fragments = m->p.max_render_blocks;
+    a.n_visible_override = nvis_override;
+    return a;
+}
+
+extern "C" fu_status fu_create_rendering_blocks(fu_map* m, float* zrange_min, float* zrange_max, int zcols,
+                                                int zrows, float depth_max, float depth_min, const fu_pose_f* pose,
+                                                uint32_t n_visible, float fx, float fy, float cx, float cy,
+                                                uint32_t* n_fragments, int* rendered, fu_stream s) {
+    if (!m || !zrange_min || !zrange_max || !pose) return set_err(FU_ERR_ARG, "fu_create_rendering_blocks: null argument");
+    if (zcols <= 0 || zrows <= 0 || zcols > 32767 || zrows > 32767) return set_err(FU_ERR_ARG, "bad z-range size");
+    if (n_visible == 0) {  // RenderScene.cu:214-215
+        if (n_fragments) *n_fragments = 0;
+        if (rendered) *rendered = 0;
+        return FU_OK;
+    }
+    int ov = n_visible == 0xffffffffu ? -1 : (int)(n_visible < m->p.num_blocks ? n_visible : m->p.num_blocks);
+    const ZRangeArgs a =
+        make_zr(m, zrange_min, zrange_max, zcols, zrows, depth_max, depth_min, posef_from(pose), nullptr, ov, fx, fy, cx, cy);
+    launch_begin_pass(m->v, 4, S(s));
+    launch_zrange(m->v, a, S(s));
+    LAUNCHCHK();
+    if (n_fragments || rendered) {
+        fu_status st = FU_OK;
+        const unsigned n = read_ctr(m, C_RENDER, S(s), &st);
+        if (n_fragments) *n_fragments = n;
+        if (rendered) *rendered = n > 0;
+        return st;
+    }
+    return FU_OK;
+}
+
+static RaycastArgs make_rc(const fu_map* m, float* vmap, float* nmap, int cols, int rows, const float* zmin,
+                           const float* zmax, int zc, int zr, const PoseF& p, const TrackState* st, float invfx,
+                           float invfy, float cx, float cy) {
+    RaycastArgs a;
+    a.pose = p;
+    a.st = st;
+    a.invfx = invfx;
+    a.invfy = invfy;
+    a.cx = cx;
+    a.cy = cy;
+    // DeviceMap::stepScale = 0.5 * TruncateDist * voxelSizeInv, evaluated in double (DeviceMap.h:151)
+    a.step_scale = (float)(0.5 * (double)m->p.trunc_dist * (double)m->v.voxel_inv);
+    a.cols = cols;
+    a.rows = rows;
+    a.zmin = (const unsigned*)zmin;
+    a.zmax = (const unsigned*)zmax;
+    a.zcols = zc;
+    a.zrows = zr;
+    a.vmap = (f4*)vmap;
+    a.nmap = (f4*)nmap;
+    a.keep_v = nullptr;
+    a.keep_n = nullptr;
+    return a;
+}
+
+extern "C" fu_status fu_raycast(fu_map* m, float* vmap, float* nmap, int cols, int rows, const float* zrange_min,
+                                const float* zrange_max, int zcols, int zrows, const fu_pose_f* pose, float invfx,
+                                float invfy, float cx, float cy, fu_stream s) {
+    if (!m || !vmap || !nmap || !zrange_min || !zrange_max || !pose) return set_err(FU_ERR_ARG, "fu_raycast: null argument");
+    if (zcols < (cols + 7) / 8 || zrows < (rows + 7) / 8) return set_err(FU_ERR_ARG, "z-range maps smaller than the image tiles");
+    const RaycastArgs a = make_rc(m, vmap, nmap, cols, rows, zrange_min, zrange_max, zcols, zrows, posef_from(pose),
+                                  nullptr, invfx, invfy, cx, cy);
+    launch_raycast(m->v, a, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_mesh_scene(fu_map* m, float* vertex, float* normal, uint8_t* color, int32_t* tri_meta,
+                                   uint32_t max_triangles, uint32_t* n_triangles, uint32_t* n_occupied, fu_stream s) {
+    if (!m || !vertex || !normal || !color) return set_err(FU_ERR_ARG, "fu_mesh_scene: null argument");
+    MeshOut o;
+    o.vertex = vertex;
+    o.normal = normal;
+    o.color = color;
+    o.tri_meta = tri_meta;
+    o.max_tri = max_triangles;
+    launch_mesh(m->v, o, m->mesh_counts, m->mesh_chunk, S(s));
+    LAUNCHCHK();
+    fu_status st = FU_OK;
+    const unsigned tot = read_ctr(m, C_TRI, S(s), &st);
+    if (st) return st;
+    if (tot > max_triangles) {
+        unsigned f = FU_FLAG_TRI_FULL;
+        HIPCHK(hipMemcpy(&f, m->v.ctr + C_FLAGS, 4, hipMemcpyDeviceToHost));
+        f |= FU_FLAG_TRI_FULL;
+        HIPCHK(hipMemcpy(m->v.ctr + C_FLAGS, &f, 4, hipMemcpyHostToDevice));
+    }
+    if (n_triangles) *n_triangles = tot < max_triangles ? tot : max_triangles;
+    if (n_occupied) *n_occupied = read_ctr(m, C_OCC, S(s), &st);
+    return st;
+}
+
+// ---- inspection
+extern "C" fu_status fu_map_export_blocks(fu_map* m, int32_t* xyz, uint32_t cap, uint32_t* n, fu_stream s) {
+    if (!m || !n) return set_err(FU_ERR_ARG, "null argument");
+    fu_status st = FU_OK;
+    unsigned na = read_ctr(m, C_ALLOC, S(s), &st);
+    if (st) return st;
+    if (na > m->p.num_blocks) na = m->p.num_blocks;
+    std::vector<unsigned long long> keys(na);
+    if (na) HIPCHK(hipMemcpy(keys.data(), m->v.block_key, na * 8ull, hipMemcpyDeviceToHost));
+    unsigned k = 0;
+    for (unsigned i = 0; i < na; ++i) {
+        if (keys[i] == kEmptyKey) continue;
+        if (xyz && k < cap) {
+            const i3 p = unpack_key(keys[i]);
+            xyz[3 * k] = p.x;
+            xyz[3 * k + 1] = p.y;
+            xyz[3 * k + 2] = p.z;
+        }
+        ++k;
+    }
+    *n = k;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_export_visible(fu_map* m, int32_t* xyz, uint32_t cap, uint32_t* n, fu_stream s) {
+    if (!m || !n) return set_err(FU_ERR_ARG, "null argument");
+    fu_status st = FU_OK;
+    unsigned nv = read_ctr(m, C_VISIBLE, S(s), &st);
+    if (st) return st;
+    if (nv > m->p.num_blocks) nv = m->p.num_blocks;
+    std::vector<unsigned> ids(nv);
+    std::vector<unsigned long long> keys(m->p.num_blocks);
+    if (nv) HIPCHK(hipMemcpy(ids.data(), m->v.visible, nv * 4ull, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(keys.data(), m->v.block_key, keys.size() * 8ull, hipMemcpyDeviceToHost));
+    for (unsigned i = 0; i < nv && xyz; ++i) {
+        if (i >= cap) break;
+        const i3 p = unpack_key(keys[ids[i]]);
+        xyz[3 * i] = p.x;
+        xyz[3 * i + 1] = p.y;
+        xyz[3 * i + 2] = p.z;
+    }
+    *n = nv;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_map_read_blocks(fu_map* m, const int32_t* xyz, uint32_t n, float* sdf, uint16_t* weight,
+                                        uint8_t* rgb, fu_stream s) {
+    if (!m || (n && !xyz)) return set_err(FU_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(S(s)));
+    // locate blocks on the host from the key array (inspection path, not performance critical)
+    fu_status st = FU_OK;
+    unsigned na = read_ctr(m, C_ALLOC, S(s), &st);
+    if (st) return st;
+    if (na > m->p.num_blocks) na = m->p.num_blocks;
+    std::vector<unsigned long long> keys(na);
+    if (na) HIPCHK(hipMemcpy(keys.data(), m->v.block_key, na * 8ull, hipMemcpyDeviceToHost));
+    std::vector<std::pair<unsigned long long, unsigned>> idx;
+    idx.reserve(na);
+    for (unsigned i = 0; i < na; ++i)
+        if (keys[i] != kEmptyKey) idx.push_back({keys[i], i});
+    std::sort(idx.begin(), idx.end());
+    const int F = m->v.fmt;
+    const int bb = vf_block_bytes(F), sb = vf_sdf_bytes(F), wb = vf_w_bytes(F);
+    std::vector<unsigned char> buf(bb);
+    for (unsigned b = 0; b < n; ++b) {
+        const i3 p = mki3(xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2]);
+        long heap = -1;
+        if (key_in_range(p)) {
+            const unsigned long long k = pack_key(p);
+            auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(k, 0u));
+            if (it != idx.end() && it->first == k) heap = it->second;
+        }
+        for (int i = 0; i < 512; ++i) {
+            float sv = NAN;
+            unsigned wv = 0, r = 0, g = 0, bl = 0;
+            if (heap >= 0) {
+                if (i == 0) HIPCHK(hipMemcpy(buf.data(), m->v.pool + (size_t)heap * bb, bb, hipMemcpyDeviceToHost));
+                if (sb == 4) {
+                    memcpy(&sv, &buf[i * 4], 4);
+                } else {
+                    uint16_t h;
+                    memcpy(&h, &buf[i * 2], 2);
+                    sv = __half2float(*reinterpret_cast<__half*>(&h));
+                }
+                if (wb == 2) {
+                    uint16_t w16;
+                    memcpy(&w16, &buf[512 * sb + i * 2], 2);
+                    wv = w16;
+                } else {
+                    wv = buf[512 * sb + i];
+                }
+                r = buf[512 * (sb + wb) + i];
+                g = buf[512 * (sb + wb) + 512 + i];
+                bl = buf[512 * (sb + wb) + 1024 + i];
+            }
+            if (sdf) sdf[(size_t)b * 512 + i] = sv;
+            if (weight) weight[(size_t)b * 512 + i] = (uint16_t)wv;
+            if (rgb) {
+                rgb[((size_t)b * 512 + i) * 3 + 0] = (uint8_t)r;
+                rgb[((size_t)b * 512 + i) * 3 + 1] = (uint8_t)g;
+                rgb[((size_t)b * 512 + i) * 3 + 2] = (uint8_t)bl;
+            }
+        }
+    }
+    return FU_OK;
+}
+
+// ------------------------------------------------------------------------------ frame functions
+static fu_status check_img(const void* a, int cols, int rows, const char* what) {
+    if (!a) return set_err(FU_ERR_ARG, "%s: null image", what);
+    if (cols <= 0 || rows <= 0 || (long)cols * rows > (1l << 28)) return set_err(FU_ERR_ARG, "%s: bad size %dx%d", what, cols, rows);
+    return FU_OK;
+}
+
+extern "C" fu_status fu_filter_depth(const uint16_t* depth, float* raw, float* filtered, int cols, int rows,
+                                     float depth_scale, float depth_cutoff, fu_stream s) {
+    fu_status st = check_img(depth, cols, rows, "fu_filter_depth");
+    if (st) return st;
+    if (!raw || !filtered) return set_err(FU_ERR_ARG, "fu_filter_depth: null output");
+    // FilterDepth passes 1.0 / depthScale computed in double (Pyrdown.cu:50)
+    launch_filter_depth(depth, raw, filtered, cols, rows, (float)(1.0 / (double)depth_scale), depth_cutoff, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_pyrdown_gauss(const float* src, int cols, int rows, float* dst, fu_stream s) {
+    fu_status st = check_img(src, cols, rows, "fu_pyrdown_gauss");
+    if (st) return st;
+    if (!dst || cols < 2 || rows < 2) return set_err(FU_ERR_ARG, "fu_pyrdown_gauss: bad destination");
+    launch_pyrdown(src, cols, rows, dst, cols / 2, rows / 2, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_compute_vmap(const float* depth, float* vmap, int cols, int rows, float fx, float fy, float cx,
+                                     float cy, float depth_cutoff, fu_stream s) {
+    fu_status st = check_img(depth, cols, rows, "fu_compute_vmap");
+    if (st) return st;
+    if (!vmap) return set_err(FU_ERR_ARG, "fu_compute_vmap: null output");
+    // ComputeVMap passes 1.0 / fx computed in double (Pyrdown.cu:87)
+    launch_vmap(depth, (f4*)vmap, cols, rows, (float)(1.0 / (double)fx), (float)(1.0 / (double)fy), cx, cy,
+                depth_cutoff, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_compute_nmap(const float* vmap, float* nmap, int cols, int rows, fu_stream s) {
+    fu_status st = check_img(vmap, cols, rows, "fu_compute_nmap");
+    if (st) return st;
+    if (!nmap) return set_err(FU_ERR_ARG, "fu_compute_nmap: null output");
+    launch_nmap((const f4*)vmap, (f4*)nmap, cols, rows, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int cols, int rows, float* vdst, float* ndst,
+                                   fu_stream s) {
+    fu_status st = check_img(vsrc, cols, rows, "fu_resize_map");
+    if (st) return st;
+    if (!nsrc || !vdst || !ndst || cols < 2 || rows < 2) return set_err(FU_ERR_ARG, "fu_resize_map: bad argument");
+    launch_resize((const f4*)vsrc, (const f4*)nsrc, cols, (f4*)vdst, (f4*)ndst, cols / 2, rows / 2, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+// ------------------------------------------------------------------------------ tracker
+struct fu_tracker {
+    int cols, rows;
+    TrackState* st = nullptr;
+    float* partials = nullptr;
+    unsigned* done = nullptr;
+    int nblocks[3];
+};
+
+static int icp_blocks(int npix) {
+    int nb = (npix + 1023) / 1024;  // about four pixels per lane
+    if (nb < 1) nb = 1;
+    if (nb > 1024) nb = 1024;
+    return nb;
+}
+
+static void init_state(TrackState* h) {
+    memset(h, 0, sizeof(*h));
+    for (int i = 0; i < 16; ++i) {
+        const double I = (i % 5 == 0) ? 1.0 : 0.0;
+        h->last_pose[i] = h->next_pose[i] = h->init_pose[i] = h->delta[i] = I;
+    }
+    h->status = TS_FIRST;
+    refresh_float_pose(h);
+}
+
+extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
+    if (!out) return set_err(FU_ERR_ARG, "null output");
+    *out = nullptr;
+    if (cols < 4 || rows < 4) return set_err(FU_ERR_ARG, "tracker size %dx%d too small", cols, rows);
+    fu_tracker* t = new fu_tracker();
+    t->cols = cols;
+    t->rows = rows;
+    for (int l = 0; l < 3; ++l) t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
+    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, 1024 * 32 * 4) != hipSuccess ||
+        hipMalloc(&t->done, 64) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(t->st);
+        (void)hipFree(t->partials);
+        (void)hipFree(t->done);
+        delete t;
+        return set_err(FU_ERR_NOMEM, "tracker allocation failed");
+    }
+    TrackState h;
+    init_state(&h);
+    HIPCHK(hipMemcpy(t->st, &h, sizeof(h), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(t->done, 0, 64));
+    *out = t;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_tracker_destroy(fu_tracker* t) {
+    if (!t) return FU_OK;
+    (void)hipFree(t->st);
+    (void)hipFree(t->partials);
+    (void)hipFree(t->done);
+    delete t;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_tracker_set_last_pose(fu_tracker* t, const double pose[16], fu_stream s) {
+    if (!t || !pose) return set_err(FU_ERR_ARG, "null argument");
+    TrackState h;
+    HIPCHK(hipStreamSynchronize(S(s)));
+    HIPCHK(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i] = pose[i];
+    refresh_float_pose(&h);
+    HIPCHK(hipMemcpy(t->st, &h, sizeof(h), hipMemcpyHostToDevice));
+    return FU_OK;
+}
+
+static IcpArgs make_icp(fu_tracker* t, int level, const float* nv, const float* nn, const float* lv, const float* ln,
+                        const float K[4], int mode, int min_count, int final_iter, float thresh) {
+    IcpArgs a;
+    a.vcurr = (const f4*)nv;
+    a.ncurr = (const f4*)nn;
+    a.vlast = (const f4*)lv;
+    a.nlast = (const f4*)ln;
+    a.cols = t->cols >> level;
+    a.rows = t->rows >> level;
+    // Intrinsics::operator()(scale): K / (1 << scale) in float (Utility/Intrinsics.h:13-16)
+    const float sc = (float)(1 << level);
+    a.fx = K[0] / sc;
+    a.fy = K[1] / sc;
+    a.cx = K[2] / sc;
+    a.cy = K[3] / sc;
+    a.st = t->st;
+    a.partials = t->partials;
+    a.done = t->done;
+    a.mode = mode;
+    a.min_count = min_count;
+    a.final_iter = final_iter;
+    a.thresh = thresh;
+    return a;
+}
+
+// queue the whole coarse-to-fine loop (device-resident)
+static void queue_se3(fu_tracker* t, const float* const nv[3], const float* const nn[3], const float* const lv[3],
+                      const float* const ln[3], const float K[4], const int32_t iters[3], const int32_t minc[3],
+                      float thresh, hipStream_t s) {
+    launch_track_begin(t->st, s);
+    int last_level = -1;
+    for (int l = 2; l >= 0; --l)
+        if (iters[l] > 0) last_level = l;
+    for (int l = 2; l >= 0; --l)
+        for (int j = 0; j < iters[l]; ++j) {
+            const int fin = (l == last_level && j == iters[l] - 1);
+            launch_icp(make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], fin, thresh), t->nblocks[l], s);
+        }
+}
+
+extern "C" fu_status fu_compute_se3(fu_tracker* t, const float* const next_vmap[3], const float* const next_nmap[3],
+                                    const float* const last_vmap[3], const float* const last_nmap[3], const float K[4],
+                                    const int32_t iters[3], const int32_t min_count[3], float thresh, fu_stream s) {
+    if (!t || !next_vmap || !next_nmap || !last_vmap || !last_nmap || !K || !iters || !min_count)
+        return set_err(FU_ERR_ARG, "fu_compute_se3: null argument");
+    for (int l = 0; l < 3; ++l)
+        if (!next_vmap[l] || !next_nmap[l] || !last_vmap[l] || !last_nmap[l] || iters[l] < 0)
+            return set_err(FU_ERR_ARG, "fu_compute_se3: bad level %d", l);
+    queue_se3(t, next_vmap, next_nmap, last_vmap, last_nmap, K, iters, min_count, thresh, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+static void fill_result(const TrackState& h, fu_track_result* out) {
+    memcpy(out->pose, h.next_pose, sizeof(out->pose));
+    memcpy(out->last_pose, h.last_pose, sizeof(out->last_pose));
+    memcpy(out->x, h.x, sizeof(out->x));
+    memcpy(out->A, h.A, sizeof(out->A));
+    memcpy(out->b, h.b, sizeof(out->b));
+    out->icp_error = h.icp_error;
+    out->count = h.count;
+    out->status = h.status;
+    out->iterations = h.iters_done;
+}
+
+extern "C" fu_status fu_tracker_result(fu_tracker* t, fu_track_result* out, fu_stream s) {
+    if (!t || !out) return set_err(FU_ERR_ARG, "null argument");
+    TrackState h;
+    HIPCHK(hipMemcpyAsync(&h, t->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    fill_result(h, out);
+    return FU_OK;
+}
+
+extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const float* next_nmap,
+                                 const float* last_nmap, int cols, int rows, const float Rcurr[9], const float tcurr[3],
+                                 const float Rlast[9], const float RlastInv[9], const float tlast[3], float fx,
+                                 float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                                 fu_stream s) {
+    if (!next_vmap || !last_vmap || !next_nmap || !last_nmap || !Rcurr || !tcurr || !Rlast || !RlastInv || !tlast)
+        return set_err(FU_ERR_ARG, "fu_icp_step: null argument");
+    if (cols <= 0 || rows <= 0) return set_err(FU_ERR_ARG, "fu_icp_step: bad size");
+    // scratch state per call: the reference's Tracker owns sumSE3/outSE3 (Tracking.cc:29-30)
+    static thread_local fu_tracker* scratch = nullptr;
+    if (!scratch) {
+        fu_status st = fu_tracker_create(8, 8, &scratch);
+        if (st) return st;
+    }
+    TrackState h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.Rcurr, Rcurr, 36);
+    memcpy(h.tcurr, tcurr, 12);
+    memcpy(h.Rlast, Rlast, 36);
+    memcpy(h.RlastInv, RlastInv, 36);
+    memcpy(h.tlast, tlast, 12);
+    h.status = TS_RUNNING;
+    HIPCHK(hipMemcpyAsync(scratch->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    IcpArgs a;
+    memset(&a, 0, sizeof(a));
+    a.vcurr = (const f4*)next_vmap;
+    a.ncurr = (const f4*)next_nmap;
+    a.vlast = (const f4*)last_vmap;
+    a.nlast = (const f4*)last_nmap;
+    a.cols = cols;
+    a.rows = rows;
+    a.fx = fx;
+    a.fy = fy;
+    a.cx = cx;
+    a.cy = cy;
+    a.st = scratch->st;
+    a.partials = scratch->partials;
+    a.done = scratch->done;
+    a.mode = 0;
+    launch_icp(a, icp_blocks(cols * rows), S(s));
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&h, scratch->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    // CreateMatrix<6,7> (Reduction.cu:6-16)
+    int shift = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) {
+            const double v = (double)h.sums[shift++];
+            if (j == 6) {
+                if (b) b[i] = v;
+            } else if (A) {
+                A[j * 6 + i] = A[i * 6 + j] = v;
+            }
+        }
+    if (residual) {
+        residual[0] = h.sums[27];
+        residual[1] = h.sums[28];
+    }
+    return FU_OK;
+}
+
+// ------------------------------------------------------------------------------ system
+struct fu_system {
+    fu_system_params p;
+    fu_map* map = nullptr;
+    fu_tracker* trk = nullptr;
+    std::vector<void*> allocs;
+    fu_system_buffers b;
+    int lc[3], lr[3];
+    unsigned frame = 0;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    const uint16_t* graph_depth = nullptr;
+    const uint8_t* graph_rgb = nullptr;
+};
+
+extern "C" void fu_system_params_default(fu_system_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->cols = 640;
+    p->rows = 480;
+    p->fx = p->fy = 525.0f;
+    p->cx = 319.5f;
+    p->cy = 239.5f;
+    p->depth_scale = 5000.0f;
+    p->depth_cutoff = 3.0f;
+    p->iters[0] = 10;
+    p->iters[1] = 5;
+    p->iters[2] = 4;
+    p->min_count[0] = 2000;
+    p->min_count[1] = 1000;
+    p->min_count[2] = 100;
+    p->icp_thresh = 1e-4f;
+    p->plain = 0;
+    p->use_graph = 0;
+    fu_map_params_default(&p->map);
+}
+
+template <typename T>
+static fu_status salloc(fu_system* s, T** out, size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T) + 16) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(FU_ERR_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    }
+    s->allocs.push_back(p);
+    *out = (T*)p;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_destroy(fu_system* sys) {
+    if (!sys) return FU_OK;
+    if (sys->graph) (void)hipGraphExecDestroy(sys->graph);
+    for (void* p : sys->allocs) (void)hipFree(p);
+    fu_map_destroy(sys->map);
+    fu_tracker_destroy(sys->trk);
+    delete sys;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_create(const fu_system_params* p, fu_system** out) {
+    if (!p || !out) return set_err(FU_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->cols < 32 || p->rows < 32 || (p->cols % 4) || (p->rows % 4))
+        return set_err(FU_ERR_ARG, "frame size %dx%d must be >= 32 and a multiple of 4", p->cols, p->rows);
+    if (!(p->depth_scale > 0) || !(p->fx > 0) || !(p->fy > 0)) return set_err(FU_ERR_ARG, "bad camera parameters");
+    fu_system* sys = new fu_system();
+    sys->p = *p;
+    fu_status st;
+    if ((st = fu_map_create(&p->map, &sys->map)) || (st = fu_tracker_create(p->cols, p->rows, &sys->trk))) {
+        fu_system_destroy(sys);
+        return st;
+    }
+    fu_system_buffers& b = sys->b;
+    memset(&b, 0, sizeof(b));
+    const size_t n0 = (size_t)p->cols * p->rows;
+    for (int l = 0; l < 3; ++l) {
+        sys->lc[l] = p->cols >> l;
+        sys->lr[l] = p->rows >> l;
+    }
+    b.zcols = (p->cols + 7) / 8;
+    b.zrows = (p->rows + 7) / 8;
+    if ((st = salloc(sys, &b.depth_in, n0)) || (st = salloc(sys, &b.rgb_in, n0 * 3)) || (st = salloc(sys, &b.raw, n0)) ||
+        (st = salloc(sys, &b.zrange_min, (size_t)b.zcols * b.zrows)) ||
+        (st = salloc(sys, &b.zrange_max, (size_t)b.zcols * b.zrows))) {
+        fu_system_destroy(sys);
+        return st;
+    }
+    for (int l = 0; l < 3; ++l) {
+        const size_t n = (size_t)sys->lc[l] * sys->lr[l];
+        if ((st = salloc(sys, &b.depth[l], n)) || (st = salloc(sys, &b.vmap[l], n * 4)) ||
+            (st = salloc(sys, &b.nmap[l], n * 4)) || (st = salloc(sys, &b.model_vmap[l], n * 4)) ||
+            (st = salloc(sys, &b.model_nmap[l], n * 4))) {
+            fu_system_destroy(sys);
+            return st;
+        }
+    }
+    *out = sys;
+    return fu_system_reset(sys, nullptr);
+}
+
+extern "C" fu_status fu_system_reset(fu_system* sys, fu_stream s) {
+    if (!sys) return set_err(FU_ERR_ARG, "null system");
+    fu_status st = fu_reset_map(sys->map, s);
+    if (st) return st;
+    TrackState h;
+    init_state(&h);
+    HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    sys->frame = 0;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_get_buffers(fu_system* sys, fu_system_buffers* out) {
+    if (!sys || !out) return set_err(FU_ERR_ARG, "null argument");
+    *out = sys->b;
+    return FU_OK;
+}
+
+extern "C" fu_map* fu_system_map(fu_system* sys) { return sys ? sys->map : nullptr; }
+
+// Frame::FillImages (Core/Frame.cc:59-78) without the intensity/derivative images
+static void queue_pyramid(fu_system* sys, const uint16_t* depth, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    launch_filter_depth(depth, b.raw, b.depth[0], p.cols, p.rows, (float)(1.0 / (double)p.depth_scale), p.depth_cutoff, s);
+    for (int l = 1; l < 3; ++l) launch_pyrdown(b.depth[l - 1], sys->lc[l - 1], sys->lr[l - 1], b.depth[l], sys->lc[l], sys->lr[l], s);
+    for (int l = 0; l < 3; ++l) {
+        const float sc = (float)(1 << l);  // Frame::SetK: K / 2^l (Core/Frame.cc:229-237)
+        const float fx = p.fx / sc, fy = p.fy / sc, cx = p.cx / sc, cy = p.cy / sc;
+        launch_vmap(b.depth[l], (f4*)b.vmap[l], sys->lc[l], sys->lr[l], (float)(1.0 / (double)fx),
+                    (float)(1.0 / (double)fy), cx, cy, p.depth_cutoff, s);
+        launch_nmap((const f4*)b.vmap[l], (f4*)b.nmap[l], sys->lc[l], sys->lr[l], s);
+    }
+}
+
+// LastFrame->ResizeImages (Core/Frame.cc:80-84) on the model maps
+static void queue_resize_model(fu_system* sys, hipStream_t s) {
+    fu_system_buffers& b = sys->b;
+    for (int l = 1; l < 3; ++l)
+        launch_resize((const f4*)b.model_vmap[l - 1], (const f4*)b.model_nmap[l - 1], sys->lc[l - 1],
+                      (f4*)b.model_vmap[l], (f4*)b.model_nmap[l], sys->lc[l], sys->lr[l], s);
+}
+
+// Mapping::FuseColor + Mapping::RayTrace (Mapping.cc:76-109), pose from the tracking state
+static void queue_fuse_raycast(fu_system* sys, const uint8_t* rgb, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    fu_map* m = sys->map;
+    const TrackState* st = sys->trk->st;
+    PoseF dummy{};
+    const FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, st, p.fx, p.fy, p.cx, p.cy);
+    const VisArgs v = make_vis(m, dummy, st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
+    launch_begin_pass(m->v, 7, s);
+    launch_alloc(m->v, a, s);
+    launch_visible(m->v, v, s);
+    launch_integrate(m->v, a, kIntegrateGrid, s);
+    const ZRangeArgs z = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min, dummy,
+                                 st, -1, p.fx, p.fy, p.cx, p.cy);
+    launch_zrange(m->v, z, s);
+    RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max, b.zcols,
+                            b.zrows, dummy, st, (float)(1.0 / (double)p.fx), (float)(1.0 / (double)p.fy), p.cx, p.cy);
+    r.keep_v = (const f4*)b.vmap[0];
+    r.keep_n = (const f4*)b.nmap[0];
+    launch_raycast(m->v, r, s);
+}
+
+static void queue_frame_device(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, bool first, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    if (!first) queue_resize_model(sys, s);
+    queue_pyramid(sys, depth, s);
+    if (!first) {
+        const float* nv[3] = {b.vmap[0], b.vmap[1], b.vmap[2]};
+        const float* nn[3] = {b.nmap[0], b.nmap[1], b.nmap[2]};
+        const float* lv[3] = {b.model_vmap[0], b.model_vmap[1], b.model_vmap[2]};
+        const float* ln[3] = {b.model_nmap[0], b.model_nmap[1], b.model_nmap[2]};
+        const float K[4] = {p.fx, p.fy, p.cx, p.cy};
+        queue_se3(sys->trk, nv, nn, lv, ln, K, p.iters, p.min_count, p.icp_thresh, s);
+    }
+    launch_track_commit(sys->trk->st, s);
+    queue_fuse_raycast(sys, rgb, s);
+}
+
+// Reference-style control: the same kernels, but every ICP iteration's sums come back to the
+// host, which solves and uploads the new pose (Tracking.cc:316-396), and each fusion/raycast
+// stage synchronises and reads its count as FuseMapColor/CreateRenderingBlocks do.
+static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, bool first, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    fu_tracker* t = sys->trk;
+    if (!first) queue_resize_model(sys, s);
+    queue_pyramid(sys, depth, s);
+    HIPCHK(hipStreamSynchronize(s));
+    TrackState h;
+    HIPCHK(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+    if (!first) {
+        for (int i = 0; i < 16; ++i) {
+            h.next_pose[i] = h.init_pose[i] = h.last_pose[i];
+            h.delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
+        }
+        h.status = TS_RUNNING;
+        h.iters_done = 0;
+        refresh_float_pose(&h);
+        const float K[4] = {p.fx, p.fy, p.cx, p.cy};
+        int last_level = -1;
+        for (int l = 2; l >= 0; --l)
+            if (p.iters[l] > 0) last_level = l;
+        for (int l = 2; l >= 0 && h.status == TS_RUNNING; --l)
+            for (int j = 0; j < p.iters[l] && h.status == TS_RUNNING; ++j) {
+                HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+                launch_icp(make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0),
+                           t->nblocks[l], s);
+                LAUNCHCHK();
+                HIPCHK(hipMemcpyAsync(&h, t->st, sizeof(h), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                float sums[32];
+                memcpy(sums, h.sums, sizeof(sums));
+                const int fin = (l == last_level && j == p.iters[l] - 1);
+                icp_update(&h, sums, p.min_count[l], fin, p.icp_thresh);
+            }
+    }
+    if (h.status == TS_OK || h.status == TS_FIRST)
+        for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i];
+    refresh_float_pose(&h);
+    HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    if (h.status != TS_OK && h.status != TS_FIRST) {
+        HIPCHK(hipStreamSynchronize(s));
+        return FU_OK;
+    }
+    // FuseMapColor with its synchronisations (FuseMap.cu:294-317)
+    fu_map* m = sys->map;
+    PoseF dummy{};
+    const FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
+    const VisArgs v = make_vis(m, dummy, t->st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
+    launch_begin_pass(m->v, 7, s);
+    launch_alloc(m->v, a, s);
+    HIPCHK(hipStreamSynchronize(s));
+    launch_visible(m->v, v, s);
+    fu_status st = FU_OK;
+    const unsigned nvis = read_ctr(m, C_VISIBLE, s, &st);
+    if (st) return st;
+    if (nvis > 0) {
+        launch_integrate(m->v, a, nvis < m->p.num_blocks ? nvis : m->p.num_blocks, s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // Mapping::RayTrace (Mapping.cc:98-109)
+    unsigned nfrag = 0;
+    if (nvis > 0) {
+        const ZRangeArgs z = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min,
+                                     dummy, t->st, -1, p.fx, p.fy, p.cx, p.cy);
+        launch_zrange(m->v, z, s);
+        nfrag = read_ctr(m, C_RENDER, s, &st);
+        if (st) return st;
+    }
+    if (nfrag > 0) {
+        const RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max,
+                                      b.zcols, b.zrows, dummy, t->st, (float)(1.0 / (double)p.fx),
+                                      (float)(1.0 / (double)p.fy), p.cx, p.cy);
+        launch_raycast(m->v, r, s);
+    } else {
+        HIPCHK(hipMemcpyAsync(b.model_vmap[0], b.vmap[0], (size_t)p.cols * p.rows * 16, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.model_nmap[0], b.nmap[0], (size_t)p.cols * p.rows * 16, hipMemcpyDeviceToDevice, s));
+    }
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(s));
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s) {
+    if (!sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_system_grab_image: null argument");
+    const bool first = sys->frame == 0;
+    hipStream_t hs = S(s);
+    fu_status st = FU_OK;
+    if (sys->p.plain) {
+        st = run_frame_plain(sys, depth, rgb, first, hs);
+    } else if (sys->p.use_graph && !first) {
+        if (!sys->graph || sys->graph_depth != depth || sys->graph_rgb != rgb) {
+            if (sys->graph) {
+                (void)hipGraphExecDestroy(sys->graph);
+                sys->graph = nullptr;
+            }
+            hipStream_t cs;
+            HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            queue_frame_device(sys, depth, rgb, false, cs);
+            hipError_t e = hipStreamEndCapture(cs, &g);
+            (void)hipStreamDestroy(cs);
+            if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&sys->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+            sys->graph_depth = depth;
+            sys->graph_rgb = rgb;
+        }
+        HIPCHK(hipGraphLaunch(sys->graph, hs));
+    } else {
+        queue_frame_device(sys, depth, rgb, first, hs);
+        LAUNCHCHK();
+    }
+    if (st) return st;
+    sys->frame++;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_grab_image(fu_system* sys, fu_stream s) {
+    if (!sys) return set_err(FU_ERR_ARG, "null system");
+    return fu_system_grab_image_from(sys, sys->b.depth_in, sys->b.rgb_in, s);
+}
+
+extern "C" fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s) {
+    if (!sys || !out) return set_err(FU_ERR_ARG, "null argument");
+    TrackState h;
+    unsigned c[C_COUNT];
+    HIPCHK(hipMemcpyAsync(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipMemcpyAsync(c, sys->map->v.ctr, sizeof(c), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    memcpy(out->pose, h.next_pose, sizeof(out->pose));
+    out->track_status = h.status;
+    out->fused = (h.status == TS_OK || h.status == TS_FIRST) ? 1 : 0;
+    out->icp_error = h.icp_error;
+    out->icp_count = h.count;
+    out->n_visible = c[C_VISIBLE];
+    out->n_allocated = c[C_ALLOC] < sys->p.map.num_blocks ? c[C_ALLOC] : sys->p.map.num_blocks;
+    out->n_fragments = c[C_RENDER];
+    out->flags = c[C_FLAGS];
+    out->frame_index = sys->frame;
+    return FU_OK;
+}

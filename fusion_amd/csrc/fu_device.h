@@ -1,0 +1,169 @@
+// Device-side views and helpers shared by the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include "fu_math.h"
+
+namespace fu {
+
+// One hash-table record, 16 bytes, read with a single dwordx4 load.  It carries what the
+// reference's 32-byte HashEntry carries (Mapping/DeviceMap.h:20-59): the block position
+// (packed in `key`), `ptr` = voxel offset of the block (heap index * 512, -1 = free) and
+// `next` = 1-based index into the excess area (0 ends the chain).
+struct alignas(16) HEntry {
+    unsigned long long key;
+    int ptr;
+    int next;
+};
+
+// Map-wide counters in device memory.
+enum Counter : int {
+    C_ALLOC = 0,    // blocks taken from the pool (bump index; may exceed capacity, clamp on read)
+    C_EXCESS = 1,   // excess slots used
+    C_VISIBLE = 2,  // entries of the visible list
+    C_NEW = 3,      // new keys found by the current allocation pass
+    C_FLAGS = 4,    // sticky overflow flags (FU_FLAG_*)
+    C_RENDER = 5,   // rendering fragments of the last z-range pass
+    C_TRI = 6,      // triangles of the last mesh pass
+    C_DONE = 7,     // work-group completion counter (last-group pattern), left at 0
+    C_OCC = 8,      // occupied blocks seen by the last mesh pass
+    C_SCRATCH = 9,
+    C_COUNT = 16
+};
+
+enum Flag : unsigned {
+    FLAG_HEAP_FULL = 1u,
+    FLAG_EXCESS_FULL = 2u,
+    FLAG_NEWSET_FULL = 4u,
+    FLAG_COORD_RANGE = 8u,
+    FLAG_RENDER_FULL = 16u,
+    FLAG_TRI_FULL = 32u,
+};
+
+// Voxel storage formats.  Each block of 8^3 voxels is stored as planes:
+//   sdf[512] (f32 or f16) | weight[512] (u8 or u16) | r[512] | g[512] | b[512]
+enum VoxelFormat : int { VF_F32_W8 = 0, VF_F32_W16 = 1, VF_F16_W8 = 2 };
+
+FU_HD int vf_sdf_bytes(int f) { return f == VF_F16_W8 ? 2 : 4; }
+FU_HD int vf_w_bytes(int f) { return f == VF_F32_W16 ? 2 : 1; }
+FU_HD int vf_block_bytes(int f) { return 512 * (vf_sdf_bytes(f) + vf_w_bytes(f) + 3); }
+
+struct MapView {
+    HEntry* entries;            // [num_buckets + num_excess]
+    unsigned* ctr;              // [C_COUNT]
+    unsigned long long* block_key;  // [num_blocks], key of each allocated block (kEmptyKey = hole)
+    unsigned* visible;          // [num_blocks] block ids of the visible list
+    unsigned long long* newset; // [newset_mask + 1] per-pass set of new keys
+    unsigned long long* newkeys;  // [max_new]
+    unsigned* newslot;          // [max_new]
+    unsigned char* pool;        // voxel blocks
+    unsigned num_buckets, num_excess, num_blocks, newset_mask, max_new;
+    unsigned max_weight;
+    int fmt;
+    float voxel_size, voxel_inv, trunc, depth_min, depth_max;
+};
+
+// ---- voxel plane access -----------------------------------------------------------------
+template <int F>
+struct VoxelIO {
+    static constexpr int kSdfB = F == VF_F16_W8 ? 2 : 4;
+    static constexpr int kWB = F == VF_F32_W16 ? 2 : 1;
+    static constexpr int kBlock = 512 * (kSdfB + kWB + 3);
+    __device__ static unsigned char* block(const MapView& m, int heap) {
+        return m.pool + (size_t)heap * kBlock;
+    }
+    __device__ static float sdf(const unsigned char* b, int i) {
+        if constexpr (kSdfB == 2)
+            return __half2float(reinterpret_cast<const __half*>(b)[i]);
+        else
+            return reinterpret_cast<const float*>(b)[i];
+    }
+    __device__ static void set_sdf(unsigned char* b, int i, float v) {
+        if constexpr (kSdfB == 2)
+            reinterpret_cast<__half*>(b)[i] = __float2half_rn(v);
+        else
+            reinterpret_cast<float*>(b)[i] = v;
+    }
+    __device__ static unsigned weight(const unsigned char* b, int i) {
+        if constexpr (kWB == 2)
+            return reinterpret_cast<const unsigned short*>(b + 512 * kSdfB)[i];
+        else
+            return b[512 * kSdfB + i];
+    }
+    __device__ static void set_weight(unsigned char* b, int i, unsigned w) {
+        if constexpr (kWB == 2)
+            reinterpret_cast<unsigned short*>(b + 512 * kSdfB)[i] = (unsigned short)w;
+        else
+            b[512 * kSdfB + i] = (unsigned char)w;
+    }
+    __device__ static unsigned char* rgb(unsigned char* b) { return b + 512 * (kSdfB + kWB); }
+    __device__ static const unsigned char* rgb(const unsigned char* b) { return b + 512 * (kSdfB + kWB); }
+};
+
+// ---- hash lookup (DeviceMap::FindEntry, DeviceMap.cu:122-135) ----------------------------
+__device__ __forceinline__ HEntry load_entry(const HEntry* e) {
+    // one 16-byte load
+    uint4 v = *reinterpret_cast<const uint4*>(e);
+    HEntry h;
+    h.key = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
+    h.ptr = (int)v.z;
+    h.next = (int)v.w;
+    return h;
+}
+
+// heap index of block `bp`, or -1 if it is not allocated
+__device__ __forceinline__ int find_block(const MapView& m, i3 bp) {
+    if (!key_in_range(bp)) return -1;
+    const unsigned long long key = pack_key(bp);
+    HEntry h = load_entry(&m.entries[block_hash(bp, m.num_buckets)]);
+    while (true) {
+        if (h.key == key && h.ptr >= 0) return h.ptr >> 9;
+        if (h.next <= 0) return -1;
+        h = load_entry(&m.entries[m.num_buckets + h.next - 1]);
+    }
+}
+
+// ---- wave64 helpers -------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+    // number of set bits of `mask` below this lane
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int lane) {
+    unsigned lo = __builtin_amdgcn_readlane((unsigned)v, lane);
+    unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), lane);
+    return (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
+// wave-aggregated atomicAdd of 1 per lane with `pred`; returns this lane's slot
+__device__ __forceinline__ unsigned wave_append(unsigned* counter, bool pred) {
+    unsigned long long m = ballot(pred);
+    if (m == 0) return 0;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, leader);
+    return base + lane_rank(m);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Pairwise tree sum over the 64 lanes of a wave: DPP within rows of 16 lanes
+// (quad swaps, half-row and row mirrors), then cross-row swizzles.  Every lane ends
+// with the wave total.
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+}  // namespace fu

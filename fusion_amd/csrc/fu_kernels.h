@@ -1,0 +1,154 @@
+// Argument structs and launchers of the HIP kernels (host <-> device interface inside the
+// library; the public C ABI is include/fusion_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fu_math.h"
+
+namespace fu {
+
+struct MapView;
+
+enum TrackStatus : int {
+    TS_RUNNING = 0,
+    TS_OK = 1,        // accepted (Tracking.cc:402-403)
+    TS_FAILED = 2,    // NaN error or too few correspondences (:371-375)
+    TS_REJECTED = 3,  // final error / motion test failed (:404-407)
+    TS_FIRST = 4,     // first frame: identity pose, no tracking (Tracking.cc:66-71)
+};
+
+// Per-session tracking state, resident in device memory.  Poses are 4x4 row-major FP64
+// camera-to-world transforms; the float views are what the kernels read.
+struct TrackState {
+    double last_pose[16];
+    double next_pose[16];
+    double init_pose[16];
+    double delta[16];
+    double A[36], b[6], x[6];
+    float Rcurr[9], RcurrInv[9], tcurr[3];
+    float Rlast[9], RlastInv[9], tlast[3];
+    float residual[2];
+    float icp_error;
+    int count;
+    int status;
+    int iters_done;
+    float sums[32];
+};
+
+// camera pose as the kernels use it: R (camera->world), Rinv = R^T, t
+struct PoseF {
+    m3 R, Rinv;
+    f3 t;
+};
+
+// When `st` is set the pose comes from the tracking state and the kernel does nothing unless
+// the frame was tracked (or is the first frame).
+__device__ __forceinline__ bool pose_from_state(const TrackState* st, PoseF& p) {
+    const int s = st->status;
+    if (s != TS_OK && s != TS_FIRST) return false;
+    const float* r = st->Rcurr;
+    const float* ri = st->RcurrInv;
+    p.R = m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
+    p.Rinv = m3{mk3(ri[0], ri[1], ri[2]), mk3(ri[3], ri[4], ri[5]), mk3(ri[6], ri[7], ri[8])};
+    p.t = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
+    return true;
+}
+
+struct FuseArgs {
+    PoseF pose;
+    const TrackState* st;
+    float fx, fy, cx, cy, invfx, invfy;
+    int cols, rows;
+    const float* depth;      // raw depth (metres, NaN = invalid)
+    const f4* nmap;          // level-0 normal map of the same frame
+    const unsigned char* rgb;  // 3 bytes per pixel
+};
+
+struct VisArgs {
+    PoseF pose;
+    const TrackState* st;
+    float fx, fy, cx, cy;
+    int cols, rows;
+    float depth_min, depth_max;
+};
+
+struct ZRangeArgs {
+    PoseF pose;
+    const TrackState* st;
+    float fx, fy, cx, cy;
+    float depth_min, depth_max;
+    unsigned* zmin;  // float bit patterns, zcols x zrows
+    unsigned* zmax;
+    int zcols, zrows;
+    unsigned max_fragments;
+    int n_visible_override;  // >= 0: use this count instead of the device counter
+};
+
+struct RaycastArgs {
+    PoseF pose;
+    const TrackState* st;
+    float invfx, invfy, cx, cy;
+    float step_scale;
+    int cols, rows;
+    const unsigned* zmin;
+    const unsigned* zmax;
+    int zcols, zrows;
+    f4* vmap;
+    f4* nmap;
+    // when set, a frame that produced no rendering fragments keeps these maps (the frame's
+    // own level-0 maps) instead of a raycast, as Mapping::RayTrace skips Raycast
+    const f4* keep_v;
+    const f4* keep_n;
+};
+
+struct MeshOut {
+    float* vertex;  // 9 floats per triangle
+    float* normal;
+    unsigned char* color;  // 9 bytes per triangle
+    int* tri_meta;         // optional: 4 ints per triangle (voxel x, y, z, cube*16 + t)
+    unsigned max_tri;
+};
+
+struct IcpArgs {
+    const f4* vcurr;
+    const f4* ncurr;
+    const f4* vlast;
+    const f4* nlast;
+    int cols, rows;
+    float fx, fy, cx, cy;
+    TrackState* st;
+    float* partials;  // 32 floats per work-group
+    unsigned* done;   // zero between launches
+    int mode;         // 0: sums only; 1: sums, solve and pose update
+    int min_count;
+    int final_iter;
+    float thresh;
+};
+
+// pyramid.hip
+void launch_filter_depth(const uint16_t* depth, float* raw, float* filt, int cols, int rows, float scale_inv,
+                         float cutoff, hipStream_t s);
+void launch_pyrdown(const float* src, int scols, int srows, float* dst, int dcols, int drows, hipStream_t s);
+void launch_vmap(const float* depth, f4* vmap, int cols, int rows, float invfx, float invfy, float cx, float cy,
+                 float cutoff, hipStream_t s);
+void launch_nmap(const f4* vmap, f4* nmap, int cols, int rows, hipStream_t s);
+void launch_resize(const f4* vsrc, const f4* nsrc, int scols, f4* vdst, f4* ndst, int dcols, int drows,
+                   hipStream_t s);
+// map.hip
+void launch_reset_map(const MapView& m, hipStream_t s);
+void launch_begin_pass(const MapView& m, int which, hipStream_t s);
+void launch_alloc(const MapView& m, const FuseArgs& a, hipStream_t s);
+void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s);
+void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s);
+// raycast.hip
+void launch_zrange(const MapView& m, const ZRangeArgs& a, hipStream_t s);
+void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s);
+// mesh.hip
+void upload_mc_tables();
+void launch_mesh(const MapView& m, const MeshOut& o, unsigned* counts, unsigned* chunk, hipStream_t s);
+// icp.hip
+void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s);
+void launch_track_begin(TrackState* st, hipStream_t s);
+void launch_track_commit(TrackState* st, hipStream_t s);
+
+}  // namespace fu

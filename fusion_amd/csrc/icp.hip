@@ -1,0 +1,165 @@
+// Point-to-plane ICP: per-pixel association and Jacobian rows, compensated FP32
+// reduction of the 29 sums, and (optionally) the FP64 solve and pose update on the device.
+//
+// Reference: Tracking/Reduction.cu ICPReduce::searchPoint (:227-257), getRow (:259-286),
+// operator() (:288-312), Reduce<float,29> (:49-64), CreateMatrix<6,7> (:6-16), ICPStep
+// (:319-375); Tracking/Tracking.cc Tracker::ComputeSE3 (:296-409).
+//
+// Each lane sums its pixels with Neumaier-compensated FP32 adds; lanes are combined by a
+// DPP pairwise tree (wave_sum), waves through LDS, and work-groups by the last group to
+// finish, which sums the group partials with the same compensated tree.  That group then
+// (mode 1) unpacks A (6x6) and b (6) in FP64, solves with an LDL^T factorisation with
+// symmetric pivoting, takes the SE(3) exponential and composes the pose, all in FP64 on
+// one lane: the only FP64 steps of the pipeline.
+#include "fu_device.h"
+#include "fu_kernels.h"
+#include "fu_track.h"
+
+namespace fu {
+
+struct Kahan {
+    float s, c;
+    __device__ void add(float v) {
+        const float t = s + v;
+        if (fabsf(s) >= fabsf(v))
+            c += (s - t) + v;
+        else
+            c += (v - t) + s;
+        s = t;
+    }
+    __device__ float val() const { return s + c; }
+};
+
+// ------------------------------------------------------------------ reduction kernel
+__device__ __forceinline__ m3 load_m3(const float* r) {
+    return m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
+}
+
+__global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
+    TrackState* st = a.st;
+    if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
+    const m3 Rc = load_m3(st->Rcurr), Rl = load_m3(st->Rlast), Rli = load_m3(st->RlastInv);
+    const f3 tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
+    const f3 tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
+    const int cols = a.cols, rows = a.rows, N = cols * rows;
+    const float angle_thresh = 0.6f, dist_thresh = 0.1f;  // Reduction.cu:352-353
+    Kahan acc[29];
+#pragma unroll
+    for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const int y = i / cols, x = i - y * cols;
+        float row[7] = {0, 0, 0, 0, 0, 0, 0};
+        bool found = false;
+        const f4 vc4 = a.vcurr[i];
+        const f3 vcurr_c = xyz(vc4);
+        if (!(isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3)) {
+            const f3 vcurr_g = add(mv(Rc, vcurr_c), tc);
+            const f3 vcurr_p = mv(Rli, sub(vcurr_g, tl));
+            const double invz = 1.0 / (double)vcurr_p.z;
+            const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
+            const int v = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
+            if (!(u < 0 || v < 0 || u >= cols || v >= rows)) {
+                const int j = v * cols + u;
+                const f3 vlast_g = add(mv(Rl, xyz(a.vlast[j])), tl);
+                const f3 ncurr_c = xyz(a.ncurr[i]);
+                const f3 ncurr_g = mv(Rc, ncurr_c);
+                const f3 nlast_c = xyz(a.nlast[j]);
+                const f3 nlast_g = mv(Rl, nlast_c);
+                const float dist = norm(sub(vlast_g, vcurr_g));
+                const float sine = norm(cross(ncurr_g, nlast_g));
+                found = sine < angle_thresh && dist <= dist_thresh && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x);
+                if (found) {
+                    const f3 nl = mv(Rli, nlast_g);
+                    const f3 vcu = mv(Rli, sub(vcurr_g, tl));
+                    const f3 vla = mv(Rli, sub(vlast_g, tl));
+                    const f3 mn = neg(nl);
+                    const f3 cr = cross(nl, vla);
+                    row[0] = mn.x;
+                    row[1] = mn.y;
+                    row[2] = mn.z;
+                    row[3] = cr.x;
+                    row[4] = cr.y;
+                    row[5] = cr.z;
+                    row[6] = dot(mn, sub(vcu, vla));
+                }
+            }
+        }
+        if (found) {
+            int k = 0;
+#pragma unroll
+            for (int r = 0; r < 7; ++r)
+#pragma unroll
+                for (int c = r; c < 7; ++c) acc[k++].add(row[r] * row[c]);
+            acc[28].add(1.0f);
+        }
+    }
+    // wave tree, then waves through LDS
+    __shared__ float wsum[4][29];
+    __shared__ int is_last;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 29; ++j) {
+        const float t = wave_sum(acc[j].val());
+        if (lane_id() == 0) wsum[wave][j] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 29) {
+        const int j = threadIdx.x;
+        a.partials[blockIdx.x * 32 + j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(a.done, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    // last group: sum the group partials (compensated per lane, then the same tree)
+    __shared__ float total[29];
+    for (int j = 0; j < 29; ++j) {
+        Kahan k{0.f, 0.f};
+        for (int g = threadIdx.x; g < (int)gridDim.x; g += blockDim.x)
+            k.add(__hip_atomic_load(&a.partials[g * 32 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const float t = wave_sum(k.val());
+        if (lane_id() == 0) wsum[wave][j] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 29) {
+        const int j = threadIdx.x;
+        total[j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *a.done = 0u;
+        for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
+        if (a.mode == 1) icp_update(st, total, a.min_count, a.final_iter, a.thresh);
+    }
+}
+
+void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_icp, dim3(nblocks), dim3(256), 0, s, a);
+}
+
+// start of ComputeSE3 (Tracking.cc:305-314): last pose = tracked pose of the previous
+// frame, initial guess = last pose (constant-position model), delta = identity
+__global__ void k_track_begin(TrackState* st) {
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < 16; ++i) {
+        st->next_pose[i] = st->last_pose[i];
+        st->init_pose[i] = st->last_pose[i];
+        st->delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    st->status = TS_RUNNING;
+    st->iters_done = 0;
+    refresh_float_pose(st);
+}
+// after fusion: the tracked frame becomes the last frame (Tracker::SwapFrame)
+__global__ void k_track_commit(TrackState* st) {
+    if (threadIdx.x != 0) return;
+    if (st->status == TS_OK || st->status == TS_FIRST)
+        for (int i = 0; i < 16; ++i) st->last_pose[i] = st->next_pose[i];
+    refresh_float_pose(st);
+}
+void launch_track_begin(TrackState* st, hipStream_t s) { hipLaunchKernelGGL(k_track_begin, dim3(1), dim3(64), 0, s, st); }
+void launch_track_commit(TrackState* st, hipStream_t s) { hipLaunchKernelGGL(k_track_commit, dim3(1), dim3(64), 0, s, st); }
+
+}  // namespace fu

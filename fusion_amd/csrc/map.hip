@@ -1,0 +1,380 @@
+// Voxel-block hash table: reset, allocation, visible-block list and TSDF integration.
+//
+// Reference: Mapping/DeviceMap.cu:3-62 (Hash, CreateEntry, CreateBlock),
+// Mapping/FuseMap.cu:36-138 (visibility, CreateBlocks, CheckFullVisibility),
+// FuseMap.cu:140-191 (integrateColor), FuseMap.cu:321-364 (ResetMap).
+//
+// Allocation runs in two passes instead of the reference's per-bucket try-lock, which
+// silently drops inserts under contention:
+//   A  one thread per pixel marches the truncation band exactly as CreateBlocks does and
+//      emits block keys; duplicates are removed first along the ray, then across the wave
+//      with ballots, the survivors probe the table read-only, and keys not present are
+//      claimed in a per-pass open-addressing set with a 64-bit CAS (one winner per key)
+//      and appended to a compact list with one wave-aggregated atomic.
+//   B  one thread per listed key takes a block from the pool and links the key into its
+//      bucket: a CAS on the bucket head, else a CAS on the tail's `next` of the excess
+//      chain with an excess slot reserved beforehand.  Keys are unique here, so the
+//      lock-free append cannot duplicate, never waits on another lane and never leaks.
+// The resulting block set is the sequential insertion set of the reference (within
+// capacity); only the order of pool indices differs.
+#include "fu_device.h"
+#include "fu_kernels.h"
+
+namespace fu {
+
+// ---------------------------------------------------------------- reset (FuseMap.cu:321-364)
+__global__ void k_reset_entries(HEntry* e, unsigned n) {
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        HEntry h;
+        h.key = kEmptyKey;
+        h.ptr = -1;
+        h.next = 0;
+        e[i] = h;
+    }
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, unsigned n) {
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_zero_u32(unsigned* p, unsigned n) {
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+void launch_reset_map(const MapView& m, hipStream_t s) {
+    const unsigned ne = m.num_buckets + m.num_excess;
+    hipLaunchKernelGGL(k_reset_entries, dim3(2048), dim3(256), 0, s, m.entries, ne);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, m.block_key, kEmptyKey, m.num_blocks);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, m.newset, kEmptyKey, m.newset_mask + 1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, m.ctr, (unsigned)C_COUNT);
+}
+
+// per-frame counters: the new-key count and the visible count start at zero
+__global__ void k_begin_pass(unsigned* ctr, int which) {
+    if (threadIdx.x == 0) {
+        if (which & 1) ctr[C_NEW] = 0;
+        if (which & 2) ctr[C_VISIBLE] = 0;
+        if (which & 4) ctr[C_RENDER] = 0;
+    }
+}
+void launch_begin_pass(const MapView& m, int which, hipStream_t s) {
+    hipLaunchKernelGGL(k_begin_pass, dim3(1), dim3(64), 0, s, m.ctr, which);
+}
+
+// ------------------------------------------------------------ allocation pass A (per pixel)
+__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// claim `key` in the per-pass set; true if this lane is the one that inserted it
+__device__ __forceinline__ bool newset_claim(const MapView& m, unsigned long long key, unsigned& slot_out) {
+    unsigned slot = (unsigned)mix64(key) & m.newset_mask;
+    for (unsigned probe = 0; probe <= m.newset_mask; ++probe) {
+        unsigned long long old = atomicCAS(&m.newset[slot], kEmptyKey, key);
+        if (old == kEmptyKey) {
+            slot_out = slot;
+            return true;
+        }
+        if (old == key) return false;
+        slot = (slot + 1) & m.newset_mask;
+    }
+    atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_NEWSET_FULL);
+    return false;
+}
+
+
+// Fusion::CreateBlocks (FuseMap.cu:80-111), one lane per pixel, 16x16 pixels per group.
+__global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) {
+    PoseF pose = p.pose;
+    if (p.st && !pose_from_state(p.st, pose)) return;
+    const float* __restrict__ depth = p.depth;
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    bool live = x < p.cols && y < p.rows;
+    f3 pt = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 0.f);
+    int nsteps = 0;
+    if (live) {
+        const float z = depth[y * p.cols + x];
+        if (!(isnanf_(z) || z < m.depth_min || z > m.depth_max)) {
+            const float thresh = m.trunc / 2.0f;
+            const float z_near = fminf(m.depth_max, z - thresh);
+            const float z_far = fminf(m.depth_max, z + thresh);
+            if (z_near < z_far) {
+                // unproject: Rview * (z(x-cx)/fx, z(y-cy)/fy, z) + tview, then to voxel units
+                f3 a = mk3(z_near * ((float)x - p.cx) * p.invfx, z_near * ((float)y - p.cy) * p.invfy, z_near);
+                f3 b = mk3(z_far * ((float)x - p.cx) * p.invfx, z_far * ((float)y - p.cy) * p.invfy, z_far);
+                a = mul(add(mv(pose.R, a), pose.t), m.voxel_inv);
+                b = mul(add(mv(pose.R, b), pose.t), m.voxel_inv);
+                dir = sub(b, a);
+                const float len = norm(dir);
+                nsteps = d2i(ceil(2.0 * (double)len));
+                dir = divs(dir, (float)(nsteps - 1));
+                pt = a;
+            }
+        }
+    }
+    // wave-uniform trip count
+    int wmax = nsteps;
+    for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+    i3 prev = mki3(0x7fffffff, 0, 0);
+    for (int i = 0; i < wmax; ++i) {
+        bool has = i < nsteps;
+        i3 bp = prev;
+        if (has) {
+            bp = voxel_to_block(f2i3(pt));
+            pt = add(pt, dir);
+            if (eqi(bp, prev)) has = false;  // same block as the previous step of this ray
+            else prev = bp;
+            if (has && !key_in_range(bp)) {
+                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_COORD_RANGE);
+                has = false;
+            }
+        }
+        const unsigned long long key = has ? pack_key(bp) : kEmptyKey;
+        // wave de-duplication: one representative lane per distinct key
+        unsigned long long pending = ballot(has);
+        bool rep = false;
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const unsigned long long k = readlane64(key, leader);
+            const unsigned long long same = ballot(has && key == k);
+            pending &= ~same;
+            if (lane_id() == leader) rep = true;
+        }
+        bool won = false;
+        unsigned slot = 0;
+        if (rep && find_block(m, bp) < 0) won = newset_claim(m, key, slot);
+        const unsigned idx = wave_append(&m.ctr[C_NEW], won);
+        if (won) {
+            if (idx < m.max_new) {
+                m.newkeys[idx] = key;
+                m.newslot[idx] = slot;
+            } else {
+                // list full: release the claim so a later pass can allocate this key
+                atomicExch(&m.newset[slot], kEmptyKey);
+                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_NEWSET_FULL);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ allocation pass B (per new key)
+// one wave per new block: lane 0 links the key, all lanes initialise the block's voxels
+// (sdf NaN, weight 0, colour 0: Voxel::release, DeviceMap.h:75-79)
+__global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackState* st) {
+    if (st && st->status != TS_OK && st->status != TS_FIRST) return;
+    const unsigned nnew = min(m.ctr[C_NEW], m.max_new);
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int lane = lane_id();
+    const int bbytes = vf_block_bytes(m.fmt);
+    const int sb = vf_sdf_bytes(m.fmt);
+    for (unsigned i = wave; i < nnew; i += nwaves) {
+        const unsigned long long key = m.newkeys[i];
+        int heap = -1;
+        if (lane == 0) {
+            m.newset[m.newslot[i]] = kEmptyKey;  // clear for the next pass
+            const unsigned h = atomicAdd(&m.ctr[C_ALLOC], 1u);
+            if (h >= m.num_blocks) {
+                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_HEAP_FULL);
+            } else {
+                heap = (int)h;
+                const i3 bp = unpack_key(key);
+                HEntry* e = &m.entries[block_hash(bp, m.num_buckets)];
+                const unsigned long long old = atomicCAS(&e->key, kEmptyKey, key);
+                bool ok = true;
+                if (old == kEmptyKey) {
+                    e->ptr = heap * 512;
+                } else {
+                    // append to the excess chain (DeviceMap.cu:50-60 without the lock)
+                    int slot = -1;
+                    HEntry* tail = e;
+                    while (true) {
+                        int nx = __hip_atomic_load(&tail->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        while (nx != 0) {
+                            tail = &m.entries[m.num_buckets + nx - 1];
+                            nx = __hip_atomic_load(&tail->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        if (slot < 0) {
+                            const unsigned sx = atomicAdd(&m.ctr[C_EXCESS], 1u);
+                            if (sx >= m.num_excess) {
+                                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_EXCESS_FULL);
+                                ok = false;
+                                break;
+                            }
+                            slot = (int)sx;
+                            HEntry ne;
+                            ne.key = key;
+                            ne.ptr = heap * 512;
+                            ne.next = 0;
+                            m.entries[m.num_buckets + slot] = ne;
+                            __threadfence();
+                        }
+                        const int prev = atomicCAS(&tail->next, 0, slot + 1);
+                        if (prev == 0) break;
+                        tail = &m.entries[m.num_buckets + prev - 1];
+                    }
+                }
+                m.block_key[heap] = ok ? key : kEmptyKey;
+            }
+        }
+        heap = __shfl(heap, 0);
+        if (heap >= 0) {
+            unsigned char* blk = m.pool + (size_t)heap * bbytes;
+            // sdf plane
+            if (sb == 4) {
+                float* sp = reinterpret_cast<float*>(blk);
+                for (int v = lane; v < 512; v += 64) sp[v] = fnan();
+            } else {
+                __half* sp = reinterpret_cast<__half*>(blk);
+                for (int v = lane; v < 512; v += 64) sp[v] = __float2half_rn(fnan());
+            }
+            // weight and colour planes are zero
+            unsigned* zp = reinterpret_cast<unsigned*>(blk + 512 * sb);
+            const int nz = (bbytes - 512 * sb) / 4;
+            for (int v = lane; v < nz; v += 64) zp[v] = 0u;
+        }
+    }
+}
+
+void launch_alloc(const MapView& m, const FuseArgs& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_alloc_pass_a, dim3((p.cols + 15) / 16, (p.rows + 15) / 16), dim3(256), 0, s, m, p);
+    hipLaunchKernelGGL(k_alloc_pass_b, dim3(512), dim3(256), 0, s, m, p.st);
+}
+
+// ------------------------------------------------------------ visible list (FuseMap.cu:36-78,113-138)
+
+__device__ __forceinline__ bool vertex_visible(const VisArgs& v, const PoseF& pose, f3 p) {
+    p = mv(pose.Rinv, sub(p, pose.t));
+    if (p.z < 1e-3f) return false;
+    const float px = v.fx * p.x / p.z + v.cx;
+    const float py = v.fy * p.y / p.z + v.cy;
+    return px >= 0 && py >= 0 && px < (float)v.cols && py < (float)v.rows && p.z >= v.depth_min &&
+           p.z <= v.depth_max;
+}
+
+// the eight corners are visited by repeated float adds/subtracts exactly as in
+// Fusion::CheckBlockVisibility (FuseMap.cu:47-78)
+__device__ __forceinline__ bool block_visible(const VisArgs& v, const PoseF& pose, i3 pos, float scale) {
+    f3 c = i3f(pos, scale);
+    if (vertex_visible(v, pose, c)) return true;
+    c.z += scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.y += scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.x += scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.z -= scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.y -= scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.x -= scale;
+    c.y += scale;
+    if (vertex_visible(v, pose, c)) return true;
+    c.x += scale;
+    c.y -= scale;
+    c.z += scale;
+    if (vertex_visible(v, pose, c)) return true;
+    return false;
+}
+
+// Scans the allocated-block array (N_alloc keys) rather than all N_E hash entries; the
+// set is the same (every allocated block has exactly one live entry).  Compaction is a
+// wave ballot plus one atomic per wave.
+__global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) {
+    PoseF pose = v.pose;
+    if (v.st && !pose_from_state(v.st, pose)) return;
+    const unsigned nalloc = min(m.ctr[C_ALLOC], m.num_blocks);
+    const float scale = m.voxel_size * 8.0f;  // DeviceMap::blockWidth
+    const unsigned stride = gridDim.x * blockDim.x;
+    // uniform trip count per wave so the ballot below sees every lane
+    const unsigned base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (unsigned b0 = base0; b0 < nalloc; b0 += stride) {
+        const unsigned i = b0 + lane_id();
+        bool vis = false;
+        if (i < nalloc) {
+            const unsigned long long key = m.block_key[i];
+            if (key != kEmptyKey) vis = block_visible(v, pose, unpack_key(key), scale);
+        }
+        const unsigned idx = wave_append(&m.ctr[C_VISIBLE], vis);
+        if (vis && idx < m.num_blocks) m.visible[idx] = i;
+    }
+}
+
+void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_visible, dim3(1024), dim3(256), 0, s, m, v);
+}
+
+// ------------------------------------------------------------ integration (FuseMap.cu:140-191)
+
+template <int F>
+__global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
+    using IO = VoxelIO<F>;
+    PoseF pose = a.pose;
+    if (a.st && !pose_from_state(a.st, pose)) return;
+    const unsigned nvis = min(m.ctr[C_VISIBLE], m.num_blocks);
+    const int cols = a.cols, rows = a.rows;
+    const float thresh = m.trunc;
+    for (unsigned vb = blockIdx.x; vb < nvis; vb += gridDim.x) {
+        const unsigned heap = m.visible[vb];
+        const i3 bp = unpack_key(m.block_key[heap]);
+        unsigned char* blk = IO::block(m, (int)heap);
+        unsigned char* rgbp = IO::rgb(blk);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int li = threadIdx.x + h * 256;  // voxel index z*64 + y*8 + x
+            const int lx = li & 7, ly = (li >> 3) & 7, lz = li >> 6;
+            const i3 vp = mki3(bp.x * 8 + lx, bp.y * 8 + ly, bp.z * 8 + lz);
+            f3 pos = i3f(vp, m.voxel_size);
+            pos = mv(pose.Rinv, sub(pos, pose.t));
+            const int u = f2i(a.fx * pos.x / pos.z + a.cx);
+            const int v = f2i(a.fy * pos.y / pos.z + a.cy);
+            if (u < 0 || v < 0 || u >= cols || v >= rows) continue;
+            const int pix = v * cols + u;
+            const float dp = a.depth[pix];
+            if (isnanf_(dp) || dp > m.depth_max || dp < m.depth_min) continue;
+            float sdf = dp - pos.z;
+            if (!(sdf >= -thresh)) continue;
+            sdf = fminf(1.0f, sdf / thresh);
+            const f4 nl = a.nmap[pix];
+            if (isnanf_(nl.x)) continue;
+            // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342)
+            const float w = dot4(nl, normalised4(mk4(pos.x, pos.y, pos.z, 1.0f)));
+            const unsigned char cr = a.rgb[pix * 3 + 0], cg = a.rgb[pix * 3 + 1], cb = a.rgb[pix * 3 + 2];
+            const unsigned wold = IO::weight(blk, li);
+            if (wold == 0) {
+                IO::set_sdf(blk, li, sdf);
+                IO::set_weight(blk, li, 1u);
+                rgbp[li] = cr;
+                rgbp[512 + li] = cg;
+                rgbp[1024 + li] = cb;
+            } else {
+                const float sold = IO::sdf(blk, li);
+                const float aw = w * 0.2f;
+                const float bw = 1.0f - w * 0.2f;
+                const float r = (aw * ((float)cr / 255.f) + bw * ((float)rgbp[li] / 255.f)) * 255.f;
+                const float g = (aw * ((float)cg / 255.f) + bw * ((float)rgbp[512 + li] / 255.f)) * 255.f;
+                const float b = (aw * ((float)cb / 255.f) + bw * ((float)rgbp[1024 + li] / 255.f)) * 255.f;
+                const float fw = (float)wold;
+                IO::set_sdf(blk, li, (sold * fw + w * sdf) / (fw + w));
+                IO::set_weight(blk, li, min(m.max_weight, wold + 1u));
+                rgbp[li] = f2u8(r);
+                rgbp[512 + li] = f2u8(g);
+                rgbp[1024 + li] = f2u8(b);
+            }
+        }
+    }
+}
+
+void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
+    switch (m.fmt) {
+        case VF_F32_W8: hipLaunchKernelGGL(k_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
+        case VF_F32_W16: hipLaunchKernelGGL(k_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, m, a); break;
+        default: hipLaunchKernelGGL(k_integrate<VF_F16_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
+    }
+}
+
+}  // namespace fu

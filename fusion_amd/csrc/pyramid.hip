@@ -1,0 +1,154 @@
+// Depth pyramid, bilateral filter, vertex/normal maps and map downsampling.
+//
+// Kernels follow Tracking/Pyrdown.cu (FilterDepthKernel :5-41, PyrDownGaussKernel
+// :134-158, ComputeVMapKernel :56-78, ComputeNMapKernel :93-114, ResizeMapKernel :251-279)
+// with the same window bounds, weights and IEEE operation order.  Images are dense
+// row-major (pitch = width).  All of these are HBM-bound stencils; a 16x16 work-group
+// covers a 16x16 output tile so each wave reads whole 64-byte row segments.
+#include "fu_device.h"
+#include "fu_kernels.h"
+
+namespace fu {
+
+// Pyrdown.cu:3-4: sigSpace = 0.5/(4*4), sigRange = 0.5/(0.5*0.5) (exact in float)
+constexpr float kSigSpace = 0.5f / 16.0f;
+constexpr float kSigRange = 2.0f;
+
+__global__ __launch_bounds__(256) void k_filter_depth(const uint16_t* __restrict__ depth, float* __restrict__ raw,
+                                                     float* __restrict__ filt, int cols, int rows, float scale_inv,
+                                                     float cutoff) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    const float center = (float)depth[y * cols + x] * scale_inv;
+    raw[y * cols + x] = center < cutoff ? center : fnan();
+    // the window is [max(p-2,0), min(p+3, n-1)) in each axis: the last row/column of the
+    // image never contributes (Pyrdown.cu:23-29)
+    const int tx = min(x + 3, cols - 1);
+    const int ty = min(y + 3, rows - 1);
+    float sum1 = 0.f, sum2 = 0.f;
+    for (int cy = max(y - 2, 0); cy < ty; ++cy) {
+        for (int cx = max(x - 2, 0); cx < tx; ++cx) {
+            const float val = (float)depth[cy * cols + cx] * scale_inv;
+            const float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
+            const float color2 = (center - val) * (center - val);
+            const float weight = fu_expf(-(space2 * kSigSpace + color2 * kSigRange));
+            sum1 += val * weight;
+            sum2 += weight;
+        }
+    }
+    const float fin = sum1 / sum2;
+    filt[y * cols + x] = fin < cutoff ? fin : fnan();
+}
+
+// binomial [1 4 6 4 1] outer product, indexed (ty-cy-1)*5 + (tx-cx-1) as in Pyrdown.cu:151,
+// which mirrors the weights at the right/bottom border
+__constant__ float c_gauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
+
+__global__ __launch_bounds__(256) void k_pyrdown(const float* __restrict__ src, int scols, int srows,
+                                                float* __restrict__ dst, int dcols, int drows) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= dcols || y >= drows) return;
+    const int tx = min(2 * x + 3, scols - 1);
+    const int ty = min(2 * y + 3, srows - 1);
+    float sum = 0.f;
+    int count = 0;
+    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
+        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+            const float v = src[cy * scols + cx];
+            if (!isnanf_(v)) {
+                const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                sum += v * g;
+                count += (int)g;
+            }
+        }
+    }
+    dst[y * dcols + x] = sum / (float)count;
+}
+
+__global__ __launch_bounds__(256) void k_vmap(const float* __restrict__ depth, f4* __restrict__ vmap, int cols,
+                                             int rows, float invfx, float invfy, float cx, float cy, float cutoff) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    const float z = depth[y * cols + x];
+    f4 v;
+    // `z > 0.1` compares in double (Pyrdown.cu:69)
+    if (!isnanf_(z) && (double)z > 0.1 && z < cutoff) {
+        v.x = z * ((float)x - cx) * invfx;
+        v.y = z * ((float)y - cy) * invfy;
+        v.z = z;
+        v.w = 1.0f;
+    } else {
+        // the reference sets only x = NaN; every consumer tests x alone
+        v = mk4(fnan(), fnan(), fnan(), fnan());
+    }
+    vmap[y * cols + x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_nmap(const f4* __restrict__ vmap, f4* __restrict__ nmap, int cols, int rows) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
+    if (x == cols - 1 || y == rows - 1) {
+        nmap[y * cols + x] = nan4;
+        return;
+    }
+    const f4 c = vmap[y * cols + x];
+    const f4 r = vmap[y * cols + x + 1];
+    const f4 d = vmap[(y + 1) * cols + x];
+    if (!isnanf_(c.x) && !isnanf_(r.x) && !isnanf_(d.x)) {
+        const f3 n = normalised(cross(sub(xyz(r), xyz(c)), sub(xyz(d), xyz(c))));
+        nmap[y * cols + x] = mk4(n.x, n.y, n.z, 1.0f);
+    } else {
+        nmap[y * cols + x] = nan4;
+    }
+}
+
+// 2x2 mean of the vertex map, 2x2 mean of the normal map normalised in 4-D (w included,
+// VectorMath.h:340-342), NaN if any of the four is NaN
+__global__ __launch_bounds__(256) void k_resize(const f4* __restrict__ vsrc, const f4* __restrict__ nsrc, int scols,
+                                               f4* __restrict__ vdst, f4* __restrict__ ndst, int dcols, int drows) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= dcols || y >= drows) return;
+    const int s0 = (2 * y) * scols + 2 * x, s1 = (2 * y + 1) * scols + 2 * x;
+    const f4 v00 = vsrc[s0], v01 = vsrc[s0 + 1], v10 = vsrc[s1], v11 = vsrc[s1 + 1];
+    const f4 n00 = nsrc[s0], n01 = nsrc[s0 + 1], n10 = nsrc[s1], n11 = nsrc[s1 + 1];
+    const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
+    if (isnanf_(v00.x) || isnanf_(v01.x) || isnanf_(v10.x) || isnanf_(v11.x))
+        vdst[y * dcols + x] = nan4;
+    else
+        vdst[y * dcols + x] = divs4(add4(add4(add4(v00, v01), v10), v11), 4.0f);
+    if (isnanf_(n00.x) || isnanf_(n01.x) || isnanf_(n10.x) || isnanf_(n11.x))
+        ndst[y * dcols + x] = nan4;
+    else
+        ndst[y * dcols + x] = normalised4(divs4(add4(add4(add4(n00, n01), n10), n11), 4.0f));
+}
+
+static inline dim3 grid16(int cols, int rows) { return dim3((cols + 15) / 16, (rows + 15) / 16); }
+
+void launch_filter_depth(const uint16_t* depth, float* raw, float* filt, int cols, int rows, float scale_inv,
+                         float cutoff, hipStream_t s) {
+    hipLaunchKernelGGL(k_filter_depth, grid16(cols, rows), dim3(256), 0, s, depth, raw, filt, cols, rows, scale_inv,
+                       cutoff);
+}
+void launch_pyrdown(const float* src, int scols, int srows, float* dst, int dcols, int drows, hipStream_t s) {
+    hipLaunchKernelGGL(k_pyrdown, grid16(dcols, drows), dim3(256), 0, s, src, scols, srows, dst, dcols, drows);
+}
+void launch_vmap(const float* depth, f4* vmap, int cols, int rows, float invfx, float invfy, float cx, float cy,
+                 float cutoff, hipStream_t s) {
+    hipLaunchKernelGGL(k_vmap, grid16(cols, rows), dim3(256), 0, s, depth, vmap, cols, rows, invfx, invfy, cx, cy,
+                       cutoff);
+}
+void launch_nmap(const f4* vmap, f4* nmap, int cols, int rows, hipStream_t s) {
+    hipLaunchKernelGGL(k_nmap, grid16(cols, rows), dim3(256), 0, s, vmap, nmap, cols, rows);
+}
+void launch_resize(const f4* vsrc, const f4* nsrc, int scols, f4* vdst, f4* ndst, int dcols, int drows,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_resize, grid16(dcols, drows), dim3(256), 0, s, vsrc, nsrc, scols, vdst, ndst, dcols, drows);
+}
+
+}  // namespace fu

@@ -1,0 +1,251 @@
+// Raycast of the TSDF model: z-range tiles, then a per-pixel ray march.
+//
+// Reference: Mapping/RenderScene.cu.  Projection::projectBlock (:54-104) bounds each visible
+// block on a grid of 8x8-pixel tiles; createRenderingBlockList/fillBlocks (:106-179) split the
+// bound into 16x16-tile fragments and min/max the fragment z-ranges into the tile maps;
+// Rendering::operator() (:354-440) marches each pixel between its tile's z-range.
+//
+// The fragment list is only a vehicle for the tile maps, so here one thread per visible block
+// writes the union of its fragments straight into the maps with integer atomics (the ranges
+// are positive floats, whose IEEE bit patterns order like integers).  The union reproduces the
+// reference's fragment loop as written (inner bound `x < ny`, lower-right corner without the
+// fragment offset, RenderScene.cu:109-116): the covered tiles are
+//   [UL.x, min(LR.x, UL.x + e)] x [UL.y, min(LR.y, UL.y + e)],  e = 16 if ny >= 2 else 15.
+// Tile maps are ceil(W/8) x ceil(H/8) (80x60 at 640x480, as RenderScene.cu:245-251).
+#include "fu_device.h"
+#include "fu_kernels.h"
+
+namespace fu {
+
+__global__ void k_zrange_init(const TrackState* st, unsigned* zmin, unsigned* zmax, int n) {
+    if (st && st->status != TS_OK && st->status != TS_FIRST) return;
+    const unsigned hundred = __float_as_uint(100.0f);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        zmin[i] = hundred;
+        zmax[i] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a) {
+    PoseF pose = a.pose;
+    if (a.st && !pose_from_state(a.st, pose)) return;
+    // a caller-given count never exceeds the list the visibility pass built
+    const unsigned built = min(m.ctr[C_VISIBLE], m.num_blocks);
+    const unsigned nvis = a.n_visible_override >= 0 ? min((unsigned)a.n_visible_override, built) : built;
+    const int zc = a.zcols, zr = a.zrows;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nvis; i += gridDim.x * blockDim.x) {
+        const i3 pos = unpack_key(m.block_key[m.visible[i]]);
+        // projectBlock (RenderScene.cu:54-104)
+        int ulx = zc, uly = zr, lrx = -1, lry = -1;
+        float zlo = a.depth_max, zhi = a.depth_min;
+        for (int c = 0; c < 8; ++c) {
+            const i3 q = mki3(pos.x + (c & 1), pos.y + ((c >> 1) & 1), pos.z + ((c >> 2) & 1));
+            // int3 * uint * float: the integer product first, then to float (VectorMath.h:219-233)
+            f3 p = i3f(mki3(q.x * 8, q.y * 8, q.z * 8), m.voxel_size);
+            p = mv(pose.Rinv, sub(p, pose.t));
+            if ((double)p.z < 2e-1) continue;
+            const float px = (a.fx * p.x / p.z + a.cx) / 8.0f;
+            const float py = (a.fy * p.y / p.z + a.cy) / 8.0f;
+            const float flx = floorf(px), clx = ceilf(px), fly = floorf(py), cly = ceilf(py);
+            if ((float)ulx > flx) ulx = (short)f2i(flx);
+            if ((float)lrx < clx) lrx = (short)f2i(clx);
+            if ((float)uly > fly) uly = (short)f2i(fly);
+            if ((float)lry < cly) lry = (short)f2i(cly);
+            if (zlo > p.z) zlo = p.z;
+            if (zhi < p.z) zhi = p.z;
+        }
+        if (ulx < 0) ulx = 0;
+        if (uly < 0) uly = 0;
+        if (lrx >= zc) lrx = zc - 1;
+        if (lry >= zr) lry = zr - 1;
+        if (ulx > lrx || uly > lry) continue;
+        if (zlo < a.depth_min) zlo = a.depth_min;
+        if (zhi < a.depth_min) continue;
+        const int nx = (int)ceilf(((float)lrx - (float)ulx + 1.0f) / 16.0f);
+        const int ny = (int)ceilf(((float)lry - (float)uly + 1.0f) / 16.0f);
+        const unsigned need = (unsigned)(nx * ny);
+        const unsigned off = atomicAdd(&m.ctr[C_RENDER], need);
+        if (off + need >= a.max_fragments) {
+            atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_RENDER_FULL);
+            continue;
+        }
+        const int e = ny >= 2 ? 16 : 15;
+        const int x1 = min(lrx, ulx + e), y1 = min(lry, uly + e);
+        const unsigned zl = __float_as_uint(zlo), zh = __float_as_uint(zhi);
+        for (int ty = uly; ty <= y1; ++ty)
+            for (int tx = ulx; tx <= x1; ++tx) {
+                atomicMin(&a.zmin[ty * zc + tx], zl);
+                atomicMax(&a.zmax[ty * zc + tx], zh);
+            }
+    }
+}
+
+void launch_zrange(const MapView& m, const ZRangeArgs& a, hipStream_t s) {
+    const int n = a.zcols * a.zrows;
+    hipLaunchKernelGGL(k_zrange_init, dim3((n + 255) / 256), dim3(256), 0, s, a.st, a.zmin, a.zmax, n);
+    hipLaunchKernelGGL(k_project_blocks, dim3(512), dim3(256), 0, s, m, a);
+}
+
+// ------------------------------------------------------------------------------ ray march
+// DeviceMap::FindVoxel(float3, cache, valid) (DeviceMap.cu:97-114) with the one-block cache
+// kept in registers.  The reference's cache starts as a default HashEntry at block (0,0,0)
+// with ptr -1 and would read voxel -1+idx on a "hit"; here an empty cache never hits.
+template <int F>
+struct Sampler {
+    using IO = VoxelIO<F>;
+    const MapView& m;
+    i3 cbp;
+    int cheap;
+    __device__ Sampler(const MapView& mm) : m(mm), cbp(mki3(0, 0, 0)), cheap(-1) {}
+    // returns sdf (NaN if the block is absent); `valid` as FindVoxel sets it, `w` the weight
+    __device__ __forceinline__ float find(f3 pt, bool& valid, unsigned& w) {
+        const i3 p = f2i3(pt);
+        const i3 bp = voxel_to_block(p);
+        int heap;
+        if (cheap >= 0 && eqi(bp, cbp)) {
+            heap = cheap;
+        } else {
+            heap = find_block(m, bp);
+            if (heap < 0) {
+                valid = false;
+                w = 0;
+                return fnan();
+            }
+            cbp = bp;
+            cheap = heap;
+        }
+        valid = true;
+        const unsigned char* blk = IO::block(m, heap);
+        const int li = voxel_local_idx(p);
+        w = IO::weight(blk, li);
+        return IO::sdf(blk, li);
+    }
+    // Rendering::readSdf (RenderScene.cu:290-295)
+    __device__ __forceinline__ float read_sdf(f3 pt, bool& valid) {
+        unsigned w;
+        const float s = find(pt, valid, w);
+        if (w == 0) valid = false;
+        return s;
+    }
+    __device__ __forceinline__ float sdf_at(f3 pt, bool& valid) {
+        unsigned w;
+        return find(pt, valid, w);
+    }
+    // Rendering::readSdfInterped (RenderScene.cu:297-319): fractions from floor, lookups by
+    // truncation; `valid` ends as the last lookup left it
+    __device__ __forceinline__ float read_interp(f3 pt, bool& valid) {
+        const f3 f = sub(pt, floor3(pt));
+        float s0, s1, r0, r1, r2, r3;
+        s0 = sdf_at(pt, valid);
+        s1 = sdf_at(add(pt, mk3(1, 0, 0)), valid);
+        r0 = (1.0f - f.x) * s0 + f.x * s1;
+        s0 = sdf_at(add(pt, mk3(0, 1, 0)), valid);
+        s1 = sdf_at(add(pt, mk3(1, 1, 0)), valid);
+        r1 = (1.0f - f.x) * s0 + f.x * s1;
+        r2 = (1.0f - f.y) * r0 + f.y * r1;
+        s0 = sdf_at(add(pt, mk3(0, 0, 1)), valid);
+        s1 = sdf_at(add(pt, mk3(1, 0, 1)), valid);
+        r0 = (1.0f - f.x) * s0 + f.x * s1;
+        s0 = sdf_at(add(pt, mk3(0, 1, 1)), valid);
+        s1 = sdf_at(add(pt, mk3(1, 1, 1)), valid);
+        r1 = (1.0f - f.x) * s0 + f.x * s1;
+        r3 = (1.0f - f.y) * r0 + f.y * r1;
+        return (1.0f - f.z) * r2 + f.z * r3;
+    }
+};
+
+template <int F>
+__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
+    PoseF pose = a.pose;
+    if (a.st && !pose_from_state(a.st, pose)) return;
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= a.cols || y >= a.rows) return;
+    const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
+    f4* vout = a.vmap + y * a.cols + x;
+    f4* nout = a.nmap + y * a.cols + x;
+    if (a.keep_v && m.ctr[C_RENDER] == 0u) {
+        // Mapping::RayTrace (Mapping.cc:102-108) does not raycast when there is nothing to
+        // render; the frame keeps its own level-0 maps
+        *vout = a.keep_v[y * a.cols + x];
+        *nout = a.keep_n[y * a.cols + x];
+        return;
+    }
+    const int tile = (y / 8) * a.zcols + (x / 8);
+    const float zr0 = __uint_as_float(a.zmin[tile]);
+    const float zr1 = __uint_as_float(a.zmax[tile]);
+    if ((double)zr1 < 1e-3 || (double)zr0 < 1e-3 || isnanf_(zr0) || isnanf_(zr1)) {
+        *vout = nan4;
+        *nout = nan4;
+        return;
+    }
+    const float vinv = m.voxel_inv;
+    const float step_scale = a.step_scale;
+    float sdf = 1.0f;
+    f3 p3 = mk3(zr0 * ((float)x - a.cx) * a.invfx, zr0 * ((float)y - a.cy) * a.invfy, zr0);
+    float dist_s = norm(p3) * vinv;
+    const f3 block_s = mul(add(mv(pose.R, p3), pose.t), vinv);
+    p3 = mk3(zr1 * ((float)x - a.cx) * a.invfx, zr1 * ((float)y - a.cy) * a.invfy, zr1);
+    const float dist_e = norm(p3) * vinv;
+    const f3 block_e = mul(add(mv(pose.R, p3), pose.t), vinv);
+    const f3 dir = normalised(sub(block_e, block_s));
+    f3 result = block_s;
+    bool valid = false;
+    bool found = false;
+    float step;
+    Sampler<F> smp(m);
+    while (dist_s < dist_e) {
+        sdf = smp.read_sdf(result, valid);
+        if (!valid) {
+            step = 8.0f;
+        } else {
+            if (sdf <= 0.1f && sdf >= -0.5f) sdf = smp.read_interp(result, valid);
+            if (sdf <= 0.0f) break;
+            if (!isnanf_(sdf))
+                step = fmaxf(sdf * step_scale, 1.0f);
+            else
+                step = 8.0f;
+        }
+        result = add(result, mul(step, dir));
+        dist_s += step;
+    }
+    if (sdf <= 0.0f) {
+        step = sdf * step_scale;
+        result = add(result, mul(step, dir));
+        sdf = smp.read_interp(result, valid);
+        step = sdf * step_scale;
+        result = add(result, mul(step, dir));
+        found = true;
+    }
+    if (found) {
+        // Rendering::readNormal (RenderScene.cu:321-352)
+        const f3 off[6] = {mk3(1, 0, 0), mk3(-1, 0, 0), mk3(0, 1, 0), mk3(0, -1, 0), mk3(0, 0, 1), mk3(0, 0, -1)};
+        float s[6];
+        bool ok = true;
+        for (int k = 0; k < 6 && ok; ++k) {
+            bool v2;
+            s[k] = smp.read_interp(add(result, off[k]), v2);
+            if (isnanf_(s[k]) || s[k] == 1.0f || !v2) ok = false;
+        }
+        if (ok) {
+            const f3 n = normalised(mv(pose.Rinv, mk3(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
+            const f3 vtx = mv(pose.Rinv, sub(mul(result, m.voxel_size), pose.t));
+            *vout = mk4(vtx.x, vtx.y, vtx.z, 1.0f);
+            *nout = mk4(n.x, n.y, n.z, 1.0f);
+            return;
+        }
+    }
+    *vout = nan4;
+    *nout = nan4;
+}
+
+void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
+    const dim3 g((a.cols + 15) / 16, (a.rows + 15) / 16);
+    switch (m.fmt) {
+        case VF_F32_W8: hipLaunchKernelGGL(k_raycast<VF_F32_W8>, g, dim3(256), 0, s, m, a); break;
+        case VF_F32_W16: hipLaunchKernelGGL(k_raycast<VF_F32_W16>, g, dim3(256), 0, s, m, a); break;
+        default: hipLaunchKernelGGL(k_raycast<VF_F16_W8>, g, dim3(256), 0, s, m, a); break;
+    }
+}
+
+}  // namespace fu

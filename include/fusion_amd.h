@@ -1,0 +1,268 @@
+/*
+ * fusion_amd.h - C ABI of the MI355X (gfx950) dense RGB-D pipeline.
+ *
+ * Drop-in boundary for the CUDA path of cnxyang/Fusion: the free functions declared in
+ * Mapping/RenderScene.h:3-52 and Tracking/Reduction.h:18-72, the forwarding methods of
+ * Mapping (Mapping/Mapping.h:20-70) and the per-frame sequence of System::GrabImage
+ * (Core/System.cc:92-138).  Each entry point names the reference function it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Image arguments are device pointers to dense row-major
+ *    images (pitch = width): depth u16, depth/range f32, vertex and normal maps as 4 floats
+ *    per pixel (x,y,z,w), colour 3 bytes per pixel.
+ *  - Every call takes a stream (a hipStream_t; NULL = the default stream) and is
+ *    asynchronous, except where a host-memory output is requested: such calls synchronise
+ *    the stream before returning, as the reference wrappers do (cudaDeviceSynchronize).
+ *  - Errors are returned as fu_status codes (never exit()); fu_last_error() gives the text.
+ *  - Poses are camera-to-world.  The float pose of the reference kernels (Matrix3f Rview,
+ *    Matrix3f RviewInv, float3 tview, Core/Frame.cc:275-294) is fu_pose_f; 4x4 poses are
+ *    row-major doubles (Core/Frame.h:78).
+ */
+#ifndef FUSION_AMD_H
+#define FUSION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int fu_status;
+#define FU_OK 0
+#define FU_ERR_ARG (-1)      /* invalid argument or size */
+#define FU_ERR_HIP (-2)      /* HIP runtime error (text in fu_last_error) */
+#define FU_ERR_NOMEM (-3)    /* device allocation failed */
+#define FU_ERR_STATE (-4)    /* object not in a state that allows the call */
+
+typedef void* fu_stream;
+
+typedef struct fu_map fu_map;
+typedef struct fu_tracker fu_tracker;
+typedef struct fu_system fu_system;
+
+/* Voxel storage formats (per voxel: sdf, weight, rgb) */
+#define FU_VOXEL_F32_W8 0  /* f32 sdf + u8 weight: the reference Voxel (DeviceMap.h:61-93), 8 B */
+#define FU_VOXEL_F32_W16 1 /* f32 sdf + u16 weight, 9 B */
+#define FU_VOXEL_F16_W8 2  /* f16 sdf + u8 weight, 6 B */
+
+/* Map capacities and constants: the compile-time constants of DeviceMap (Mapping/DeviceMap.h:135-151)
+ * as run-time parameters. */
+typedef struct fu_map_params {
+    uint32_t num_buckets;        /* NumBuckets (:140) */
+    uint32_t num_excess;         /* NumExcess (:139) */
+    uint32_t num_blocks;         /* NumSdfBlocks (:141): voxel blocks in the pool */
+    float voxel_size;            /* VoxelSize (:145), metres */
+    float trunc_dist;            /* TruncateDist (:146), metres */
+    float depth_min, depth_max;  /* DepthMin/DepthMax (:137-138) */
+    uint32_t voxel_format;       /* FU_VOXEL_* */
+    uint32_t max_weight;         /* weight cap (FuseMap.cu:186 uses 255) */
+    uint32_t max_render_blocks;  /* MaxRenderingBlocks (:147) */
+    uint32_t max_new_blocks;     /* per-frame capacity of newly allocated blocks (no reference counterpart) */
+} fu_map_params;
+
+typedef struct fu_map_stats {
+    uint32_t n_allocated;  /* blocks taken from the pool (heapCounter in the reference) */
+    uint32_t n_excess;     /* excess slots used (entryPtr - 1) */
+    uint32_t n_visible;    /* noVisibleBlocks of the last visibility pass */
+    uint32_t n_new;        /* blocks allocated by the last fusion pass */
+    uint32_t n_fragments;  /* rendering fragments of the last z-range pass (noRenderingBlocks) */
+    uint32_t n_triangles;  /* triangles of the last mesh pass (noTriangles) */
+    uint32_t flags;        /* FU_FLAG_*: capacity overflows since the last reset */
+    uint32_t capacity;     /* num_blocks */
+} fu_map_stats;
+
+#define FU_FLAG_HEAP_FULL 1u     /* CreateEntry found the heap empty (DeviceMap.cu:14-20) */
+#define FU_FLAG_EXCESS_FULL 2u   /* excess list full (DeviceMap.cu:54) */
+#define FU_FLAG_NEWSET_FULL 4u   /* more new blocks in one frame than max_new_blocks */
+#define FU_FLAG_COORD_RANGE 8u   /* a block coordinate outside [-2^20, 2^20) was skipped */
+#define FU_FLAG_RENDER_FULL 16u  /* rendering fragment list full (RenderScene.cu:144-146) */
+#define FU_FLAG_TRI_FULL 32u     /* triangle output full (MeshScene.cu:263,281) */
+
+typedef struct fu_pose_f {
+    float R[9];    /* rotation, row-major (Frame::GpuRotation) */
+    float Rinv[9]; /* its inverse (Frame::GpuInvRotation) */
+    float t[3];    /* translation (Frame::GpuTranslation) */
+} fu_pose_f;
+
+/* ---------------------------------------------------------------- library */
+const char* fu_last_error(void);
+const char* fu_version(void);
+/* The marching-cubes tables the library uses, in the reference's form (Mapping/Constant.h):
+ * edge[256], tri[256*16] (-1 padded), vert[256]. Host memory; no device needed. */
+fu_status fu_mc_tables(int32_t* edge, int32_t* tri, int32_t* vert);
+void fu_map_params_default(fu_map_params* p);
+/* bytes of device memory a map with these parameters allocates */
+size_t fu_map_bytes(const fu_map_params* p);
+
+/* ---------------------------------------------------------------- map (Mapping) */
+/* Mapping::Create (Mapping/Mapping.cc:11-51): allocate every map buffer and reset the map. */
+fu_status fu_map_create(const fu_map_params* p, fu_map** out);
+fu_status fu_map_destroy(fu_map* m);
+fu_status fu_map_get_params(const fu_map* m, fu_map_params* out);
+/* ResetMap (Mapping/RenderScene.h:3, FuseMap.cu:352-364) */
+fu_status fu_reset_map(fu_map* m, fu_stream s);
+/* counters; synchronises */
+fu_status fu_map_stats_get(fu_map* m, fu_map_stats* out, fu_stream s);
+
+/* FuseMapColor (RenderScene.h:47-52, FuseMap.cu:251-319): allocate the blocks the frame's
+ * truncation band touches, build the visible-block list, integrate depth and colour.
+ * depth: raw range image (metres, NaN invalid), nmap: level-0 normals of the same frame.
+ * As in the reference the map's depth_min/depth_max are used and depth_max/depth_min
+ * arguments are ignored (FuseMap.cu:288-289).  n_visible (host, optional): synchronise
+ * and return the visible count. */
+fu_status fu_fuse_map_color(fu_map* m, const float* depth, const uint8_t* rgb, const float* nmap, int cols, int rows,
+                            const fu_pose_f* pose, float fx, float fy, float cx, float cy, float depth_max,
+                            float depth_min, uint32_t* n_visible, fu_stream s);
+/* CheckBlockVisibility (RenderScene.h:42-45, FuseMap.cu:206-249) */
+fu_status fu_check_block_visibility(fu_map* m, const fu_pose_f* pose, int cols, int rows, float fx, float fy, float cx,
+                                    float cy, float depth_max, float depth_min, uint32_t* n_visible, fu_stream s);
+/* CreateRenderingBlocks (RenderScene.h:21-27, RenderScene.cu:199-276): z-range tile maps
+ * (zcols x zrows floats, tile = 8x8 pixels) from the visible list.  n_visible: number of
+ * visible entries to use (the reference's argument); UINT32_MAX = the map's own count.
+ * n_fragments/rendered (host, optional): synchronise, return noRenderingBlocks and
+ * whether anything is to be rendered. */
+fu_status fu_create_rendering_blocks(fu_map* m, float* zrange_min, float* zrange_max, int zcols, int zrows,
+                                     float depth_max, float depth_min, const fu_pose_f* pose, uint32_t n_visible,
+                                     float fx, float fy, float cx, float cy, uint32_t* n_fragments, int* rendered,
+                                     fu_stream s);
+/* Raycast (RenderScene.h:13-19, RenderScene.cu:447-489) */
+fu_status fu_raycast(fu_map* m, float* vmap, float* nmap, int cols, int rows, const float* zrange_min,
+                     const float* zrange_max, int zcols, int zrows, const fu_pose_f* pose, float invfx, float invfy,
+                     float cx, float cy, fu_stream s);
+/* MeshScene (RenderScene.h:29-40, MeshScene.cu:306-355) + Mapping::CreateModel: triangle
+ * soup of the whole map.  vertex/normal: 9 floats per triangle, color: 9 bytes per
+ * triangle, tri_meta (optional): 4 ints per triangle (voxel x, y, z, cube_index*16 + t).
+ * Synchronises; n_triangles = min(total, max_triangles) as the reference returns. */
+fu_status fu_mesh_scene(fu_map* m, float* vertex, float* normal, uint8_t* color, int32_t* tri_meta,
+                        uint32_t max_triangles, uint32_t* n_triangles, uint32_t* n_occupied, fu_stream s);
+
+/* Inspection (no reference counterpart beyond the raw map dump of System::WriteMapToDisk,
+ * Core/System.cc:195-234).  Host outputs; synchronise.
+ *  export_blocks: block coordinates (3 ints each) of the allocated blocks, pool order.
+ *  export_visible: block coordinates of the visible list.
+ *  read_blocks: the 512 voxels (index z*64+y*8+x) of each listed block; absent blocks
+ *  read as sdf NaN, weight 0, colour 0. */
+fu_status fu_map_export_blocks(fu_map* m, int32_t* xyz, uint32_t cap, uint32_t* n, fu_stream s);
+fu_status fu_map_export_visible(fu_map* m, int32_t* xyz, uint32_t cap, uint32_t* n, fu_stream s);
+fu_status fu_map_read_blocks(fu_map* m, const int32_t* xyz, uint32_t n, float* sdf, uint16_t* weight, uint8_t* rgb,
+                             fu_stream s);
+
+/* ---------------------------------------------------------------- frame (Reduction.h) */
+/* FilterDepth (Reduction.h:18-20, Pyrdown.cu:43-54) */
+fu_status fu_filter_depth(const uint16_t* depth, float* raw, float* filtered, int cols, int rows, float depth_scale,
+                          float depth_cutoff, fu_stream s);
+/* PyrDownGauss (Reduction.h:29, Pyrdown.cu:160-169); dst is (cols/2) x (rows/2) */
+fu_status fu_pyrdown_gauss(const float* src, int cols, int rows, float* dst, fu_stream s);
+/* ComputeVMap (Reduction.h:22-24, Pyrdown.cu:80-91) */
+fu_status fu_compute_vmap(const float* depth, float* vmap, int cols, int rows, float fx, float fy, float cx, float cy,
+                          float depth_cutoff, fu_stream s);
+/* ComputeNMap (Reduction.h:26-27, Pyrdown.cu:116-125) */
+fu_status fu_compute_nmap(const float* vmap, float* nmap, int cols, int rows, fu_stream s);
+/* ResizeMap (Reduction.h:40-42, Pyrdown.cu:281-291); destination (cols/2) x (rows/2) */
+fu_status fu_resize_map(const float* vsrc, const float* nsrc, int cols, int rows, float* vdst, float* ndst,
+                        fu_stream s);
+/* ICPStep (Reduction.h:67-72, Reduction.cu:319-375): one point-to-plane reduction.
+ * residual[2] = {sum r^2, count}, A (6x6 row-major) and b (6) in FP64 on the host.
+ * Synchronises. */
+fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const float* next_nmap, const float* last_nmap,
+                      int cols, int rows, const float Rcurr[9], const float tcurr[3], const float Rlast[9],
+                      const float RlastInv[9], const float tlast[3], float fx, float fy, float cx, float cy,
+                      float residual[2], double A[36], double b[6], fu_stream s);
+
+/* ---------------------------------------------------------------- tracker (Tracker::ComputeSE3) */
+typedef struct fu_track_result {
+    double pose[16];       /* NextFrame->pose after tracking (rolled back on failure) */
+    double last_pose[16];  /* LastFrame->pose */
+    double x[6];           /* last solved increment (upsilon, omega) */
+    double A[36], b[6];    /* last normal equations */
+    float icp_error;       /* sqrt(sum r^2) / count of the last iteration */
+    int32_t count;         /* correspondences of the last iteration */
+    int32_t status;        /* FU_TRACK_* */
+    int32_t iterations;    /* iterations run */
+} fu_track_result;
+
+#define FU_TRACK_RUNNING 0
+#define FU_TRACK_OK 1       /* accepted (Tracking.cc:402-403) */
+#define FU_TRACK_FAILED 2   /* NaN error or too few correspondences: pose rolled back (:371-375) */
+#define FU_TRACK_REJECTED 3 /* final error or motion test failed: pose rolled back (:404-407) */
+#define FU_TRACK_FIRST 4    /* first frame: identity pose (Tracking.cc:66-71) */
+
+/* Tracker state and ICP scratch (Tracker::Tracker, Tracking.cc:19-43) */
+fu_status fu_tracker_create(int cols, int rows, fu_tracker** out);
+fu_status fu_tracker_destroy(fu_tracker* t);
+/* set LastFrame->pose (4x4 row-major) */
+fu_status fu_tracker_set_last_pose(fu_tracker* t, const double pose[16], fu_stream s);
+/* Tracker::ComputeSE3(icpOnly=true, iter, thresh) (Tracking.cc:296-409) run entirely on the
+ * device: per iteration one reduction kernel whose last work-group solves the 6x6 system
+ * and composes the pose in FP64.  Maps: 3 levels each, level l is (cols>>l) x (rows>>l);
+ * K = {fx, fy, cx, cy} of level 0 (level l uses K/2^l). */
+fu_status fu_compute_se3(fu_tracker* t, const float* const next_vmap[3], const float* const next_nmap[3],
+                         const float* const last_vmap[3], const float* const last_nmap[3], const float K[4],
+                         const int32_t iters[3], const int32_t min_count[3], float thresh, fu_stream s);
+/* the result of the last compute_se3; synchronises */
+fu_status fu_tracker_result(fu_tracker* t, fu_track_result* out, fu_stream s);
+
+/* ---------------------------------------------------------------- per-frame pipeline (System) */
+typedef struct fu_system_params {
+    int32_t cols, rows;
+    float fx, fy, cx, cy;
+    float depth_scale;   /* Frame::mDepthScale: depth units per metre */
+    float depth_cutoff;  /* Frame::mDepthCutoff, metres */
+    int32_t iters[3];    /* ICP iterations per level (Tracking.h:100) */
+    int32_t min_count[3]; /* MIN_ICP_COUNT (Tracking.h:102) */
+    float icp_thresh;    /* THRESH_ICP_SE3 (Tracking.h:103) */
+    int32_t plain;       /* 1: reference-style control (host solve, one synchronisation per
+                            stage); 0: device-resident (on-device solve, no host round trip) */
+    int32_t use_graph;   /* device-resident mode only: replay the frame as one hipGraph */
+    fu_map_params map;
+} fu_system_params;
+
+typedef struct fu_frame_result {
+    double pose[16];        /* camera-to-world pose of the frame (after tracking) */
+    int32_t track_status;   /* FU_TRACK_* */
+    int32_t fused;          /* 1 if the frame was fused and raycast */
+    float icp_error;
+    int32_t icp_count;
+    uint32_t n_visible;
+    uint32_t n_allocated;
+    uint32_t n_fragments;
+    uint32_t flags;
+    uint32_t frame_index;
+} fu_frame_result;
+
+/* maps owned by a system (device pointers) */
+typedef struct fu_system_buffers {
+    uint16_t* depth_in;  /* input slot: cols*rows u16 */
+    uint8_t* rgb_in;     /* input slot: cols*rows*3 u8 */
+    float* raw;          /* range image (metres) */
+    float* depth[3];     /* filtered depth pyramid */
+    float* vmap[3];      /* current frame maps */
+    float* nmap[3];
+    float* model_vmap[3]; /* raycast maps of the model (LastFrame after tracking) */
+    float* model_nmap[3];
+    float* zrange_min;
+    float* zrange_max;
+    int32_t zcols, zrows;
+} fu_system_buffers;
+
+void fu_system_params_default(fu_system_params* p);
+fu_status fu_system_create(const fu_system_params* p, fu_system** out);
+fu_status fu_system_destroy(fu_system* sys);
+/* reset map and tracking (System request "reboot" -> Mapping::Reset, Tracker::ResetTracking) */
+fu_status fu_system_reset(fu_system* sys, fu_stream s);
+fu_status fu_system_get_buffers(fu_system* sys, fu_system_buffers* out);
+fu_map* fu_system_map(fu_system* sys);
+/* System::GrabImage (Core/System.cc:92-138) in ICP-only mode on the frame in the input slot
+ * (copy the frame into depth_in/rgb_in first, on the same stream): pyramid, tracking
+ * against the model raycast, fusion, raycast. Asynchronous. */
+fu_status fu_system_grab_image(fu_system* sys, fu_stream s);
+/* the same on caller-provided device images */
+fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s);
+/* outcome of the last frame; synchronises */
+fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FUSION_AMD_H */

@@ -1,0 +1,358 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) - TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, as
+the checker.  Arrays are numpy, host memory.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+P = C.c_void_p
+I = C.c_int
+U = C.c_uint32
+F = C.c_float
+FP = C.POINTER(C.c_float)
+DP = C.POINTER(C.c_double)
+
+_SIG = {
+    "or_expf": (F, [F]),
+    "or_filter_depth": (None, [P, P, P, I, I, F, F]),
+    "or_pyrdown_gauss": (None, [P, I, I, P]),
+    "or_compute_vmap": (None, [P, P, I, I, F, F, F, F, F]),
+    "or_compute_nmap": (None, [P, P, I, I]),
+    "or_resize_map": (None, [P, P, I, I, P, P]),
+    "or_icp_sums": (None, [P, P, P, P, I, I, FP, FP, FP, FP, FP, F, F, F, F, FP, DP]),
+    "or_solve6": (I, [DP, DP, DP]),
+    "or_se3_exp": (None, [DP, DP]),
+    "or_compute_se3": (I, [C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), I, I, FP,
+                           C.POINTER(I), C.POINTER(I), F, DP, DP, FP, C.POINTER(I)]),
+    "or_pose_f": (None, [DP, FP, FP, FP]),
+    "or_map_create": (P, [U, U, U, F, F, F, F, U, I]),
+    "or_map_destroy": (None, [P]),
+    "or_map_reset": (None, [P]),
+    "or_map_counters": (None, [P, C.POINTER(U)]),
+    "or_create_blocks": (None, [P, P, I, I, FP, FP, F, F, F, F]),
+    "or_check_visibility": (U, [P, FP, FP, I, I, F, F, F, F, F, F]),
+    "or_integrate": (None, [P, P, P, P, I, I, FP, FP, F, F, F, F]),
+    "or_fuse_map_color": (U, [P, P, P, P, I, I, FP, FP, FP, F, F, F, F]),
+    "or_create_rendering_blocks": (I, [P, P, P, I, I, F, F, FP, FP, U, F, F, F, F, C.POINTER(U)]),
+    "or_raycast": (None, [P, P, P, I, I, P, P, I, I, FP, FP, FP, F, F, F, F]),
+    "or_mesh": (U, [P, P, P, P, P, U]),
+    "or_map_export_blocks": (U, [P, P, U]),
+    "or_map_export_visible": (U, [P, P, U]),
+    "or_map_read_blocks": (None, [P, P, U, P, P, P]),
+    "or_map_read_blocks_f64": (None, [P, P, U, P]),
+    "or_hash": (U, [C.c_int32, C.c_int32, C.c_int32, U]),
+    "or_system_create": (P, [I, I, F, F, F, F, F, F, C.POINTER(I), C.POINTER(I), F, U, U, U, F, F, U, I]),
+    "or_system_destroy": (None, [P]),
+    "or_system_grab": (I, [P, P, P, DP]),
+    "or_system_map": (P, [P]),
+    "or_system_buffer": (FP, [P, I, I]),
+    "or_set_threads": (None, [I]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: %s (make -C oracle)" % LIB_PATH)
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _fa(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.ctypes.data_as(FP), a
+
+
+def pose_f(pose):
+    """(R, Rinv, t) float32 views of a 4x4 pose (Core/Frame.cc:275-294)"""
+    pose = np.ascontiguousarray(pose, dtype=np.float64)
+    R = np.zeros(9, np.float32)
+    Ri = np.zeros(9, np.float32)
+    t = np.zeros(3, np.float32)
+    lib().or_pose_f(pose.ctypes.data_as(DP), R.ctypes.data_as(FP), Ri.ctypes.data_as(FP), t.ctypes.data_as(FP))
+    return R, Ri, t
+
+
+def filter_depth(depth_u16, depth_scale, cutoff):
+    rows, cols = depth_u16.shape
+    d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
+    raw = np.zeros((rows, cols), np.float32)
+    filt = np.zeros((rows, cols), np.float32)
+    lib().or_filter_depth(_ptr(d), _ptr(raw), _ptr(filt), cols, rows, depth_scale, cutoff)
+    return raw, filt
+
+
+def pyrdown_gauss(src):
+    rows, cols = src.shape
+    s = np.ascontiguousarray(src, dtype=np.float32)
+    dst = np.zeros((rows // 2, cols // 2), np.float32)
+    lib().or_pyrdown_gauss(_ptr(s), cols, rows, _ptr(dst))
+    return dst
+
+
+def compute_vmap(depth, fx, fy, cx, cy, cutoff):
+    rows, cols = depth.shape
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    v = np.zeros((rows, cols, 4), np.float32)
+    lib().or_compute_vmap(_ptr(d), _ptr(v), cols, rows, fx, fy, cx, cy, cutoff)
+    return v
+
+
+def compute_nmap(vmap):
+    rows, cols, _ = vmap.shape
+    v = np.ascontiguousarray(vmap, dtype=np.float32)
+    n = np.zeros_like(v)
+    lib().or_compute_nmap(_ptr(v), _ptr(n), cols, rows)
+    return n
+
+
+def resize_map(vsrc, nsrc):
+    rows, cols, _ = vsrc.shape
+    v = np.ascontiguousarray(vsrc, dtype=np.float32)
+    n = np.ascontiguousarray(nsrc, dtype=np.float32)
+    vd = np.zeros((rows // 2, cols // 2, 4), np.float32)
+    nd = np.zeros_like(vd)
+    lib().or_resize_map(_ptr(v), _ptr(n), cols, rows, _ptr(vd), _ptr(nd))
+    return vd, nd
+
+
+def icp_sums(nv, lv, nn, ln, pose_curr, pose_last, K):
+    rows, cols, _ = nv.shape
+    Rc, _, tc = pose_f(pose_curr)
+    Rl, Rli, tl = pose_f(pose_last)
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (nv, lv, nn, ln)]
+    s32 = np.zeros(29, np.float32)
+    s64 = np.zeros(29, np.float64)
+    fp = lambda a: a.ctypes.data_as(FP)
+    lib().or_icp_sums(*[_ptr(a) for a in arrs], cols, rows, fp(Rc), fp(tc), fp(Rl), fp(Rli), fp(tl),
+                      K[0], K[1], K[2], K[3], fp(s32), s64.ctypes.data_as(DP))
+    return s32, s64
+
+
+def unpack_sums(s):
+    """CreateMatrix<6,7> (Reduction.cu:6-16): A (6x6), b (6)"""
+    A = np.zeros((6, 6))
+    b = np.zeros(6)
+    k = 0
+    for i in range(6):
+        for j in range(i, 7):
+            if j == 6:
+                b[i] = s[k]
+            else:
+                A[i, j] = A[j, i] = s[k]
+            k += 1
+    return A, b
+
+
+def solve6(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(6)
+    if lib().or_solve6(A.ctypes.data_as(DP), b.ctypes.data_as(DP), x.ctypes.data_as(DP)) != 0:
+        raise np.linalg.LinAlgError("not positive definite")
+    return x
+
+
+def se3_exp(xi):
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    T = np.zeros(16)
+    lib().or_se3_exp(xi.ctypes.data_as(DP), T.ctypes.data_as(DP))
+    return T.reshape(4, 4)
+
+
+def compute_se3(next_v, next_n, last_v, last_n, K, last_pose, iters=(10, 5, 4), min_count=(2000, 1000, 100),
+                thresh=1e-4):
+    keep = [np.ascontiguousarray(a, dtype=np.float32) for a in list(next_v) + list(next_n) + list(last_v) + list(last_n)]
+    arr = lambda k: (C.c_void_p * 3)(*[keep[k * 3 + i].ctypes.data for i in range(3)])
+    rows, cols, _ = keep[0].shape
+    lp = np.ascontiguousarray(last_pose, dtype=np.float64).ravel()
+    out = np.zeros(16)
+    err = C.c_float()
+    cnt = C.c_int()
+    st = lib().or_compute_se3(arr(0), arr(1), arr(2), arr(3), cols, rows, (C.c_float * 4)(*K),
+                              (C.c_int * 3)(*iters), (C.c_int * 3)(*min_count), thresh, lp.ctypes.data_as(DP),
+                              out.ctypes.data_as(DP), C.byref(err), C.byref(cnt))
+    return st, out.reshape(4, 4), err.value, cnt.value
+
+
+class Map:
+    def __init__(self, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
+                 trunc_dist=None, depth_min=0.1, depth_max=3.0, max_weight=255, precision=0):
+        if trunc_dist is None:
+            trunc_dist = float(np.float32(voxel_size) * np.float32(3.0))
+        self.h = lib().or_map_create(num_buckets, num_excess, num_blocks, voxel_size, trunc_dist, depth_min,
+                                     depth_max, max_weight, precision)
+        self.owned = True
+        self.depth_min, self.depth_max = depth_min, depth_max
+
+    def __del__(self):
+        try:
+            if self.owned and self.h:
+                lib().or_map_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().or_map_reset(self.h)
+
+    def counters(self):
+        c = (U * 5)()
+        lib().or_map_counters(self.h, c)
+        return {"n_allocated": c[0], "n_excess": c[1], "n_visible": c[2], "heap_full": c[3], "excess_full": c[4]}
+
+    def create_blocks(self, depth, pose, K):
+        rows, cols = depth.shape
+        R, _, t = pose_f(pose)
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        lib().or_create_blocks(self.h, _ptr(d), cols, rows, R.ctypes.data_as(FP), t.ctypes.data_as(FP), *K)
+
+    def check_visibility(self, pose, cols, rows, K, depth_max=None, depth_min=None):
+        _, Ri, t = pose_f(pose)
+        return lib().or_check_visibility(self.h, Ri.ctypes.data_as(FP), t.ctypes.data_as(FP), cols, rows, *K,
+                                         self.depth_max if depth_max is None else depth_max,
+                                         self.depth_min if depth_min is None else depth_min)
+
+    def integrate(self, depth, rgb, nmap, pose, K):
+        rows, cols = depth.shape
+        _, Ri, t = pose_f(pose)
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        c = np.ascontiguousarray(rgb, dtype=np.uint8)
+        n = np.ascontiguousarray(nmap, dtype=np.float32)
+        lib().or_integrate(self.h, _ptr(d), _ptr(c), _ptr(n), cols, rows, Ri.ctypes.data_as(FP),
+                           t.ctypes.data_as(FP), *K)
+
+    def fuse_map_color(self, depth, rgb, nmap, pose, K):
+        rows, cols = depth.shape
+        R, Ri, t = pose_f(pose)
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        c = np.ascontiguousarray(rgb, dtype=np.uint8)
+        n = np.ascontiguousarray(nmap, dtype=np.float32)
+        return lib().or_fuse_map_color(self.h, _ptr(d), _ptr(c), _ptr(n), cols, rows, R.ctypes.data_as(FP),
+                                       Ri.ctypes.data_as(FP), t.ctypes.data_as(FP), *K)
+
+    def create_rendering_blocks(self, zcols, zrows, pose, n_visible, K, depth_max=None, depth_min=None):
+        _, Ri, t = pose_f(pose)
+        zmin = np.zeros((zrows, zcols), np.float32)
+        zmax = np.zeros((zrows, zcols), np.float32)
+        nf = U()
+        r = lib().or_create_rendering_blocks(self.h, _ptr(zmin), _ptr(zmax), zcols, zrows,
+                                             self.depth_max if depth_max is None else depth_max,
+                                             self.depth_min if depth_min is None else depth_min,
+                                             Ri.ctypes.data_as(FP), t.ctypes.data_as(FP), n_visible, *K, C.byref(nf))
+        return zmin, zmax, nf.value, bool(r)
+
+    def raycast(self, cols, rows, zmin, zmax, pose, K):
+        R, Ri, t = pose_f(pose)
+        v = np.zeros((rows, cols, 4), np.float32)
+        n = np.zeros_like(v)
+        zmin = np.ascontiguousarray(zmin, np.float32)
+        zmax = np.ascontiguousarray(zmax, np.float32)
+        zrows, zcols = zmin.shape
+        invfx = np.float32(1.0 / np.float64(np.float32(K[0])))
+        invfy = np.float32(1.0 / np.float64(np.float32(K[1])))
+        lib().or_raycast(self.h, _ptr(v), _ptr(n), cols, rows, _ptr(zmin), _ptr(zmax), zcols, zrows,
+                         R.ctypes.data_as(FP), Ri.ctypes.data_as(FP), t.ctypes.data_as(FP), invfx, invfy, K[2], K[3])
+        return v, n
+
+    def mesh(self, max_tri=2_000_000, with_meta=True):
+        total = lib().or_mesh(self.h, None, None, None, None, 0)
+        k = min(total, max_tri)
+        vert = np.zeros((k, 3, 3), np.float32)
+        norm = np.zeros((k, 3, 3), np.float32)
+        col = np.zeros((k, 3, 3), np.uint8)
+        meta = np.zeros((k, 4), np.int32)
+        lib().or_mesh(self.h, _ptr(vert), _ptr(norm), _ptr(col), _ptr(meta) if with_meta else None, k)
+        return vert, norm, col, meta, total
+
+    def export_blocks(self):
+        n = lib().or_map_export_blocks(self.h, None, 0)
+        out = np.zeros((n, 3), np.int32)
+        lib().or_map_export_blocks(self.h, _ptr(out), n)
+        return out
+
+    def export_visible(self):
+        n = lib().or_map_export_visible(self.h, None, 0)
+        out = np.zeros((n, 3), np.int32)
+        lib().or_map_export_visible(self.h, _ptr(out), n)
+        return out
+
+    def read_blocks(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        n = xyz.shape[0]
+        sdf = np.zeros((n, 512), np.float32)
+        w = np.zeros((n, 512), np.uint16)
+        rgb = np.zeros((n, 512, 3), np.uint8)
+        lib().or_map_read_blocks(self.h, _ptr(xyz), n, _ptr(sdf), _ptr(w), _ptr(rgb))
+        return sdf, w, rgb
+
+    def read_blocks_f64(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        sdf = np.zeros((xyz.shape[0], 512), np.float64)
+        lib().or_map_read_blocks_f64(self.h, _ptr(xyz), xyz.shape[0], _ptr(sdf))
+        return sdf
+
+
+class System:
+    """System::GrabImage (ICP-only) on the CPU"""
+
+    def __init__(self, cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), depth_scale=5000.0, depth_cutoff=3.0,
+                 iters=(10, 5, 4), min_count=(2000, 1000, 100), icp_thresh=1e-4, num_buckets=1 << 18,
+                 num_excess=1 << 17, num_blocks=200_000, voxel_size=0.01, trunc_dist=None, max_weight=255,
+                 precision=0):
+        if trunc_dist is None:
+            trunc_dist = float(np.float32(voxel_size) * np.float32(3.0))
+        self.cols, self.rows = cols, rows
+        self.h = lib().or_system_create(cols, rows, *K, depth_scale, depth_cutoff, (I * 3)(*iters),
+                                        (I * 3)(*min_count), icp_thresh, num_buckets, num_excess, num_blocks,
+                                        voxel_size, trunc_dist, max_weight, precision)
+        self.map = Map.__new__(Map)
+        self.map.h = lib().or_system_map(self.h)
+        self.map.owned = False
+        self.map.depth_min, self.map.depth_max = 0.1, 3.0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def close(self):
+        if self.h:
+            self.map.h = None
+            lib().or_system_destroy(self.h)
+            self.h = None
+
+    def grab(self, depth_u16, rgb):
+        d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
+        c = np.ascontiguousarray(rgb, dtype=np.uint8)
+        pose = np.zeros(16)
+        st = lib().or_system_grab(self.h, _ptr(d), _ptr(c), pose.ctypes.data_as(DP))
+        return st, pose.reshape(4, 4)
+
+    def buffer(self, which, level=0):
+        rows, cols = self.rows >> level, self.cols >> level
+        shape = (rows, cols) if which in (0, 5) else (rows, cols, 4)
+        ptr = lib().or_system_buffer(self.h, which, level)
+        return np.ctypeslib.as_array(ptr, shape=shape).copy()
+
+
+def set_threads(n):
+    lib().or_set_threads(n)

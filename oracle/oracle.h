@@ -1,0 +1,120 @@
+/*
+ * CPU oracle of the dense RGB-D pipeline - TEST INFRASTRUCTURE ONLY.
+ *
+ * A scalar C++ restatement of the reference's CUDA algorithm, used by tests/ (as the
+ * checker), by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
+ * library never links or calls it.  Every function cites the reference file:line it follows.
+ * Floating-point arithmetic is written in the reference's operation order and the library is
+ * built with -ffp-contract=off, so single-precision results are those of a strict IEEE
+ * evaluation of the reference's expressions.
+ *
+ * Defined behaviour where the reference has none (DEVIATIONS, also listed in DESIGN.md):
+ *  D1 float->int conversions saturate and map NaN to 0 (the GPU conversion instruction).
+ *  D2 the block hash reduces the wrapped int products modulo the *unsigned* bucket count
+ *     (DeviceMap.cu:3-10: NumBuckets is `uint`).
+ *  D3 allocation is sequential in pixel order; no insert is dropped (the reference's
+ *     try-lock may drop one under contention, DeviceMap.cu:45-60).
+ *  D4 the raycast's one-block cache starts empty (the reference's default cache "hits" block
+ *     (0,0,0) with ptr -1, DeviceMap.cu:100-103).
+ *  D5 rendering fragments are neither lost to overlapping writes nor read from stale slots
+ *     (RenderScene.cu:109-116 writes ny*ny fragments into nx*ny slots; fillBlocksKernel
+ *     covers ceil(n/4)*4 slots, :269); the loop-bound and corner quirks themselves are kept.
+ *  D6 ResizeMap bounds its grid by the destination size (the reference checks the source
+ *     size, Pyrdown.cu:256, which reads past the source at 1280x720).
+ *  D7 invalid vertex-map pixels are all-NaN (the reference sets only x, Pyrdown.cu:75).
+ */
+#ifndef FU_ORACLE_H
+#define FU_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_map or_map;
+
+/* the bilateral filter's exp: the same published method as the library (Cody-Waite
+ * reduction + degree-7 Taylor polynomial) */
+float or_expf(float x);
+
+/* Tracking/Pyrdown.cu */
+void or_filter_depth(const uint16_t* depth, float* raw, float* filt, int cols, int rows, float depth_scale,
+                     float cutoff);
+void or_pyrdown_gauss(const float* src, int cols, int rows, float* dst);
+void or_compute_vmap(const float* depth, float* vmap, int cols, int rows, float fx, float fy, float cx, float cy,
+                     float cutoff);
+void or_compute_nmap(const float* vmap, float* nmap, int cols, int rows);
+void or_resize_map(const float* vsrc, const float* nsrc, int cols, int rows, float* vdst, float* ndst);
+
+/* Tracking/Reduction.cu ICPReduce: sums32 = the 29 sums accumulated in FP32 in pixel order,
+ * sums64 = the same FP32 row products accumulated in FP64 */
+void or_icp_sums(const float* next_vmap, const float* last_vmap, const float* next_nmap, const float* last_nmap,
+                 int cols, int rows, const float Rcurr[9], const float tcurr[3], const float Rlast[9],
+                 const float RlastInv[9], const float tlast[3], float fx, float fy, float cx, float cy,
+                 float sums32[29], double sums64[29]);
+/* solve A x = b for symmetric positive definite A (Cholesky, FP64) */
+int or_solve6(const double A[36], const double b[6], double x[6]);
+/* SE(3) exponential of (upsilon, omega) */
+void or_se3_exp(const double xi[6], double T[16]);
+/* Tracker::ComputeSE3 in ICP-only mode (Tracking.cc:296-409) with FP64 accumulation.
+ * Maps: 3 levels. last_pose in, pose out (rolled back on failure). Returns FU_TRACK_* code
+ * (1 ok, 2 failed, 3 rejected); err/count of the last iteration. */
+int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap[3], const float* const last_vmap[3],
+                   const float* const last_nmap[3], int cols, int rows, const float K[4], const int iters[3],
+                   const int min_count[3], float thresh, const double last_pose[16], double pose[16],
+                   float* err, int* count);
+/* float views of a 4x4 pose: R, Rinv (transpose in double, then float), t */
+void or_pose_f(const double pose[16], float R[9], float Rinv[9], float t[3]);
+
+/* map (Mapping/DeviceMap.*, FuseMap.cu) */
+or_map* or_map_create(uint32_t num_buckets, uint32_t num_excess, uint32_t num_blocks, float voxel_size,
+                      float trunc_dist, float depth_min, float depth_max, uint32_t max_weight, int precision);
+void or_map_destroy(or_map* m);
+void or_map_reset(or_map* m);
+/* counters: [0] allocated, [1] excess used, [2] visible, [3] heap-full events, [4] excess-full events */
+void or_map_counters(const or_map* m, uint32_t out[5]);
+void or_create_blocks(or_map* m, const float* depth, int cols, int rows, const float R[9], const float t[3], float fx,
+                      float fy, float cx, float cy);
+uint32_t or_check_visibility(or_map* m, const float Rinv[9], const float t[3], int cols, int rows, float fx, float fy,
+                             float cx, float cy, float depth_max, float depth_min);
+void or_integrate(or_map* m, const float* depth, const uint8_t* rgb, const float* nmap, int cols, int rows,
+                  const float Rinv[9], const float t[3], float fx, float fy, float cx, float cy);
+uint32_t or_fuse_map_color(or_map* m, const float* depth, const uint8_t* rgb, const float* nmap, int cols, int rows,
+                           const float R[9], const float Rinv[9], const float t[3], float fx, float fy, float cx,
+                           float cy);
+int or_create_rendering_blocks(or_map* m, float* zmin, float* zmax, int zcols, int zrows, float depth_max,
+                               float depth_min, const float Rinv[9], const float t[3], uint32_t n_visible, float fx,
+                               float fy, float cx, float cy, uint32_t* n_fragments);
+void or_raycast(or_map* m, float* vmap, float* nmap, int cols, int rows, const float* zmin, const float* zmax,
+                int zcols, int zrows, const float R[9], const float Rinv[9], const float t[3], float invfx,
+                float invfy, float cx, float cy);
+/* marching cubes over all allocated blocks; returns the total triangle count (outputs are
+ * written for the first max_tri); meta: 4 ints per triangle (voxel x,y,z, cube*16+t) */
+uint32_t or_mesh(or_map* m, float* vertex, float* normal, uint8_t* color, int32_t* meta, uint32_t max_tri);
+uint32_t or_map_export_blocks(const or_map* m, int32_t* xyz, uint32_t cap);
+uint32_t or_map_export_visible(const or_map* m, int32_t* xyz, uint32_t cap);
+/* sdf as float (FP64 maps: rounded), weight, rgb of the listed blocks; absent -> NaN/0/0 */
+void or_map_read_blocks(const or_map* m, const int32_t* xyz, uint32_t n, float* sdf, uint16_t* weight, uint8_t* rgb);
+void or_map_read_blocks_f64(const or_map* m, const int32_t* xyz, uint32_t n, double* sdf);
+/* the bucket index of a block (DeviceMap::Hash) */
+uint32_t or_hash(int32_t x, int32_t y, int32_t z, uint32_t num_buckets);
+
+/* the per-frame pipeline (System::GrabImage, ICP-only) */
+typedef struct or_system or_system;
+or_system* or_system_create(int cols, int rows, float fx, float fy, float cx, float cy, float depth_scale,
+                            float depth_cutoff, const int iters[3], const int min_count[3], float icp_thresh,
+                            uint32_t num_buckets, uint32_t num_excess, uint32_t num_blocks, float voxel_size,
+                            float trunc_dist, uint32_t max_weight, int precision);
+void or_system_destroy(or_system* s);
+/* returns the track status (1 ok, 2 failed, 3 rejected, 4 first frame) */
+int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t* rgb, double pose_out[16]);
+or_map* or_system_map(or_system* s);
+/* level-0 maps: 0 raw depth, 1 vmap, 2 nmap, 3 model vmap, 4 model nmap */
+const float* or_system_buffer(or_system* s, int which, int level);
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""GPU ICP (Tracking/Reduction.cu, Tracking/Tracking.cc:296-409) against the FP64 oracle.
+
+Bar (north star): each ICP step's pose update within 1e-5 rad and 1e-5 m of the FP64
+oracle on the same inputs; the reduced normal equations within FP32 summation error.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fusion_amd import api as A
+from oracle import binding as O
+from tests.helpers import scene_frames
+from tests.test_gpu_pyramid import _pyr_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _rot_err(Ra, Rb):
+    c = (np.trace(Ra.T @ Rb) - 1) / 2
+    return float(np.arccos(np.clip(c, -1, 1)))
+
+
+def _pair(f0=0, f1=2, cols=640, rows=480):
+    sc, K, fr = scene_frames(frames=(f0, f1), cols=cols, rows=rows)
+    _, _, v0, n0 = _pyr_oracle(fr[0][0], K)
+    _, _, v1, n1 = _pyr_oracle(fr[1][0], K)
+    gt = np.linalg.inv(sc.pose(f0)) @ sc.pose(f1)
+    return K, (v0, n0), (v1, n1), gt
+
+
+def test_icp_step_sums(gpu):
+    K, (v0, n0), (v1, n1), gt = _pair()
+    cu = lambda a: torch.from_numpy(a).cuda()
+    A_g, b_g, r_g = A.ICPStep(cu(v1[0]), cu(v0[0]), cu(n1[0]), cu(n0[0]), np.eye(4), np.eye(4), K)
+    s32, s64 = O.icp_sums(v1[0], v0[0], n1[0], n0[0], np.eye(4), np.eye(4), K)
+    A_o, b_o = O.unpack_sums(s64)
+    assert r_g[1] == s64[28]  # the correspondence count is exact
+    scale = np.abs(A_o).max()
+    assert np.abs(A_g - A_o).max() <= 1e-6 * scale
+    assert np.abs(b_g - b_o).max() <= 1e-6 * max(np.abs(b_o).max(), 1e-12) + 1e-6 * scale * 1e-3
+    x_g = np.linalg.solve(A_g, b_g)
+    x_o = O.solve6(A_o, b_o)
+    assert np.abs(x_g - x_o).max() < 1e-6
+
+
+@pytest.mark.parametrize("cols,rows,f1", [(640, 480, 2), (320, 240, 3), (1280, 720, 1)])
+def test_compute_se3_vs_fp64(gpu, cols, rows, f1):
+    print("icp %dx%d" % (cols, rows), flush=True)
+    K, (v0, n0), (v1, n1), gt = _pair(0, f1, cols, rows)
+    t = A.Tracker(cols, rows)
+    cu = lambda xs: [torch.from_numpy(x).cuda() for x in xs]
+    t.set_last_pose(np.eye(4))
+    r = t.ComputeSE3(cu(v1), cu(n1), cu(v0), cu(n0), K)
+    st, pose_o, err_o, cnt_o = O.compute_se3(v1, n1, v0, n0, K, np.eye(4))
+    assert r["status"] == st == 1
+    assert _rot_err(r["pose"][:3, :3], pose_o[:3, :3]) <= 1e-5
+    assert np.linalg.norm(r["pose"][:3, 3] - pose_o[:3, 3]) <= 1e-5
+    # and the motion itself is recovered (at 1280x720 the depth noise of 0.0015 z^2 m spans
+    # several pixel spacings, so the reference algorithm's estimate itself is off by ~1 cm
+    # on this view; the requirement there is agreement with the FP64 oracle above)
+    if cols <= 640:
+        assert np.linalg.norm(r["pose"][:3, 3] - gt[:3, 3]) < 3e-3
+
+
+def test_icp_too_few_points_rolls_back(gpu):
+    K, (v0, n0), (v1, n1), gt = _pair()
+    empty = [np.full_like(v, np.nan) for v in v1]
+    t = A.Tracker(640, 480)
+    P = np.eye(4)
+    P[:3, 3] = [0.1, 0.2, 0.3]
+    t.set_last_pose(P)
+    cu = lambda xs: [torch.from_numpy(x).cuda() for x in xs]
+    r = t.ComputeSE3(cu(empty), cu(empty), cu(v0), cu(n0), K)
+    assert r["status"] == 2
+    assert np.array_equal(r["pose"], P)
+    st, pose_o, _, _ = O.compute_se3(empty, empty, v0, n0, K, P)
+    assert st == 2 and np.array_equal(pose_o, P)
+
+
+def test_icp_large_motion_rejected(gpu):
+    """a 0.2 m jump between model and frame is not accepted (Tracking.cc:398-408)"""
+    sc, K, fr = scene_frames(frames=(0,))
+    _, _, v0, n0 = _pyr_oracle(fr[0][0], K)
+    pose = sc.pose(0).copy()
+    pose[:3, 3] += pose[:3, :3] @ np.array([0.0, 0.0, 0.2])
+    d1, _ = sc.render_pose(pose)
+    _, _, v1, n1 = _pyr_oracle(d1, K)
+    t = A.Tracker(640, 480)
+    cu = lambda xs: [torch.from_numpy(x).cuda() for x in xs]
+    r = t.ComputeSE3(cu(v1), cu(n1), cu(v0), cu(n0), K)
+    st, _, _, _ = O.compute_se3(v1, n1, v0, n0, K, np.eye(4))
+    assert r["status"] == st
+    assert r["status"] in (2, 3)
+    assert np.array_equal(r["pose"], np.eye(4))
