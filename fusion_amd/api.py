@@ -329,6 +329,15 @@ class System:
         d["pose"] = np.array(r.pose[:]).reshape(4, 4)
         return d
 
+    def set_profiling(self, enable=True):
+        check(N.lib().fu_system_set_profiling(self.handle, int(enable)), "fu_system_set_profiling")
+
+    def stage_times(self):
+        t = N.StageTimes()
+        check(N.lib().fu_system_get_stage_times(self.handle, C.byref(t)), "fu_system_get_stage_times")
+        return {"ms": dict(zip(N.STAGES, t.ms[:])), "frames": t.frames, "n_visible": t.n_visible,
+                "n_new": t.n_new, "icp_iterations": t.icp_iterations, "icp_pixels": t.icp_pixels}
+
     def view(self, name, level=0):
         """a torch view of one of the system's device maps"""
         b = self.buffers

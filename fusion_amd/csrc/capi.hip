@@ -591,10 +591,14 @@ struct fu_tracker {
     int nblocks[3];
 };
 
+// One pixel per lane up to 2048 work-groups: every pixel's association is a dependent
+// gather, so latency is hidden across waves (about 5 work-groups per CU at 640x480)
+// rather than by a per-lane loop.
+static constexpr int kIcpMaxBlocks = 2048;
 static int icp_blocks(int npix) {
-    int nb = (npix + 1023) / 1024;  // about four pixels per lane
+    int nb = (npix + 255) / 256;
     if (nb < 1) nb = 1;
-    if (nb > 1024) nb = 1024;
+    if (nb > kIcpMaxBlocks) nb = kIcpMaxBlocks;
     return nb;
 }
 
@@ -616,7 +620,7 @@ extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
     t->cols = cols;
     t->rows = rows;
     for (int l = 0; l < 3; ++l) t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
-    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, 1024 * 32 * 4) != hipSuccess ||
+    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 32 * 4) != hipSuccess ||
         hipMalloc(&t->done, 64) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(t->st);
@@ -798,7 +802,9 @@ struct fu_system {
     int lc[3], lr[3];
     unsigned frame = 0;
     hipGraphExec_t graph = nullptr;
-    hipStream_t graph_stream = nullptr;
+    bool prof = false;
+    hipEvent_t ev[FU_NUM_STAGES + 1] = {};
+    fu_stage_times times{};
     const uint16_t* graph_depth = nullptr;
     const uint8_t* graph_rgb = nullptr;
 };
@@ -839,6 +845,8 @@ static fu_status salloc(fu_system* s, T** out, size_t count) {
 extern "C" fu_status fu_system_destroy(fu_system* sys) {
     if (!sys) return FU_OK;
     if (sys->graph) (void)hipGraphExecDestroy(sys->graph);
+    for (int k = 0; k <= FU_NUM_STAGES; ++k)
+        if (sys->ev[k]) (void)hipEventDestroy(sys->ev[k]);
     for (void* p : sys->allocs) (void)hipFree(p);
     fu_map_destroy(sys->map);
     fu_tracker_destroy(sys->trk);
@@ -930,6 +938,10 @@ static void queue_resize_model(fu_system* sys, hipStream_t s) {
                       (f4*)b.model_vmap[l], (f4*)b.model_nmap[l], sys->lc[l], sys->lr[l], s);
 }
 
+static void mark(fu_system* sys, int boundary, hipStream_t s) {
+    if (sys->prof) (void)hipEventRecord(sys->ev[boundary], s);
+}
+
 // Mapping::FuseColor + Mapping::RayTrace (Mapping.cc:76-109), pose from the tracking state
 static void queue_fuse_raycast(fu_system* sys, const uint8_t* rgb, hipStream_t s) {
     const fu_system_params& p = sys->p;
@@ -941,23 +953,31 @@ static void queue_fuse_raycast(fu_system* sys, const uint8_t* rgb, hipStream_t s
     const VisArgs v = make_vis(m, dummy, st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
     launch_begin_pass(m->v, 7, s);
     launch_alloc(m->v, a, s);
+    mark(sys, FU_STAGE_ALLOC + 1, s);
     launch_visible(m->v, v, s);
+    mark(sys, FU_STAGE_VISIBLE + 1, s);
     launch_integrate(m->v, a, kIntegrateGrid, s);
+    mark(sys, FU_STAGE_INTEGRATE + 1, s);
     const ZRangeArgs z = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min, dummy,
                                  st, -1, p.fx, p.fy, p.cx, p.cy);
     launch_zrange(m->v, z, s);
+    mark(sys, FU_STAGE_ZRANGE + 1, s);
     RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max, b.zcols,
                             b.zrows, dummy, st, (float)(1.0 / (double)p.fx), (float)(1.0 / (double)p.fy), p.cx, p.cy);
     r.keep_v = (const f4*)b.vmap[0];
     r.keep_n = (const f4*)b.nmap[0];
     launch_raycast(m->v, r, s);
+    mark(sys, FU_STAGE_RAYCAST + 1, s);
 }
 
 static void queue_frame_device(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, bool first, hipStream_t s) {
     const fu_system_params& p = sys->p;
     fu_system_buffers& b = sys->b;
+    mark(sys, 0, s);
     if (!first) queue_resize_model(sys, s);
+    mark(sys, FU_STAGE_RESIZE + 1, s);
     queue_pyramid(sys, depth, s);
+    mark(sys, FU_STAGE_PYRAMID + 1, s);
     if (!first) {
         const float* nv[3] = {b.vmap[0], b.vmap[1], b.vmap[2]};
         const float* nn[3] = {b.nmap[0], b.nmap[1], b.nmap[2]};
@@ -967,6 +987,7 @@ static void queue_frame_device(fu_system* sys, const uint16_t* depth, const uint
         queue_se3(sys->trk, nv, nn, lv, ln, K, p.iters, p.min_count, p.icp_thresh, s);
     }
     launch_track_commit(sys->trk->st, s);
+    mark(sys, FU_STAGE_ICP + 1, s);
     queue_fuse_raycast(sys, rgb, s);
 }
 
@@ -1055,6 +1076,25 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     return FU_OK;
 }
 
+static fu_status capture_graph(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, hipGraphExec_t* out) {
+    hipStream_t cs;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    queue_frame_device(sys, depth, rgb, false, cs);
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return FU_OK;
+}
+
 extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s) {
     if (!sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_system_grab_image: null argument");
     const bool first = sys->frame == 0;
@@ -1062,23 +1102,13 @@ extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* d
     fu_status st = FU_OK;
     if (sys->p.plain) {
         st = run_frame_plain(sys, depth, rgb, first, hs);
-    } else if (sys->p.use_graph && !first) {
-        if (!sys->graph || sys->graph_depth != depth || sys->graph_rgb != rgb) {
-            if (sys->graph) {
-                (void)hipGraphExecDestroy(sys->graph);
-                sys->graph = nullptr;
-            }
-            hipStream_t cs;
-            HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            hipGraph_t g;
-            HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            queue_frame_device(sys, depth, rgb, false, cs);
-            hipError_t e = hipStreamEndCapture(cs, &g);
-            (void)hipStreamDestroy(cs);
-            if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-            e = hipGraphInstantiate(&sys->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    } else if (sys->p.use_graph && !first && !sys->prof) {
+        if (sys->graph && (sys->graph_depth != depth || sys->graph_rgb != rgb)) {
+            (void)hipGraphExecDestroy(sys->graph);
+            sys->graph = nullptr;
+        }
+        if (!sys->graph) {
+            if ((st = capture_graph(sys, depth, rgb, &sys->graph))) return st;
             sys->graph_depth = depth;
             sys->graph_rgb = rgb;
         }
@@ -1088,7 +1118,49 @@ extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* d
         LAUNCHCHK();
     }
     if (st) return st;
+    if (sys->prof && !sys->p.plain && !first) {
+        HIPCHK(hipEventSynchronize(sys->ev[FU_NUM_STAGES]));
+        for (int k = 0; k < FU_NUM_STAGES; ++k) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, sys->ev[k], sys->ev[k + 1]));
+            sys->times.ms[k] += ms;
+        }
+        unsigned c[C_COUNT];
+        TrackState h;
+        HIPCHK(hipMemcpy(c, sys->map->v.ctr, sizeof(c), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost));
+        sys->times.frames += 1;
+        sys->times.n_visible += c[C_VISIBLE];
+        sys->times.n_new += c[C_NEW];
+        if (!first) {
+            sys->times.icp_iterations += h.iters_done;
+            // pixel-iterations actually reduced (levels run to completion unless tracking failed)
+            double px = 0;
+            int left = h.iters_done;
+            for (int l = 2; l >= 0 && left > 0; --l) {
+                const int it = std::min(left, sys->p.iters[l]);
+                px += (double)it * sys->lc[l] * sys->lr[l];
+                left -= it;
+            }
+            sys->times.icp_pixels += px;
+        }
+    }
     sys->frame++;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_set_profiling(fu_system* sys, int enable) {
+    if (!sys) return set_err(FU_ERR_ARG, "null system");
+    if (enable && !sys->ev[0])
+        for (int k = 0; k <= FU_NUM_STAGES; ++k) HIPCHK(hipEventCreate(&sys->ev[k]));
+    sys->prof = enable != 0;
+    memset(&sys->times, 0, sizeof(sys->times));
+    return FU_OK;
+}
+
+extern "C" fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* out) {
+    if (!sys || !out) return set_err(FU_ERR_ARG, "null argument");
+    *out = sys->times;
     return FU_OK;
 }
 

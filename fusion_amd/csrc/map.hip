@@ -116,22 +116,28 @@ __global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) {
             }
         }
     }
-    // wave-uniform trip count
-    int wmax = nsteps;
-    for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+    // Rounds: every lane advances its march to its next block that differs from the block
+    // of its previous step (consecutive repeats are no-ops for CreateBlock), so a wave runs
+    // as many rounds as its rays cross distinct blocks (about 3) instead of one per step.
+    int i = 0;
     i3 prev = mki3(0x7fffffff, 0, 0);
-    for (int i = 0; i < wmax; ++i) {
-        bool has = i < nsteps;
+    while (ballot(i < nsteps)) {
+        bool has = false;
         i3 bp = prev;
-        if (has) {
-            bp = voxel_to_block(f2i3(pt));
+        while (i < nsteps) {
+            const i3 b = voxel_to_block(f2i3(pt));
             pt = add(pt, dir);
-            if (eqi(bp, prev)) has = false;  // same block as the previous step of this ray
-            else prev = bp;
-            if (has && !key_in_range(bp)) {
-                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_COORD_RANGE);
-                has = false;
+            ++i;
+            if (!eqi(b, prev)) {
+                prev = b;
+                bp = b;
+                has = true;
+                break;
             }
+        }
+        if (has && !key_in_range(bp)) {
+            atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_COORD_RANGE);
+            has = false;
         }
         const unsigned long long key = has ? pack_key(bp) : kEmptyKey;
         // wave de-duplication: one representative lane per distinct key

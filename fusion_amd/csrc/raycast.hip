@@ -132,24 +132,40 @@ struct Sampler {
         return find(pt, valid, w);
     }
     // Rendering::readSdfInterped (RenderScene.cu:297-319): fractions from floor, lookups by
-    // truncation; `valid` ends as the last lookup left it
+    // truncation; `valid` ends as the last lookup left it.  The eight corner voxels are
+    // computed exactly as the reference computes them (make_int3(pt + offset)); their blocks
+    // are resolved first (through the cache, usually one probe) and the eight sdf loads are
+    // then issued together.
     __device__ __forceinline__ float read_interp(f3 pt, bool& valid) {
         const f3 f = sub(pt, floor3(pt));
-        float s0, s1, r0, r1, r2, r3;
-        s0 = sdf_at(pt, valid);
-        s1 = sdf_at(add(pt, mk3(1, 0, 0)), valid);
-        r0 = (1.0f - f.x) * s0 + f.x * s1;
-        s0 = sdf_at(add(pt, mk3(0, 1, 0)), valid);
-        s1 = sdf_at(add(pt, mk3(1, 1, 0)), valid);
-        r1 = (1.0f - f.x) * s0 + f.x * s1;
-        r2 = (1.0f - f.y) * r0 + f.y * r1;
-        s0 = sdf_at(add(pt, mk3(0, 0, 1)), valid);
-        s1 = sdf_at(add(pt, mk3(1, 0, 1)), valid);
-        r0 = (1.0f - f.x) * s0 + f.x * s1;
-        s0 = sdf_at(add(pt, mk3(0, 1, 1)), valid);
-        s1 = sdf_at(add(pt, mk3(1, 1, 1)), valid);
-        r1 = (1.0f - f.x) * s0 + f.x * s1;
-        r3 = (1.0f - f.y) * r0 + f.y * r1;
+        i3 c[8];
+        int hp[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            c[k] = f2i3(add(pt, mk3((float)(k & 1), (float)((k >> 1) & 1), (float)((k >> 2) & 1))));
+            const i3 bp = voxel_to_block(c[k]);
+            if (cheap >= 0 && eqi(bp, cbp)) {
+                hp[k] = cheap;
+            } else {
+                hp[k] = find_block(m, bp);
+                if (hp[k] >= 0) {
+                    cbp = bp;
+                    cheap = hp[k];
+                }
+            }
+        }
+        float sv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            sv[k] = hp[k] >= 0 ? IO::sdf(IO::block(m, hp[k]), voxel_local_idx(c[k])) : fnan();
+        valid = hp[7] >= 0;
+        // corner k = (x, y, z) bits; same blend order as the reference
+        const float r0 = (1.0f - f.x) * sv[0] + f.x * sv[1];
+        const float r1 = (1.0f - f.x) * sv[2] + f.x * sv[3];
+        const float r2 = (1.0f - f.y) * r0 + f.y * r1;
+        const float r0b = (1.0f - f.x) * sv[4] + f.x * sv[5];
+        const float r1b = (1.0f - f.x) * sv[6] + f.x * sv[7];
+        const float r3 = (1.0f - f.y) * r0b + f.y * r1b;
         return (1.0f - f.z) * r2 + f.z * r3;
     }
 };

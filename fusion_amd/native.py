@@ -65,6 +65,13 @@ class SystemBuffers(C.Structure):
                 ("zrange_max", C.c_void_p), ("zcols", C.c_int32), ("zrows", C.c_int32)]
 
 
+class StageTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("frames", C.c_uint32), ("n_visible", C.c_double), ("n_new", C.c_double),
+                ("icp_iterations", C.c_double), ("icp_pixels", C.c_double)]
+
+
+STAGES = ["resize", "pyramid", "icp", "alloc", "visible", "integrate", "zrange", "raycast"]
+
 P = C.c_void_p
 I = C.c_int
 U = C.c_uint32
@@ -114,6 +121,8 @@ SIGNATURES = {
     "fu_system_grab_image": (I, [P, P]),
     "fu_system_grab_image_from": (I, [P, P, P, P]),
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
+    "fu_system_set_profiling": (I, [P, I]),
+    "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
 }
 
 SCENE_SIGNATURES = {

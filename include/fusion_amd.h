@@ -262,6 +262,31 @@ fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const
 /* outcome of the last frame; synchronises */
 fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
 
+/* Per-stage timing with HIP events on the frame's stream (the reference defines a StopWatch,
+ * Utility/StopWatch.h:8-28, but never calls it).  While enabled, every grab launches its
+ * kernels directly (no graph), records an event at each stage boundary, synchronises at the
+ * end of the frame and accumulates the stage times and the frame's map counters; the first
+ * frame of a sequence is not accumulated. */
+#define FU_STAGE_RESIZE 0    /* ResizeMap of the model maps */
+#define FU_STAGE_PYRAMID 1   /* FilterDepth, PyrDownGauss, ComputeVMap/NMap */
+#define FU_STAGE_ICP 2       /* all ICP iterations */
+#define FU_STAGE_ALLOC 3     /* block allocation (both passes) */
+#define FU_STAGE_VISIBLE 4   /* visible-block list */
+#define FU_STAGE_INTEGRATE 5 /* TSDF integration */
+#define FU_STAGE_ZRANGE 6    /* z-range tiles */
+#define FU_STAGE_RAYCAST 7   /* ray march */
+#define FU_NUM_STAGES 8
+typedef struct fu_stage_times {
+    double ms[FU_NUM_STAGES]; /* summed milliseconds per stage */
+    uint32_t frames;          /* frames accumulated */
+    double n_visible;         /* summed visible blocks */
+    double n_new;             /* summed newly allocated blocks */
+    double icp_iterations;    /* summed ICP iterations run */
+    double icp_pixels;        /* summed pixel-iterations of ICP */
+} fu_stage_times;
+fu_status fu_system_set_profiling(fu_system* sys, int enable);
+fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* out);
+
 #ifdef __cplusplus
 }
 #endif

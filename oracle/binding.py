@@ -47,6 +47,7 @@ _SIG = {
     "or_map_read_blocks": (None, [P, P, U, P, P, P]),
     "or_map_read_blocks_f64": (None, [P, P, U, P]),
     "or_hash": (U, [C.c_int32, C.c_int32, C.c_int32, U]),
+    "or_map_set_sphere": (None, [P, F, F, F, F]),
     "or_system_create": (P, [I, I, F, F, F, F, F, F, C.POINTER(I), C.POINTER(I), F, U, U, U, F, F, U, I]),
     "or_system_destroy": (None, [P]),
     "or_system_grab": (I, [P, P, P, DP]),
@@ -281,6 +282,9 @@ class Map:
         meta = np.zeros((k, 4), np.int32)
         lib().or_mesh(self.h, _ptr(vert), _ptr(norm), _ptr(col), _ptr(meta) if with_meta else None, k)
         return vert, norm, col, meta, total
+
+    def set_sphere(self, c, r):
+        lib().or_map_set_sphere(self.h, c[0], c[1], c[2], r)
 
     def export_blocks(self):
         n = lib().or_map_export_blocks(self.h, None, 0)

@@ -1137,6 +1137,31 @@ extern "C" uint32_t or_mesh(or_map* m, float* vertex, float* normal, uint8_t* co
     return tri;
 }
 
+// test hook: allocate the blocks within `trunc` of a sphere (through CreateBlock) and store its
+// exact truncated signed distance, weight 1, colour (100,150,200)
+extern "C" void or_map_set_sphere(or_map* m, float cx, float cy, float cz, float r) {
+    const double v = m->voxel, T = m->trunc;
+    const int lo[3] = {(int)floor((cx - r - T) / v) - 1, (int)floor((cy - r - T) / v) - 1, (int)floor((cz - r - T) / v) - 1};
+    const int hi[3] = {(int)ceil((cx + r + T) / v) + 1, (int)ceil((cy + r + T) / v) + 1, (int)ceil((cz + r + T) / v) + 1};
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                const double d = sqrt((x * v - cx) * (x * v - cx) + (y * v - cy) * (y * v - cy) + (z * v - cz) * (z * v - cz)) - r;
+                if (fabs(d) > T) continue;
+                const I3 p{x, y, z};
+                create_block(m, voxel_to_block(p));
+                const int ptr = find_ptr(m, voxel_to_block(p));
+                if (ptr < 0) continue;
+                const size_t vi = (size_t)ptr + voxel_local_idx(p);
+                m->sdf[vi] = (float)(d / T);
+                if (m->precision) m->sdf64[vi] = d / T;
+                m->weight[vi] = 1;
+                m->rgb[vi * 3] = 100;
+                m->rgb[vi * 3 + 1] = 150;
+                m->rgb[vi * 3 + 2] = 200;
+            }
+}
+
 extern "C" uint32_t or_map_export_blocks(const or_map* m, int32_t* xyz, uint32_t cap) {
     uint32_t k = 0;
     for (const OrEntry& e : m->entries)

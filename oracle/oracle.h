@@ -97,6 +97,8 @@ uint32_t or_map_export_visible(const or_map* m, int32_t* xyz, uint32_t cap);
 /* sdf as float (FP64 maps: rounded), weight, rgb of the listed blocks; absent -> NaN/0/0 */
 void or_map_read_blocks(const or_map* m, const int32_t* xyz, uint32_t n, float* sdf, uint16_t* weight, uint8_t* rgb);
 void or_map_read_blocks_f64(const or_map* m, const int32_t* xyz, uint32_t n, double* sdf);
+/* test hook: the blocks within trunc of a sphere, holding its exact truncated SDF */
+void or_map_set_sphere(or_map* m, float cx, float cy, float cz, float r);
 /* the bucket index of a block (DeviceMap::Hash) */
 uint32_t or_hash(int32_t x, int32_t y, int32_t z, uint32_t num_buckets);
 
