@@ -36,8 +36,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300, help="timed frames")
     ap.add_argument("--warmup", type=int, default=20, help="untimed frames before the timed region")
-    ap.add_argument("--plain-steps", type=int, default=100, help="frames timed in the plain (reference-style) mode")
-    ap.add_argument("--profile-steps", type=int, default=40, help="frames of the per-stage event pass")
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="frames timed in the plain (reference-style) mode (0: the same as --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -65,24 +65,21 @@ def run_gpu(args):
     import torch
     from fusion_amd import api as A
 
-    dev_frames = []
     scene = A.Scene(1, (12.0, 8.0, 3.0), args.seed)
-    total = args.warmup + args.steps + args.profile_steps
+    total = args.warmup + args.steps
     t0 = time.time()
     host = render_frames(scene, 0, total)
     gen_s = time.time() - t0
-    for d16, rgb in host:
-        dev_frames.append(A.to_device(d16, rgb))
+    dev_frames = [A.to_device(d16, rgb) for d16, rgb in host]
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
 
-    # ---- served path: device-resident control, one hipGraph per frame
+    # ---- served path: device-resident control, the frame replayed as one hipGraph
     sysg = A.System(A.system_params(use_graph=True, map=MAP5))
     for i in range(args.warmup):
         sysg.GrabImage(*dev_frames[i])
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    stream = torch.cuda.current_stream()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev[0].record(stream)
     for i in range(args.steps):
@@ -92,16 +89,22 @@ def run_gpu(args):
     wall = time.perf_counter() - t0
     lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
     res = sysg.result()
-
-    # ---- per-stage event pass on the continuing sequence (eager launches, same kernels)
-    sysg.set_profiling(True)
-    for i in range(args.profile_steps):
-        sysg.GrabImage(*dev_frames[args.warmup + args.steps + i])
-    st = sysg.stage_times()
-    sysg.set_profiling(False)
     stats = sysg.mapping().stats()
     sysg.close()
     del sysg
+    torch.cuda.synchronize()
+
+    # ---- the same frames again with HIP events at every stage boundary (kernels launched
+    # directly, same kernels and launch shapes as the graph); gives per-kernel durations
+    sysq = A.System(A.system_params(use_graph=False, map=MAP5))
+    for i in range(args.warmup):
+        sysq.GrabImage(*dev_frames[i])
+    sysq.set_profiling(True)
+    for i in range(args.steps):
+        sysq.GrabImage(*dev_frames[args.warmup + i])
+    st = sysq.stage_times()
+    sysq.close()
+    del sysq
     torch.cuda.synchronize()
 
     # ---- plain path: reference-style control (host solve, synchronise per stage)
@@ -109,7 +112,7 @@ def run_gpu(args):
     for i in range(args.warmup):
         sysp.GrabImage(*dev_frames[i])
     torch.cuda.synchronize()
-    nplain = min(args.plain_steps, args.steps)
+    nplain = min(args.plain_steps, args.steps) if args.plain_steps > 0 else args.steps
     lat_p = []
     t0 = time.perf_counter()
     for i in range(nplain):

@@ -103,6 +103,13 @@ extern "C" void fu_map_params_default(fu_map_params* p) {
     p->max_new_blocks = 1u << 18;
 }
 
+// occupancy bitmap: about 8 bits per pool block, 2^20..2^25 bits
+static unsigned occ_bits(const fu_map_params* p) {
+    unsigned n = 1u << 20;
+    while (n < (1u << 25) && (unsigned long long)n < 8ull * p->num_blocks) n <<= 1;
+    return n;
+}
+
 static unsigned newset_size(const fu_map_params* p) {
     unsigned n = 1024;
     while (n < 2u * p->max_new_blocks) n <<= 1;
@@ -113,7 +120,8 @@ extern "C" size_t fu_map_bytes(const fu_map_params* p) {
     const size_t ne = (size_t)p->num_buckets + p->num_excess;
     const size_t chunks = (p->num_blocks + 2047) / 2048;
     return ne * sizeof(HEntry) + 64 + (size_t)p->num_blocks * (8 + 4 + 4) + newset_size(p) * 8ull +
-           (size_t)p->max_new_blocks * 12 + chunks * 4 + (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format);
+           (size_t)p->max_new_blocks * 12 + chunks * 4 + occ_bits(p) / 8 +
+           (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format);
 }
 
 static fu_status check_params(const fu_map_params* p) {
@@ -166,6 +174,7 @@ extern "C" fu_status fu_map_create(const fu_map_params* p, fu_map** out) {
         (st = dalloc(m, &v.visible, p->num_blocks)) || (st = dalloc(m, &v.newset, ns)) ||
         (st = dalloc(m, &v.newkeys, p->max_new_blocks)) || (st = dalloc(m, &v.newslot, p->max_new_blocks)) ||
         (st = dalloc(m, &m->mesh_counts, p->num_blocks)) || (st = dalloc(m, &m->mesh_chunk, chunks)) ||
+        (st = dalloc(m, &v.occ, occ_bits(p) / 32)) ||
         (st = dalloc(m, &v.pool, (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format)))) {
         fu_map_destroy(m);
         return st;
@@ -174,6 +183,9 @@ extern "C" fu_status fu_map_create(const fu_map_params* p, fu_map** out) {
     v.num_excess = p->num_excess;
     v.num_blocks = p->num_blocks;
     v.newset_mask = ns - 1;
+    v.occ_mask = occ_bits(p) - 1;
+    v.occ_shift = 32u - (unsigned)__builtin_ctz(occ_bits(p));
+    v.bucket_mask = (p->num_buckets & (p->num_buckets - 1)) == 0 ? p->num_buckets - 1 : 0u;
     v.max_new = p->max_new_blocks;
     v.max_weight = p->max_weight;
     v.fmt = (int)p->voxel_format;
@@ -501,16 +513,19 @@ extern "C" fu_status fu_map_read_blocks(fu_map* m, const int32_t* xyz, uint32_t 
                     memcpy(&h, &buf[i * 2], 2);
                     sv = __half2float(*reinterpret_cast<__half*>(&h));
                 }
-                if (wb == 2) {
+                if (wb == 2) {  // sdf | u16 weight | r | g | b planes
                     uint16_t w16;
                     memcpy(&w16, &buf[512 * sb + i * 2], 2);
                     wv = w16;
-                } else {
-                    wv = buf[512 * sb + i];
+                    r = buf[512 * (sb + wb) + i];
+                    g = buf[512 * (sb + wb) + 512 + i];
+                    bl = buf[512 * (sb + wb) + 1024 + i];
+                } else {  // sdf | u32 (weight | r << 8 | g << 16 | b << 24) plane
+                    wv = buf[512 * sb + 4 * i];
+                    r = buf[512 * sb + 4 * i + 1];
+                    g = buf[512 * sb + 4 * i + 2];
+                    bl = buf[512 * sb + 4 * i + 3];
                 }
-                r = buf[512 * (sb + wb) + i];
-                g = buf[512 * (sb + wb) + 512 + i];
-                bl = buf[512 * (sb + wb) + 1024 + i];
             }
             if (sdf) sdf[(size_t)b * 512 + i] = sv;
             if (weight) weight[(size_t)b * 512 + i] = (uint16_t)wv;
@@ -587,7 +602,6 @@ struct fu_tracker {
     int cols, rows;
     TrackState* st = nullptr;
     float* partials = nullptr;
-    unsigned* done = nullptr;
     int nblocks[3];
 };
 
@@ -620,19 +634,16 @@ extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
     t->cols = cols;
     t->rows = rows;
     for (int l = 0; l < 3; ++l) t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
-    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 32 * 4) != hipSuccess ||
-        hipMalloc(&t->done, 64) != hipSuccess) {
+    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 32 * 4) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(t->st);
         (void)hipFree(t->partials);
-        (void)hipFree(t->done);
         delete t;
         return set_err(FU_ERR_NOMEM, "tracker allocation failed");
     }
     TrackState h;
     init_state(&h);
     HIPCHK(hipMemcpy(t->st, &h, sizeof(h), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(t->done, 0, 64));
     *out = t;
     return FU_OK;
 }
@@ -641,7 +652,6 @@ extern "C" fu_status fu_tracker_destroy(fu_tracker* t) {
     if (!t) return FU_OK;
     (void)hipFree(t->st);
     (void)hipFree(t->partials);
-    (void)hipFree(t->done);
     delete t;
     return FU_OK;
 }
@@ -674,7 +684,6 @@ static IcpArgs make_icp(fu_tracker* t, int level, const float* nv, const float* 
     a.cy = K[3] / sc;
     a.st = t->st;
     a.partials = t->partials;
-    a.done = t->done;
     a.mode = mode;
     a.min_count = min_count;
     a.final_iter = final_iter;
@@ -768,7 +777,6 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
     a.cy = cy;
     a.st = scratch->st;
     a.partials = scratch->partials;
-    a.done = scratch->done;
     a.mode = 0;
     launch_icp(a, icp_blocks(cols * rows), S(s));
     LAUNCHCHK();

@@ -41,8 +41,12 @@ enum Flag : unsigned {
     FLAG_TRI_FULL = 32u,
 };
 
-// Voxel storage formats.  Each block of 8^3 voxels is stored as planes:
-//   sdf[512] (f32 or f16) | weight[512] (u8 or u16) | r[512] | g[512] | b[512]
+// Voxel storage formats.  Each block of 8^3 voxels is stored as planes of 512 values in
+// voxel order z*64 + y*8 + x:
+//   u8 weights:  sdf[512] (f32 or f16) | wrgb[512] (u32: weight | r << 8 | g << 16 | b << 24)
+//   u16 weights: sdf[512] (f32)        | weight[512] (u16) | r[512] | g[512] | b[512]
+// so a wave touches 64 consecutive values of a plane, and an update is one sdf and one
+// weight+colour access in the u8 formats.
 enum VoxelFormat : int { VF_F32_W8 = 0, VF_F32_W16 = 1, VF_F16_W8 = 2 };
 
 FU_HD int vf_sdf_bytes(int f) { return f == VF_F16_W8 ? 2 : 4; }
@@ -58,6 +62,9 @@ struct MapView {
     unsigned long long* newkeys;  // [max_new]
     unsigned* newslot;          // [max_new]
     unsigned char* pool;        // voxel blocks
+    unsigned* occ;              // occupancy bitmap, one bit per hash of every allocated key
+    unsigned occ_mask, occ_shift;
+    unsigned bucket_mask;       // num_buckets - 1 when it is a power of two, else 0
     unsigned num_buckets, num_excess, num_blocks, newset_mask, max_new;
     unsigned max_weight;
     int fmt;
@@ -65,11 +72,16 @@ struct MapView {
 };
 
 // ---- voxel plane access -----------------------------------------------------------------
+struct VoxAux {
+    unsigned w;
+    unsigned char r, g, b;
+};
+
 template <int F>
 struct VoxelIO {
     static constexpr int kSdfB = F == VF_F16_W8 ? 2 : 4;
-    static constexpr int kWB = F == VF_F32_W16 ? 2 : 1;
-    static constexpr int kBlock = 512 * (kSdfB + kWB + 3);
+    static constexpr bool kW16 = F == VF_F32_W16;
+    static constexpr int kBlock = 512 * (kSdfB + (kW16 ? 2 : 1) + 3);
     __device__ static unsigned char* block(const MapView& m, int heap) {
         return m.pool + (size_t)heap * kBlock;
     }
@@ -85,20 +97,41 @@ struct VoxelIO {
         else
             reinterpret_cast<float*>(b)[i] = v;
     }
+    __device__ static VoxAux aux(const unsigned char* b, int i) {
+        VoxAux a;
+        if constexpr (kW16) {
+            const unsigned char* c = b + 512 * 6;
+            a.w = reinterpret_cast<const unsigned short*>(b + 512 * 4)[i];
+            a.r = c[i];
+            a.g = c[512 + i];
+            a.b = c[1024 + i];
+        } else {
+            const unsigned v = reinterpret_cast<const unsigned*>(b + 512 * kSdfB)[i];
+            a.w = v & 0xFFu;
+            a.r = (unsigned char)(v >> 8);
+            a.g = (unsigned char)(v >> 16);
+            a.b = (unsigned char)(v >> 24);
+        }
+        return a;
+    }
+    __device__ static void set_aux(unsigned char* b, int i, VoxAux a) {
+        if constexpr (kW16) {
+            unsigned char* c = b + 512 * 6;
+            reinterpret_cast<unsigned short*>(b + 512 * 4)[i] = (unsigned short)a.w;
+            c[i] = a.r;
+            c[512 + i] = a.g;
+            c[1024 + i] = a.b;
+        } else {
+            reinterpret_cast<unsigned*>(b + 512 * kSdfB)[i] =
+                (a.w & 0xFFu) | ((unsigned)a.r << 8) | ((unsigned)a.g << 16) | ((unsigned)a.b << 24);
+        }
+    }
     __device__ static unsigned weight(const unsigned char* b, int i) {
-        if constexpr (kWB == 2)
-            return reinterpret_cast<const unsigned short*>(b + 512 * kSdfB)[i];
+        if constexpr (kW16)
+            return reinterpret_cast<const unsigned short*>(b + 512 * 4)[i];
         else
-            return b[512 * kSdfB + i];
+            return reinterpret_cast<const unsigned*>(b + 512 * kSdfB)[i] & 0xFFu;
     }
-    __device__ static void set_weight(unsigned char* b, int i, unsigned w) {
-        if constexpr (kWB == 2)
-            reinterpret_cast<unsigned short*>(b + 512 * kSdfB)[i] = (unsigned short)w;
-        else
-            b[512 * kSdfB + i] = (unsigned char)w;
-    }
-    __device__ static unsigned char* rgb(unsigned char* b) { return b + 512 * (kSdfB + kWB); }
-    __device__ static const unsigned char* rgb(const unsigned char* b) { return b + 512 * (kSdfB + kWB); }
 };
 
 // ---- hash lookup (DeviceMap::FindEntry, DeviceMap.cu:122-135) ----------------------------
@@ -112,11 +145,32 @@ __device__ __forceinline__ HEntry load_entry(const HEntry* e) {
     return h;
 }
 
+FU_HD unsigned long long mix64(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// A clear occupancy bit proves the key is not allocated (bits are set for every allocated
+// key; collisions only cost a probe).  The bitmap is small enough to stay in L2, so rays
+// stepping through empty space skip the dependent hash-table load.  Bit index: the top bits
+// of a multiplicative scramble of the 32-bit block hash.
+FU_HD unsigned occ_bit(const MapView& m, uint32_t raw) { return (raw * 0x9E3779B1u) >> m.occ_shift; }
+__device__ __forceinline__ bool maybe_allocated(const MapView& m, uint32_t raw) {
+    const unsigned h = occ_bit(m, raw);
+    return (m.occ[h >> 5] >> (h & 31)) & 1u;
+}
+
 // heap index of block `bp`, or -1 if it is not allocated
 __device__ __forceinline__ int find_block(const MapView& m, i3 bp) {
     if (!key_in_range(bp)) return -1;
+    const uint32_t raw = block_hash_raw(bp);
+    if (!maybe_allocated(m, raw)) return -1;
     const unsigned long long key = pack_key(bp);
-    HEntry h = load_entry(&m.entries[block_hash(bp, m.num_buckets)]);
+    HEntry h = load_entry(&m.entries[bucket_of(raw, m.num_buckets, m.bucket_mask)]);
     while (true) {
         if (h.key == key && h.ptr >= 0) return h.ptr >> 9;
         if (h.next <= 0) return -1;

@@ -118,7 +118,6 @@ struct IcpArgs {
     float fx, fy, cx, cy;
     TrackState* st;
     float* partials;  // 32 floats per work-group
-    unsigned* done;   // zero between launches
     int mode;         // 0: sums only; 1: sums, solve and pose update
     int min_count;
     int final_iter;

@@ -124,29 +124,23 @@ FU_HD float fu_expf(float x) {
 
 // ---------------------------------------------------------------------------------------
 // Voxel-block coordinates (Mapping/DeviceMap.cu:137-201, block edge 8 voxels).
-FU_HD i3 voxel_to_block(i3 v) {
-    // DeviceMap.cu:150-161: shift negatives by 7 then truncating division by 8
-    if (v.x < 0) v.x -= 7;
-    if (v.y < 0) v.y -= 7;
-    if (v.z < 0) v.z -= 7;
-    return i3{v.x / 8, v.y / 8, v.z / 8};
-}
-FU_HD i3 voxel_to_local(i3 v) {
-    // DeviceMap.cu:167-178
-    i3 l{v.x % 8, v.y % 8, v.z % 8};
-    if (l.x < 0) l.x += 8;
-    if (l.y < 0) l.y += 8;
-    if (l.z < 0) l.z += 8;
-    return l;
-}
+// voxelPosToBlockPos (DeviceMap.cu:150-161) shifts negatives by 7 and divides by 8 with
+// truncation, i.e. floor division by 8: an arithmetic shift right by 3 on two's complement
+// ints.  voxelPosToLocalPos (:167-178) is the non-negative remainder, i.e. the low 3 bits.
+FU_HD i3 voxel_to_block(i3 v) { return i3{v.x >> 3, v.y >> 3, v.z >> 3}; }
+FU_HD i3 voxel_to_local(i3 v) { return i3{v.x & 7, v.y & 7, v.z & 7}; }
 FU_HD int local_idx(i3 l) { return l.z * 64 + l.y * 8 + l.x; }  // DeviceMap.cu:180-182
-FU_HD int voxel_local_idx(i3 v) { return local_idx(voxel_to_local(v)); }
+FU_HD int voxel_local_idx(i3 v) { return ((v.z & 7) << 6) | ((v.y & 7) << 3) | (v.x & 7); }
 
 // DeviceMap::Hash (DeviceMap.cu:3-10).  NumBuckets is an unsigned constant there, so the
 // xor of the wrapped int products is reduced modulo it as an unsigned value.
-FU_HD uint32_t block_hash(i3 p, uint32_t num_buckets) {
-    uint32_t h = ((uint32_t)p.x * 73856093u) ^ ((uint32_t)p.y * 19349669u) ^ ((uint32_t)p.z * 83492791u);
-    return h % num_buckets;
+FU_HD uint32_t block_hash_raw(i3 p) {
+    return ((uint32_t)p.x * 73856093u) ^ ((uint32_t)p.y * 19349669u) ^ ((uint32_t)p.z * 83492791u);
+}
+FU_HD uint32_t block_hash(i3 p, uint32_t num_buckets) { return block_hash_raw(p) % num_buckets; }
+// the same bucket with a mask when the bucket count is a power of two (mask != 0)
+FU_HD uint32_t bucket_of(uint32_t raw, uint32_t num_buckets, uint32_t mask) {
+    return mask ? (raw & mask) : (raw % num_buckets);
 }
 
 // Block coordinates packed into one 64-bit key, 21 bits per axis (two's complement), so a

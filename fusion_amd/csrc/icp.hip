@@ -6,11 +6,10 @@
 // (:319-375); Tracking/Tracking.cc Tracker::ComputeSE3 (:296-409).
 //
 // Each lane sums its pixels with Neumaier-compensated FP32 adds; lanes are combined by a
-// DPP pairwise tree (wave_sum), waves through LDS, and work-groups by the last group to
-// finish, which sums the group partials with the same compensated tree.  That group then
-// (mode 1) unpacks A (6x6) and b (6) in FP64, solves with an LDL^T factorisation with
-// symmetric pivoting, takes the SE(3) exponential and composes the pose, all in FP64 on
-// one lane: the only FP64 steps of the pipeline.
+// DPP pairwise tree (wave_sum) and waves through LDS into one partial row per work-group.
+// A one-group kernel then sums the rows with the same compensated tree and (mode 1) unpacks
+// A (6x6) and b (6) in FP64, solves by Cholesky, takes the SE(3) exponential and composes
+// the pose on one lane: the only FP64 steps of the pipeline.
 #include "fu_device.h"
 #include "fu_kernels.h"
 #include "fu_track.h"
@@ -93,9 +92,8 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
             acc[28].add(1.0f);
         }
     }
-    // wave tree, then waves through LDS
+    // wave tree, then waves through LDS; one 128-byte partial row per work-group
     __shared__ float wsum[4][29];
-    __shared__ int is_last;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < 29; ++j) {
@@ -107,19 +105,39 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
         const int j = threadIdx.x;
         a.partials[blockIdx.x * 32 + j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(a.done, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // last group: sum the group partials (compensated per lane, then the same tree)
+}
+
+// One work-group: sums the partial rows of k_icp (compensated per lane, then the same
+// tree), stores the 29 sums and, in mode 1, runs the FP64 update on one lane.  Running it as
+// its own kernel makes the partials visible through the kernel boundary; a last-group ticket
+// inside k_icp would need a device-scope release per work-group, which on MI355X writes back
+// the work-group's L2 (one per XCD) every time.
+__global__ __launch_bounds__(256) void k_icp_final(IcpArgs a, int nparts) {
+    TrackState* st = a.st;
+    if (a.mode != 0 && st->status != TS_RUNNING) return;
+    __shared__ float wsum[4][29];
     __shared__ float total[29];
+    const int wave = threadIdx.x >> 6;
+    Kahan fin[29];
+#pragma unroll
+    for (int j = 0; j < 29; ++j) fin[j] = Kahan{0.f, 0.f};
+    for (int g = threadIdx.x; g < nparts; g += blockDim.x) {
+        const float4* row = reinterpret_cast<const float4*>(a.partials + g * 32);
+        float v[32];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 t = row[q];
+            v[4 * q + 0] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 29; ++j) fin[j].add(v[j]);
+    }
+#pragma unroll
     for (int j = 0; j < 29; ++j) {
-        Kahan k{0.f, 0.f};
-        for (int g = threadIdx.x; g < (int)gridDim.x; g += blockDim.x)
-            k.add(__hip_atomic_load(&a.partials[g * 32 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const float t = wave_sum(k.val());
+        const float t = wave_sum(fin[j].val());
         if (lane_id() == 0) wsum[wave][j] = t;
     }
     __syncthreads();
@@ -129,7 +147,6 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        *a.done = 0u;
         for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
         if (a.mode == 1) icp_update(st, total, a.min_count, a.final_iter, a.thresh);
     }
@@ -137,6 +154,7 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
 
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_icp, dim3(nblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_icp_final, dim3(1), dim3(256), 0, s, a, nblocks);
 }
 
 // start of ComputeSE3 (Tracking.cc:305-314): last pose = tracked pose of the previous

@@ -44,6 +44,7 @@ void launch_reset_map(const MapView& m, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_entries, dim3(2048), dim3(256), 0, s, m.entries, ne);
     hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, m.block_key, kEmptyKey, m.num_blocks);
     hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, m.newset, kEmptyKey, m.newset_mask + 1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1024), dim3(256), 0, s, m.occ, (m.occ_mask + 1) / 32);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, m.ctr, (unsigned)C_COUNT);
 }
 
@@ -60,15 +61,6 @@ void launch_begin_pass(const MapView& m, int which, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ allocation pass A (per pixel)
-__device__ __forceinline__ unsigned long long mix64(unsigned long long k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ull;
-    k ^= k >> 33;
-    return k;
-}
-
 // claim `key` in the per-pass set; true if this lane is the one that inserted it
 __device__ __forceinline__ bool newset_claim(const MapView& m, unsigned long long key, unsigned& slot_out) {
     unsigned slot = (unsigned)mix64(key) & m.newset_mask;
@@ -189,7 +181,7 @@ __global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackStat
             } else {
                 heap = (int)h;
                 const i3 bp = unpack_key(key);
-                HEntry* e = &m.entries[block_hash(bp, m.num_buckets)];
+                HEntry* e = &m.entries[bucket_of(block_hash_raw(bp), m.num_buckets, m.bucket_mask)];
                 const unsigned long long old = atomicCAS(&e->key, kEmptyKey, key);
                 bool ok = true;
                 if (old == kEmptyKey) {
@@ -225,6 +217,10 @@ __global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackStat
                     }
                 }
                 m.block_key[heap] = ok ? key : kEmptyKey;
+                if (ok) {
+                    const unsigned hb = occ_bit(m, block_hash_raw(bp));
+                    atomicOr(&m.occ[hb >> 5], 1u << (hb & 31));
+                }
             }
         }
         heap = __shfl(heap, 0);
@@ -328,49 +324,78 @@ __global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
         const unsigned heap = m.visible[vb];
         const i3 bp = unpack_key(m.block_key[heap]);
         unsigned char* blk = IO::block(m, (int)heap);
-        unsigned char* rgbp = IO::rgb(blk);
+        // each lane owns voxels li and li + 256 (index z*64 + y*8 + x); the work is split in
+        // phases so that both voxels' gathers are in flight together
+        int pix[2];
+        bool inb[2];
+        float pz[2];
+        f3 pos[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int li = threadIdx.x + h * 256;  // voxel index z*64 + y*8 + x
-            const int lx = li & 7, ly = (li >> 3) & 7, lz = li >> 6;
-            const i3 vp = mki3(bp.x * 8 + lx, bp.y * 8 + ly, bp.z * 8 + lz);
-            f3 pos = i3f(vp, m.voxel_size);
-            pos = mv(pose.Rinv, sub(pos, pose.t));
-            const int u = f2i(a.fx * pos.x / pos.z + a.cx);
-            const int v = f2i(a.fy * pos.y / pos.z + a.cy);
-            if (u < 0 || v < 0 || u >= cols || v >= rows) continue;
-            const int pix = v * cols + u;
-            const float dp = a.depth[pix];
-            if (isnanf_(dp) || dp > m.depth_max || dp < m.depth_min) continue;
-            float sdf = dp - pos.z;
-            if (!(sdf >= -thresh)) continue;
-            sdf = fminf(1.0f, sdf / thresh);
-            const f4 nl = a.nmap[pix];
-            if (isnanf_(nl.x)) continue;
+            const int li = threadIdx.x + h * 256;
+            const i3 vp = mki3(bp.x * 8 + (li & 7), bp.y * 8 + ((li >> 3) & 7), bp.z * 8 + (li >> 6));
+            pos[h] = mv(pose.Rinv, sub(i3f(vp, m.voxel_size), pose.t));
+            const int u = f2i(a.fx * pos[h].x / pos[h].z + a.cx);
+            const int v = f2i(a.fy * pos[h].y / pos[h].z + a.cy);
+            inb[h] = !(u < 0 || v < 0 || u >= cols || v >= rows);
+            pix[h] = inb[h] ? v * cols + u : 0;
+            pz[h] = pos[h].z;
+        }
+        float dp[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) dp[h] = a.depth[pix[h]];
+        bool upd[2];
+        float sdf[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool dv = inb[h] && !(isnanf_(dp[h]) || dp[h] > m.depth_max || dp[h] < m.depth_min);
+            sdf[h] = dp[h] - pz[h];
+            upd[h] = dv && (sdf[h] >= -thresh);
+            if (!upd[h]) pix[h] = 0;
+        }
+        if (!(upd[0] || upd[1])) continue;
+        f4 nl[2];
+        unsigned char cr[2], cg[2], cb[2];
+        float sold[2];
+        VoxAux old[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int li = threadIdx.x + h * 256;
+            nl[h] = a.nmap[pix[h]];
+            cr[h] = a.rgb[pix[h] * 3 + 0];
+            cg[h] = a.rgb[pix[h] * 3 + 1];
+            cb[h] = a.rgb[pix[h] * 3 + 2];
+            sold[h] = IO::sdf(blk, li);
+            old[h] = IO::aux(blk, li);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!upd[h] || isnanf_(nl[h].x)) continue;
+            const int li = threadIdx.x + h * 256;
+            const float s = fminf(1.0f, sdf[h] / thresh);
             // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342)
-            const float w = dot4(nl, normalised4(mk4(pos.x, pos.y, pos.z, 1.0f)));
-            const unsigned char cr = a.rgb[pix * 3 + 0], cg = a.rgb[pix * 3 + 1], cb = a.rgb[pix * 3 + 2];
-            const unsigned wold = IO::weight(blk, li);
-            if (wold == 0) {
-                IO::set_sdf(blk, li, sdf);
-                IO::set_weight(blk, li, 1u);
-                rgbp[li] = cr;
-                rgbp[512 + li] = cg;
-                rgbp[1024 + li] = cb;
+            const float w = dot4(nl[h], normalised4(mk4(pos[h].x, pos[h].y, pos[h].z, 1.0f)));
+            VoxAux nv;
+            if (old[h].w == 0) {
+                IO::set_sdf(blk, li, s);
+                nv.w = 1u;
+                nv.r = cr[h];
+                nv.g = cg[h];
+                nv.b = cb[h];
             } else {
-                const float sold = IO::sdf(blk, li);
                 const float aw = w * 0.2f;
                 const float bw = 1.0f - w * 0.2f;
-                const float r = (aw * ((float)cr / 255.f) + bw * ((float)rgbp[li] / 255.f)) * 255.f;
-                const float g = (aw * ((float)cg / 255.f) + bw * ((float)rgbp[512 + li] / 255.f)) * 255.f;
-                const float b = (aw * ((float)cb / 255.f) + bw * ((float)rgbp[1024 + li] / 255.f)) * 255.f;
-                const float fw = (float)wold;
-                IO::set_sdf(blk, li, (sold * fw + w * sdf) / (fw + w));
-                IO::set_weight(blk, li, min(m.max_weight, wold + 1u));
-                rgbp[li] = f2u8(r);
-                rgbp[512 + li] = f2u8(g);
-                rgbp[1024 + li] = f2u8(b);
+                const float fr = (aw * ((float)cr[h] / 255.f) + bw * ((float)old[h].r / 255.f)) * 255.f;
+                const float fg = (aw * ((float)cg[h] / 255.f) + bw * ((float)old[h].g / 255.f)) * 255.f;
+                const float fb = (aw * ((float)cb[h] / 255.f) + bw * ((float)old[h].b / 255.f)) * 255.f;
+                const float fw = (float)old[h].w;
+                IO::set_sdf(blk, li, (sold[h] * fw + w * s) / (fw + w));
+                nv.w = min(m.max_weight, old[h].w + 1u);
+                nv.r = f2u8(fr);
+                nv.g = f2u8(fg);
+                nv.b = f2u8(fb);
             }
+            IO::set_aux(blk, li, nv);
         }
     }
 }

@@ -81,8 +81,8 @@ __device__ void stage_block(const MapView& m, i3 bp, MeshSmem& s, bool with_rgb)
             const int li = ((rz & 7) * 8 + (ry & 7)) * 8 + (rx & 7);
             sd = IO::sdf(blk, li);
             if (with_rgb) {
-                const unsigned char* cp = IO::rgb(blk);
-                c = (unsigned)cp[li] | ((unsigned)cp[512 + li] << 8) | ((unsigned)cp[1024 + li] << 16);
+                const VoxAux x = IO::aux(blk, li);
+                c = (unsigned)x.r | ((unsigned)x.g << 8) | ((unsigned)x.b << 16);
             }
         }
         s.sdf[i] = sd;

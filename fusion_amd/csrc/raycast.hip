@@ -154,10 +154,14 @@ struct Sampler {
                 }
             }
         }
+        // unconditional loads (an absent block reads pool block 0 and is masked to NaN), so
+        // the eight loads issue back to back
         float sv[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            sv[k] = hp[k] >= 0 ? IO::sdf(IO::block(m, hp[k]), voxel_local_idx(c[k])) : fnan();
+        for (int k = 0; k < 8; ++k) {
+            const float raw = IO::sdf(IO::block(m, hp[k] >= 0 ? hp[k] : 0), voxel_local_idx(c[k]));
+            sv[k] = hp[k] >= 0 ? raw : fnan();
+        }
         valid = hp[7] >= 0;
         // corner k = (x, y, z) bits; same blend order as the reference
         const float r0 = (1.0f - f.x) * sv[0] + f.x * sv[1];
