@@ -360,6 +360,48 @@ class System:
         return m
 
 
+class Batch:
+    """A batch context: one frame of each of many attached Systems per call, every stage one
+    launch for all of them (no reference counterpart; the reference runs one System)."""
+
+    MAX = 64
+
+    def __init__(self, capacity=256):
+        _dev()
+        h = C.c_void_p()
+        check(N.lib().fu_batch_create(int(capacity), C.byref(h)), "fu_batch_create")
+        self.handle = h
+        self.systems = []
+
+    def attach(self, system):
+        check(N.lib().fu_batch_attach(self.handle, system.handle), "fu_batch_attach")
+        self.systems.append(system)
+
+    def detach(self, system):
+        check(N.lib().fu_batch_detach(self.handle, system.handle), "fu_batch_detach")
+        self.systems.remove(system)
+
+    def close(self):
+        if self.handle:
+            N.lib().fu_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grab(self, systems, depths, rgbs):
+        """one frame per system: device depth (H, W) u16-in-int16 and rgb (H, W, 3) u8 tensors"""
+        n = len(systems)
+        arr = C.c_void_p * n
+        sy = arr(*[s.handle.value for s in systems])
+        dp = arr(*[d.data_ptr() for d in depths])
+        cp = arr(*[c.data_ptr() for c in rgbs])
+        check(N.lib().fu_batch_grab(self.handle, n, sy, dp, cp, _s()), "fu_batch_grab")
+
+
 class _DevBuf:
     """__cuda_array_interface__ over a device buffer owned by the library (zero-copy view)"""
 

@@ -13,6 +13,7 @@
 #include "fu_device.h"
 #include "fu_kernels.h"
 #include "fu_track.h"
+#include "fu_batch.h"
 #include "mc_tables.h"
 
 using namespace fu;
@@ -815,6 +816,8 @@ struct fu_system {
     fu_stage_times times{};
     const uint16_t* graph_depth = nullptr;
     const uint8_t* graph_rgb = nullptr;
+    fu_batch* batch = nullptr;  // batch context the system is attached to
+    int batch_slot = -1;
 };
 
 extern "C" void fu_system_params_default(fu_system_params* p) {
@@ -852,6 +855,7 @@ static fu_status salloc(fu_system* s, T** out, size_t count) {
 
 extern "C" fu_status fu_system_destroy(fu_system* sys) {
     if (!sys) return FU_OK;
+    if (sys->batch) fu_batch_detach(sys->batch, sys);
     if (sys->graph) (void)hipGraphExecDestroy(sys->graph);
     for (int k = 0; k <= FU_NUM_STAGES; ++k)
         if (sys->ev[k]) (void)hipEventDestroy(sys->ev[k]);
@@ -1194,5 +1198,208 @@ extern "C" fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_s
     out->n_fragments = c[C_RENDER];
     out->flags = c[C_FLAGS];
     out->frame_index = sys->frame;
+    return FU_OK;
+}
+
+// ------------------------------------------------------------------------------ batches
+namespace fu {
+int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, int fixed) {
+    int total = 0, uni = -1;
+    for (int i = 0; i < b.n; ++i) {
+        const SessDesc& d = h[b.sess[i]];
+        int gx = 1, cnt = 1;
+        switch (t) {
+            case BT_TILES:
+                gx = (d.lc[level] + 15) / 16;
+                cnt = gx * ((d.lr[level] + 15) / 16);
+                break;
+            case BT_PYR3:
+                cnt = 0;
+                for (int l = 0; l < 3; ++l) cnt += ((d.lc[l] + 15) / 16) * ((d.lr[l] + 15) / 16);
+                gx = cnt;
+                break;
+            case BT_ICP: gx = cnt = d.icp_blocks[level]; break;
+            case BT_ONE: gx = cnt = 1; break;
+            case BT_FIXED: gx = cnt = fixed; break;
+            case BT_ZTILES: gx = cnt = (d.zr.zcols * d.zr.zrows + 255) / 256; break;
+        }
+        b.pre[i] = total;
+        b.gx[i] = (unsigned short)gx;
+        total += cnt;
+        uni = i == 0 ? cnt : (uni == cnt ? uni : 0);
+    }
+    b.pre[b.n] = total;
+    b.uniform = uni > 0 ? uni : 0;
+    return total;
+}
+}  // namespace fu
+
+struct fu_batch {
+    int cap = 0;
+    int fmt = -1;
+    SessDesc* d_desc = nullptr;
+    std::vector<SessDesc> h_desc;
+    std::vector<fu_system*> owner;
+    int attached = 0;
+};
+
+extern "C" fu_status fu_batch_create(int capacity, fu_batch** out) {
+    if (!out) return set_err(FU_ERR_ARG, "null output");
+    *out = nullptr;
+    if (capacity < 1 || capacity > 65535) return set_err(FU_ERR_ARG, "batch capacity %d out of range", capacity);
+    fu_batch* bt = new fu_batch();
+    bt->cap = capacity;
+    bt->h_desc.resize(capacity);
+    bt->owner.assign(capacity, nullptr);
+    if (hipMalloc(&bt->d_desc, sizeof(SessDesc) * capacity) != hipSuccess) {
+        (void)hipGetLastError();
+        delete bt;
+        return set_err(FU_ERR_NOMEM, "batch descriptor allocation failed");
+    }
+    *out = bt;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_destroy(fu_batch* bt) {
+    if (!bt) return FU_OK;
+    for (fu_system* sys : bt->owner)
+        if (sys) {
+            sys->batch = nullptr;
+            sys->batch_slot = -1;
+        }
+    (void)hipFree(bt->d_desc);
+    delete bt;
+    return FU_OK;
+}
+
+static void fill_desc(fu_system* sys, SessDesc& d) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    fu_map* m = sys->map;
+    fu_tracker* t = sys->trk;
+    memset(&d, 0, sizeof(d));
+    d.m = m->v;
+    d.st = t->st;
+    d.partials = t->partials;
+    d.cols = p.cols;
+    d.rows = p.rows;
+    d.depth_scale_inv = (float)(1.0 / (double)p.depth_scale);
+    d.cutoff = p.depth_cutoff;
+    d.raw = b.raw;
+    const float K[4] = {p.fx, p.fy, p.cx, p.cy};
+    d.last_level = -1;
+    for (int l = 2; l >= 0; --l)
+        if (p.iters[l] > 0) d.last_level = l;
+    for (int l = 0; l < 3; ++l) {
+        d.lc[l] = sys->lc[l];
+        d.lr[l] = sys->lr[l];
+        const float sc = (float)(1 << l);
+        d.Kl[l][0] = p.fx / sc;
+        d.Kl[l][1] = p.fy / sc;
+        d.Kl[l][2] = p.cx / sc;
+        d.Kl[l][3] = p.cy / sc;
+        d.invf[l][0] = (float)(1.0 / (double)d.Kl[l][0]);
+        d.invf[l][1] = (float)(1.0 / (double)d.Kl[l][1]);
+        d.depth[l] = b.depth[l];
+        d.vmap[l] = (f4*)b.vmap[l];
+        d.nmap[l] = (f4*)b.nmap[l];
+        d.model_v[l] = (f4*)b.model_vmap[l];
+        d.model_n[l] = (f4*)b.model_nmap[l];
+        d.icp[l] = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 1, p.min_count[l], 0,
+                            p.icp_thresh);
+        d.icp_blocks[l] = t->nblocks[l];
+        d.iters[l] = p.iters[l];
+    }
+    PoseF dummy{};
+    d.fuse = make_fuse(m, b.raw, nullptr, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
+    d.vis = make_vis(m, dummy, t->st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
+    d.zr = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min, dummy, t->st, -1,
+                   p.fx, p.fy, p.cx, p.cy);
+    d.rc = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max, b.zcols, b.zrows,
+                   dummy, t->st, d.invf[0][0], d.invf[0][1], p.cx, p.cy);
+    d.rc.keep_v = (const f4*)b.vmap[0];
+    d.rc.keep_n = (const f4*)b.nmap[0];
+}
+
+extern "C" fu_status fu_batch_attach(fu_batch* bt, fu_system* sys) {
+    if (!bt || !sys) return set_err(FU_ERR_ARG, "null argument");
+    if (sys->batch) return set_err(FU_ERR_STATE, "system is already attached to a batch context");
+    if (bt->fmt >= 0 && bt->fmt != sys->p.map.voxel_format)
+        return set_err(FU_ERR_ARG, "voxel format %d differs from the batch context's %d", sys->p.map.voxel_format, bt->fmt);
+    int tiles = 0;
+    for (int l = 0; l < 3; ++l) tiles += ((sys->lc[l] + 15) / 16) * ((sys->lr[l] + 15) / 16);
+    if (tiles > 65535) return set_err(FU_ERR_ARG, "frame %dx%d too large for batched launches", sys->p.cols, sys->p.rows);
+    int slot = -1;
+    for (int i = 0; i < bt->cap && slot < 0; ++i)
+        if (!bt->owner[i]) slot = i;
+    if (slot < 0) return set_err(FU_ERR_STATE, "batch context full (%d systems)", bt->cap);
+    fill_desc(sys, bt->h_desc[slot]);
+    HIPCHK(hipMemcpy(bt->d_desc + slot, &bt->h_desc[slot], sizeof(SessDesc), hipMemcpyHostToDevice));
+    bt->owner[slot] = sys;
+    bt->fmt = sys->p.map.voxel_format;
+    bt->attached++;
+    sys->batch = bt;
+    sys->batch_slot = slot;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
+    if (!bt || !sys) return set_err(FU_ERR_ARG, "null argument");
+    if (sys->batch != bt) return set_err(FU_ERR_STATE, "system is not attached to this batch context");
+    bt->owner[sys->batch_slot] = nullptr;
+    if (--bt->attached == 0) bt->fmt = -1;
+    sys->batch = nullptr;
+    sys->batch_slot = -1;
+    return FU_OK;
+}
+
+// One frame of each of `n` systems as one sequence of batched launches: the same stages as
+// queue_frame_device, each launched once for all sessions.
+extern "C" fu_status fu_batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
+                                   const uint8_t* const* rgb, fu_stream s) {
+    if (!bt || !sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_batch_grab: null argument");
+    if (n < 1 || n > kMaxBatch) return set_err(FU_ERR_ARG, "fu_batch_grab: %d frames (1..%d)", n, kMaxBatch);
+    BatchLaunch b;
+    memset(&b, 0, sizeof(b));
+    b.desc = bt->d_desc;
+    b.n = n;
+    int maxit[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        fu_system* x = sys[i];
+        if (!x || x->batch != bt) return set_err(FU_ERR_STATE, "fu_batch_grab: system %d not attached to this context", i);
+        if (!depth[i] || !rgb[i]) return set_err(FU_ERR_ARG, "fu_batch_grab: null frame %d", i);
+        for (int j = 0; j < i; ++j)
+            if (sys[j] == x) return set_err(FU_ERR_ARG, "fu_batch_grab: system %d appears twice", i);
+        b.sess[i] = (unsigned short)x->batch_slot;
+        if (x->frame == 0) {
+            b.first |= 1ull << i;
+        } else {
+            for (int l = 0; l < 3; ++l) maxit[l] = std::max(maxit[l], x->p.iters[l]);
+        }
+    }
+    const bool track = b.first != (n == 64 ? ~0ull : ((1ull << n) - 1));
+    const SessDesc* h = bt->h_desc.data();
+    hipStream_t hs = S(s);
+    for (int i = 0; i < n; ++i) b.in[i] = depth[i];
+    if (track) {
+        launch_b_resize(b, h, 1, hs);
+        launch_b_resize(b, h, 2, hs);
+    }
+    launch_b_filter(b, h, hs);
+    launch_b_pyrdown(b, h, 1, hs);
+    launch_b_pyrdown(b, h, 2, hs);
+    launch_b_vmap(b, h, hs);
+    launch_b_nmap(b, h, hs);
+    if (track) {
+        launch_b_track_begin(b, h, hs);
+        for (int l = 2; l >= 0; --l)
+            for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
+    }
+    launch_b_track_commit(b, h, hs);
+    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
+    launch_b_fuse(b, h, bt->fmt, hs);
+    launch_b_raycast(b, h, bt->fmt, hs);
+    LAUNCHCHK();
+    for (int i = 0; i < n; ++i) sys[i]->frame++;
     return FU_OK;
 }

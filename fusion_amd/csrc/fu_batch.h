@@ -1,0 +1,134 @@
+// Batched launches: one launch runs the same stage for the frames of many sessions.
+//
+// Every kernel body takes its work-group coordinates as a WG value instead of reading
+// blockIdx/gridDim.  A single-session launch passes the hardware ids; a batched launch splits
+// its grid into consecutive shares, one per session of the step, and passes the ids within
+// the share.  A session therefore sees exactly the work-group decomposition it would get
+// alone, so its results (including the ICP partial sums and their order) are bit-identical
+// alone and in a batch.
+#pragma once
+
+#include "fu_device.h"
+#include "fu_kernels.h"
+
+namespace fu {
+
+constexpr int kMaxBatch = 64;
+
+struct WG {
+    int x, y, nx, ny;
+};
+__device__ __forceinline__ WG hw_wg() { return WG{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y}; }
+
+// Everything a session's kernels need, resident in device memory (written when the session
+// joins a batch context and read with scalar loads by every batched kernel).
+struct SessDesc {
+    MapView m;
+    TrackState* st;
+    float* partials;
+    int cols, rows;  // level 0
+    int lc[3], lr[3];
+    float depth_scale_inv, cutoff;
+    float Kl[3][4];    // fx, fy, cx, cy per level (Frame::SetK, K / 2^l)
+    float invf[3][2];  // (float)(1.0 / (double)fx_l), same for fy
+    float* raw;
+    float* depth[3];
+    f4* vmap[3];
+    f4* nmap[3];
+    f4* model_v[3];
+    f4* model_n[3];
+    IcpArgs icp[3];
+    int icp_blocks[3];
+    int iters[3];
+    int last_level;
+    FuseArgs fuse;
+    VisArgs vis;
+    ZRangeArgs zr;
+    RaycastArgs rc;
+};
+
+// Kernel argument of a batched launch (passed by value; about 1.1 KB).
+struct BatchLaunch {
+    const SessDesc* desc;
+    int n;                    // sessions in this step
+    int level;                // pyramid / ICP level of the launch
+    int iter;                 // ICP iteration of the launch
+    int uniform;              // > 0: every share has this many work-groups
+    unsigned long long first;  // bit i: slot i is its session's first frame
+    unsigned short sess[kMaxBatch];  // slot -> session descriptor index
+    unsigned short gx[kMaxBatch];    // work-groups in x of the slot's share (2-D tilings)
+    int pre[kMaxBatch + 1];          // prefix of the shares' work-group counts
+    const void* in[kMaxBatch];       // per-slot input of the step (depth or colour)
+};
+
+// slot of hardware work-group `wg` and the work-group ids within its share
+__device__ __forceinline__ int batch_slot(const BatchLaunch& b, int wg, WG& g) {
+    int s;
+    if (b.uniform > 0) {
+        s = wg / b.uniform;
+    } else {
+        int lo = 0, hi = b.n;  // pre[lo] <= wg < pre[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (b.pre[mid] <= wg)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        s = lo;
+    }
+    const int local = wg - b.pre[s];
+    const int gx = b.gx[s];
+    const int cnt = b.pre[s + 1] - b.pre[s];
+    g.x = local % gx;
+    g.y = local / gx;
+    g.nx = gx;
+    g.ny = cnt / gx;
+    return s;
+}
+
+// Copy of a descriptor part, read through the constant address space: the descriptors do not
+// change while a kernel runs, so the loads are scalar and the values live in SGPRs.
+template <class T>
+__device__ __forceinline__ T cload(const T& src) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized");
+    T v;
+    typedef const __attribute__((address_space(4))) unsigned CU;
+    CU* q = (CU*)(const unsigned*)&src;
+    unsigned* d = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) d[i] = q[i];
+    return v;
+}
+
+__device__ __forceinline__ bool slot_first(const BatchLaunch& b, int s) { return (b.first >> s) & 1ull; }
+
+// ---- batched launchers (host) ------------------------------------------------------------
+// `h` is the host mirror of the descriptor array; the launchers size each share from it.
+enum BatchTiling : int {
+    BT_TILES = 0,   // 16x16 tiles of the level-`level` image
+    BT_PYR3 = 1,    // 16x16 tiles of all three levels, level 0 first
+    BT_ICP = 2,     // icp_blocks[level] work-groups
+    BT_ONE = 3,     // one work-group per session
+    BT_FIXED = 4,   // a fixed number of work-groups per session (grid-stride bodies)
+    BT_ZTILES = 5,  // 256-thread groups over the z-range tiles
+};
+// fills b.pre, b.gx and b.uniform for one launch; returns the total work-group count
+int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, int fixed);
+
+// pyramid.hip
+void launch_b_resize(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
+void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
+void launch_b_vmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+// icp.hip
+void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStream_t s);
+void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+// map.hip
+void launch_b_fuse(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
+// raycast.hip
+void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
+
+}  // namespace fu

@@ -13,6 +13,7 @@
 #include "fu_device.h"
 #include "fu_kernels.h"
 #include "fu_track.h"
+#include "fu_batch.h"
 
 namespace fu {
 
@@ -34,7 +35,7 @@ __device__ __forceinline__ m3 load_m3(const float* r) {
     return m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
 }
 
-__global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
+__device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
     TrackState* st = a.st;
     if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
     const m3 Rc = load_m3(st->Rcurr), Rl = load_m3(st->Rlast), Rli = load_m3(st->RlastInv);
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
     Kahan acc[29];
 #pragma unroll
     for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = g.x * blockDim.x + threadIdx.x; i < N; i += g.nx * blockDim.x) {
         const int y = i / cols, x = i - y * cols;
         float row[7] = {0, 0, 0, 0, 0, 0, 0};
         bool found = false;
@@ -103,16 +104,17 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) {
     __syncthreads();
     if (threadIdx.x < 29) {
         const int j = threadIdx.x;
-        a.partials[blockIdx.x * 32 + j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
+        a.partials[g.x * 32 + j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
     }
 }
+__global__ __launch_bounds__(256) void k_icp(IcpArgs a) { icp_body(hw_wg(), a); }
 
 // One work-group: sums the partial rows of k_icp (compensated per lane, then the same
 // tree), stores the 29 sums and, in mode 1, runs the FP64 update on one lane.  Running it as
 // its own kernel makes the partials visible through the kernel boundary; a last-group ticket
 // inside k_icp would need a device-scope release per work-group, which on MI355X writes back
 // the work-group's L2 (one per XCD) every time.
-__global__ __launch_bounds__(256) void k_icp_final(IcpArgs a, int nparts) {
+__device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     TrackState* st = a.st;
     if (a.mode != 0 && st->status != TS_RUNNING) return;
     __shared__ float wsum[4][29];
@@ -151,6 +153,7 @@ __global__ __launch_bounds__(256) void k_icp_final(IcpArgs a, int nparts) {
         if (a.mode == 1) icp_update(st, total, a.min_count, a.final_iter, a.thresh);
     }
 }
+__global__ __launch_bounds__(256) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
 
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_icp, dim3(nblocks), dim3(256), 0, s, a);
@@ -159,7 +162,7 @@ void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
 
 // start of ComputeSE3 (Tracking.cc:305-314): last pose = tracked pose of the previous
 // frame, initial guess = last pose (constant-position model), delta = identity
-__global__ void k_track_begin(TrackState* st) {
+__device__ __forceinline__ void track_begin_body(TrackState* st) {
     if (threadIdx.x != 0) return;
     for (int i = 0; i < 16; ++i) {
         st->next_pose[i] = st->last_pose[i];
@@ -170,14 +173,69 @@ __global__ void k_track_begin(TrackState* st) {
     st->iters_done = 0;
     refresh_float_pose(st);
 }
+__global__ void k_track_begin(TrackState* st) { track_begin_body(st); }
 // after fusion: the tracked frame becomes the last frame (Tracker::SwapFrame)
-__global__ void k_track_commit(TrackState* st) {
+__device__ __forceinline__ void track_commit_body(TrackState* st) {
     if (threadIdx.x != 0) return;
     if (st->status == TS_OK || st->status == TS_FIRST)
         for (int i = 0; i < 16; ++i) st->last_pose[i] = st->next_pose[i];
     refresh_float_pose(st);
 }
+__global__ void k_track_commit(TrackState* st) { track_commit_body(st); }
 void launch_track_begin(TrackState* st, hipStream_t s) { hipLaunchKernelGGL(k_track_begin, dim3(1), dim3(64), 0, s, st); }
 void launch_track_commit(TrackState* st, hipStream_t s) { hipLaunchKernelGGL(k_track_commit, dim3(1), dim3(64), 0, s, st); }
+
+// ------------------------------------------------------------------ batched launches
+// The iteration schedule is the union of the sessions' schedules: a session runs iteration
+// `iter` of `level` only if it has that many, and its last iteration is final.
+__device__ __forceinline__ bool icp_slot_args(const BatchLaunch& b, int s, IcpArgs& a, int& nblocks) {
+    if (slot_first(b, s)) return false;
+    const SessDesc& d = b.desc[b.sess[s]];
+    const int l = b.level;
+    if (b.iter >= d.iters[l]) return false;
+    a = cload(d.icp[l]);
+    a.final_iter = (l == d.last_level && b.iter == d.iters[l] - 1) ? 1 : 0;
+    nblocks = d.icp_blocks[l];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void kb_icp(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, s, a, nb)) return;
+    icp_body(g, a);
+}
+
+__global__ __launch_bounds__(256) void kb_icp_final(BatchLaunch b) {
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, blockIdx.x, a, nb)) return;
+    icp_final_body(a, nb);
+}
+
+__global__ void kb_track_begin(BatchLaunch b) {
+    const int s = blockIdx.x;
+    if (slot_first(b, s)) return;
+    track_begin_body(b.desc[b.sess[s]].st);
+}
+__global__ void kb_track_commit(BatchLaunch b) { track_commit_body(b.desc[b.sess[blockIdx.x]].st); }
+
+void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    (void)h;
+    hipLaunchKernelGGL(kb_track_begin, dim3(b.n), dim3(64), 0, s, b);
+}
+void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStream_t s) {
+    b.level = level;
+    b.iter = iter;
+    const int grid = batch_layout(b, h, BT_ICP, level, 0);
+    hipLaunchKernelGGL(kb_icp, dim3(grid), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(kb_icp_final, dim3(b.n), dim3(256), 0, s, b);
+}
+void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    (void)h;
+    hipLaunchKernelGGL(kb_track_commit, dim3(b.n), dim3(64), 0, s, b);
+}
 
 }  // namespace fu

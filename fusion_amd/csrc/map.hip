@@ -19,6 +19,7 @@
 // capacity); only the order of pool indices differs.
 #include "fu_device.h"
 #include "fu_kernels.h"
+#include "fu_batch.h"
 
 namespace fu {
 
@@ -49,13 +50,14 @@ void launch_reset_map(const MapView& m, hipStream_t s) {
 }
 
 // per-frame counters: the new-key count and the visible count start at zero
-__global__ void k_begin_pass(unsigned* ctr, int which) {
+__device__ __forceinline__ void begin_pass_body(unsigned* ctr, int which) {
     if (threadIdx.x == 0) {
         if (which & 1) ctr[C_NEW] = 0;
         if (which & 2) ctr[C_VISIBLE] = 0;
         if (which & 4) ctr[C_RENDER] = 0;
     }
 }
+__global__ void k_begin_pass(unsigned* ctr, int which) { begin_pass_body(ctr, which); }
 void launch_begin_pass(const MapView& m, int which, hipStream_t s) {
     hipLaunchKernelGGL(k_begin_pass, dim3(1), dim3(64), 0, s, m.ctr, which);
 }
@@ -79,12 +81,12 @@ __device__ __forceinline__ bool newset_claim(const MapView& m, unsigned long lon
 
 
 // Fusion::CreateBlocks (FuseMap.cu:80-111), one lane per pixel, 16x16 pixels per group.
-__global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) {
+__device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseArgs& p) {
     PoseF pose = p.pose;
     if (p.st && !pose_from_state(p.st, pose)) return;
     const float* __restrict__ depth = p.depth;
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
     bool live = x < p.cols && y < p.rows;
     f3 pt = mk3(0.f, 0.f, 0.f), dir = mk3(0.f, 0.f, 0.f);
     int nsteps = 0;
@@ -158,15 +160,16 @@ __global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) { alloc_a_body(hw_wg(), m, p); }
 
 // ------------------------------------------------------------ allocation pass B (per new key)
 // one wave per new block: lane 0 links the key, all lanes initialise the block's voxels
 // (sdf NaN, weight 0, colour 0: Voxel::release, DeviceMap.h:75-79)
-__global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackState* st) {
+__device__ __forceinline__ void alloc_b_body(WG g, const MapView& m, const TrackState* st) {
     if (st && st->status != TS_OK && st->status != TS_FIRST) return;
     const unsigned nnew = min(m.ctr[C_NEW], m.max_new);
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int wave = (g.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (g.nx * blockDim.x) >> 6;
     const int lane = lane_id();
     const int bbytes = vf_block_bytes(m.fmt);
     const int sb = vf_sdf_bytes(m.fmt);
@@ -241,6 +244,7 @@ __global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackStat
         }
     }
 }
+__global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackState* st) { alloc_b_body(hw_wg(), m, st); }
 
 void launch_alloc(const MapView& m, const FuseArgs& p, hipStream_t s) {
     hipLaunchKernelGGL(k_alloc_pass_a, dim3((p.cols + 15) / 16, (p.rows + 15) / 16), dim3(256), 0, s, m, p);
@@ -286,14 +290,14 @@ __device__ __forceinline__ bool block_visible(const VisArgs& v, const PoseF& pos
 // Scans the allocated-block array (N_alloc keys) rather than all N_E hash entries; the
 // set is the same (every allocated block has exactly one live entry).  Compaction is a
 // wave ballot plus one atomic per wave.
-__global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) {
+__device__ __forceinline__ void visible_body(WG g, const MapView& m, const VisArgs& v) {
     PoseF pose = v.pose;
     if (v.st && !pose_from_state(v.st, pose)) return;
     const unsigned nalloc = min(m.ctr[C_ALLOC], m.num_blocks);
     const float scale = m.voxel_size * 8.0f;  // DeviceMap::blockWidth
-    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned stride = g.nx * blockDim.x;
     // uniform trip count per wave so the ballot below sees every lane
-    const unsigned base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    const unsigned base0 = g.x * blockDim.x + (threadIdx.x & ~63u);
     for (unsigned b0 = base0; b0 < nalloc; b0 += stride) {
         const unsigned i = b0 + lane_id();
         bool vis = false;
@@ -305,6 +309,7 @@ __global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) {
         if (vis && idx < m.num_blocks) m.visible[idx] = i;
     }
 }
+__global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) { visible_body(hw_wg(), m, v); }
 
 void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
     hipLaunchKernelGGL(k_visible, dim3(1024), dim3(256), 0, s, m, v);
@@ -313,14 +318,14 @@ void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
 // ------------------------------------------------------------ integration (FuseMap.cu:140-191)
 
 template <int F>
-__global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
+__device__ __forceinline__ void integrate_body(WG g, const MapView& m, const FuseArgs& a) {
     using IO = VoxelIO<F>;
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
     const unsigned nvis = min(m.ctr[C_VISIBLE], m.num_blocks);
     const int cols = a.cols, rows = a.rows;
     const float thresh = m.trunc;
-    for (unsigned vb = blockIdx.x; vb < nvis; vb += gridDim.x) {
+    for (unsigned vb = g.x; vb < nvis; vb += g.nx) {
         const unsigned heap = m.visible[vb];
         const i3 bp = unpack_key(m.block_key[heap]);
         unsigned char* blk = IO::block(m, (int)heap);
@@ -399,12 +404,79 @@ __global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
         }
     }
 }
+template <int F>
+__global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
+    integrate_body<F>(hw_wg(), m, a);
+}
 
 void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
     switch (m.fmt) {
         case VF_F32_W8: hipLaunchKernelGGL(k_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
         case VF_F32_W16: hipLaunchKernelGGL(k_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, m, a); break;
         default: hipLaunchKernelGGL(k_integrate<VF_F16_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
+    }
+}
+
+// ------------------------------------------------------------------ batched launches
+// FuseMapColor for every session of a step: counters, allocation (both passes), visible
+// list and integration, each stage one launch over all sessions.
+__global__ void kb_begin_pass(BatchLaunch b) { begin_pass_body(b.desc[b.sess[blockIdx.x]].m.ctr, 7); }
+
+__device__ __forceinline__ FuseArgs slot_fuse(const BatchLaunch& b, const SessDesc& d, int s) {
+    FuseArgs a = d.fuse;
+    a.rgb = (const unsigned char*)b.in[s];
+    return a;
+}
+
+__global__ __launch_bounds__(256) void kb_alloc_a(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
+    alloc_a_body(g, m, d.fuse);
+}
+__global__ __launch_bounds__(256) void kb_alloc_b(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
+    alloc_b_body(g, m, d.st);
+}
+__global__ __launch_bounds__(256) void kb_visible(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
+    visible_body(g, m, d.vis);
+}
+template <int F>
+__global__ __launch_bounds__(256) void kb_integrate(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
+    integrate_body<F>(g, m, slot_fuse(b, d, s));
+}
+
+// work-groups per session for the grid-stride stages
+static int per_slot(int total, int n, int lo, int hi) {
+    int v = total / (n > 0 ? n : 1);
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+void launch_b_fuse(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+    hipLaunchKernelGGL(kb_begin_pass, dim3(b.n), dim3(64), 0, s, b);
+    int grid = batch_layout(b, h, BT_TILES, 0, 0);
+    hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(256), 0, s, b);
+    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(2048, b.n, 16, 512));
+    hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);
+    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(8192, b.n, 64, 1024));
+    hipLaunchKernelGGL(kb_visible, dim3(grid), dim3(256), 0, s, b);
+    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(16384, b.n, 256, 8192));
+    switch (fmt) {
+        case VF_F32_W8: hipLaunchKernelGGL(kb_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
+        case VF_F32_W16: hipLaunchKernelGGL(kb_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_integrate<VF_F16_W8>, dim3(grid), dim3(256), 0, s, b); break;
     }
 }
 

@@ -14,26 +14,30 @@
 // Tile maps are ceil(W/8) x ceil(H/8) (80x60 at 640x480, as RenderScene.cu:245-251).
 #include "fu_device.h"
 #include "fu_kernels.h"
+#include "fu_batch.h"
 
 namespace fu {
 
-__global__ void k_zrange_init(const TrackState* st, unsigned* zmin, unsigned* zmax, int n) {
+__device__ __forceinline__ void zrange_init_body(WG g, const TrackState* st, unsigned* zmin, unsigned* zmax, int n) {
     if (st && st->status != TS_OK && st->status != TS_FIRST) return;
     const unsigned hundred = __float_as_uint(100.0f);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = g.x * blockDim.x + threadIdx.x; i < n; i += g.nx * blockDim.x) {
         zmin[i] = hundred;
         zmax[i] = 0u;
     }
 }
+__global__ void k_zrange_init(const TrackState* st, unsigned* zmin, unsigned* zmax, int n) {
+    zrange_init_body(hw_wg(), st, zmin, zmax, n);
+}
 
-__global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a) {
+__device__ __forceinline__ void project_body(WG g, const MapView& m, const ZRangeArgs& a) {
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
     // a caller-given count never exceeds the list the visibility pass built
     const unsigned built = min(m.ctr[C_VISIBLE], m.num_blocks);
     const unsigned nvis = a.n_visible_override >= 0 ? min((unsigned)a.n_visible_override, built) : built;
     const int zc = a.zcols, zr = a.zrows;
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nvis; i += gridDim.x * blockDim.x) {
+    for (unsigned i = g.x * blockDim.x + threadIdx.x; i < nvis; i += g.nx * blockDim.x) {
         const i3 pos = unpack_key(m.block_key[m.visible[i]]);
         // projectBlock (RenderScene.cu:54-104)
         int ulx = zc, uly = zr, lrx = -1, lry = -1;
@@ -79,6 +83,7 @@ __global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a)
             }
     }
 }
+__global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a) { project_body(hw_wg(), m, a); }
 
 void launch_zrange(const MapView& m, const ZRangeArgs& a, hipStream_t s) {
     const int n = a.zcols * a.zrows;
@@ -175,11 +180,11 @@ struct Sampler {
 };
 
 template <int F>
-__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
+__device__ __forceinline__ void raycast_body(WG g, const MapView& m, const RaycastArgs& a) {
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= a.cols || y >= a.rows) return;
     const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
     f4* vout = a.vmap + y * a.cols + x;
@@ -258,6 +263,10 @@ __global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
     *vout = nan4;
     *nout = nan4;
 }
+template <int F>
+__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
+    raycast_body<F>(hw_wg(), m, a);
+}
 
 void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
     const dim3 g((a.cols + 15) / 16, (a.rows + 15) / 16);
@@ -265,6 +274,42 @@ void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
         case VF_F32_W8: hipLaunchKernelGGL(k_raycast<VF_F32_W8>, g, dim3(256), 0, s, m, a); break;
         case VF_F32_W16: hipLaunchKernelGGL(k_raycast<VF_F32_W16>, g, dim3(256), 0, s, m, a); break;
         default: hipLaunchKernelGGL(k_raycast<VF_F16_W8>, g, dim3(256), 0, s, m, a); break;
+    }
+}
+
+// ------------------------------------------------------------------ batched launches
+// Mapping::RayTrace for every session of a step: z-range tiles, block projection, ray march.
+__global__ void kb_zrange_init(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    zrange_init_body(g, d.st, d.zr.zmin, d.zr.zmax, d.zr.zcols * d.zr.zrows);
+}
+__global__ __launch_bounds__(256) void kb_project(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    project_body(g, cload(d.m), cload(d.zr));
+}
+template <int F>
+__global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    raycast_body<F>(g, cload(d.m), cload(d.rc));
+}
+
+void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+    int grid = batch_layout(b, h, BT_ZTILES, 0, 0);
+    hipLaunchKernelGGL(kb_zrange_init, dim3(grid), dim3(256), 0, s, b);
+    const int per = b.n > 0 ? 4096 / b.n : 64;
+    grid = batch_layout(b, h, BT_FIXED, 0, per < 32 ? 32 : (per > 512 ? 512 : per));
+    hipLaunchKernelGGL(kb_project, dim3(grid), dim3(256), 0, s, b);
+    grid = batch_layout(b, h, BT_TILES, 0, 0);
+    switch (fmt) {
+        case VF_F32_W8: hipLaunchKernelGGL(kb_raycast<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
+        case VF_F32_W16: hipLaunchKernelGGL(kb_raycast<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_raycast<VF_F16_W8>, dim3(grid), dim3(256), 0, s, b); break;
     }
 }
 

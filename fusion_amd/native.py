@@ -123,6 +123,11 @@ SIGNATURES = {
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
     "fu_system_set_profiling": (I, [P, I]),
     "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_batch_create": (I, [I, C.POINTER(P)]),
+    "fu_batch_destroy": (I, [P]),
+    "fu_batch_attach": (I, [P, P]),
+    "fu_batch_detach": (I, [P, P]),
+    "fu_batch_grab": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
 }
 
 SCENE_SIGNATURES = {

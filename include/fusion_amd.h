@@ -287,6 +287,24 @@ typedef struct fu_stage_times {
 fu_status fu_system_set_profiling(fu_system* sys, int enable);
 fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* out);
 
+/* ---- batched frames of many systems -------------------------------------------------------
+ * A batch context runs one frame of each of up to FU_MAX_BATCH attached systems as one
+ * sequence of launches (each stage of System::GrabImage, Core/System.cc:92-138, launched once
+ * for all of them).  Systems keep their own map, pose, pyramid and model maps; each one's
+ * results are bit-identical to running the same frames through fu_system_grab_image_from.
+ * Systems of one context share a voxel format and may differ in frame size and intrinsics.
+ * No reference counterpart: the reference processes one frame of one system at a time. */
+#define FU_MAX_BATCH 64
+typedef struct fu_batch fu_batch;
+fu_status fu_batch_create(int capacity, fu_batch** out);  /* capacity = attachable systems */
+fu_status fu_batch_destroy(fu_batch* b);                  /* detaches every system */
+fu_status fu_batch_attach(fu_batch* b, fu_system* sys);
+fu_status fu_batch_detach(fu_batch* b, fu_system* sys);
+/* n distinct attached systems, one device-resident frame each (depth u16, RGB u8x3 of the
+ * system's size); asynchronous on `s` */
+fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_t* const* depth,
+                        const uint8_t* const* rgb, fu_stream s);
+
 #ifdef __cplusplus
 }
 #endif

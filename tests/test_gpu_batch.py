@@ -1,0 +1,127 @@
+"""Batched frames of many systems (fu_batch_grab) against the same systems run alone.
+
+Each session's work-group decomposition is the same alone and in a batch, so every result
+is required to be bit-identical: poses, tracking status, model maps, block sets and voxels.
+The batch composition changes from step to step (sessions join late, skip a step, differ in
+frame size), as it does under continuous batching.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fusion_amd import api as A
+from tests.helpers import rows_set, same_bits, scene_frames
+
+pytestmark = pytest.mark.gpu
+
+VOX = 0.01
+MAP = dict(num_buckets=1 << 17, num_excess=1 << 16, num_blocks=120_000, voxel_size=VOX,
+           trunc_dist=float(np.float32(VOX) * np.float32(3.0)), max_new_blocks=1 << 15)
+
+# (scene kind, size, seed, cols, rows, first step, steps skipped)
+SESSIONS = [
+    (0, (4.0, 4.0, 3.0), 0, 640, 480, 0, ()),
+    (0, (3.0, 3.0, 2.5), 3, 320, 240, 0, (3,)),
+    (0, (5.0, 4.0, 3.0), 7, 640, 480, 2, ()),
+    (1, (8.0, 6.0, 3.0), 2, 1280, 720, 1, (4,)),
+]
+STEPS = 6
+
+
+def _frames():
+    out = []
+    for kind, size, seed, cols, rows, start, skip in SESSIONS:
+        n = STEPS - start - len(skip)
+        _, K, fr = scene_frames(kind, size, seed, frames=range(n), cols=cols, rows=rows)
+        out.append((K, [A.to_device(*f) for f in fr]))
+    return out
+
+
+def _system(i, K):
+    kind, size, seed, cols, rows, start, skip = SESSIONS[i]
+    return A.System(A.system_params(cols=cols, rows=rows, K=K, map=MAP))
+
+
+def _state(s):
+    m = s.mapping()
+    xyz = m.export_blocks()
+    order = np.lexsort(xyz.T[::-1])
+    xyz = xyz[order]
+    sdf, w, rgb = m.read_blocks(xyz)
+    vm = s.view("model_vmap").cpu().numpy()
+    nm = s.view("model_nmap").cpu().numpy()
+    return xyz, sdf, w, rgb, vm, nm
+
+
+def test_batch_equals_alone(gpu):
+    frames = _frames()
+    alone_res, alone_state = [], []
+    for i, (K, fr) in enumerate(frames):
+        print("alone session", i, flush=True)
+        s = _system(i, K)
+        res = []
+        for d, c in fr:
+            s.GrabImage(d, c)
+            res.append(s.result())
+        alone_res.append(res)
+        alone_state.append(_state(s))
+        s.close()
+    torch.cuda.synchronize()
+
+    print("batched", flush=True)
+    b = A.Batch(8)
+    systems = [_system(i, K) for i, (K, _) in enumerate(frames)]
+    for s in systems:
+        b.attach(s)
+    nxt = [0] * len(systems)
+    batch_res = [[] for _ in systems]
+    for step in range(STEPS):
+        members = [i for i, (_, _, _, _, _, start, skip) in enumerate(SESSIONS) if step >= start and step not in skip]
+        print("step", step, "sessions", members, flush=True)
+        b.grab([systems[i] for i in members], [frames[i][1][nxt[i]][0] for i in members],
+               [frames[i][1][nxt[i]][1] for i in members])
+        for i in members:
+            batch_res[i].append(systems[i].result())
+            nxt[i] += 1
+    for i, s in enumerate(systems):
+        assert nxt[i] == len(frames[i][1])
+        for f, (ra, rb) in enumerate(zip(alone_res[i], batch_res[i])):
+            assert ra["track_status"] == rb["track_status"], (i, f)
+            assert np.array_equal(ra["pose"], rb["pose"]), (i, f, ra["pose"], rb["pose"])
+            assert ra["n_visible"] == rb["n_visible"] and ra["n_allocated"] == rb["n_allocated"], (i, f)
+        xa, sa, wa, ca, va, na = alone_state[i]
+        xb, sb, wb, cb, vb, nb = _state(s)
+        assert np.array_equal(xa, xb), i
+        assert same_bits(sa, sb) and np.array_equal(wa, wb) and np.array_equal(ca, cb), i
+        assert same_bits(va, vb) and same_bits(na, nb), i
+        assert alone_res[i][-1]["track_status"] in (1, 4), i  # the sequences do track
+    b.close()
+
+
+def test_batch_argument_errors(gpu):
+    K = (525.0, 525.0, 319.5, 239.5)
+    b = A.Batch(2)
+    s1 = A.System(A.system_params(map=MAP))
+    s2 = A.System(A.system_params(map=MAP))
+    s3 = A.System(A.system_params(map=dict(MAP, voxel_format=2)))
+    b.attach(s1)
+    with pytest.raises(A.N.FusionError):
+        b.attach(s1)  # already attached
+    with pytest.raises(A.N.FusionError):
+        b.attach(s3)  # other voxel format
+    d = torch.zeros((480, 640), dtype=torch.int16, device="cuda")
+    c = torch.zeros((480, 640, 3), dtype=torch.uint8, device="cuda")
+    with pytest.raises(A.N.FusionError):
+        b.grab([s1, s2], [d, d], [c, c])  # s2 not attached
+    b.attach(s2)
+    with pytest.raises(A.N.FusionError):
+        b.grab([s1, s1], [d, d], [c, c])  # twice in one step
+    b.grab([s1, s2], [d, d], [c, c])  # empty frames: nothing to fuse, still valid
+    r = s1.result()
+    assert r["track_status"] == 4 and r["frame_index"] == 1
+    b.detach(s1)
+    s1.close()
+    s2.close()  # destroying an attached system detaches it
+    s3.close()
+    b.close()
+    del K
