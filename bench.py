@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark: tracked+fused 640x480 frames/s at 5 mm voxels, with p50/p99 frame latency,
-served (device-resident, hipGraph-replayed) vs plain (reference-style host control).
+"""Benchmark: tracked+fused 640x480 frames/s at 5 mm voxels across served sessions, with
+p50/p99 frame latency, served vs plain.
 
-Workload (the single-stream fusion-at-scale configuration): seeded synthetic multi-room
-scene 12 x 8 x 3 m, 640x480 u16 depth at 5000 units/m with sigma = 0.0015 z^2 noise and 1%
-dropout, fx = fy = 525, cx = 319.5, cy = 239.5; ICP 10/5/4 point-to-plane iterations in
-FP32; 5 mm voxels in 8^3 blocks, truncation 8 voxels (the reference's TruncateDist), 2^21
-hash buckets and 2M blocks.
-A step is one frame: pyramid, ICP against the model raycast, allocation, visibility,
-integration and raycast.  Frames are generated on the CPU and resident in HBM before the
-timed region starts.
+Served workload (the served-reconstruction configuration at 640x480): 64 concurrent sessions,
+session i observes synthetic room i % 16 (seeded, 3x3x2.5 m up to 10x8x3 m, random spheres and
+boxes), starting (i // 16) * 100 frames into its camera path so no two sessions see the same
+frames.  Per session: 640x480 u16 depth at 5000 units/m with sigma = 0.0015 z^2 noise and 1%
+dropout plus RGB, fx = fy = 525, cx = 319.5, cy = 239.5; ICP 10/5/4 point-to-plane iterations
+in FP32; 5 mm voxels in 8^3 blocks, truncation 8 voxels, a hash table of 2^19 buckets and
+500k blocks (2 GB) per session, all resident in HBM; admission under a 200 GB block budget.
+Frames are generated on the CPU and resident in HBM before any timed region.
+
+A step is one frame of every session (64 frames).  `value` is the number of frames that were
+tracked and fused per second with every session's frames queued from the start (the server's
+capacity); `latency_ms` is submit-to-result latency with 30 fps Poisson arrivals per session
+(about 1,900 frames/s offered).  `plain` runs the same kind of session one frame at a time with
+the reference's host control (host solve per ICP iteration, synchronisation per stage).
 
 Prints one JSON line.  --gpus N > 1 is refused: this build targets one GPU.
 """
@@ -27,17 +33,32 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 K640 = (525.0, 525.0, 319.5, 239.5)
 VOX = 0.005
-MAP5 = dict(num_buckets=1 << 21, num_excess=1 << 21, num_blocks=2_000_000, voxel_size=VOX,
-            trunc_dist=float(np.float32(VOX) * np.float32(8.0)), max_new_blocks=1 << 18)
+TRUNC = float(np.float32(VOX) * np.float32(8.0))
+MAP_SESSION = dict(num_buckets=1 << 19, num_excess=1 << 19, num_blocks=500_000, voxel_size=VOX, trunc_dist=TRUNC,
+                   max_new_blocks=1 << 17)
+MAP_SINGLE = dict(num_buckets=1 << 21, num_excess=1 << 21, num_blocks=2_000_000, voxel_size=VOX, trunc_dist=TRUNC,
+                  max_new_blocks=1 << 18)
+BUDGET = 200 * 10**9
+TRACKED = (1, 4)  # FU_TRACK_OK, FU_TRACK_FIRST: frames that are tracked and fused
+# frame size mix of the mixed-resolution served run: (cols, rows, probability)
+MIX = [(640, 480, 0.5), (320, 240, 0.3), (1280, 720, 0.2)]
 
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300, help="timed frames")
-    ap.add_argument("--warmup", type=int, default=20, help="untimed frames before the timed region")
-    ap.add_argument("--plain-steps", type=int, default=0,
-                    help="frames timed in the plain (reference-style) mode (0: the same as --steps)")
+    ap.add_argument("--steps", type=int, default=60, help="timed steps (one frame of every session each)")
+    ap.add_argument("--warmup", type=int, default=4, help="untimed steps before the timed region")
+    ap.add_argument("--sessions", type=int, default=64)
+    ap.add_argument("--realtime-frames", type=int, default=30, help="frames per session of the 30 fps latency run")
+    ap.add_argument("--profile-steps", type=int, default=8, help="steps of the per-stage event profile")
+    ap.add_argument("--plain-sessions", type=int, default=8)
+    ap.add_argument("--plain-steps", type=int, default=12)
+    ap.add_argument("--mixed-steps", type=int, default=30, help="timed steps of the mixed-resolution run (0: skip)")
+    ap.add_argument("--single-steps", type=int, default=200, help="frames of the single-stream run (0: skip)")
+    ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=32)
+    ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -50,116 +71,211 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
-def render_frames(scene, first, n, cols=640, rows=480):
+def pct(a, q):
+    return round(float(np.percentile(np.asarray(a, dtype=np.float64), q)), 4) if len(a) else None
+
+
+def room(k):
+    f = k / 15.0
+    return (3.0 + 7.0 * f, 3.0 + 5.0 * f, 2.5 + 0.5 * f)
+
+
+def kmat(cols, rows):
+    s = cols / 640.0
+    return (525.0 * s, 525.0 * s, (cols - 1) / 2.0, (rows - 1) / 2.0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def render_sessions(A, sizes, nframes):
+    """device frames of every session: list (per session) of (depth, rgb) tensors"""
+    scenes = {}
     out = []
-    for f in range(first, first + n):
-        out.append(scene.render(f, cols, rows, K640, 5000.0, True))
+    for i, (cols, rows) in enumerate(sizes):
+        k = i % 16
+        if k not in scenes:
+            scenes[k] = A.Scene(0, room(k), k)
+        start = (i // 16) * 100
+        K = kmat(cols, rows)
+        out.append([A.to_device(*scenes[k].render(start + f, cols, rows, K, 5000.0, True)) for f in range(nframes)])
     return out
 
 
-def pct(a, q):
-    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+def trace(sids, frames, lo, hi, arrivals=None):
+    tr = []
+    for j, sid in enumerate(sids):
+        for f in range(lo, hi):
+            t = 0.0 if arrivals is None else arrivals[j][f - lo]
+            d, c = frames[j][f]
+            tr.append((sid, d, c, t, j * 100000 + f))
+    return tr
 
 
-def run_gpu(args):
-    import torch
-    from fusion_amd import api as A
+def poisson_arrivals(rng, n_sessions, n, fps=30.0):
+    out = []
+    for _ in range(n_sessions):
+        gaps = rng.exponential(1e3 / fps, size=n)
+        gaps[0] = rng.uniform(0, 1e3 / fps)
+        out.append(np.cumsum(gaps).tolist())
+    return out
 
+
+def served(A, torch, args, sizes, frames, steps, realtime, profile):
+    """open the sessions, warm up, then the backlog (capacity) run, the 30 fps run and the
+    stage profile; returns a dict of results"""
+    sv = A.Server(max_batch=args.max_batch, max_inflight=args.inflight, lanes=args.lanes, block_budget=BUDGET)
+    sids = []
+    for cols, rows in sizes:
+        sids.append(sv.open(A.system_params(cols=cols, rows=rows, K=kmat(cols, rows), map=MAP_SESSION)))
+    W = args.warmup
+    sv.run_trace(trace(sids, frames, 0, W), closed_loop=True)
+    torch.cuda.synchronize()
+    # capacity: every session's timed frames queued at t = 0
+    tr = trace(sids, frames, W, W + steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = sv.run_trace(tr)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = np.array([c["track_status"] for c in done])
+    lat_b = [c["t_done_ms"] - c["t_submit_ms"] for c in done]
+    flags = int(np.bitwise_or.reduce([c["flags"] for c in done])) if done else 0
+    nalloc = [c["n_allocated"] for c in done]
+    res = dict(wall=wall, frames=len(done), tracked=int(np.isin(st, TRACKED).sum()), lat_backlog=lat_b, flags=flags,
+               n_allocated_max=int(max(nalloc)), n_allocated_mean=float(np.mean(nalloc)))
+    # latency: 30 fps Poisson arrivals per session
+    lo = W + steps
+    if realtime > 0:
+        rng = np.random.default_rng(args.seed + 7)
+        arr = poisson_arrivals(rng, len(sids), realtime)
+        done = sv.run_trace(trace(sids, frames, lo, lo + realtime, arr))
+        res["lat_rt"] = [c["t_done_ms"] - c["t_submit_ms"] for c in done]
+        span = max(c["t_done_ms"] for c in done) - min(c["t_submit_ms"] for c in done)
+        res["rt_fps"] = len(done) / span * 1e3
+        res["rt_tracked"] = float(np.isin([c["track_status"] for c in done], TRACKED).mean())
+        lo += realtime
+    # per-stage HIP events (each step synchronises), same sessions, next frames
+    if profile > 0:
+        sv.set_profiling(True)
+        sv.run_trace(trace(sids, frames, lo, lo + profile))
+        res["stages"] = sv.stage_times()
+        sv.set_profiling(False)
+    sv.close()
+    torch.cuda.synchronize()
+    return res
+
+
+def plain(A, torch, args, frames):
+    """reference-style control, one frame at a time, sessions in turn"""
+    n = min(args.plain_sessions, len(frames))
+    systems = [A.System(A.system_params(plain=True, map=MAP_SESSION)) for _ in range(n)]
+    W = args.warmup
+    for f in range(W):
+        for j, s in enumerate(systems):
+            s.GrabImage(*frames[j][f])
+    torch.cuda.synchronize()
+    lat = []
+    tracked = 0
+    t0 = time.perf_counter()
+    for f in range(W, W + args.plain_steps):
+        for j, s in enumerate(systems):
+            a = time.perf_counter()
+            s.GrabImage(*frames[j][f])
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - a) * 1e3)
+    wall = time.perf_counter() - t0
+    for s in systems:
+        tracked += s.result()["track_status"] in TRACKED
+        s.close()
+    torch.cuda.synchronize()
+    return dict(value=round(len(lat) / wall, 2), frames=len(lat), sessions=n,
+                latency_ms={"p50": pct(lat, 50), "p99": pct(lat, 99)})
+
+
+def single_stream(A, torch, args):
+    """one session on a 12x8x3 m multi-room scene (2^21 buckets, 2M blocks), frames replayed
+    as one hipGraph each: the latency of a frame alone on the GPU"""
     scene = A.Scene(1, (12.0, 8.0, 3.0), args.seed)
-    total = args.warmup + args.steps
-    t0 = time.time()
-    host = render_frames(scene, 0, total)
-    gen_s = time.time() - t0
-    dev_frames = [A.to_device(d16, rgb) for d16, rgb in host]
+    n = args.single_steps
+    W = 10
+    dev = [A.to_device(*scene.render(f, 640, 480, K640, 5000.0, True)) for f in range(W + n)]
+    s = A.System(A.system_params(use_graph=True, map=MAP_SINGLE))
+    for i in range(W):
+        s.GrabImage(*dev[i])
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-
-    # ---- served path: device-resident control, the frame replayed as one hipGraph
-    sysg = A.System(A.system_params(use_graph=True, map=MAP5))
-    for i in range(args.warmup):
-        sysg.GrabImage(*dev_frames[i])
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
     t0 = time.perf_counter()
     ev[0].record(stream)
-    for i in range(args.steps):
-        sysg.GrabImage(*dev_frames[args.warmup + i])
+    for i in range(n):
+        s.GrabImage(*dev[W + i])
         ev[i + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
-    res = sysg.result()
-    stats = sysg.mapping().stats()
-    sysg.close()
-    del sysg
+    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(n)]
+    r = s.result()
+    s.close()
     torch.cuda.synchronize()
-
-    # ---- the same frames again with HIP events at every stage boundary (kernels launched
-    # directly, same kernels and launch shapes as the graph); gives per-kernel durations
-    sysq = A.System(A.system_params(use_graph=False, map=MAP5))
-    for i in range(args.warmup):
-        sysq.GrabImage(*dev_frames[i])
-    sysq.set_profiling(True)
-    for i in range(args.steps):
-        sysq.GrabImage(*dev_frames[args.warmup + i])
-    st = sysq.stage_times()
-    sysq.close()
-    del sysq
-    torch.cuda.synchronize()
-
-    # ---- plain path: reference-style control (host solve, synchronise per stage)
-    sysp = A.System(A.system_params(plain=True, map=MAP5))
-    for i in range(args.warmup):
-        sysp.GrabImage(*dev_frames[i])
-    torch.cuda.synchronize()
-    nplain = min(args.plain_steps, args.steps) if args.plain_steps > 0 else args.steps
-    lat_p = []
-    t0 = time.perf_counter()
-    for i in range(nplain):
-        a = time.perf_counter()
-        sysp.GrabImage(*dev_frames[args.warmup + i])
-        torch.cuda.synchronize()
-        lat_p.append((time.perf_counter() - a) * 1e3)
-    wall_p = time.perf_counter() - t0
-    sysp.close()
-    return dict(wall=wall, lat=lat, res=res, stages=st, stats=stats, wall_p=wall_p, lat_p=lat_p, nplain=nplain,
-                gen_s=gen_s, host=host)
+    return dict(value=round(n / wall, 2), frames=n, latency_ms={"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                last_track_status=r["track_status"],
+                workload="12x8x3 m multi-room, 640x480, 2^21 buckets, 2M blocks, hipGraph per frame")
 
 
-def integrate_roofline(st, cols=640, rows=480):
-    """algorithmic bytes of TSDF integration per frame: every voxel of every visible block
-    read and written (8 B each in the f32/u8 layout), the block id and key (16 B), and the
-    frame's depth (4 B), normal (16 B) and colour (3 B) read once"""
-    frames = max(1, st["frames"])
+def rooflines(st, sizes, nsess):
+    """algorithmic bytes of the batched stage kernels per launch, over their measured average
+    launch duration (HIP events around the launch, profiling steps)"""
+    steps = max(1.0, st["steps"])
+    frames = max(1.0, st["frames"])
+    per_step = frames / steps
+    px = float(np.mean([c * r for c, r in sizes]))
     nvis = st["n_visible"] / frames
-    bytes_per_frame = nvis * 512 * 2 * 8 + nvis * 16 + cols * rows * (4 + 16 + 3)
-    ms = st["ms"]["integrate"] / frames
-    achieved = bytes_per_frame / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_integrate<VF_F32_W8>",
-                bytes_per_launch=int(bytes_per_frame), avg_launch_us=round(ms * 1e3, 2), n_visible=round(nvis, 1))
+    out = {}
+    # integrate: every voxel of every visible block read and written (8 B each in the f32 sdf +
+    # u8 weight/rgb layout), the visible id and block key (16 B), depth, normal and colour of
+    # the frame read once (4 + 16 + 3 B per pixel)
+    b_int = per_step * (nvis * 512 * 2 * 8 + nvis * 16 + px * (4 + 16 + 3))
+    # raycast: lower bound of W*H*32 B written (vertex + normal) plus each visible block's
+    # sdf and weight planes read once (8 B per voxel)
+    b_rc = per_step * (px * 32 + nvis * 512 * 8)
+    # ICP: 64 B per pixel-iteration (current vertex+normal, last vertex+normal)
+    b_icp = (st["icp_pixels"] / steps) * 64
+    for name, b, kern in (("integrate", b_int, "kb_integrate<VF_F32_W8>"), ("raycast", b_rc, "kb_raycast<VF_F32_W8>"),
+                          ("icp", b_icp, "kb_icp + kb_icp_final (stage)")):
+        ms = st["ms"][name] / steps
+        ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[name] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kern, bytes_per_launch=int(b),
+                         avg_launch_us=round(ms * 1e3, 2), frames_per_launch=round(per_step, 2),
+                         n_visible_per_frame=round(nvis, 1))
+    return out
 
 
-def cpu_baseline(host, seconds):
-    """the CPU oracle (scalar C++ restatement, OpenMP over pixels and blocks) on the first
-    frames of the same sequence, run until `seconds` of CPU time have passed"""
+def cpu_baseline(A, seconds):
+    """the CPU oracle (scalar C++ restatement, OpenMP over pixels and blocks) on session 0's
+    frames (room 0, 640x480, 5 mm, 2^19 buckets, 500k blocks), until `seconds` have passed"""
     from oracle import binding as O
 
     cores = cpu_cores()
     O.set_threads(cores)
-    s = O.System(num_buckets=MAP5["num_buckets"], num_excess=MAP5["num_excess"], num_blocks=MAP5["num_blocks"],
-                 voxel_size=VOX, trunc_dist=MAP5["trunc_dist"])
+    sc = A.Scene(0, room(0), 0)
+    s = O.System(num_buckets=MAP_SESSION["num_buckets"], num_excess=MAP_SESSION["num_excess"],
+                 num_blocks=MAP_SESSION["num_blocks"], voxel_size=VOX, trunc_dist=TRUNC)
     n = 0
     t0 = time.perf_counter()
-    for d16, rgb in host:
+    busy = 0.0
+    while True:
+        d16, rgb = sc.render(n, 640, 480, K640, 5000.0, True)
+        a = time.perf_counter()
         s.grab(d16, rgb)
+        busy += time.perf_counter() - a
         n += 1
-        if time.perf_counter() - t0 > seconds and n >= 3:
+        if busy > seconds and n >= 3:
             break
-    dt = time.perf_counter() - t0
     s.close()
-    return dict(value=round(n / dt, 3), unit="frames/s", cores=cores, kind="port",
-                sample="first %d frames of the same sequence (%.1f s)" % (n, dt))
+    return dict(value=round(n / busy, 3), unit="frames/s", cores=cores, kind="port",
+                sample="first %d frames of session 0 (room 0, 640x480, 5 mm), %.1f s of oracle time" % (n, busy))
 
 
 def main():
@@ -168,12 +284,50 @@ def main():
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
               "(--gpus %d refused)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    r = run_gpu(args)
-    fps = args.steps / r["wall"]
-    fps_plain = r["nplain"] / r["wall_p"]
-    cpu = cpu_baseline(r["host"], args.cpu_seconds) if args.cpu_seconds > 0 else None
-    st = r["stages"]
-    per_stage = {k: round(v / max(1, st["frames"]), 4) for k, v in st["ms"].items()}
+    import torch
+    from fusion_amd import api as A
+
+    ns = args.sessions
+    per = args.warmup + args.steps + args.realtime_frames + args.profile_steps
+    t = time.time()
+    sizes = [(640, 480)] * ns
+    frames = render_sessions(A, sizes, per)
+    torch.cuda.synchronize()
+    log("rendered %d frames in %.1f s" % (ns * per, time.time() - t))
+    r = served(A, torch, args, sizes, frames, args.steps, args.realtime_frames, args.profile_steps)
+    log("served: %.0f frames/s" % (r["tracked"] / r["wall"]))
+    p = plain(A, torch, args, frames)
+    log("plain: %.0f frames/s" % p["value"])
+    del frames
+    torch.cuda.empty_cache()
+
+    mixed = None
+    if args.mixed_steps > 0:
+        rng = np.random.default_rng(args.seed)
+        pick = rng.choice(len(MIX), size=ns, p=[m[2] for m in MIX])
+        msz = [(MIX[k][0], MIX[k][1]) for k in pick]
+        mper = args.warmup + args.mixed_steps + args.realtime_frames
+        mf = render_sessions(A, msz, mper)
+        torch.cuda.synchronize()
+        m = served(A, torch, args, msz, mf, args.mixed_steps, args.realtime_frames, 0)
+        del mf
+        torch.cuda.empty_cache()
+        mixed = dict(value=round(m["tracked"] / m["wall"], 2), frames_per_s_all=round(m["frames"] / m["wall"], 2),
+                     sessions={"%dx%d" % s: int(sum(1 for z in msz if z == s)) for s in sorted(set(msz))},
+                     mean_pixels_per_frame=float(np.mean([c * r_ for c, r_ in msz])),
+                     latency_ms={"p50": pct(m["lat_rt"], 50), "p99": pct(m["lat_rt"], 99)},
+                     realtime_frames_per_s=round(m["rt_fps"], 1), flags=m["flags"])
+        log("mixed: %.0f frames/s" % mixed["value"])
+
+    single = single_stream(A, torch, args) if args.single_steps > 0 else None
+    cpu = cpu_baseline(A, args.cpu_seconds) if args.cpu_seconds > 0 else None
+
+    fps = r["tracked"] / r["wall"]
+    st = r.get("stages")
+    rl = rooflines(st, sizes, ns) if st else None
+    # the single kernel with the largest time per step (the ICP entry is a stage of many launches)
+    dominant = max(("integrate", "raycast"), key=lambda k: rl[k]["avg_launch_us"]) if rl else None
+    steps_p = max(1.0, st["steps"]) if st else 1.0
     out = {
         "metric": "tracked+fused 640x480 frames/s @5mm; p50/p99 frame latency, served vs plain",
         "value": round(fps, 2),
@@ -186,18 +340,26 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded multi-room scene, noisy 640x480 depth + RGB)",
-        "config": {"workload": "single-stream fusion at scale: 12x8x3 m multi-room, 640x480, 5 mm voxels, "
-                               "trunc 8 voxels, 2^21 buckets, 2M blocks, ICP 10/5/4",
-                   "frames": args.steps, "voxel_format": "f32 sdf + u8 weight + rgb", "mode": "device-resident + hipGraph"},
-        "latency_ms": {"p50": round(pct(r["lat"], 50), 4), "p99": round(pct(r["lat"], 99), 4)},
-        "plain": {"value": round(fps_plain, 2), "frames": r["nplain"],
-                  "latency_ms": {"p50": round(pct(r["lat_p"], 50), 4), "p99": round(pct(r["lat_p"], 99), 4)}},
-        "served_over_plain": round(fps / fps_plain, 3),
-        "stage_ms_per_frame": per_stage,
-        "map": {"n_allocated": r["stats"]["n_allocated"], "flags": r["stats"]["flags"],
-                "last_track_status": r["res"]["track_status"]},
-        "roofline": integrate_roofline(st),
+        "data": "synthetic (16 seeded rooms, noisy 640x480 depth + RGB, 64 sessions)",
+        "config": {"workload": "served reconstruction: %d sessions of 640x480 at 5 mm, trunc 8 voxels, 2^19 buckets "
+                               "and 500k blocks per session, ICP 10/5/4, 200 GB block budget" % ns,
+                   "sessions": ns, "frames_per_session": args.steps, "voxel_format": "f32 sdf + u8 weight + rgb",
+                   "lanes": args.lanes, "max_batch": args.max_batch, "max_inflight": args.inflight},
+        "frames_per_s_all": round(r["frames"] / r["wall"], 2),
+        "tracked_fraction": round(r["tracked"] / max(1, r["frames"]), 4),
+        "latency_ms": {"p50": pct(r.get("lat_rt", []), 50), "p99": pct(r.get("lat_rt", []), 99),
+                       "arrivals": "30 fps Poisson per session",
+                       "offered_frames_per_s": round(r.get("rt_fps", 0.0), 1)},
+        "latency_backlog_ms": {"p50": pct(r["lat_backlog"], 50), "p99": pct(r["lat_backlog"], 99)},
+        "plain": p,
+        "served_over_plain": round(fps / p["value"], 3),
+        "mixed_resolution": mixed,
+        "single_stream": single,
+        "stage_ms_per_step": ({k: round(v / steps_p, 4) for k, v in st["ms"].items()} if st else None),
+        "map": {"n_allocated_max": r["n_allocated_max"], "n_allocated_mean": round(r["n_allocated_mean"], 1),
+                "flags": r["flags"]},
+        "roofline": rl[dominant] if rl else None,
+        "kernels": rl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -335,8 +335,7 @@ class System:
     def stage_times(self):
         t = N.StageTimes()
         check(N.lib().fu_system_get_stage_times(self.handle, C.byref(t)), "fu_system_get_stage_times")
-        return {"ms": dict(zip(N.STAGES, t.ms[:])), "frames": t.frames, "n_visible": t.n_visible,
-                "n_new": t.n_new, "icp_iterations": t.icp_iterations, "icp_pixels": t.icp_pixels}
+        return _times(t)
 
     def view(self, name, level=0):
         """a torch view of one of the system's device maps"""
@@ -392,6 +391,14 @@ class Batch:
         except Exception:
             pass
 
+    def set_profiling(self, enable=True):
+        check(N.lib().fu_batch_set_profiling(self.handle, int(enable)), "fu_batch_set_profiling")
+
+    def stage_times(self):
+        t = N.StageTimes()
+        check(N.lib().fu_batch_get_stage_times(self.handle, C.byref(t)), "fu_batch_get_stage_times")
+        return _times(t)
+
     def grab(self, systems, depths, rgbs):
         """one frame per system: device depth (H, W) u16-in-int16 and rgb (H, W, 3) u8 tensors"""
         n = len(systems)
@@ -400,6 +407,112 @@ class Batch:
         dp = arr(*[d.data_ptr() for d in depths])
         cp = arr(*[c.data_ptr() for c in rgbs])
         check(N.lib().fu_batch_grab(self.handle, n, sy, dp, cp, _s()), "fu_batch_grab")
+
+
+class BudgetExceeded(N.FusionError):
+    """a session was not admitted: the server's voxel-block budget is in use"""
+
+
+class Server:
+    """Continuous-batching server of many reconstruction sessions (fu_server_*)."""
+
+    def __init__(self, max_batch=32, max_inflight=2, lanes=2, block_budget=0):
+        _dev()
+        p = N.ServerParams()
+        N.lib().fu_server_params_default(C.byref(p))
+        p.max_batch, p.max_inflight, p.lanes, p.block_budget = max_batch, max_inflight, lanes, int(block_budget)
+        h = C.c_void_p()
+        check(N.lib().fu_server_create(C.byref(p), C.byref(h)), "fu_server_create")
+        self.handle = h
+        self.params = {}
+
+    def close(self):
+        if self.handle:
+            N.lib().fu_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def open(self, params):
+        sid = C.c_int()
+        st = N.lib().fu_server_open(self.handle, C.byref(params), C.byref(sid))
+        if st == N.FU_ERR_BUDGET:
+            raise BudgetExceeded(N.lib().fu_last_error().decode(errors="replace"))
+        check(st, "fu_server_open")
+        self.params[sid.value] = params
+        return sid.value
+
+    def close_session(self, sid):
+        check(N.lib().fu_server_close(self.handle, int(sid)), "fu_server_close")
+        del self.params[sid]
+
+    def system(self, sid):
+        """a non-owning System view of a session (result(), mapping(), view())"""
+        s = System.__new__(System)
+        s.handle = C.c_void_p(N.lib().fu_server_system(self.handle, int(sid)))
+        s.params = self.params[sid]
+        b = N.SystemBuffers()
+        check(N.lib().fu_system_get_buffers(s.handle, C.byref(b)), "fu_system_get_buffers")
+        s.buffers = b
+        s.close = lambda: None
+        return s
+
+    def now_ms(self):
+        return N.lib().fu_server_now_ms(self.handle)
+
+    def set_profiling(self, enable=True):
+        check(N.lib().fu_server_set_profiling(self.handle, int(enable)), "fu_server_set_profiling")
+
+    def stage_times(self):
+        t = N.StageTimes()
+        check(N.lib().fu_server_get_stage_times(self.handle, C.byref(t)), "fu_server_get_stage_times")
+        return _times(t)
+
+    def submit(self, sid, depth, rgb, tag=0, t_submit_ms=-1.0):
+        check(N.lib().fu_server_submit(self.handle, int(sid), _p(depth), _p(rgb), int(tag), float(t_submit_ms)),
+              "fu_server_submit")
+
+    def poll(self, cap=256):
+        out = (N.Completion * cap)()
+        n = C.c_int()
+        check(N.lib().fu_server_poll(self.handle, out, cap, C.byref(n)), "fu_server_poll")
+        return [_completion(out[i]) for i in range(n.value)]
+
+    def pending(self):
+        n = C.c_int()
+        check(N.lib().fu_server_pending(self.handle, C.byref(n)), "fu_server_pending")
+        return n.value
+
+    def drain(self):
+        out = []
+        while self.pending():
+            out += self.poll()
+        return out
+
+    def run_trace(self, frames, closed_loop=False):
+        """frames: list of (session, depth, rgb, arrival_ms, tag); returns completions in order"""
+        n = len(frames)
+        tr = (N.TraceFrame * n)()
+        for i, (sid, d, c, t, tag) in enumerate(frames):
+            tr[i].session, tr[i].depth, tr[i].rgb, tr[i].arrival_ms, tr[i].tag = sid, d.data_ptr(), c.data_ptr(), t, tag
+        out = (N.Completion * n)()
+        check(N.lib().fu_server_run_trace(self.handle, tr, n, int(closed_loop), out), "fu_server_run_trace")
+        return [_completion(out[i]) for i in range(n)]
+
+
+def _completion(c):
+    d = {f: getattr(c, f) for f, _ in N.Completion._fields_ if f != "pose"}
+    d["pose"] = np.array(c.pose[:]).reshape(4, 4)
+    return d
+
+
+def _times(t):
+    return {"ms": dict(zip(N.STAGES, t.ms[:])), "frames": t.frames, "n_visible": t.n_visible,
+            "n_new": t.n_new, "icp_iterations": t.icp_iterations, "icp_pixels": t.icp_pixels, "steps": t.steps}
 
 
 class _DevBuf:

@@ -43,6 +43,14 @@ static fu_status set_err(fu_status code, const char* fmt, ...) {
                                             __FILE__, __LINE__);                                  \
     } while (0)
 
+fu_status fu::api_error(fu_status code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 static hipStream_t S(fu_stream s) { return (hipStream_t)s; }
 
 extern "C" const char* fu_last_error(void) { return g_err; }
@@ -1142,6 +1150,7 @@ extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* d
         HIPCHK(hipMemcpy(c, sys->map->v.ctr, sizeof(c), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost));
         sys->times.frames += 1;
+        sys->times.steps += 1;
         sys->times.n_visible += c[C_VISIBLE];
         sys->times.n_new += c[C_NEW];
         if (!first) {
@@ -1241,6 +1250,9 @@ struct fu_batch {
     std::vector<SessDesc> h_desc;
     std::vector<fu_system*> owner;
     int attached = 0;
+    bool prof = false;
+    hipEvent_t ev[FU_NUM_STAGES + 1] = {};
+    fu_stage_times times{};
 };
 
 extern "C" fu_status fu_batch_create(int capacity, fu_batch** out) {
@@ -1267,8 +1279,25 @@ extern "C" fu_status fu_batch_destroy(fu_batch* bt) {
             sys->batch = nullptr;
             sys->batch_slot = -1;
         }
+    for (int k = 0; k <= FU_NUM_STAGES; ++k)
+        if (bt->ev[k]) (void)hipEventDestroy(bt->ev[k]);
     (void)hipFree(bt->d_desc);
     delete bt;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_set_profiling(fu_batch* bt, int enable) {
+    if (!bt) return set_err(FU_ERR_ARG, "null batch");
+    if (enable && !bt->ev[0])
+        for (int k = 0; k <= FU_NUM_STAGES; ++k) HIPCHK(hipEventCreate(&bt->ev[k]));
+    bt->prof = enable != 0;
+    memset(&bt->times, 0, sizeof(bt->times));
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_get_stage_times(fu_batch* bt, fu_stage_times* out) {
+    if (!bt || !out) return set_err(FU_ERR_ARG, "null argument");
+    *out = bt->times;
     return FU_OK;
 }
 
@@ -1354,9 +1383,10 @@ extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
 }
 
 // One frame of each of `n` systems as one sequence of batched launches: the same stages as
-// queue_frame_device, each launched once for all sessions.
-extern "C" fu_status fu_batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
-                                   const uint8_t* const* rgb, fu_stream s) {
+// queue_frame_device, each launched once for all sessions.  With `brief` set, the frames'
+// results are gathered into brief[0..n).
+fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
+                         const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s) {
     if (!bt || !sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_batch_grab: null argument");
     if (n < 1 || n > kMaxBatch) return set_err(FU_ERR_ARG, "fu_batch_grab: %d frames (1..%d)", n, kMaxBatch);
     BatchLaunch b;
@@ -1379,27 +1409,77 @@ extern "C" fu_status fu_batch_grab(fu_batch* bt, int n, fu_system* const* sys, c
     }
     const bool track = b.first != (n == 64 ? ~0ull : ((1ull << n) - 1));
     const SessDesc* h = bt->h_desc.data();
-    hipStream_t hs = S(s);
+    hipStream_t hs = s;
+    const bool prof = bt->prof;
+    auto mark = [&](int k) {
+        if (prof) (void)hipEventRecord(bt->ev[k], hs);
+    };
+    mark(0);
     for (int i = 0; i < n; ++i) b.in[i] = depth[i];
     if (track) {
         launch_b_resize(b, h, 1, hs);
         launch_b_resize(b, h, 2, hs);
     }
+    mark(FU_STAGE_RESIZE + 1);
     launch_b_filter(b, h, hs);
     launch_b_pyrdown(b, h, 1, hs);
     launch_b_pyrdown(b, h, 2, hs);
     launch_b_vmap(b, h, hs);
     launch_b_nmap(b, h, hs);
+    mark(FU_STAGE_PYRAMID + 1);
     if (track) {
         launch_b_track_begin(b, h, hs);
         for (int l = 2; l >= 0; --l)
             for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
     }
     launch_b_track_commit(b, h, hs);
+    mark(FU_STAGE_ICP + 1);
     for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
-    launch_b_fuse(b, h, bt->fmt, hs);
+    launch_b_alloc(b, h, hs);
+    mark(FU_STAGE_ALLOC + 1);
+    launch_b_visible(b, h, hs);
+    mark(FU_STAGE_VISIBLE + 1);
+    launch_b_integrate(b, h, bt->fmt, hs);
+    mark(FU_STAGE_INTEGRATE + 1);
+    launch_b_zrange(b, h, hs);
+    mark(FU_STAGE_ZRANGE + 1);
     launch_b_raycast(b, h, bt->fmt, hs);
+    mark(FU_STAGE_RAYCAST + 1);
+    if (brief) launch_b_results(b, brief, hs);
     LAUNCHCHK();
     for (int i = 0; i < n; ++i) sys[i]->frame++;
+    if (prof) {
+        // profiling steps synchronise: stage times, visible blocks and ICP work of the step
+        HIPCHK(hipEventSynchronize(bt->ev[FU_NUM_STAGES]));
+        for (int k = 0; k < FU_NUM_STAGES; ++k) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, bt->ev[k], bt->ev[k + 1]));
+            bt->times.ms[k] += ms;
+        }
+        bt->times.steps += 1;
+        for (int i = 0; i < n; ++i) {
+            fu_system* x = sys[i];
+            unsigned c[C_COUNT];
+            TrackState t;
+            HIPCHK(hipMemcpy(c, x->map->v.ctr, sizeof(c), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&t, x->trk->st, sizeof(t), hipMemcpyDeviceToHost));
+            bt->times.frames += 1;
+            bt->times.n_visible += c[C_VISIBLE];
+            bt->times.n_new += c[C_NEW];
+            bt->times.icp_iterations += t.iters_done;
+            int left = t.iters_done;
+            for (int l = 2; l >= 0 && left > 0; --l) {
+                const int it = std::min(left, x->p.iters[l]);
+                bt->times.icp_pixels += (double)it * x->lc[l] * x->lr[l];
+                left -= it;
+            }
+        }
+    }
     return FU_OK;
+}
+
+
+extern "C" fu_status fu_batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
+                                   const uint8_t* const* rgb, fu_stream s) {
+    return batch_grab(bt, n, sys, depth, rgb, nullptr, S(s));
 }

@@ -10,6 +10,7 @@
 
 #include "fu_device.h"
 #include "fu_kernels.h"
+#include "../../include/fusion_amd.h"
 
 namespace fu {
 
@@ -116,6 +117,18 @@ enum BatchTiling : int {
 // fills b.pre, b.gx and b.uniform for one launch; returns the total work-group count
 int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, int fixed);
 
+// per-frame result of a batched step, gathered on the device and copied back in one transfer
+struct FrameBrief {
+    double pose[16];
+    int status;
+    float icp_error;
+    int icp_count;
+    unsigned n_visible;
+    unsigned n_alloc;
+    unsigned flags;
+    unsigned pad[2];
+};
+
 // pyramid.hip
 void launch_b_resize(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
 void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
@@ -126,9 +139,19 @@ void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStream_t s);
 void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_results(BatchLaunch& b, FrameBrief* out, hipStream_t s);
 // map.hip
-void launch_b_fuse(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
+void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_visible(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_integrate(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
 // raycast.hip
+void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
+
+// capi.hip: batch_grab with an optional device result gather (brief[0..n)), and the error
+// slot behind fu_last_error for the other translation units of the library
+fu_status batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
+                     const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s);
+fu_status api_error(fu_status code, const char* fmt, ...);
 
 }  // namespace fu

@@ -222,6 +222,25 @@ __global__ void kb_track_begin(BatchLaunch b) {
 }
 __global__ void kb_track_commit(BatchLaunch b) { track_commit_body(b.desc[b.sess[blockIdx.x]].st); }
 
+// tracking result and map counters of every slot, into out[slot]
+__global__ void kb_results(BatchLaunch b, FrameBrief* out) {
+    const int s = blockIdx.x;
+    const SessDesc& d = b.desc[b.sess[s]];
+    const TrackState* st = d.st;
+    FrameBrief* o = out + s;
+    const int t = threadIdx.x;
+    if (t < 16) o->pose[t] = st->next_pose[t];
+    if (t == 16) o->status = st->status;
+    if (t == 17) o->icp_error = st->icp_error;
+    if (t == 18) o->icp_count = st->count;
+    if (t == 19) o->n_visible = d.m.ctr[C_VISIBLE];
+    if (t == 20) o->n_alloc = min(d.m.ctr[C_ALLOC], d.m.num_blocks);
+    if (t == 21) o->flags = d.m.ctr[C_FLAGS];
+}
+void launch_b_results(BatchLaunch& b, FrameBrief* out, hipStream_t s) {
+    hipLaunchKernelGGL(kb_results, dim3(b.n), dim3(64), 0, s, b, out);
+}
+
 void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     (void)h;
     hipLaunchKernelGGL(kb_track_begin, dim3(b.n), dim3(64), 0, s, b);

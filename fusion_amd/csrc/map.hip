@@ -464,15 +464,19 @@ static int per_slot(int total, int n, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-void launch_b_fuse(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     hipLaunchKernelGGL(kb_begin_pass, dim3(b.n), dim3(64), 0, s, b);
     int grid = batch_layout(b, h, BT_TILES, 0, 0);
     hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(256), 0, s, b);
     grid = batch_layout(b, h, BT_FIXED, 0, per_slot(2048, b.n, 16, 512));
     hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);
-    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(8192, b.n, 64, 1024));
+}
+void launch_b_visible(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(8192, b.n, 64, 1024));
     hipLaunchKernelGGL(kb_visible, dim3(grid), dim3(256), 0, s, b);
-    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(16384, b.n, 256, 8192));
+}
+void launch_b_integrate(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(16384, b.n, 256, 8192));
     switch (fmt) {
         case VF_F32_W8: hipLaunchKernelGGL(kb_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
         case VF_F32_W16: hipLaunchKernelGGL(kb_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;

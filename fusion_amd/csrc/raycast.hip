@@ -299,13 +299,15 @@ __global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
     raycast_body<F>(g, cload(d.m), cload(d.rc));
 }
 
-void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     int grid = batch_layout(b, h, BT_ZTILES, 0, 0);
     hipLaunchKernelGGL(kb_zrange_init, dim3(grid), dim3(256), 0, s, b);
     const int per = b.n > 0 ? 4096 / b.n : 64;
     grid = batch_layout(b, h, BT_FIXED, 0, per < 32 ? 32 : (per > 512 ? 512 : per));
     hipLaunchKernelGGL(kb_project, dim3(grid), dim3(256), 0, s, b);
-    grid = batch_layout(b, h, BT_TILES, 0, 0);
+}
+void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+    const int grid = batch_layout(b, h, BT_TILES, 0, 0);
     switch (fmt) {
         case VF_F32_W8: hipLaunchKernelGGL(kb_raycast<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
         case VF_F32_W16: hipLaunchKernelGGL(kb_raycast<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;

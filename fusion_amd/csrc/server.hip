@@ -1,0 +1,385 @@
+// Serving many reconstruction sessions with continuous batching.
+//
+// Each session is a system (map, pose, pyramid, last raycast, all resident in HBM).  Sessions
+// are spread over a few lanes; a lane owns a HIP stream and a batch context.  A lane forms a
+// step from the oldest pending frame of each of its sessions whenever fewer than
+// `max_inflight` of its steps are on the device, so frames that arrive while steps run join
+// the next step.  Steps of different lanes run concurrently on the device.  A step ends with
+// a gather of every frame's result (pose, status, counters) into one device array and one
+// copy to pinned host memory; an event per step tells the host when it is done.
+//
+// The per-session order of frames is the stream order of its lane.  The reference has no
+// counterpart: it drives one System from its main loop (Core/System.cc:92-138).
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <vector>
+
+#include "../../include/fusion_amd.h"
+#include "fu_batch.h"
+
+using namespace fu;
+
+namespace {
+
+struct Req {
+    uint64_t tag;
+    int session;
+    const uint16_t* depth;
+    const uint8_t* rgb;
+    double t_submit;
+};
+
+struct Step {
+    int n = 0;
+    int slot = 0;
+    Req req[kMaxBatch];
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    fu_batch* batch = nullptr;
+    std::vector<int> sessions;
+    std::deque<Step> inflight;
+    std::vector<hipEvent_t> ev;  // one per ring slot
+    FrameBrief* d_res = nullptr;  // ring of max_inflight x kMaxBatch
+    FrameBrief* h_res = nullptr;  // pinned mirror
+    uint64_t launched = 0;
+    double pixels = 0;  // load measure for placing sessions
+};
+
+struct Session {
+    fu_system* sys = nullptr;
+    int lane = -1;
+    size_t bytes = 0;
+    double pixels = 0;
+    std::deque<Req> pending;  // submitted, not yet in a step
+    int in_steps = 0;         // frames in launched steps
+};
+
+}  // namespace
+
+struct fu_server {
+    fu_server_params p;
+    std::vector<Lane> lanes;
+    std::vector<Session> sess;
+    uint64_t used = 0;
+    std::deque<fu_completion> done;
+    int64_t outstanding = 0;
+    std::chrono::steady_clock::time_point t0;
+};
+
+#define SV_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return api_error(FU_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" void fu_server_params_default(fu_server_params* p) {
+    p->max_batch = 32;
+    p->max_inflight = 2;
+    p->lanes = 2;
+    p->block_budget = 0;
+}
+
+extern "C" double fu_server_now_ms(fu_server* sv) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sv->t0).count();
+}
+
+extern "C" fu_status fu_server_destroy(fu_server* sv) {
+    if (!sv) return FU_OK;
+    for (Lane& L : sv->lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+    for (Session& s : sv->sess)
+        if (s.sys) fu_system_destroy(s.sys);
+    for (Lane& L : sv->lanes) {
+        fu_batch_destroy(L.batch);
+        for (hipEvent_t e : L.ev) (void)hipEventDestroy(e);
+        (void)hipFree(L.d_res);
+        (void)hipHostFree(L.h_res);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    delete sv;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_create(const fu_server_params* p, fu_server** out) {
+    if (!p || !out) return api_error(FU_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->max_batch < 1 || p->max_batch > kMaxBatch) return api_error(FU_ERR_ARG, "max_batch %d not in 1..%d", p->max_batch, kMaxBatch);
+    if (p->max_inflight < 1 || p->max_inflight > 16) return api_error(FU_ERR_ARG, "max_inflight %d not in 1..16", p->max_inflight);
+    if (p->lanes < 1 || p->lanes > 8) return api_error(FU_ERR_ARG, "lanes %d not in 1..8", p->lanes);
+    fu_server* sv = new fu_server();
+    sv->p = *p;
+    sv->t0 = std::chrono::steady_clock::now();
+    sv->lanes.resize(p->lanes);
+    for (Lane& L : sv->lanes) {
+        fu_status st = fu_batch_create(4096, &L.batch);
+        if (st) {
+            fu_server_destroy(sv);
+            return st;
+        }
+        const size_t nb = sizeof(FrameBrief) * kMaxBatch * p->max_inflight;
+        hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipMalloc(&L.d_res, nb);
+        if (e == hipSuccess) e = hipHostMalloc(&L.h_res, nb, hipHostMallocDefault);
+        for (int k = 0; k < p->max_inflight && e == hipSuccess; ++k) {
+            hipEvent_t ev;
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e == hipSuccess) L.ev.push_back(ev);
+        }
+        if (e != hipSuccess) {
+            fu_server_destroy(sv);
+            return api_error(FU_ERR_HIP, "server lane set-up: %s", hipGetErrorString(e));
+        }
+    }
+    *out = sv;
+    return FU_OK;
+}
+
+static Session* get_session(fu_server* sv, int32_t id) {
+    if (!sv || id < 0 || id >= (int32_t)sv->sess.size() || !sv->sess[id].sys) return nullptr;
+    return &sv->sess[id];
+}
+
+extern "C" fu_status fu_server_open(fu_server* sv, const fu_system_params* p, int32_t* session) {
+    if (!sv || !p || !session) return api_error(FU_ERR_ARG, "null argument");
+    const size_t bytes = fu_map_bytes(&p->map);
+    if (bytes == 0) return api_error(FU_ERR_ARG, "bad map parameters");
+    if (sv->p.block_budget && sv->used + bytes > sv->p.block_budget)
+        return api_error(FU_ERR_BUDGET, "session needs %zu bytes, %llu of %llu in use", bytes,
+                         (unsigned long long)sv->used, (unsigned long long)sv->p.block_budget);
+    Session s;
+    fu_status st = fu_system_create(p, &s.sys);
+    if (st) return st;
+    // the lane with the least pixel load per step
+    int best = 0;
+    for (int i = 1; i < (int)sv->lanes.size(); ++i)
+        if (sv->lanes[i].pixels < sv->lanes[best].pixels) best = i;
+    if ((st = fu_batch_attach(sv->lanes[best].batch, s.sys))) {
+        fu_system_destroy(s.sys);
+        return st;
+    }
+    s.lane = best;
+    s.bytes = bytes;
+    s.pixels = (double)p->cols * p->rows;
+    sv->lanes[best].pixels += s.pixels;
+    const int id = (int)sv->sess.size();
+    sv->lanes[best].sessions.push_back(id);
+    sv->sess.push_back(std::move(s));
+    sv->used += bytes;
+    *session = id;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_close(fu_server* sv, int32_t id) {
+    Session* s = get_session(sv, id);
+    if (!s) return api_error(FU_ERR_ARG, "no session %d", id);
+    if (!s->pending.empty() || s->in_steps) return api_error(FU_ERR_STATE, "session %d has frames in flight", id);
+    Lane& L = sv->lanes[s->lane];
+    L.sessions.erase(std::find(L.sessions.begin(), L.sessions.end(), id));
+    L.pixels -= s->pixels;
+    fu_system_destroy(s->sys);  // detaches it from the lane's batch context
+    s->sys = nullptr;
+    sv->used -= s->bytes;
+    return FU_OK;
+}
+
+extern "C" fu_system* fu_server_system(fu_server* sv, int32_t id) {
+    Session* s = get_session(sv, id);
+    return s ? s->sys : nullptr;
+}
+
+extern "C" fu_status fu_server_submit(fu_server* sv, int32_t id, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
+                                      double t_submit_ms) {
+    Session* s = get_session(sv, id);
+    if (!s) return api_error(FU_ERR_ARG, "no session %d", id);
+    if (!depth || !rgb) return api_error(FU_ERR_ARG, "null frame");
+    s->pending.push_back(Req{tag, id, depth, rgb, t_submit_ms < 0 ? fu_server_now_ms(sv) : t_submit_ms});
+    sv->outstanding++;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_pending(fu_server* sv, int32_t* frames) {
+    if (!sv || !frames) return api_error(FU_ERR_ARG, "null argument");
+    *frames = (int32_t)sv->outstanding;
+    return FU_OK;
+}
+
+// finished steps of a lane, in launch order
+static fu_status retire(fu_server* sv, Lane& L) {
+    while (!L.inflight.empty()) {
+        Step& st = L.inflight.front();
+        const hipError_t e = hipEventQuery(L.ev[st.slot]);
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) return api_error(FU_ERR_HIP, "step completion: %s", hipGetErrorString(e));
+        const double now = fu_server_now_ms(sv);
+        const FrameBrief* r = L.h_res + (size_t)st.slot * kMaxBatch;
+        for (int i = 0; i < st.n; ++i) {
+            fu_completion c;
+            memset(&c, 0, sizeof(c));
+            c.tag = st.req[i].tag;
+            c.session = st.req[i].session;
+            c.track_status = r[i].status;
+            memcpy(c.pose, r[i].pose, sizeof(c.pose));
+            c.icp_error = r[i].icp_error;
+            c.n_visible = r[i].n_visible;
+            c.n_allocated = r[i].n_alloc;
+            c.flags = r[i].flags;
+            c.t_submit_ms = st.req[i].t_submit;
+            c.t_done_ms = now;
+            sv->done.push_back(c);
+            sv->sess[c.session].in_steps--;
+        }
+        L.inflight.pop_front();
+    }
+    return FU_OK;
+}
+
+// next steps of a lane while it has room: the oldest pending frame of each session, oldest
+// sessions first
+static fu_status launch(fu_server* sv, Lane& L) {
+    std::vector<int> cand;
+    while ((int)L.inflight.size() < sv->p.max_inflight) {
+        cand.clear();
+        for (int id : L.sessions)
+            if (!sv->sess[id].pending.empty()) cand.push_back(id);
+        if (cand.empty()) break;
+        if ((int)cand.size() > sv->p.max_batch) {
+            std::partial_sort(cand.begin(), cand.begin() + sv->p.max_batch, cand.end(), [&](int a, int b) {
+                return sv->sess[a].pending.front().t_submit < sv->sess[b].pending.front().t_submit;
+            });
+            cand.resize(sv->p.max_batch);
+        }
+        Step st;
+        st.n = (int)cand.size();
+        st.slot = (int)(L.launched % (uint64_t)sv->p.max_inflight);
+        fu_system* sys[kMaxBatch];
+        const uint16_t* dp[kMaxBatch];
+        const uint8_t* cp[kMaxBatch];
+        for (int i = 0; i < st.n; ++i) {
+            Session& s = sv->sess[cand[i]];
+            st.req[i] = s.pending.front();
+            s.pending.pop_front();
+            s.in_steps++;
+            sys[i] = s.sys;
+            dp[i] = st.req[i].depth;
+            cp[i] = st.req[i].rgb;
+        }
+        FrameBrief* dres = L.d_res + (size_t)st.slot * kMaxBatch;
+        FrameBrief* hres = L.h_res + (size_t)st.slot * kMaxBatch;
+        fu_status e = batch_grab(L.batch, st.n, sys, dp, cp, dres, L.stream);
+        if (e) return e;
+        SV_HIP(hipMemcpyAsync(hres, dres, sizeof(FrameBrief) * st.n, hipMemcpyDeviceToHost, L.stream));
+        SV_HIP(hipEventRecord(L.ev[st.slot], L.stream));
+        L.launched++;
+        L.inflight.push_back(st);
+    }
+    return FU_OK;
+}
+
+static fu_status progress(fu_server* sv) {
+    for (Lane& L : sv->lanes) {
+        fu_status st = retire(sv, L);
+        if (st) return st;
+        if ((st = launch(sv, L))) return st;
+    }
+    return FU_OK;
+}
+
+static int take_done(fu_server* sv, fu_completion* out, int cap) {
+    int k = 0;
+    while (k < cap && !sv->done.empty()) {
+        out[k++] = sv->done.front();
+        sv->done.pop_front();
+    }
+    sv->outstanding -= k;
+    return k;
+}
+
+extern "C" fu_status fu_server_poll(fu_server* sv, fu_completion* out, int32_t cap, int32_t* n_out) {
+    if (!sv || !n_out || (cap > 0 && !out)) return api_error(FU_ERR_ARG, "null argument");
+    fu_status st = progress(sv);
+    *n_out = take_done(sv, out, cap);
+    return st;
+}
+
+extern "C" fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* frames, int32_t n, int32_t closed_loop,
+                                         fu_completion* out) {
+    if (!sv || (n > 0 && (!frames || !out))) return api_error(FU_ERR_ARG, "null argument");
+    for (int32_t i = 0; i < n; ++i)
+        if (!get_session(sv, frames[i].session) || !frames[i].depth || !frames[i].rgb)
+            return api_error(FU_ERR_ARG, "trace frame %d: bad session or frame", i);
+    if (sv->outstanding) return api_error(FU_ERR_STATE, "run_trace needs an idle server");
+    std::vector<int32_t> order(n);
+    for (int32_t i = 0; i < n; ++i) order[i] = i;
+    // per-session queues keep the given order; open loop releases frames by arrival time
+    std::vector<std::deque<int32_t>> per(sv->sess.size());
+    for (int32_t i : order) per[frames[i].session].push_back(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return frames[a].arrival_ms < frames[b].arrival_ms; });
+    const double t0 = fu_server_now_ms(sv);
+    auto submit = [&](int32_t i, double t) {
+        return fu_server_submit(sv, frames[i].session, frames[i].depth, frames[i].rgb, frames[i].tag, t);
+    };
+    fu_status st = FU_OK;
+    int32_t next = 0, got = 0;
+    if (closed_loop)
+        for (auto& q : per)
+            if (!q.empty()) {
+                if ((st = submit(q.front(), -1.0))) return st;
+                q.pop_front();
+            }
+    std::vector<fu_completion> buf(256);
+    while (got < n) {
+        if (!closed_loop) {
+            const double now = fu_server_now_ms(sv) - t0;
+            while (next < n && frames[order[next]].arrival_ms <= now) {
+                if ((st = submit(order[next], t0 + frames[order[next]].arrival_ms))) return st;
+                ++next;
+            }
+        }
+        int32_t k = 0;
+        if ((st = fu_server_poll(sv, buf.data(), (int32_t)buf.size(), &k))) return st;
+        for (int32_t j = 0; j < k; ++j) {
+            out[got++] = buf[j];
+            if (closed_loop) {
+                auto& q = per[buf[j].session];
+                if (!q.empty()) {
+                    if ((st = submit(q.front(), -1.0))) return st;
+                    q.pop_front();
+                }
+            }
+        }
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_set_profiling(fu_server* sv, int enable) {
+    if (!sv) return api_error(FU_ERR_ARG, "null server");
+    for (Lane& L : sv->lanes) {
+        fu_status st = fu_batch_set_profiling(L.batch, enable);
+        if (st) return st;
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_get_stage_times(fu_server* sv, fu_stage_times* out) {
+    if (!sv || !out) return api_error(FU_ERR_ARG, "null argument");
+    memset(out, 0, sizeof(*out));
+    for (Lane& L : sv->lanes) {
+        fu_stage_times t;
+        fu_status st = fu_batch_get_stage_times(L.batch, &t);
+        if (st) return st;
+        for (int k = 0; k < FU_NUM_STAGES; ++k) out->ms[k] += t.ms[k];
+        out->frames += t.frames;
+        out->n_visible += t.n_visible;
+        out->n_new += t.n_new;
+        out->icp_iterations += t.icp_iterations;
+        out->icp_pixels += t.icp_pixels;
+        out->steps += t.steps;
+    }
+    return FU_OK;
+}

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libfusion_amd.so")
 SCENE_PATH = os.path.join(LIB_DIR, "libfu_scene.so")
 
 FU_OK = 0
+FU_ERR_BUDGET = -5
 FU_TRACK_RUNNING, FU_TRACK_OK, FU_TRACK_FAILED, FU_TRACK_REJECTED, FU_TRACK_FIRST = 0, 1, 2, 3, 4
 FU_VOXEL_F32_W8, FU_VOXEL_F32_W16, FU_VOXEL_F16_W8 = 0, 1, 2
 FLAGS = {1: "heap_full", 2: "excess_full", 4: "newset_full", 8: "coord_range", 16: "render_full", 32: "tri_full"}
@@ -67,7 +68,7 @@ class SystemBuffers(C.Structure):
 
 class StageTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("frames", C.c_uint32), ("n_visible", C.c_double), ("n_new", C.c_double),
-                ("icp_iterations", C.c_double), ("icp_pixels", C.c_double)]
+                ("icp_iterations", C.c_double), ("icp_pixels", C.c_double), ("steps", C.c_double)]
 
 
 STAGES = ["resize", "pyramid", "icp", "alloc", "visible", "integrate", "zrange", "raycast"]
@@ -82,6 +83,23 @@ UP = C.POINTER(C.c_uint32)
 IP = C.POINTER(C.c_int32)
 
 # name -> (restype, argtypes); every function declared in include/fusion_amd.h
+class ServerParams(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("max_inflight", C.c_int32), ("lanes", C.c_int32),
+                ("block_budget", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("session", C.c_int32), ("track_status", C.c_int32),
+                ("pose", C.c_double * 16), ("icp_error", C.c_float), ("n_visible", C.c_uint32),
+                ("n_allocated", C.c_uint32), ("flags", C.c_uint32), ("t_submit_ms", C.c_double),
+                ("t_done_ms", C.c_double)]
+
+
+class TraceFrame(C.Structure):
+    _fields_ = [("session", C.c_int32), ("depth", C.c_void_p), ("rgb", C.c_void_p), ("arrival_ms", C.c_double),
+                ("tag", C.c_uint64)]
+
+
 SIGNATURES = {
     "fu_last_error": (C.c_char_p, []),
     "fu_version": (C.c_char_p, []),
@@ -128,6 +146,21 @@ SIGNATURES = {
     "fu_batch_attach": (I, [P, P]),
     "fu_batch_detach": (I, [P, P]),
     "fu_batch_grab": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
+    "fu_batch_set_profiling": (I, [P, I]),
+    "fu_batch_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_server_set_profiling": (I, [P, I]),
+    "fu_server_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_server_params_default": (None, [C.POINTER(ServerParams)]),
+    "fu_server_create": (I, [C.POINTER(ServerParams), C.POINTER(P)]),
+    "fu_server_destroy": (I, [P]),
+    "fu_server_open": (I, [P, C.POINTER(SystemParams), C.POINTER(I)]),
+    "fu_server_close": (I, [P, I]),
+    "fu_server_system": (P, [P, I]),
+    "fu_server_now_ms": (C.c_double, [P]),
+    "fu_server_submit": (I, [P, I, P, P, C.c_uint64, C.c_double]),
+    "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),
+    "fu_server_pending": (I, [P, C.POINTER(I)]),
+    "fu_server_run_trace": (I, [P, C.POINTER(TraceFrame), I, I, C.POINTER(Completion)]),
 }
 
 SCENE_SIGNATURES = {

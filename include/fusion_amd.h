@@ -283,6 +283,7 @@ typedef struct fu_stage_times {
     double n_new;             /* summed newly allocated blocks */
     double icp_iterations;    /* summed ICP iterations run */
     double icp_pixels;        /* summed pixel-iterations of ICP */
+    double steps;             /* profiled launches of the stage sequence (frames, or batched steps) */
 } fu_stage_times;
 fu_status fu_system_set_profiling(fu_system* sys, int enable);
 fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* out);
@@ -304,6 +305,70 @@ fu_status fu_batch_detach(fu_batch* b, fu_system* sys);
  * system's size); asynchronous on `s` */
 fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_t* const* depth,
                         const uint8_t* const* rgb, fu_stream s);
+/* per-stage HIP-event times of batched steps, as fu_system_set_profiling; a profiled step
+ * synchronises at its end.  ms[] are per step, summed over steps; the counts sum over frames. */
+fu_status fu_batch_set_profiling(fu_batch* b, int enable);
+fu_status fu_batch_get_stage_times(fu_batch* b, fu_stage_times* out);
+
+/* ---- serving many reconstruction sessions -------------------------------------------------
+ * A server keeps every session's map, pose, pyramid and last raycast resident in HBM and
+ * processes frames as they arrive (continuous batching): sessions are spread over `lanes`
+ * HIP streams; on each lane, whenever fewer than `max_inflight` steps are queued, the oldest
+ * pending frame of each session (up to `max_batch` sessions, oldest first) forms the next
+ * batched step, so frames arriving while others run join the next step and the lanes' steps
+ * overlap on the device.  Sessions are admitted while the sum of their map sizes
+ * (fu_map_bytes) stays within `block_budget`.  Times are host-clock milliseconds since the
+ * server was created.  Not thread-safe: one host thread drives a server.
+ * No reference counterpart: the reference drives one System from its main loop
+ * (Core/System.cc:92-138). */
+#define FU_ERR_BUDGET (-5) /* session not admitted: voxel-block budget exhausted */
+typedef struct fu_server fu_server;
+typedef struct {
+    int32_t max_batch;     /* sessions per step, 1..FU_MAX_BATCH */
+    int32_t max_inflight;  /* steps queued per lane, >= 1 */
+    int32_t lanes;         /* HIP streams, 1..8 */
+    uint64_t block_budget; /* bytes of session maps admitted at once; 0 = no limit */
+} fu_server_params;
+typedef struct {
+    uint64_t tag;          /* caller's tag of the frame */
+    int32_t session;
+    int32_t track_status;  /* FU_TRACK_* */
+    double pose[16];       /* camera-to-world after the frame, row-major */
+    float icp_error;
+    uint32_t n_visible;
+    uint32_t n_allocated;
+    uint32_t flags;        /* FU_FLAG_* of the session's map */
+    double t_submit_ms;    /* arrival time the latency is measured from */
+    double t_done_ms;      /* when the host saw the frame's step complete */
+} fu_completion;
+typedef struct {
+    int32_t session;
+    const uint16_t* depth; /* device-resident frame of the session's size */
+    const uint8_t* rgb;
+    double arrival_ms;     /* offset from the start of the trace */
+    uint64_t tag;
+} fu_trace_frame;
+void fu_server_params_default(fu_server_params* p);
+fu_status fu_server_create(const fu_server_params* p, fu_server** out);
+fu_status fu_server_destroy(fu_server* sv);
+fu_status fu_server_open(fu_server* sv, const fu_system_params* p, int32_t* session); /* FU_ERR_BUDGET if full */
+fu_status fu_server_close(fu_server* sv, int32_t session);  /* its submitted frames must have completed */
+fu_system* fu_server_system(fu_server* sv, int32_t session);
+double fu_server_now_ms(fu_server* sv);
+/* queue one frame; t_submit_ms < 0 means now */
+fu_status fu_server_submit(fu_server* sv, int32_t session, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
+                           double t_submit_ms);
+/* launch steps where lanes have room and collect finished frames (at most cap) */
+fu_status fu_server_poll(fu_server* sv, fu_completion* out, int32_t cap, int32_t* n_out);
+fu_status fu_server_pending(fu_server* sv, int32_t* frames); /* submitted, not yet returned by poll */
+/* Replays a trace in real time: open loop submits each frame at its arrival offset; closed
+ * loop submits a session's next frame when its previous one completes (arrivals ignored).
+ * Fills out[0..n) in completion order and returns when every frame has completed. */
+fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* frames, int32_t n, int32_t closed_loop,
+                              fu_completion* out);
+/* stage profiling of every lane's steps (fu_batch_set_profiling); times summed over lanes */
+fu_status fu_server_set_profiling(fu_server* sv, int enable);
+fu_status fu_server_get_stage_times(fu_server* sv, fu_stage_times* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,103 @@
+"""The continuous-batching server: every session's results are the same alone and in the mix
+(bit-identical poses, maps and model maps), admission by the voxel-block budget, and the
+submit/poll interface.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fusion_amd import api as A
+from tests.helpers import same_bits
+from tests.test_gpu_batch import MAP, SESSIONS, _frames, _state, _system
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(frames):
+    res, state = [], []
+    for i, (K, fr) in enumerate(frames):
+        s = _system(i, K)
+        r = []
+        for d, c in fr:
+            s.GrabImage(d, c)
+            r.append(s.result())
+        res.append(r)
+        state.append(_state(s))
+        s.close()
+    return res, state
+
+
+def _params(i, K):
+    kind, size, seed, cols, rows, start, skip = SESSIONS[i]
+    return A.system_params(cols=cols, rows=rows, K=K, map=MAP)
+
+
+@pytest.mark.parametrize("closed_loop", [False, True])
+def test_server_same_alone_and_in_mix(gpu, closed_loop):
+    frames = _frames()
+    alone_res, alone_state = _alone(frames)
+    sv = A.Server(max_batch=3, max_inflight=2, lanes=2)
+    sids = [sv.open(_params(i, K)) for i, (K, _) in enumerate(frames)]
+    trace = []
+    rng = np.random.default_rng(5)
+    for i, (K, fr) in enumerate(frames):
+        t = float(rng.uniform(0, 5))
+        for f, (d, c) in enumerate(fr):
+            trace.append((sids[i], d, c, t, i * 1000 + f))
+            t += float(rng.exponential(4.0))  # about 250 frames/s per session, jittered
+    torch.cuda.synchronize()
+    print("trace", len(trace), "frames, closed loop", closed_loop, flush=True)
+    done = sv.run_trace(trace, closed_loop=closed_loop)
+    assert len(done) == len(trace)
+    seen = {}
+    for c in done:
+        i, f = divmod(c["tag"], 1000)
+        assert c["session"] == sids[i]
+        assert f == seen.get(i, -1) + 1, "frames of a session complete in order"
+        seen[i] = f
+        ra = alone_res[i][f]
+        assert c["track_status"] == ra["track_status"], (i, f)
+        assert np.array_equal(c["pose"], ra["pose"]), (i, f)
+        assert c["n_visible"] == ra["n_visible"], (i, f)
+        assert c["t_done_ms"] >= c["t_submit_ms"]
+    for i, sid in enumerate(sids):
+        xa, sa, wa, ca, va, na = alone_state[i]
+        xb, sb, wb, cb, vb, nb = _state(sv.system(sid))
+        assert np.array_equal(xa, xb), i
+        assert same_bits(sa, sb) and np.array_equal(wa, wb) and np.array_equal(ca, cb), i
+        assert same_bits(va, vb) and same_bits(na, nb), i
+    sv.close()
+
+
+def test_server_budget_admission(gpu):
+    p = A.system_params(map=MAP)
+    one = A.N.lib().fu_map_bytes(A.C.byref(p.map))
+    sv = A.Server(block_budget=2 * one + one // 2)
+    a = sv.open(p)
+    b = sv.open(p)
+    with pytest.raises(A.BudgetExceeded):
+        sv.open(p)
+    sv.close_session(a)
+    c = sv.open(p)  # admitted once the budget frees up
+    assert c not in (a, b)
+    sv.close()
+
+
+def test_server_submit_poll(gpu):
+    frames = _frames()
+    K, fr = frames[0]
+    sv = A.Server(max_batch=4, max_inflight=2, lanes=1)
+    sid = sv.open(_params(0, K))
+    for f, (d, c) in enumerate(fr):
+        sv.submit(sid, d, c, tag=f)
+    assert sv.pending() == len(fr)
+    with pytest.raises(A.N.FusionError):
+        sv.close_session(sid)  # frames still queued
+    done = sv.drain()
+    assert [c["tag"] for c in done] == list(range(len(fr)))
+    assert sv.pending() == 0
+    r = sv.system(sid).result()
+    assert np.array_equal(r["pose"], done[-1]["pose"])
+    sv.close_session(sid)
+    sv.close()
+    torch.cuda.synchronize()
