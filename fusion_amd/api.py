@@ -267,6 +267,35 @@ class Tracker:
                 "icp_error": r.icp_error, "count": r.count, "status": r.status, "iterations": r.iterations}
 
 
+class IcpBatch:
+    """fu_compute_se3_batch: ComputeSE3 for many trackers, each iteration one launch pair"""
+
+    def __init__(self, capacity=256):
+        _dev()
+        h = C.c_void_p()
+        check(N.lib().fu_icp_batch_create(int(capacity), C.byref(h)), "fu_icp_batch_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                N.lib().fu_icp_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+    def ComputeSE3(self, trackers, next_vmaps, next_nmaps, last_vmaps, last_nmaps, Ks, iters=(10, 5, 4),
+                   min_count=(2000, 1000, 100), thresh=1e-4):
+        """per pair: 3-level lists of device maps and (fx, fy, cx, cy); returns each tracker's result"""
+        n = len(trackers)
+        arr = lambda lists: (C.c_void_p * (3 * n))(*[t.data_ptr() for ts in lists for t in ts])
+        Kf = (C.c_float * (4 * n))(*[float(v) for K in Ks for v in K])
+        check(N.lib().fu_compute_se3_batch(self.handle, n, (C.c_void_p * n)(*[t.handle.value for t in trackers]),
+                                           arr(next_vmaps), arr(next_nmaps), arr(last_vmaps), arr(last_nmaps), Kf,
+                                           (C.c_int32 * 3)(*iters), (C.c_int32 * 3)(*min_count), thresh, _s()),
+              "fu_compute_se3_batch")
+        return [t.result() for t in trackers]
+
+
 # --------------------------------------------------------------------- system
 def system_params(cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), plain=False, use_graph=False, map=None,
                   **kw):
@@ -475,6 +504,11 @@ class Server:
     def submit(self, sid, depth, rgb, tag=0, t_submit_ms=-1.0):
         check(N.lib().fu_server_submit(self.handle, int(sid), _p(depth), _p(rgb), int(tag), float(t_submit_ms)),
               "fu_server_submit")
+
+    def submit_mesh(self, sid, vertex, normal, color, max_triangles, tag=0, t_submit_ms=-1.0):
+        """queue a mesh extraction (device buffers: float (T, 9), float (T, 9), uint8 (T, 9))"""
+        check(N.lib().fu_server_submit_mesh(self.handle, int(sid), _p(vertex), _p(normal), _p(color),
+                                            int(max_triangles), int(tag), float(t_submit_ms)), "fu_server_submit_mesh")
 
     def poll(self, cap=256):
         out = (N.Completion * cap)()

@@ -614,12 +614,12 @@ struct fu_tracker {
     int nblocks[3];
 };
 
-// One pixel per lane up to 2048 work-groups: every pixel's association is a dependent
-// gather, so latency is hidden across waves (about 5 work-groups per CU at 640x480)
-// rather than by a per-lane loop.
-static constexpr int kIcpMaxBlocks = 2048;
+// One pixel per lane (a kIcpThreads-pixel work-group each) up to 4096 work-groups (4M
+// pixels): every pixel's association is a dependent gather, so latency is hidden across waves
+// rather than by a per-lane loop, and lanes keep no running sums.  Larger frames loop.
+static constexpr int kIcpMaxBlocks = 4096;
 static int icp_blocks(int npix) {
-    int nb = (npix + 255) / 256;
+    int nb = (npix + kIcpThreads - 1) / kIcpThreads;
     if (nb < 1) nb = 1;
     if (nb > kIcpMaxBlocks) nb = kIcpMaxBlocks;
     return nb;
@@ -1482,4 +1482,105 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
 extern "C" fu_status fu_batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
                                    const uint8_t* const* rgb, fu_stream s) {
     return batch_grab(bt, n, sys, depth, rgb, nullptr, S(s));
+}
+
+fu_status fu::mesh_async(fu_system* sys, float* vertex, float* normal, uint8_t* color, uint32_t max_triangles,
+                         hipStream_t s) {
+    if (!sys || !vertex || !normal || !color) return set_err(FU_ERR_ARG, "mesh request: null argument");
+    MeshOut o;
+    o.vertex = vertex;
+    o.normal = normal;
+    o.color = color;
+    o.tri_meta = nullptr;
+    o.max_tri = max_triangles;
+    launch_mesh(sys->map->v, o, sys->map->mesh_counts, sys->map->mesh_chunk, s);
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+const unsigned* fu::system_counters(fu_system* sys) { return sys->map->v.ctr; }
+
+// ------------------------------------------------------------------------------ batched ICP
+struct fu_icp_batch {
+    int cap = 0;
+    SessDesc* d_desc = nullptr;
+    std::vector<SessDesc> h_desc;
+};
+
+extern "C" fu_status fu_icp_batch_create(int capacity, fu_icp_batch** out) {
+    if (!out) return set_err(FU_ERR_ARG, "null output");
+    *out = nullptr;
+    if (capacity < 1 || capacity > 65535) return set_err(FU_ERR_ARG, "capacity %d out of range", capacity);
+    fu_icp_batch* b = new fu_icp_batch();
+    b->cap = capacity;
+    b->h_desc.resize(capacity);
+    if (hipMalloc(&b->d_desc, sizeof(SessDesc) * capacity) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return set_err(FU_ERR_NOMEM, "descriptor allocation failed");
+    }
+    *out = b;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_icp_batch_destroy(fu_icp_batch* b) {
+    if (!b) return FU_OK;
+    (void)hipFree(b->d_desc);
+    delete b;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_compute_se3_batch(fu_icp_batch* ib, int n, fu_tracker* const* t,
+                                          const float* const* next_vmap, const float* const* next_nmap,
+                                          const float* const* last_vmap, const float* const* last_nmap,
+                                          const float* K, const int32_t iters[3], const int32_t min_count[3],
+                                          float thresh, fu_stream s) {
+    if (!ib || !t || !next_vmap || !next_nmap || !last_vmap || !last_nmap || !K || !iters || !min_count)
+        return set_err(FU_ERR_ARG, "fu_compute_se3_batch: null argument");
+    if (n < 1 || n > ib->cap) return set_err(FU_ERR_ARG, "fu_compute_se3_batch: %d pairs (capacity %d)", n, ib->cap);
+    for (int l = 0; l < 3; ++l)
+        if (iters[l] < 0) return set_err(FU_ERR_ARG, "fu_compute_se3_batch: negative iterations");
+    int last_level = -1;
+    for (int l = 2; l >= 0; --l)
+        if (iters[l] > 0) last_level = l;
+    for (int i = 0; i < n; ++i) {
+        if (!t[i]) return set_err(FU_ERR_ARG, "fu_compute_se3_batch: null tracker %d", i);
+        for (int j = 0; j < i; ++j)
+            if (t[j] == t[i]) return set_err(FU_ERR_ARG, "fu_compute_se3_batch: tracker %d appears twice", i);
+        SessDesc& d = ib->h_desc[i];
+        memset(&d, 0, sizeof(d));
+        d.st = t[i]->st;
+        d.partials = t[i]->partials;
+        d.cols = t[i]->cols;
+        d.rows = t[i]->rows;
+        d.last_level = last_level;
+        for (int l = 0; l < 3; ++l) {
+            const int k = i * 3 + l;
+            if (!next_vmap[k] || !next_nmap[k] || !last_vmap[k] || !last_nmap[k])
+                return set_err(FU_ERR_ARG, "fu_compute_se3_batch: pair %d level %d: null map", i, l);
+            d.lc[l] = t[i]->cols >> l;
+            d.lr[l] = t[i]->rows >> l;
+            d.icp[l] = make_icp(t[i], l, next_vmap[k], next_nmap[k], last_vmap[k], last_nmap[k], K + 4 * i, 1,
+                                min_count[l], 0, thresh);
+            d.icp_blocks[l] = t[i]->nblocks[l];
+            d.iters[l] = iters[l];
+        }
+    }
+    hipStream_t hs = S(s);
+    // the descriptors of this call; copied before any kernel of the call reads them
+    HIPCHK(hipMemcpyAsync(ib->d_desc, ib->h_desc.data(), sizeof(SessDesc) * n, hipMemcpyHostToDevice, hs));
+    HIPCHK(hipStreamSynchronize(hs));
+    for (int g0 = 0; g0 < n; g0 += kMaxBatch) {
+        BatchLaunch b;
+        memset(&b, 0, sizeof(b));
+        b.desc = ib->d_desc;
+        b.n = std::min(kMaxBatch, n - g0);
+        for (int i = 0; i < b.n; ++i) b.sess[i] = (unsigned short)(g0 + i);
+        const SessDesc* h = ib->h_desc.data();
+        launch_b_track_begin(b, h, hs);
+        for (int l = 2; l >= 0; --l)
+            for (int j = 0; j < iters[l]; ++j) launch_b_icp(b, h, l, j, hs);
+    }
+    LAUNCHCHK();
+    return FU_OK;
 }

@@ -153,5 +153,10 @@ void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s)
 fu_status batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
                      const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s);
 fu_status api_error(fu_status code, const char* fmt, ...);
+// mesh extraction of a system's map queued on `s` without synchronising; the triangle and
+// occupied-block counts land in the map's counters, whose device address `counters` returns
+fu_status mesh_async(fu_system* sys, float* vertex, float* normal, uint8_t* color, uint32_t max_triangles,
+                     hipStream_t s);
+const unsigned* system_counters(fu_system* sys);
 
 }  // namespace fu

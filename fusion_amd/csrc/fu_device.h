@@ -7,6 +7,17 @@
 
 namespace fu {
 
+// Global-memory view of a pointer.  Pointers read from descriptors in memory are generic to
+// the compiler (flat instructions, 64-bit VGPR addresses, no scalar loads); the explicit
+// global address space gives global loads with a scalar base, and scalar loads for uniform
+// addresses.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FU_GPTR(T) __attribute__((address_space(1))) T*
+#else
+#define FU_GPTR(T) T*
+#endif
+#define FU_G(T, p) ((FU_GPTR(T))(p))
+
 // One hash-table record, 16 bytes, read with a single dwordx4 load.  It carries what the
 // reference's 32-byte HashEntry carries (Mapping/DeviceMap.h:20-59): the block position
 // (packed in `key`), `ptr` = voxel offset of the block (heap index * 512, -1 = free) and
@@ -87,26 +98,26 @@ struct VoxelIO {
     }
     __device__ static float sdf(const unsigned char* b, int i) {
         if constexpr (kSdfB == 2)
-            return __half2float(reinterpret_cast<const __half*>(b)[i]);
+            return __half2float(FU_G(const __half, b)[i]);
         else
-            return reinterpret_cast<const float*>(b)[i];
+            return FU_G(const float, b)[i];
     }
     __device__ static void set_sdf(unsigned char* b, int i, float v) {
         if constexpr (kSdfB == 2)
-            reinterpret_cast<__half*>(b)[i] = __float2half_rn(v);
+            FU_G(__half, b)[i] = __float2half_rn(v);
         else
-            reinterpret_cast<float*>(b)[i] = v;
+            FU_G(float, b)[i] = v;
     }
     __device__ static VoxAux aux(const unsigned char* b, int i) {
         VoxAux a;
         if constexpr (kW16) {
-            const unsigned char* c = b + 512 * 6;
-            a.w = reinterpret_cast<const unsigned short*>(b + 512 * 4)[i];
+            const auto c = FU_G(const unsigned char, b + 512 * 6);
+            a.w = FU_G(const unsigned short, b + 512 * 4)[i];
             a.r = c[i];
             a.g = c[512 + i];
             a.b = c[1024 + i];
         } else {
-            const unsigned v = reinterpret_cast<const unsigned*>(b + 512 * kSdfB)[i];
+            const unsigned v = FU_G(const unsigned, b + 512 * kSdfB)[i];
             a.w = v & 0xFFu;
             a.r = (unsigned char)(v >> 8);
             a.g = (unsigned char)(v >> 16);
@@ -116,28 +127,28 @@ struct VoxelIO {
     }
     __device__ static void set_aux(unsigned char* b, int i, VoxAux a) {
         if constexpr (kW16) {
-            unsigned char* c = b + 512 * 6;
-            reinterpret_cast<unsigned short*>(b + 512 * 4)[i] = (unsigned short)a.w;
+            const auto c = FU_G(unsigned char, b + 512 * 6);
+            FU_G(unsigned short, b + 512 * 4)[i] = (unsigned short)a.w;
             c[i] = a.r;
             c[512 + i] = a.g;
             c[1024 + i] = a.b;
         } else {
-            reinterpret_cast<unsigned*>(b + 512 * kSdfB)[i] =
+            FU_G(unsigned, b + 512 * kSdfB)[i] =
                 (a.w & 0xFFu) | ((unsigned)a.r << 8) | ((unsigned)a.g << 16) | ((unsigned)a.b << 24);
         }
     }
     __device__ static unsigned weight(const unsigned char* b, int i) {
         if constexpr (kW16)
-            return reinterpret_cast<const unsigned short*>(b + 512 * 4)[i];
+            return FU_G(const unsigned short, b + 512 * 4)[i];
         else
-            return reinterpret_cast<const unsigned*>(b + 512 * kSdfB)[i] & 0xFFu;
+            return FU_G(const unsigned, b + 512 * kSdfB)[i] & 0xFFu;
     }
 };
 
 // ---- hash lookup (DeviceMap::FindEntry, DeviceMap.cu:122-135) ----------------------------
 __device__ __forceinline__ HEntry load_entry(const HEntry* e) {
     // one 16-byte load
-    uint4 v = *reinterpret_cast<const uint4*>(e);
+    const uint4 v = *FU_G(const uint4, e);
     HEntry h;
     h.key = (unsigned long long)v.x | ((unsigned long long)v.y << 32);
     h.ptr = (int)v.z;
@@ -161,7 +172,7 @@ FU_HD unsigned long long mix64(unsigned long long k) {
 FU_HD unsigned occ_bit(const MapView& m, uint32_t raw) { return (raw * 0x9E3779B1u) >> m.occ_shift; }
 __device__ __forceinline__ bool maybe_allocated(const MapView& m, uint32_t raw) {
     const unsigned h = occ_bit(m, raw);
-    return (m.occ[h >> 5] >> (h & 31)) & 1u;
+    return (FU_G(const unsigned, m.occ)[h >> 5] >> (h & 31)) & 1u;
 }
 
 // heap index of block `bp`, or -1 if it is not allocated
@@ -218,6 +229,44 @@ __device__ __forceinline__ float wave_sum(float v) {
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
     return v;
+}
+
+// Sums 32 per-lane values over the 64 lanes of a wave as a reduce-scatter: each step adds
+// partial sums of disjoint lane sets (a pairwise tree) and halves the number of values a lane
+// carries.  Step 1 swaps the wave halves (v_permlane32_swap), step 2 odd and even rows
+// (v_permlane16_swap), steps 3-6 use DPP within rows (row_ror:8, row_half_mirror, quad
+// permutes).  Afterwards lanes 2k and 2k+1 hold the total of value k; about 70 instructions
+// instead of 32 separate wave trees.  `v` is clobbered.
+__device__ __forceinline__ float wave_reduce_scatter32(float (&v)[32]) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {  // lanes 0-31 keep values 0-15, lanes 32-63 values 16-31
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 16]), false, false);
+        v[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {  // even rows keep the lower 8 of their 16, odd rows the upper 8
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 8]), false, false);
+        v[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const int lane = lane_id();
+    bool hi = (lane & 8) != 0;  // partner lane ^ 8
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float keep = hi ? v[j + 4] : v[j], send = hi ? v[j] : v[j + 4];
+        v[j] = keep + dpp_f<0x128>(send);  // row_ror:8
+    }
+    hi = (lane & 4) != 0;  // partner 7 - q within each half row
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float keep = hi ? v[j + 2] : v[j], send = hi ? v[j] : v[j + 2];
+        v[j] = keep + dpp_f<0x141>(send);  // row_half_mirror
+    }
+    hi = (lane & 2) != 0;  // partner lane ^ 2
+    {
+        const float keep = hi ? v[1] : v[0], send = hi ? v[0] : v[1];
+        v[0] = keep + dpp_f<0x4E>(send);  // quad_perm [2,3,0,1]
+    }
+    return v[0] + dpp_f<0xB1>(v[0]);  // quad_perm [1,0,3,2]
 }
 
 }  // namespace fu

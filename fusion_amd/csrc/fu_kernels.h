@@ -43,11 +43,17 @@ struct PoseF {
 
 // When `st` is set the pose comes from the tracking state and the kernel does nothing unless
 // the frame was tracked (or is the first frame).
-__device__ __forceinline__ bool pose_from_state(const TrackState* st, PoseF& p) {
+__device__ __forceinline__ bool pose_from_state(const TrackState* st_, PoseF& p) {
+    // global view: the state pointer may come from a descriptor in memory (scalar loads)
+#if defined(__HIP_DEVICE_COMPILE__)
+    const auto st = (const __attribute__((address_space(1))) TrackState*)st_;
+#else
+    const TrackState* st = st_;
+#endif
     const int s = st->status;
     if (s != TS_OK && s != TS_FIRST) return false;
-    const float* r = st->Rcurr;
-    const float* ri = st->RcurrInv;
+    const auto r = st->Rcurr;
+    const auto ri = st->RcurrInv;
     p.R = m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
     p.Rinv = m3{mk3(ri[0], ri[1], ri[2]), mk3(ri[3], ri[4], ri[5]), mk3(ri[6], ri[7], ri[8])};
     p.t = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
@@ -108,6 +114,10 @@ struct MeshOut {
     int* tri_meta;         // optional: 4 ints per triangle (voxel x, y, z, cube*16 + t)
     unsigned max_tri;
 };
+
+// work-group sizes of the ICP reduction (one pixel per lane) and of its one-group final sum
+constexpr int kIcpThreads = 1024;
+constexpr int kIcpFinalThreads = 256;
 
 struct IcpArgs {
     const f4* vcurr;

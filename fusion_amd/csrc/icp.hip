@@ -31,83 +31,143 @@ struct Kahan {
 };
 
 // ------------------------------------------------------------------ reduction kernel
-__device__ __forceinline__ m3 load_m3(const float* r) {
+__device__ __forceinline__ m3 load_m3(FU_GPTR(const float) r) {
     return m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
 }
 
-__device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
-    TrackState* st = a.st;
-    if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
-    const m3 Rc = load_m3(st->Rcurr), Rl = load_m3(st->Rlast), Rli = load_m3(st->RlastInv);
-    const f3 tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
-    const f3 tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
-    const int cols = a.cols, rows = a.rows, N = cols * rows;
+struct IcpPose {
+    m3 Rc, Rl, Rli;
+    f3 tc, tl;
+};
+
+// ICPReduce::searchPoint + getRow (Reduction.cu:227-286) for pixel i: the 7-vector
+// [-n, n x v_l, -n.(v_c - v_l)] in the last camera frame, or false
+__device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int i, float (&row)[7]) {
+    const int cols = a.cols, rows = a.rows;
     const float angle_thresh = 0.6f, dist_thresh = 0.1f;  // Reduction.cu:352-353
-    Kahan acc[29];
-#pragma unroll
-    for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
-    for (int i = g.x * blockDim.x + threadIdx.x; i < N; i += g.nx * blockDim.x) {
-        const int y = i / cols, x = i - y * cols;
-        float row[7] = {0, 0, 0, 0, 0, 0, 0};
-        bool found = false;
-        const f4 vc4 = a.vcurr[i];
-        const f3 vcurr_c = xyz(vc4);
-        if (!(isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3)) {
-            const f3 vcurr_g = add(mv(Rc, vcurr_c), tc);
-            const f3 vcurr_p = mv(Rli, sub(vcurr_g, tl));
-            const double invz = 1.0 / (double)vcurr_p.z;
-            const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
-            const int v = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
-            if (!(u < 0 || v < 0 || u >= cols || v >= rows)) {
-                const int j = v * cols + u;
-                const f3 vlast_g = add(mv(Rl, xyz(a.vlast[j])), tl);
-                const f3 ncurr_c = xyz(a.ncurr[i]);
-                const f3 ncurr_g = mv(Rc, ncurr_c);
-                const f3 nlast_c = xyz(a.nlast[j]);
-                const f3 nlast_g = mv(Rl, nlast_c);
-                const float dist = norm(sub(vlast_g, vcurr_g));
-                const float sine = norm(cross(ncurr_g, nlast_g));
-                found = sine < angle_thresh && dist <= dist_thresh && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x);
-                if (found) {
-                    const f3 nl = mv(Rli, nlast_g);
-                    const f3 vcu = mv(Rli, sub(vcurr_g, tl));
-                    const f3 vla = mv(Rli, sub(vlast_g, tl));
-                    const f3 mn = neg(nl);
-                    const f3 cr = cross(nl, vla);
-                    row[0] = mn.x;
-                    row[1] = mn.y;
-                    row[2] = mn.z;
-                    row[3] = cr.x;
-                    row[4] = cr.y;
-                    row[5] = cr.z;
-                    row[6] = dot(mn, sub(vcu, vla));
-                }
-            }
-        }
-        if (found) {
-            int k = 0;
-#pragma unroll
-            for (int r = 0; r < 7; ++r)
-#pragma unroll
-                for (int c = r; c < 7; ++c) acc[k++].add(row[r] * row[c]);
-            acc[28].add(1.0f);
-        }
-    }
-    // wave tree, then waves through LDS; one 128-byte partial row per work-group
-    __shared__ float wsum[4][29];
-    const int wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < 29; ++j) {
-        const float t = wave_sum(acc[j].val());
-        if (lane_id() == 0) wsum[wave][j] = t;
-    }
+    const auto vcurr = FU_G(const f4, a.vcurr);
+    const auto ncurr = FU_G(const f4, a.ncurr);
+    const auto vlast = FU_G(const f4, a.vlast);
+    const auto nlast = FU_G(const f4, a.nlast);
+    const f4 vc4 = vcurr[i];
+    const f3 vcurr_c = xyz(vc4);
+    if (isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
+    const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
+    const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
+    const double invz = 1.0 / (double)vcurr_p.z;
+    const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
+    const int v = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
+    if (u < 0 || v < 0 || u >= cols || v >= rows) return false;
+    const int j = v * cols + u;
+    const f3 vlast_g = add(mv(P.Rl, xyz(vlast[j])), P.tl);
+    const f3 ncurr_c = xyz(ncurr[i]);
+    const f3 ncurr_g = mv(P.Rc, ncurr_c);
+    const f3 nlast_c = xyz(nlast[j]);
+    const f3 nlast_g = mv(P.Rl, nlast_c);
+    const float dist = norm(sub(vlast_g, vcurr_g));
+    const float sine = norm(cross(ncurr_g, nlast_g));
+    if (!(sine < angle_thresh && dist <= dist_thresh && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x))) return false;
+    const f3 nl = mv(P.Rli, nlast_g);
+    const f3 vcu = mv(P.Rli, sub(vcurr_g, P.tl));
+    const f3 vla = mv(P.Rli, sub(vlast_g, P.tl));
+    const f3 mn = neg(nl);
+    const f3 cr = cross(nl, vla);
+    row[0] = mn.x;
+    row[1] = mn.y;
+    row[2] = mn.z;
+    row[3] = cr.x;
+    row[4] = cr.y;
+    row[5] = cr.z;
+    row[6] = dot(mn, sub(vcu, vla));
+    return true;
+}
+
+// sums of a work-group of NW waves: each wave's totals into LDS, then one 29-float row
+// (v[29..31] must be 0).  Value j of a wave's totals is held by lanes 2j and 2j+1 after the
+// reduce-scatter; the waves are added as a pairwise tree.
+template <int NW, class Out>
+__device__ __forceinline__ void block_rows29(float (&v)[32], float (&wsum)[NW][32], Out out) {
+    const float t = wave_reduce_scatter32(v);
+    const int lane = lane_id();
+    if (!(lane & 1)) wsum[threadIdx.x >> 6][lane >> 1] = t;
     __syncthreads();
     if (threadIdx.x < 29) {
         const int j = threadIdx.x;
-        a.partials[g.x * 32 + j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
+        float w[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = wsum[k][j];
+#pragma unroll
+        for (int h = NW / 2; h >= 1; h /= 2)
+#pragma unroll
+            for (int k = 0; k < h; ++k) w[k] = w[2 * k] + w[2 * k + 1];
+        out[j] = w[0];
     }
 }
-__global__ __launch_bounds__(256) void k_icp(IcpArgs a) { icp_body(hw_wg(), a); }
+
+__device__ __forceinline__ void icp_products(const float (&row)[7], float (&v)[32]) {
+    int k = 0;
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+#pragma unroll
+        for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
+    v[28] = 1.0f;
+}
+
+// kOne: the grid has a lane for every pixel (one pixel per lane, no per-lane sums); otherwise
+// lanes loop over pixels with compensated sums seeded exactly by their first pixel.  Both
+// give the same result when every lane has at most one pixel.
+template <bool kOne>
+__device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
+    const auto st = FU_G(const TrackState, a.st);
+    if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
+    IcpPose P;
+    P.Rc = load_m3(st->Rcurr);
+    P.Rl = load_m3(st->Rlast);
+    P.Rli = load_m3(st->RlastInv);
+    P.tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
+    P.tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
+    const int N = a.cols * a.rows;
+    __shared__ float wsum[kIcpThreads / 64][32];
+    float v[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) v[j] = 0.f;
+    int i = g.x * blockDim.x + threadIdx.x;
+    if constexpr (kOne) {
+        float row[7];
+        if (i < N && icp_row(a, P, i, row)) icp_products(row, v);
+    } else {
+        Kahan acc[29];
+#pragma unroll
+        for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
+        const int stride = g.nx * blockDim.x;
+        if (i < N) {
+            // a lane's first pixel seeds its sums: a compensated add into zero is exact
+            float row[7];
+            if (icp_row(a, P, i, row)) {
+                icp_products(row, v);
+#pragma unroll
+                for (int j = 0; j < 29; ++j) acc[j].s = v[j];
+            }
+            i += stride;
+        }
+        for (; i < N; i += stride) {
+            float row[7];
+            if (icp_row(a, P, i, row)) {
+                icp_products(row, v);
+#pragma unroll
+                for (int j = 0; j < 29; ++j) acc[j].add(v[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 29; ++j) v[j] = acc[j].val();
+        v[29] = v[30] = v[31] = 0.f;
+    }
+    block_rows29(v, wsum, FU_G(float, a.partials) + g.x * 32);
+}
+template <bool kOne>
+__global__ __launch_bounds__(kIcpThreads) void k_icp(IcpArgs a) {
+    icp_body<kOne>(hw_wg(), a);
+}
 
 // One work-group: sums the partial rows of k_icp (compensated per lane, then the same
 // tree), stores the 29 sums and, in mode 1, runs the FP64 update on one lane.  Running it as
@@ -117,9 +177,8 @@ __global__ __launch_bounds__(256) void k_icp(IcpArgs a) { icp_body(hw_wg(), a); 
 __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     TrackState* st = a.st;
     if (a.mode != 0 && st->status != TS_RUNNING) return;
-    __shared__ float wsum[4][29];
-    __shared__ float total[29];
-    const int wave = threadIdx.x >> 6;
+    __shared__ float wsum[kIcpFinalThreads / 64][32];
+    __shared__ float total[32];
     Kahan fin[29];
 #pragma unroll
     for (int j = 0; j < 29; ++j) fin[j] = Kahan{0.f, 0.f};
@@ -137,27 +196,25 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
 #pragma unroll
         for (int j = 0; j < 29; ++j) fin[j].add(v[j]);
     }
+    float v[32];
 #pragma unroll
-    for (int j = 0; j < 29; ++j) {
-        const float t = wave_sum(fin[j].val());
-        if (lane_id() == 0) wsum[wave][j] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 29) {
-        const int j = threadIdx.x;
-        total[j] = (wsum[0][j] + wsum[1][j]) + (wsum[2][j] + wsum[3][j]);
-    }
+    for (int j = 0; j < 29; ++j) v[j] = fin[j].val();
+    v[29] = v[30] = v[31] = 0.f;
+    block_rows29(v, wsum, total);
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
         if (a.mode == 1) icp_update(st, total, a.min_count, a.final_iter, a.thresh);
     }
 }
-__global__ __launch_bounds__(256) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
+__global__ __launch_bounds__(kIcpFinalThreads) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
 
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_icp, dim3(nblocks), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_icp_final, dim3(1), dim3(256), 0, s, a, nblocks);
+    if ((long long)nblocks * kIcpThreads >= (long long)a.cols * a.rows)
+        hipLaunchKernelGGL(k_icp<true>, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_icp<false>, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
+    hipLaunchKernelGGL(k_icp_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, nblocks);
 }
 
 // start of ComputeSE3 (Tracking.cc:305-314): last pose = tracked pose of the previous
@@ -199,16 +256,17 @@ __device__ __forceinline__ bool icp_slot_args(const BatchLaunch& b, int s, IcpAr
     return true;
 }
 
-__global__ __launch_bounds__(256) void kb_icp(BatchLaunch b) {
+template <bool kOne>
+__global__ __launch_bounds__(kIcpThreads) void kb_icp(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, s, a, nb)) return;
-    icp_body(g, a);
+    icp_body<kOne>(g, a);
 }
 
-__global__ __launch_bounds__(256) void kb_icp_final(BatchLaunch b) {
+__global__ __launch_bounds__(kIcpFinalThreads) void kb_icp_final(BatchLaunch b) {
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, blockIdx.x, a, nb)) return;
@@ -249,8 +307,16 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
     b.level = level;
     b.iter = iter;
     const int grid = batch_layout(b, h, BT_ICP, level, 0);
-    hipLaunchKernelGGL(kb_icp, dim3(grid), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(kb_icp_final, dim3(b.n), dim3(256), 0, s, b);
+    bool one = true;
+    for (int i = 0; i < b.n; ++i) {
+        const SessDesc& d = h[b.sess[i]];
+        one = one && (long long)d.icp_blocks[level] * kIcpThreads >= (long long)d.lc[level] * d.lr[level];
+    }
+    if (one)
+        hipLaunchKernelGGL(kb_icp<true>, dim3(grid), dim3(kIcpThreads), 0, s, b);
+    else
+        hipLaunchKernelGGL(kb_icp<false>, dim3(grid), dim3(kIcpThreads), 0, s, b);
+    hipLaunchKernelGGL(kb_icp_final, dim3(b.n), dim3(kIcpFinalThreads), 0, s, b);
 }
 void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     (void)h;

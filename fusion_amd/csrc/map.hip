@@ -84,7 +84,7 @@ __device__ __forceinline__ bool newset_claim(const MapView& m, unsigned long lon
 __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseArgs& p) {
     PoseF pose = p.pose;
     if (p.st && !pose_from_state(p.st, pose)) return;
-    const float* __restrict__ depth = p.depth;
+    const auto depth = FU_G(const float, p.depth);
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     bool live = x < p.cols && y < p.rows;
@@ -293,7 +293,7 @@ __device__ __forceinline__ bool block_visible(const VisArgs& v, const PoseF& pos
 __device__ __forceinline__ void visible_body(WG g, const MapView& m, const VisArgs& v) {
     PoseF pose = v.pose;
     if (v.st && !pose_from_state(v.st, pose)) return;
-    const unsigned nalloc = min(m.ctr[C_ALLOC], m.num_blocks);
+    const unsigned nalloc = min(FU_G(const unsigned, m.ctr)[C_ALLOC], m.num_blocks);
     const float scale = m.voxel_size * 8.0f;  // DeviceMap::blockWidth
     const unsigned stride = g.nx * blockDim.x;
     // uniform trip count per wave so the ballot below sees every lane
@@ -302,11 +302,11 @@ __device__ __forceinline__ void visible_body(WG g, const MapView& m, const VisAr
         const unsigned i = b0 + lane_id();
         bool vis = false;
         if (i < nalloc) {
-            const unsigned long long key = m.block_key[i];
+            const unsigned long long key = FU_G(const unsigned long long, m.block_key)[i];
             if (key != kEmptyKey) vis = block_visible(v, pose, unpack_key(key), scale);
         }
         const unsigned idx = wave_append(&m.ctr[C_VISIBLE], vis);
-        if (vis && idx < m.num_blocks) m.visible[idx] = i;
+        if (vis && idx < m.num_blocks) FU_G(unsigned, m.visible)[idx] = i;
     }
 }
 __global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) { visible_body(hw_wg(), m, v); }
@@ -322,12 +322,15 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     using IO = VoxelIO<F>;
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
-    const unsigned nvis = min(m.ctr[C_VISIBLE], m.num_blocks);
+    const unsigned nvis = min(FU_G(const unsigned, m.ctr)[C_VISIBLE], m.num_blocks);
     const int cols = a.cols, rows = a.rows;
+    const auto depth = FU_G(const float, a.depth);
+    const auto nmap = FU_G(const f4, a.nmap);
+    const auto rgb = FU_G(const unsigned char, a.rgb);
     const float thresh = m.trunc;
     for (unsigned vb = g.x; vb < nvis; vb += g.nx) {
-        const unsigned heap = m.visible[vb];
-        const i3 bp = unpack_key(m.block_key[heap]);
+        const unsigned heap = FU_G(const unsigned, m.visible)[vb];
+        const i3 bp = unpack_key(FU_G(const unsigned long long, m.block_key)[heap]);
         unsigned char* blk = IO::block(m, (int)heap);
         // each lane owns voxels li and li + 256 (index z*64 + y*8 + x); the work is split in
         // phases so that both voxels' gathers are in flight together
@@ -348,7 +351,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
         }
         float dp[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) dp[h] = a.depth[pix[h]];
+        for (int h = 0; h < 2; ++h) dp[h] = depth[pix[h]];
         bool upd[2];
         float sdf[2];
 #pragma unroll
@@ -366,10 +369,10 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int li = threadIdx.x + h * 256;
-            nl[h] = a.nmap[pix[h]];
-            cr[h] = a.rgb[pix[h] * 3 + 0];
-            cg[h] = a.rgb[pix[h] * 3 + 1];
-            cb[h] = a.rgb[pix[h] * 3 + 2];
+            nl[h] = nmap[pix[h]];
+            cr[h] = rgb[pix[h] * 3 + 0];
+            cg[h] = rgb[pix[h] * 3 + 1];
+            cb[h] = rgb[pix[h] * 3 + 2];
             sold[h] = IO::sdf(blk, li);
             old[h] = IO::aux(blk, li);
         }

@@ -15,12 +15,15 @@ namespace fu {
 constexpr float kSigSpace = 0.5f / 16.0f;
 constexpr float kSigRange = 2.0f;
 
-__device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restrict__ depth, float* __restrict__ raw,
-                                                  float* __restrict__ filt, int cols, int rows, float scale_inv,
+__device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restrict__ depth_, float* __restrict__ raw_,
+                                                  float* __restrict__ filt_, int cols, int rows, float scale_inv,
                                                   float cutoff) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
+    const auto depth = FU_G(const uint16_t, depth_);
+    const auto raw = FU_G(float, raw_);
+    const auto filt = FU_G(float, filt_);
     const float center = (float)depth[y * cols + x] * scale_inv;
     raw[y * cols + x] = center < cutoff ? center : fnan();
     // the window is [max(p-2,0), min(p+3, n-1)) in each axis: the last row/column of the
@@ -51,11 +54,13 @@ __global__ __launch_bounds__(256) void k_filter_depth(const uint16_t* __restrict
 // which mirrors the weights at the right/bottom border
 __constant__ float c_gauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
 
-__device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src, int scols, int srows,
-                                             float* __restrict__ dst, int dcols, int drows) {
+__device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src_, int scols, int srows,
+                                             float* __restrict__ dst_, int dcols, int drows) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= dcols || y >= drows) return;
+    const auto src = FU_G(const float, src_);
+    const auto dst = FU_G(float, dst_);
     const int tx = min(2 * x + 3, scols - 1);
     const int ty = min(2 * y + 3, srows - 1);
     float sum = 0.f;
@@ -77,11 +82,13 @@ __global__ __launch_bounds__(256) void k_pyrdown(const float* __restrict__ src, 
     pyrdown_body(hw_wg(), src, scols, srows, dst, dcols, drows);
 }
 
-__device__ __forceinline__ void vmap_body(WG g, const float* __restrict__ depth, f4* __restrict__ vmap, int cols,
+__device__ __forceinline__ void vmap_body(WG g, const float* __restrict__ depth_, f4* __restrict__ vmap_, int cols,
                                           int rows, float invfx, float invfy, float cx, float cy, float cutoff) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
+    const auto depth = FU_G(const float, depth_);
+    const auto vmap = FU_G(f4, vmap_);
     const float z = depth[y * cols + x];
     f4 v;
     // `z > 0.1` compares in double (Pyrdown.cu:69)
@@ -101,10 +108,12 @@ __global__ __launch_bounds__(256) void k_vmap(const float* __restrict__ depth, f
     vmap_body(hw_wg(), depth, vmap, cols, rows, invfx, invfy, cx, cy, cutoff);
 }
 
-__device__ __forceinline__ void nmap_body(WG g, const f4* __restrict__ vmap, f4* __restrict__ nmap, int cols, int rows) {
+__device__ __forceinline__ void nmap_body(WG g, const f4* __restrict__ vmap_, f4* __restrict__ nmap_, int cols, int rows) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
+    const auto vmap = FU_G(const f4, vmap_);
+    const auto nmap = FU_G(f4, nmap_);
     const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
     if (x == cols - 1 || y == rows - 1) {
         nmap[y * cols + x] = nan4;
@@ -126,11 +135,15 @@ __global__ __launch_bounds__(256) void k_nmap(const f4* __restrict__ vmap, f4* _
 
 // 2x2 mean of the vertex map, 2x2 mean of the normal map normalised in 4-D (w included,
 // VectorMath.h:340-342), NaN if any of the four is NaN
-__device__ __forceinline__ void resize_body(WG g, const f4* __restrict__ vsrc, const f4* __restrict__ nsrc, int scols,
-                                            f4* __restrict__ vdst, f4* __restrict__ ndst, int dcols, int drows) {
+__device__ __forceinline__ void resize_body(WG g, const f4* __restrict__ vsrc_, const f4* __restrict__ nsrc_, int scols,
+                                            f4* __restrict__ vdst_, f4* __restrict__ ndst_, int dcols, int drows) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= dcols || y >= drows) return;
+    const auto vsrc = FU_G(const f4, vsrc_);
+    const auto nsrc = FU_G(const f4, nsrc_);
+    const auto vdst = FU_G(f4, vdst_);
+    const auto ndst = FU_G(f4, ndst_);
     const int s0 = (2 * y) * scols + 2 * x, s1 = (2 * y + 1) * scols + 2 * x;
     const f4 v00 = vsrc[s0], v01 = vsrc[s0 + 1], v10 = vsrc[s1], v11 = vsrc[s1 + 1];
     const f4 n00 = nsrc[s0], n01 = nsrc[s0 + 1], n10 = nsrc[s1], n11 = nsrc[s1 + 1];

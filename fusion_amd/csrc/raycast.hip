@@ -30,46 +30,91 @@ __global__ void k_zrange_init(const TrackState* st, unsigned* zmin, unsigned* zm
     zrange_init_body(hw_wg(), st, zmin, zmax, n);
 }
 
+// exclusive prefix sum of `v` over the lanes of a wave, and the wave total in `total`
+__device__ __forceinline__ unsigned wave_exclusive_scan(unsigned v, unsigned& total) {
+    const int lane = lane_id();
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// Tile maps staged in LDS when they fit: every block's min/max goes to the work-group's copy,
+// which is flushed into the global maps once, only where it changed.  Min and max are exact
+// and order-free, so the maps are the same as with direct global atomics.
+constexpr int kZTilesLds = 6144;  // 48 KB of LDS for both maps; 640x480 needs 4800 tiles
+constexpr int kProjectGroups = 16;  // work-groups per session with LDS tile maps
+
+template <bool kLds>
 __device__ __forceinline__ void project_body(WG g, const MapView& m, const ZRangeArgs& a) {
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
     // a caller-given count never exceeds the list the visibility pass built
-    const unsigned built = min(m.ctr[C_VISIBLE], m.num_blocks);
+    const unsigned built = min(FU_G(const unsigned, m.ctr)[C_VISIBLE], m.num_blocks);
     const unsigned nvis = a.n_visible_override >= 0 ? min((unsigned)a.n_visible_override, built) : built;
     const int zc = a.zcols, zr = a.zrows;
-    for (unsigned i = g.x * blockDim.x + threadIdx.x; i < nvis; i += g.nx * blockDim.x) {
-        const i3 pos = unpack_key(m.block_key[m.visible[i]]);
-        // projectBlock (RenderScene.cu:54-104)
+    const int nt = zc * zr;
+    __shared__ unsigned lmin[kLds ? kZTilesLds : 1], lmax[kLds ? kZTilesLds : 1];
+    if constexpr (kLds) {
+        for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+            lmin[t] = 0xFFFFFFFFu;
+            lmax[t] = 0u;
+        }
+        __syncthreads();
+    }
+    // uniform trip count per wave: the fragment counter is advanced once per wave
+    const unsigned stride = g.nx * blockDim.x;
+    for (unsigned i0 = g.x * blockDim.x + (threadIdx.x & ~63u); i0 < nvis; i0 += stride) {
+        const unsigned i = i0 + lane_id();
+        bool live = i < nvis;
         int ulx = zc, uly = zr, lrx = -1, lry = -1;
         float zlo = a.depth_max, zhi = a.depth_min;
-        for (int c = 0; c < 8; ++c) {
-            const i3 q = mki3(pos.x + (c & 1), pos.y + ((c >> 1) & 1), pos.z + ((c >> 2) & 1));
-            // int3 * uint * float: the integer product first, then to float (VectorMath.h:219-233)
-            f3 p = i3f(mki3(q.x * 8, q.y * 8, q.z * 8), m.voxel_size);
-            p = mv(pose.Rinv, sub(p, pose.t));
-            if ((double)p.z < 2e-1) continue;
-            const float px = (a.fx * p.x / p.z + a.cx) / 8.0f;
-            const float py = (a.fy * p.y / p.z + a.cy) / 8.0f;
-            const float flx = floorf(px), clx = ceilf(px), fly = floorf(py), cly = ceilf(py);
-            if ((float)ulx > flx) ulx = (short)f2i(flx);
-            if ((float)lrx < clx) lrx = (short)f2i(clx);
-            if ((float)uly > fly) uly = (short)f2i(fly);
-            if ((float)lry < cly) lry = (short)f2i(cly);
-            if (zlo > p.z) zlo = p.z;
-            if (zhi < p.z) zhi = p.z;
+        if (live) {
+            const i3 pos = unpack_key(FU_G(const unsigned long long, m.block_key)[FU_G(const unsigned, m.visible)[i]]);
+            // projectBlock (RenderScene.cu:54-104)
+            for (int c = 0; c < 8; ++c) {
+                const i3 q = mki3(pos.x + (c & 1), pos.y + ((c >> 1) & 1), pos.z + ((c >> 2) & 1));
+                // int3 * uint * float: the integer product first, then to float (VectorMath.h:219-233)
+                f3 p = i3f(mki3(q.x * 8, q.y * 8, q.z * 8), m.voxel_size);
+                p = mv(pose.Rinv, sub(p, pose.t));
+                if ((double)p.z < 2e-1) continue;
+                const float px = (a.fx * p.x / p.z + a.cx) / 8.0f;
+                const float py = (a.fy * p.y / p.z + a.cy) / 8.0f;
+                const float flx = floorf(px), clx = ceilf(px), fly = floorf(py), cly = ceilf(py);
+                if ((float)ulx > flx) ulx = (short)f2i(flx);
+                if ((float)lrx < clx) lrx = (short)f2i(clx);
+                if ((float)uly > fly) uly = (short)f2i(fly);
+                if ((float)lry < cly) lry = (short)f2i(cly);
+                if (zlo > p.z) zlo = p.z;
+                if (zhi < p.z) zhi = p.z;
+            }
+            if (ulx < 0) ulx = 0;
+            if (uly < 0) uly = 0;
+            if (lrx >= zc) lrx = zc - 1;
+            if (lry >= zr) lry = zr - 1;
+            if (ulx > lrx || uly > lry) live = false;
+            if (zlo < a.depth_min) zlo = a.depth_min;
+            if (zhi < a.depth_min) live = false;
         }
-        if (ulx < 0) ulx = 0;
-        if (uly < 0) uly = 0;
-        if (lrx >= zc) lrx = zc - 1;
-        if (lry >= zr) lry = zr - 1;
-        if (ulx > lrx || uly > lry) continue;
-        if (zlo < a.depth_min) zlo = a.depth_min;
-        if (zhi < a.depth_min) continue;
-        const int nx = (int)ceilf(((float)lrx - (float)ulx + 1.0f) / 16.0f);
-        const int ny = (int)ceilf(((float)lry - (float)uly + 1.0f) / 16.0f);
-        const unsigned need = (unsigned)(nx * ny);
-        const unsigned off = atomicAdd(&m.ctr[C_RENDER], need);
-        if (off + need >= a.max_fragments) {
+        int nx = 0, ny = 0;
+        unsigned need = 0;
+        if (live) {
+            nx = (int)ceilf(((float)lrx - (float)ulx + 1.0f) / 16.0f);
+            ny = (int)ceilf(((float)lry - (float)uly + 1.0f) / 16.0f);
+            need = (unsigned)(nx * ny);
+        }
+        // fragment slots (createRenderingBlockList, RenderScene.cu:106-154): one atomic per wave
+        unsigned total;
+        const unsigned pre = wave_exclusive_scan(need, total);
+        unsigned base = 0;
+        if (lane_id() == 0 && total) base = atomicAdd(&m.ctr[C_RENDER], total);
+        base = __shfl(base, 0);
+        if (!live) continue;
+        if (base + pre + need >= a.max_fragments) {
             atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_RENDER_FULL);
             continue;
         }
@@ -78,17 +123,40 @@ __device__ __forceinline__ void project_body(WG g, const MapView& m, const ZRang
         const unsigned zl = __float_as_uint(zlo), zh = __float_as_uint(zhi);
         for (int ty = uly; ty <= y1; ++ty)
             for (int tx = ulx; tx <= x1; ++tx) {
-                atomicMin(&a.zmin[ty * zc + tx], zl);
-                atomicMax(&a.zmax[ty * zc + tx], zh);
+                if constexpr (kLds) {
+                    atomicMin(&lmin[ty * zc + tx], zl);
+                    atomicMax(&lmax[ty * zc + tx], zh);
+                } else {
+                    __hip_atomic_fetch_min(FU_G(unsigned, a.zmin) + ty * zc + tx, zl, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_max(FU_G(unsigned, a.zmax) + ty * zc + tx, zh, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
     }
+    if constexpr (kLds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+            const unsigned lo = lmin[t];
+            if (lo != 0xFFFFFFFFu) {
+                __hip_atomic_fetch_min(FU_G(unsigned, a.zmin) + t, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_max(FU_G(unsigned, a.zmax) + t, lmax[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
 }
-__global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a) { project_body(hw_wg(), m, a); }
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_project_blocks(MapView m, ZRangeArgs a) {
+    project_body<kLds>(hw_wg(), m, a);
+}
 
 void launch_zrange(const MapView& m, const ZRangeArgs& a, hipStream_t s) {
     const int n = a.zcols * a.zrows;
     hipLaunchKernelGGL(k_zrange_init, dim3((n + 255) / 256), dim3(256), 0, s, a.st, a.zmin, a.zmax, n);
-    hipLaunchKernelGGL(k_project_blocks, dim3(512), dim3(256), 0, s, m, a);
+    if (n <= kZTilesLds)
+        hipLaunchKernelGGL(k_project_blocks<true>, dim3(kProjectGroups), dim3(256), 0, s, m, a);
+    else
+        hipLaunchKernelGGL(k_project_blocks<false>, dim3(512), dim3(256), 0, s, m, a);
 }
 
 // ------------------------------------------------------------------------------ ray march
@@ -187,18 +255,18 @@ __device__ __forceinline__ void raycast_body(WG g, const MapView& m, const Rayca
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= a.cols || y >= a.rows) return;
     const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
-    f4* vout = a.vmap + y * a.cols + x;
-    f4* nout = a.nmap + y * a.cols + x;
-    if (a.keep_v && m.ctr[C_RENDER] == 0u) {
+    const auto vout = FU_G(f4, a.vmap) + y * a.cols + x;
+    const auto nout = FU_G(f4, a.nmap) + y * a.cols + x;
+    if (a.keep_v && FU_G(const unsigned, m.ctr)[C_RENDER] == 0u) {
         // Mapping::RayTrace (Mapping.cc:102-108) does not raycast when there is nothing to
         // render; the frame keeps its own level-0 maps
-        *vout = a.keep_v[y * a.cols + x];
-        *nout = a.keep_n[y * a.cols + x];
+        *vout = FU_G(const f4, a.keep_v)[y * a.cols + x];
+        *nout = FU_G(const f4, a.keep_n)[y * a.cols + x];
         return;
     }
     const int tile = (y / 8) * a.zcols + (x / 8);
-    const float zr0 = __uint_as_float(a.zmin[tile]);
-    const float zr1 = __uint_as_float(a.zmax[tile]);
+    const float zr0 = __uint_as_float(FU_G(const unsigned, a.zmin)[tile]);
+    const float zr1 = __uint_as_float(FU_G(const unsigned, a.zmax)[tile]);
     if ((double)zr1 < 1e-3 || (double)zr0 < 1e-3 || isnanf_(zr0) || isnanf_(zr1)) {
         *vout = nan4;
         *nout = nan4;
@@ -285,11 +353,12 @@ __global__ void kb_zrange_init(BatchLaunch b) {
     const SessDesc& d = b.desc[b.sess[s]];
     zrange_init_body(g, d.st, d.zr.zmin, d.zr.zmax, d.zr.zcols * d.zr.zrows);
 }
+template <bool kLds>
 __global__ __launch_bounds__(256) void kb_project(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
-    project_body(g, cload(d.m), cload(d.zr));
+    project_body<kLds>(g, cload(d.m), cload(d.zr));
 }
 template <int F>
 __global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
@@ -302,9 +371,16 @@ __global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
 void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     int grid = batch_layout(b, h, BT_ZTILES, 0, 0);
     hipLaunchKernelGGL(kb_zrange_init, dim3(grid), dim3(256), 0, s, b);
-    const int per = b.n > 0 ? 4096 / b.n : 64;
-    grid = batch_layout(b, h, BT_FIXED, 0, per < 32 ? 32 : (per > 512 ? 512 : per));
-    hipLaunchKernelGGL(kb_project, dim3(grid), dim3(256), 0, s, b);
+    bool lds = true;
+    for (int i = 0; i < b.n; ++i) lds = lds && h[b.sess[i]].zr.zcols * h[b.sess[i]].zr.zrows <= kZTilesLds;
+    if (lds) {
+        grid = batch_layout(b, h, BT_FIXED, 0, kProjectGroups);
+        hipLaunchKernelGGL(kb_project<true>, dim3(grid), dim3(256), 0, s, b);
+    } else {
+        const int per = b.n > 0 ? 4096 / b.n : 64;
+        grid = batch_layout(b, h, BT_FIXED, 0, per < 32 ? 32 : (per > 512 ? 512 : per));
+        hipLaunchKernelGGL(kb_project<false>, dim3(grid), dim3(256), 0, s, b);
+    }
 }
 void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
     const int grid = batch_layout(b, h, BT_TILES, 0, 0);

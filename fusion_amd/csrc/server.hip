@@ -30,11 +30,17 @@ struct Req {
     const uint16_t* depth;
     const uint8_t* rgb;
     double t_submit;
+    int kind;  // FU_REQ_FRAME or FU_REQ_MESH
+    float* mv;
+    float* mn;
+    uint8_t* mc;
+    uint32_t max_tri;
 };
 
 struct Step {
     int n = 0;
     int slot = 0;
+    int kind = FU_REQ_FRAME;  // a batched frame step, or one session's mesh extraction
     Req req[kMaxBatch];
 };
 
@@ -197,7 +203,33 @@ extern "C" fu_status fu_server_submit(fu_server* sv, int32_t id, const uint16_t*
     Session* s = get_session(sv, id);
     if (!s) return api_error(FU_ERR_ARG, "no session %d", id);
     if (!depth || !rgb) return api_error(FU_ERR_ARG, "null frame");
-    s->pending.push_back(Req{tag, id, depth, rgb, t_submit_ms < 0 ? fu_server_now_ms(sv) : t_submit_ms});
+    Req r{};
+    r.tag = tag;
+    r.session = id;
+    r.depth = depth;
+    r.rgb = rgb;
+    r.t_submit = t_submit_ms < 0 ? fu_server_now_ms(sv) : t_submit_ms;
+    r.kind = FU_REQ_FRAME;
+    s->pending.push_back(r);
+    sv->outstanding++;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_submit_mesh(fu_server* sv, int32_t id, float* vertex, float* normal, uint8_t* color,
+                                           uint32_t max_triangles, uint64_t tag, double t_submit_ms) {
+    Session* s = get_session(sv, id);
+    if (!s) return api_error(FU_ERR_ARG, "no session %d", id);
+    if (!vertex || !normal || !color) return api_error(FU_ERR_ARG, "null mesh buffer");
+    Req r{};
+    r.tag = tag;
+    r.session = id;
+    r.t_submit = t_submit_ms < 0 ? fu_server_now_ms(sv) : t_submit_ms;
+    r.kind = FU_REQ_MESH;
+    r.mv = vertex;
+    r.mn = normal;
+    r.mc = color;
+    r.max_tri = max_triangles;
+    s->pending.push_back(r);
     sv->outstanding++;
     return FU_OK;
 }
@@ -217,6 +249,26 @@ static fu_status retire(fu_server* sv, Lane& L) {
         if (e != hipSuccess) return api_error(FU_ERR_HIP, "step completion: %s", hipGetErrorString(e));
         const double now = fu_server_now_ms(sv);
         const FrameBrief* r = L.h_res + (size_t)st.slot * kMaxBatch;
+        if (st.kind == FU_REQ_MESH) {
+            // the slot holds the map's counters after the extraction
+            const unsigned* ctr = reinterpret_cast<const unsigned*>(r);
+            fu_completion c;
+            memset(&c, 0, sizeof(c));
+            c.tag = st.req[0].tag;
+            c.session = st.req[0].session;
+            c.kind = FU_REQ_MESH;
+            c.track_status = -1;
+            c.n_triangles_total = ctr[C_TRI];
+            c.n_triangles = ctr[C_TRI] < st.req[0].max_tri ? ctr[C_TRI] : st.req[0].max_tri;
+            c.n_occupied = ctr[C_OCC];
+            c.flags = ctr[C_FLAGS] | (ctr[C_TRI] > st.req[0].max_tri ? (unsigned)FU_FLAG_TRI_FULL : 0u);
+            c.t_submit_ms = st.req[0].t_submit;
+            c.t_done_ms = now;
+            sv->done.push_back(c);
+            sv->sess[c.session].in_steps--;
+            L.inflight.pop_front();
+            continue;
+        }
         for (int i = 0; i < st.n; ++i) {
             fu_completion c;
             memset(&c, 0, sizeof(c));
@@ -243,9 +295,35 @@ static fu_status retire(fu_server* sv, Lane& L) {
 static fu_status launch(fu_server* sv, Lane& L) {
     std::vector<int> cand;
     while ((int)L.inflight.size() < sv->p.max_inflight) {
+        // a mesh request at the head of a session's queue runs as a step of its own
+        int mesh = -1;
+        for (int id : L.sessions) {
+            const Session& s = sv->sess[id];
+            if (!s.pending.empty() && s.pending.front().kind == FU_REQ_MESH &&
+                (mesh < 0 || s.pending.front().t_submit < sv->sess[mesh].pending.front().t_submit))
+                mesh = id;
+        }
+        if (mesh >= 0) {
+            Session& s = sv->sess[mesh];
+            Step st;
+            st.kind = FU_REQ_MESH;
+            st.n = 1;
+            st.slot = (int)(L.launched % (uint64_t)sv->p.max_inflight);
+            st.req[0] = s.pending.front();
+            s.pending.pop_front();
+            s.in_steps++;
+            fu_status e = mesh_async(s.sys, st.req[0].mv, st.req[0].mn, st.req[0].mc, st.req[0].max_tri, L.stream);
+            if (e) return e;
+            SV_HIP(hipMemcpyAsync(L.h_res + (size_t)st.slot * kMaxBatch, system_counters(s.sys), sizeof(unsigned) * 16,
+                                  hipMemcpyDeviceToHost, L.stream));
+            SV_HIP(hipEventRecord(L.ev[st.slot], L.stream));
+            L.launched++;
+            L.inflight.push_back(st);
+            continue;
+        }
         cand.clear();
         for (int id : L.sessions)
-            if (!sv->sess[id].pending.empty()) cand.push_back(id);
+            if (!sv->sess[id].pending.empty() && sv->sess[id].pending.front().kind == FU_REQ_FRAME) cand.push_back(id);
         if (cand.empty()) break;
         if ((int)cand.size() > sv->p.max_batch) {
             std::partial_sort(cand.begin(), cand.begin() + sv->p.max_batch, cand.end(), [&](int a, int b) {

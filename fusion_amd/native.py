@@ -92,7 +92,8 @@ class Completion(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("session", C.c_int32), ("track_status", C.c_int32),
                 ("pose", C.c_double * 16), ("icp_error", C.c_float), ("n_visible", C.c_uint32),
                 ("n_allocated", C.c_uint32), ("flags", C.c_uint32), ("t_submit_ms", C.c_double),
-                ("t_done_ms", C.c_double)]
+                ("t_done_ms", C.c_double), ("kind", C.c_int32), ("n_triangles", C.c_uint32),
+                ("n_triangles_total", C.c_uint32), ("n_occupied", C.c_uint32)]
 
 
 class TraceFrame(C.Structure):
@@ -141,6 +142,10 @@ SIGNATURES = {
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
     "fu_system_set_profiling": (I, [P, I]),
     "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_icp_batch_create": (I, [I, C.POINTER(P)]),
+    "fu_icp_batch_destroy": (I, [P]),
+    "fu_compute_se3_batch": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), FP,
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_float, P]),
     "fu_batch_create": (I, [I, C.POINTER(P)]),
     "fu_batch_destroy": (I, [P]),
     "fu_batch_attach": (I, [P, P]),
@@ -158,6 +163,7 @@ SIGNATURES = {
     "fu_server_system": (P, [P, I]),
     "fu_server_now_ms": (C.c_double, [P]),
     "fu_server_submit": (I, [P, I, P, P, C.c_uint64, C.c_double]),
+    "fu_server_submit_mesh": (I, [P, I, P, P, P, C.c_uint32, C.c_uint64, C.c_double]),
     "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),
     "fu_server_pending": (I, [P, C.POINTER(I)]),
     "fu_server_run_trace": (I, [P, C.POINTER(TraceFrame), I, I, C.POINTER(Completion)]),

@@ -288,6 +288,21 @@ typedef struct fu_stage_times {
 fu_status fu_system_set_profiling(fu_system* sys, int enable);
 fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* out);
 
+/* ---- batched ICP of many frame pairs ------------------------------------------------------
+ * Tracker::ComputeSE3 (Tracking/Tracking.cc:296-409) for n trackers at once: levels coarse to
+ * fine, every iteration one reduction launch and one solve launch covering up to
+ * FU_MAX_BATCH trackers (larger n runs in groups).  Each tracker's pyramids have its own size
+ * (the tracker's cols x rows); K holds fx, fy, cx, cy of each pair's level 0.  Results are
+ * those of fu_compute_se3 on each tracker alone (fu_tracker_result); asynchronous on `s`.
+ * No reference counterpart for the batching itself. */
+typedef struct fu_icp_batch fu_icp_batch;
+fu_status fu_icp_batch_create(int capacity, fu_icp_batch** out);
+fu_status fu_icp_batch_destroy(fu_icp_batch* b);
+fu_status fu_compute_se3_batch(fu_icp_batch* b, int n, fu_tracker* const* trackers,
+                               const float* const* next_vmap, const float* const* next_nmap,
+                               const float* const* last_vmap, const float* const* last_nmap, const float* K,
+                               const int32_t iters[3], const int32_t min_count[3], float thresh, fu_stream s);
+
 /* ---- batched frames of many systems -------------------------------------------------------
  * A batch context runs one frame of each of up to FU_MAX_BATCH attached systems as one
  * sequence of launches (each stage of System::GrabImage, Core/System.cc:92-138, launched once
@@ -340,7 +355,13 @@ typedef struct {
     uint32_t flags;        /* FU_FLAG_* of the session's map */
     double t_submit_ms;    /* arrival time the latency is measured from */
     double t_done_ms;      /* when the host saw the frame's step complete */
+    int32_t kind;          /* FU_REQ_FRAME or FU_REQ_MESH */
+    uint32_t n_triangles;  /* mesh requests: triangles written (min(total, max_triangles)) */
+    uint32_t n_triangles_total;
+    uint32_t n_occupied;   /* mesh requests: blocks with at least one triangle */
 } fu_completion;
+#define FU_REQ_FRAME 0
+#define FU_REQ_MESH 1
 typedef struct {
     int32_t session;
     const uint16_t* depth; /* device-resident frame of the session's size */
@@ -358,6 +379,11 @@ double fu_server_now_ms(fu_server* sv);
 /* queue one frame; t_submit_ms < 0 means now */
 fu_status fu_server_submit(fu_server* sv, int32_t session, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
                            double t_submit_ms);
+/* queue a mesh extraction of the session's map (fu_mesh_scene), ordered after the session's
+ * frames submitted before it; the output buffers (9 floats, 9 floats and 9 bytes per
+ * triangle) must stay untouched until its completion is returned */
+fu_status fu_server_submit_mesh(fu_server* sv, int32_t session, float* vertex, float* normal, uint8_t* color,
+                                uint32_t max_triangles, uint64_t tag, double t_submit_ms);
 /* launch steps where lanes have room and collect finished frames (at most cap) */
 fu_status fu_server_poll(fu_server* sv, fu_completion* out, int32_t cap, int32_t* n_out);
 fu_status fu_server_pending(fu_server* sv, int32_t* frames); /* submitted, not yet returned by poll */

@@ -92,3 +92,44 @@ def test_icp_large_motion_rejected(gpu):
     assert r["status"] == st
     assert r["status"] in (2, 3)
     assert np.array_equal(r["pose"], np.eye(4))
+
+
+def test_compute_se3_batch_uneven_sizes(gpu):
+    """Batched ComputeSE3 over frame pairs of uneven size (320x240, 640x480, 1280x720; more
+    pairs than one launch takes, so they run in groups): every pair's result is identical to
+    the tracker run alone, and within 1e-5 rad / 1e-5 m of the FP64 oracle"""
+    sizes = [(320, 240)] * 50 + [(640, 480)] * 12 + [(1280, 720)] * 4
+    rng = np.random.default_rng(3)
+    rng.shuffle(sizes)
+    cu = lambda xs: [torch.from_numpy(x).cuda() for x in xs]
+    pairs, trackers, alone = [], [], []
+    print("pairs", len(sizes), flush=True)
+    for i, (cols, rows) in enumerate(sizes):
+        f0 = int(rng.integers(0, 40))
+        sc, K, fr = scene_frames(frames=(f0, f0 + 2), cols=cols, rows=rows)
+        _, _, v0, n0 = _pyr_oracle(fr[0][0], K)
+        _, _, v1, n1 = _pyr_oracle(fr[1][0], K)
+        pairs.append((K, cu(v1), cu(n1), cu(v0), cu(n0), (v0, n0, v1, n1)))
+        t = A.Tracker(cols, rows)
+        alone.append(t.ComputeSE3(*[pairs[-1][k] for k in (1, 2, 3, 4)], K))
+        t2 = A.Tracker(cols, rows)
+        t2.set_last_pose(np.eye(4))
+        trackers.append(t2)
+    b = A.IcpBatch(128)
+    res = b.ComputeSE3(trackers, [p[1] for p in pairs], [p[2] for p in pairs], [p[3] for p in pairs],
+                       [p[4] for p in pairs], [p[0] for p in pairs])
+    for i, (ra, rb) in enumerate(zip(alone, res)):
+        assert ra["status"] == rb["status"], i
+        assert np.array_equal(ra["pose"], rb["pose"]), i
+    checked = 0
+    for i, (cols, rows) in enumerate(sizes):
+        if checked >= 6 or (cols == 320 and checked >= 2):
+            continue
+        K, _, _, _, _, (v0, n0, v1, n1) = pairs[i]
+        st, pose_o, _, _ = O.compute_se3(v1, n1, v0, n0, K, np.eye(4))
+        assert res[i]["status"] == st, i
+        if st == 1:
+            assert _rot_err(res[i]["pose"][:3, :3], pose_o[:3, :3]) <= 1e-5, i
+            assert np.linalg.norm(res[i]["pose"][:3, 3] - pose_o[:3, 3]) <= 1e-5, i
+        checked += 1
+    assert sum(r["status"] == 1 for r in res) >= 0.9 * len(res)

@@ -143,3 +143,37 @@ def test_empty_frame(gpu):
     rgb = np.zeros((480, 640, 3), np.uint8)
     g, o = _fuse_both(_params(), [(d16, rgb)], [np.eye(4)], K)
     assert len(g.export_blocks()) == 0 and len(o.export_blocks()) == 0
+
+
+@pytest.mark.parametrize("fmt,bound", [(0, 1e-3), (1, 1e-3), (2, 2.0 ** -8)])
+def test_precision_vs_fp64_twin(gpu, fmt, bound):
+    """Per-voxel TSDF of each storage format against an FP64 re-integration of the same
+    frames (the oracle's FP64 twin: same allocation, the update arithmetic in double), 5 mm
+    voxels, truncation 8 voxels, 8 frames with ground-truth poses.  Bar: |sdf - sdf64| within
+    1e-3 of the truncation distance for the FP32 forms; FP16 within 2^-8 (storage rounding,
+    spacing 2^-11 near 1).  Weights must match the FP64 twin's exactly."""
+    sc, K, frames = scene_frames(frames=range(0, 16, 2))
+    P0 = sc.pose(0)
+    poses = [np.linalg.inv(P0) @ sc.pose(f) for f in range(0, 16, 2)]
+    vox = 0.005
+    p = _params(voxel=vox, buckets=1 << 19, blocks=300_000, excess=1 << 18)
+    p["trunc_dist"] = float(np.float32(vox) * np.float32(8.0))
+    g = A.Mapping(**p, voxel_format=fmt)
+    o = O.Map(num_buckets=p["num_buckets"], num_excess=p["num_excess"], num_blocks=p["num_blocks"], voxel_size=vox,
+              trunc_dist=p["trunc_dist"], precision=1)
+    for (d16, rgb), pose in zip(frames, poses):
+        raw, n = _inputs(d16, rgb, K)
+        g.FuseMapColor(torch.from_numpy(raw).cuda(), torch.from_numpy(rgb).cuda(), torch.from_numpy(n).cuda(), pose, K)
+        o.fuse_map_color(raw, rgb, n, pose, K)
+    ob = o.export_blocks()
+    assert rows_set(g.export_blocks()) == rows_set(ob)
+    sg, wg, _ = g.read_blocks(ob)
+    s64 = o.read_blocks_f64(ob)
+    _, wo, _ = o.read_blocks(ob)
+    assert np.array_equal(wg, wo)
+    m = ~np.isnan(s64)
+    assert np.array_equal(np.isnan(sg), ~m)
+    err = np.abs(sg[m].astype(np.float64) - s64[m])
+    print("format %d: %d voxels, max |sdf - sdf64| = %.3g T, mean %.3g T, 99.9%% %.3g T" % (
+        fmt, int(m.sum()), err.max(), err.mean(), np.percentile(err, 99.9)), flush=True)
+    assert err.max() <= bound

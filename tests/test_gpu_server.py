@@ -101,3 +101,39 @@ def test_server_submit_poll(gpu):
     sv.close_session(sid)
     sv.close()
     torch.cuda.synchronize()
+
+
+def test_server_mesh_request(gpu):
+    """a mesh request queued between a session's frames extracts the map as it is after the
+    frames before it: the same triangles (as a set) as fu_mesh_scene on a system that ran
+    those frames alone"""
+    frames = _frames()
+    K, fr = frames[1]
+    cut = 3
+    s = _system(1, K)
+    for d, c in fr[:cut]:
+        s.GrabImage(d, c)
+    v_a, _, c_a, _ = s.mapping().MeshScene(max_triangles=1_000_000)
+    v_a = v_a.reshape(-1, 9).cpu().numpy()
+    s.close()
+    sv = A.Server(max_batch=4, max_inflight=2, lanes=1)
+    sid = sv.open(_params(1, K))
+    cap = 1_000_000
+    vb = torch.empty((cap, 9), dtype=torch.float32, device="cuda")
+    nb = torch.empty_like(vb)
+    cb = torch.empty((cap, 9), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    for f, (d, c) in enumerate(fr):
+        if f == cut:
+            sv.submit_mesh(sid, vb, nb, cb, cap, tag=999)
+        sv.submit(sid, d, c, tag=f)
+    done = sv.drain()
+    kinds = [(c["kind"], c["tag"]) for c in done]
+    assert kinds == [(0, f) for f in range(cut)] + [(1, 999)] + [(0, f) for f in range(cut, len(fr))]
+    m = done[cut]
+    assert m["n_triangles"] == m["n_triangles_total"] == v_a.shape[0] > 0
+    v_b = vb[: m["n_triangles"]].cpu().numpy()
+    key = lambda v: np.unique(v.view(np.uint32), axis=0)
+    assert np.array_equal(key(v_a), key(v_b))
+    sv.close_session(sid)
+    sv.close()
