@@ -14,7 +14,8 @@ Frames are generated on the CPU and resident in HBM before any timed region.
 A step is one frame of every session (64 frames).  `value` is the number of frames that were
 tracked and fused per second with every session's frames queued from the start (the server's
 capacity); `latency_ms` is submit-to-result latency with 30 fps Poisson arrivals per session
-(about 1,900 frames/s offered).  `plain` runs the same kind of session one frame at a time with
+(about 1,900 frames/s offered) plus mesh extractions requested at 0.1 Hz per session.
+`plain` runs the same kind of session one frame at a time with
 the reference's host control (host solve per ICP iteration, synchronisation per stage).
 
 Prints one JSON line.  --gpus N > 1 is refused: this build targets one GPU.
@@ -40,6 +41,8 @@ MAP_SINGLE = dict(num_buckets=1 << 21, num_excess=1 << 21, num_blocks=2_000_000,
                   max_new_blocks=1 << 18)
 BUDGET = 200 * 10**9
 TRACKED = (1, 4)  # FU_TRACK_OK, FU_TRACK_FIRST: frames that are tracked and fused
+MESH_HZ = 0.1  # mesh extraction requests per session per second
+MESH_MAX_TRI = 1_000_000
 # frame size mix of the mixed-resolution served run: (cols, rows, probability)
 MIX = [(640, 480, 0.5), (320, 240, 0.3), (1280, 720, 0.2)]
 
@@ -150,11 +153,32 @@ def served(A, torch, args, sizes, frames, steps, realtime, profile):
     if realtime > 0:
         rng = np.random.default_rng(args.seed + 7)
         arr = poisson_arrivals(rng, len(sids), realtime)
-        done = sv.run_trace(trace(sids, frames, lo, lo + realtime, arr))
-        res["lat_rt"] = [c["t_done_ms"] - c["t_submit_ms"] for c in done]
-        span = max(c["t_done_ms"] for c in done) - min(c["t_submit_ms"] for c in done)
-        res["rt_fps"] = len(done) / span * 1e3
-        res["rt_tracked"] = float(np.isin([c["track_status"] for c in done], TRACKED).mean())
+        tr = trace(sids, frames, lo, lo + realtime, arr)
+        # mesh extraction requests, Poisson at MESH_HZ per session over the run
+        horizon = max(a[-1] for a in arr)
+        meshes = []
+        for j, sid in enumerate(sids):
+            t = float(rng.exponential(1e3 / MESH_HZ))
+            while t < horizon:
+                meshes.append((sid, t))
+                t += float(rng.exponential(1e3 / MESH_HZ))
+        bufs = []
+        for k, (sid, t) in enumerate(meshes):
+            v = torch.empty((MESH_MAX_TRI, 9), dtype=torch.float32, device="cuda")
+            bufs.append((v, torch.empty_like(v), torch.empty((MESH_MAX_TRI, 9), dtype=torch.uint8, device="cuda")))
+            tr.append((sid, bufs[-1], None, t, 10**9 + k))
+        torch.cuda.synchronize()
+        done = sv.run_trace(tr)
+        fr = [c for c in done if c["kind"] == 0]
+        ms = [c for c in done if c["kind"] == 1]
+        res["lat_rt"] = [c["t_done_ms"] - c["t_submit_ms"] for c in fr]
+        span = max(c["t_done_ms"] for c in fr) - min(c["t_submit_ms"] for c in fr)
+        res["rt_fps"] = len(fr) / span * 1e3
+        res["rt_tracked"] = float(np.isin([c["track_status"] for c in fr], TRACKED).mean())
+        res["mesh"] = dict(requests=len(ms), latency_ms={"p50": pct([c["t_done_ms"] - c["t_submit_ms"] for c in ms], 50),
+                                                         "max": pct([c["t_done_ms"] - c["t_submit_ms"] for c in ms], 100)},
+                           triangles_mean=(float(np.mean([c["n_triangles"] for c in ms])) if ms else None))
+        del bufs
         lo += realtime
     # per-stage HIP events (each step synchronises), same sessions, next frames
     if profile > 0:
@@ -316,7 +340,7 @@ def main():
                      sessions={"%dx%d" % s: int(sum(1 for z in msz if z == s)) for s in sorted(set(msz))},
                      mean_pixels_per_frame=float(np.mean([c * r_ for c, r_ in msz])),
                      latency_ms={"p50": pct(m["lat_rt"], 50), "p99": pct(m["lat_rt"], 99)},
-                     realtime_frames_per_s=round(m["rt_fps"], 1), flags=m["flags"])
+                     realtime_frames_per_s=round(m["rt_fps"], 1), mesh_requests=m.get("mesh"), flags=m["flags"])
         log("mixed: %.0f frames/s" % mixed["value"])
 
     single = single_stream(A, torch, args) if args.single_steps > 0 else None
@@ -349,7 +373,8 @@ def main():
         "tracked_fraction": round(r["tracked"] / max(1, r["frames"]), 4),
         "latency_ms": {"p50": pct(r.get("lat_rt", []), 50), "p99": pct(r.get("lat_rt", []), 99),
                        "arrivals": "30 fps Poisson per session",
-                       "offered_frames_per_s": round(r.get("rt_fps", 0.0), 1)},
+                       "offered_frames_per_s": round(r.get("rt_fps", 0.0), 1),
+                       "mesh_requests": r.get("mesh")},
         "latency_backlog_ms": {"p50": pct(r["lat_backlog"], 50), "p99": pct(r["lat_backlog"], 99)},
         "plain": p,
         "served_over_plain": round(fps / p["value"], 3),

@@ -528,11 +528,19 @@ class Server:
         return out
 
     def run_trace(self, frames, closed_loop=False):
-        """frames: list of (session, depth, rgb, arrival_ms, tag); returns completions in order"""
+        """frames: list of (session, depth, rgb, arrival_ms, tag) frame requests, or
+        (session, (vertex, normal, color), None, arrival_ms, tag) mesh requests (buffers of
+        shape (T, 9)); returns completions in completion order"""
         n = len(frames)
         tr = (N.TraceFrame * n)()
         for i, (sid, d, c, t, tag) in enumerate(frames):
-            tr[i].session, tr[i].depth, tr[i].rgb, tr[i].arrival_ms, tr[i].tag = sid, d.data_ptr(), c.data_ptr(), t, tag
+            tr[i].session, tr[i].arrival_ms, tr[i].tag = sid, t, tag
+            if c is None:
+                v, nm, col = d
+                tr[i].kind, tr[i].max_triangles = 1, v.shape[0]
+                tr[i].vertex, tr[i].normal, tr[i].color = v.data_ptr(), nm.data_ptr(), col.data_ptr()
+            else:
+                tr[i].depth, tr[i].rgb = d.data_ptr(), c.data_ptr()
         out = (N.Completion * n)()
         check(N.lib().fu_server_run_trace(self.handle, tr, n, int(closed_loop), out), "fu_server_run_trace")
         return [_completion(out[i]) for i in range(n)]

@@ -387,9 +387,11 @@ extern "C" fu_status fu_server_poll(fu_server* sv, fu_completion* out, int32_t c
 extern "C" fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* frames, int32_t n, int32_t closed_loop,
                                          fu_completion* out) {
     if (!sv || (n > 0 && (!frames || !out))) return api_error(FU_ERR_ARG, "null argument");
-    for (int32_t i = 0; i < n; ++i)
-        if (!get_session(sv, frames[i].session) || !frames[i].depth || !frames[i].rgb)
-            return api_error(FU_ERR_ARG, "trace frame %d: bad session or frame", i);
+    for (int32_t i = 0; i < n; ++i) {
+        const fu_trace_frame& f = frames[i];
+        const bool ok = f.kind == FU_REQ_MESH ? (f.vertex && f.normal && f.color) : (f.kind == FU_REQ_FRAME && f.depth && f.rgb);
+        if (!get_session(sv, f.session) || !ok) return api_error(FU_ERR_ARG, "trace request %d: bad session or buffers", i);
+    }
     if (sv->outstanding) return api_error(FU_ERR_STATE, "run_trace needs an idle server");
     std::vector<int32_t> order(n);
     for (int32_t i = 0; i < n; ++i) order[i] = i;
@@ -400,7 +402,10 @@ extern "C" fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* fr
                      [&](int32_t a, int32_t b) { return frames[a].arrival_ms < frames[b].arrival_ms; });
     const double t0 = fu_server_now_ms(sv);
     auto submit = [&](int32_t i, double t) {
-        return fu_server_submit(sv, frames[i].session, frames[i].depth, frames[i].rgb, frames[i].tag, t);
+        const fu_trace_frame& f = frames[i];
+        if (f.kind == FU_REQ_MESH)
+            return fu_server_submit_mesh(sv, f.session, f.vertex, f.normal, f.color, f.max_triangles, f.tag, t);
+        return fu_server_submit(sv, f.session, f.depth, f.rgb, f.tag, t);
     };
     fu_status st = FU_OK;
     int32_t next = 0, got = 0;

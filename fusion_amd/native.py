@@ -98,7 +98,8 @@ class Completion(C.Structure):
 
 class TraceFrame(C.Structure):
     _fields_ = [("session", C.c_int32), ("depth", C.c_void_p), ("rgb", C.c_void_p), ("arrival_ms", C.c_double),
-                ("tag", C.c_uint64)]
+                ("tag", C.c_uint64), ("kind", C.c_int32), ("max_triangles", C.c_uint32), ("vertex", C.c_void_p),
+                ("normal", C.c_void_p), ("color", C.c_void_p)]
 
 
 SIGNATURES = {

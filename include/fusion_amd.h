@@ -368,6 +368,11 @@ typedef struct {
     const uint8_t* rgb;
     double arrival_ms;     /* offset from the start of the trace */
     uint64_t tag;
+    int32_t kind;          /* FU_REQ_FRAME, or FU_REQ_MESH with the buffers below */
+    uint32_t max_triangles;
+    float* vertex;
+    float* normal;
+    uint8_t* color;
 } fu_trace_frame;
 void fu_server_params_default(fu_server_params* p);
 fu_status fu_server_create(const fu_server_params* p, fu_server** out);
@@ -387,8 +392,8 @@ fu_status fu_server_submit_mesh(fu_server* sv, int32_t session, float* vertex, f
 /* launch steps where lanes have room and collect finished frames (at most cap) */
 fu_status fu_server_poll(fu_server* sv, fu_completion* out, int32_t cap, int32_t* n_out);
 fu_status fu_server_pending(fu_server* sv, int32_t* frames); /* submitted, not yet returned by poll */
-/* Replays a trace in real time: open loop submits each frame at its arrival offset; closed
- * loop submits a session's next frame when its previous one completes (arrivals ignored).
+/* Replays a trace in real time: open loop submits each request at its arrival offset; closed
+ * loop submits a session's next request when its previous one completes (arrivals ignored).
  * Fills out[0..n) in completion order and returns when every frame has completed. */
 fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* frames, int32_t n, int32_t closed_loop,
                               fu_completion* out);
