@@ -1083,8 +1083,8 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
         if (st) return st;
     }
     if (nfrag > 0) {
-        const RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max,
-                                      b.zcols, b.zrows, dummy, t->st, (float)(1.0 / (double)p.fx),
+        const RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min,
+                                      b.zrange_max, b.zcols, b.zrows, dummy, t->st, (float)(1.0 / (double)p.fx),
                                       (float)(1.0 / (double)p.fy), p.cx, p.cy);
         launch_raycast(m->v, r, s);
     } else {

@@ -39,7 +39,7 @@ enum Counter : int {
     C_TRI = 6,      // triangles of the last mesh pass
     C_DONE = 7,     // work-group completion counter (last-group pattern), left at 0
     C_OCC = 8,      // occupied blocks seen by the last mesh pass
-    C_SCRATCH = 9,
+    C_SPARE = 9,
     C_COUNT = 16
 };
 

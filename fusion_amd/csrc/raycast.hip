@@ -211,19 +211,41 @@ struct Sampler {
     // then issued together.
     __device__ __forceinline__ float read_interp(f3 pt, bool& valid) {
         const f3 f = sub(pt, floor3(pt));
-        i3 c[8];
+        // corner k = (x, y, z) bits sits at make_int3(pt + (kx, ky, kz)): two coordinates per axis
+        const int x0 = f2i(pt.x), x1 = f2i(pt.x + 1.0f);
+        const int y0 = f2i(pt.y), y1 = f2i(pt.y + 1.0f);
+        const int z0 = f2i(pt.z), z1 = f2i(pt.z + 1.0f);
+        const i3 b0 = mki3(x0 >> 3, y0 >> 3, z0 >> 3);
+        const i3 b1 = mki3(x1 >> 3, y1 >> 3, z1 >> 3);
         int hp[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            c[k] = f2i3(add(pt, mk3((float)(k & 1), (float)((k >> 1) & 1), (float)((k >> 2) & 1))));
-            const i3 bp = voxel_to_block(c[k]);
-            if (cheap >= 0 && eqi(bp, cbp)) {
-                hp[k] = cheap;
+        if (eqi(b0, b1)) {
+            // every corner in one block (about two samples in three): one resolution
+            int h;
+            if (cheap >= 0 && eqi(b0, cbp)) {
+                h = cheap;
             } else {
-                hp[k] = find_block(m, bp);
-                if (hp[k] >= 0) {
-                    cbp = bp;
-                    cheap = hp[k];
+                h = find_block(m, b0);
+                if (h >= 0) {
+                    cbp = b0;
+                    cheap = h;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) hp[k] = h;
+        } else {
+            // blocks resolved in corner order, each through the cache (as a corner-by-corner
+            // lookup would leave it)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const i3 bp = mki3((k & 1) ? b1.x : b0.x, (k & 2) ? b1.y : b0.y, (k & 4) ? b1.z : b0.z);
+                if (cheap >= 0 && eqi(bp, cbp)) {
+                    hp[k] = cheap;
+                } else {
+                    hp[k] = find_block(m, bp);
+                    if (hp[k] >= 0) {
+                        cbp = bp;
+                        cheap = hp[k];
+                    }
                 }
             }
         }
@@ -232,7 +254,8 @@ struct Sampler {
         float sv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const float raw = IO::sdf(IO::block(m, hp[k] >= 0 ? hp[k] : 0), voxel_local_idx(c[k]));
+            const int li = (((k & 4) ? z1 : z0) & 7) << 6 | (((k & 2) ? y1 : y0) & 7) << 3 | (((k & 1) ? x1 : x0) & 7);
+            const float raw = IO::sdf(IO::block(m, hp[k] >= 0 ? hp[k] : 0), li);
             sv[k] = hp[k] >= 0 ? raw : fnan();
         }
         valid = hp[7] >= 0;
@@ -245,15 +268,76 @@ struct Sampler {
         const float r3 = (1.0f - f.y) * r0b + f.y * r1b;
         return (1.0f - f.z) * r2 + f.z * r3;
     }
+
+    // Heap indices of the 2x2x2 blocks lo + (dx, dy, dz), dx, dy, dz in {0, 1}, probed
+    // together: the occupancy words, then the bucket heads, as independent loads; only excess
+    // chains are walked one step at a time.
+    __device__ __forceinline__ void resolve8(i3 lo, int (&h)[8]) {
+        uint32_t raw[8];
+        bool maybe[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const i3 bp = mki3(lo.x + (k & 1), lo.y + ((k >> 1) & 1), lo.z + ((k >> 2) & 1));
+            raw[k] = block_hash_raw(bp);
+            maybe[k] = key_in_range(bp) && maybe_allocated(m, raw[k]);
+        }
+        HEntry e[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (maybe[k]) e[k] = load_entry(&m.entries[bucket_of(raw[k], m.num_buckets, m.bucket_mask)]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            h[k] = -1;
+            if (!maybe[k]) continue;
+            const i3 bp = mki3(lo.x + (k & 1), lo.y + ((k >> 1) & 1), lo.z + ((k >> 2) & 1));
+            const unsigned long long key = pack_key(bp);
+            HEntry c = e[k];
+            while (true) {
+                if (c.key == key && c.ptr >= 0) {
+                    h[k] = c.ptr >> 9;
+                    break;
+                }
+                if (c.next <= 0) break;
+                c = load_entry(&m.entries[m.num_buckets + c.next - 1]);
+            }
+        }
+    }
+
+    // read_interp with every corner's block taken from a resolved neighbourhood (the corners
+    // must lie in lo .. lo + 1 per axis)
+    __device__ __forceinline__ float interp_nb(f3 pt, i3 lo, const int (&h)[8], bool& valid) const {
+        const f3 f = sub(pt, floor3(pt));
+        const int x0 = f2i(pt.x), x1 = f2i(pt.x + 1.0f);
+        const int y0 = f2i(pt.y), y1 = f2i(pt.y + 1.0f);
+        const int z0 = f2i(pt.z), z1 = f2i(pt.z + 1.0f);
+        const int ix0 = (x0 >> 3) - lo.x, ix1 = (x1 >> 3) - lo.x;
+        const int iy0 = ((y0 >> 3) - lo.y) << 1, iy1 = ((y1 >> 3) - lo.y) << 1;
+        const int iz0 = ((z0 >> 3) - lo.z) << 2, iz1 = ((z1 >> 3) - lo.z) << 2;
+        float sv[8];
+        int hl = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int hk = h[((k & 1) ? ix1 : ix0) | ((k & 2) ? iy1 : iy0) | ((k & 4) ? iz1 : iz0)];
+            const int li = (((k & 4) ? z1 : z0) & 7) << 6 | (((k & 2) ? y1 : y0) & 7) << 3 | (((k & 1) ? x1 : x0) & 7);
+            const float raw = IO::sdf(IO::block(m, hk >= 0 ? hk : 0), li);
+            sv[k] = hk >= 0 ? raw : fnan();
+            hl = hk;
+        }
+        valid = hl >= 0;
+        const float r0 = (1.0f - f.x) * sv[0] + f.x * sv[1];
+        const float r1 = (1.0f - f.x) * sv[2] + f.x * sv[3];
+        const float r2 = (1.0f - f.y) * r0 + f.y * r1;
+        const float r0b = (1.0f - f.x) * sv[4] + f.x * sv[5];
+        const float r1b = (1.0f - f.x) * sv[6] + f.x * sv[7];
+        const float r3 = (1.0f - f.y) * r0b + f.y * r1b;
+        return (1.0f - f.z) * r2 + f.z * r3;
+    }
 };
 
-template <int F>
-__device__ __forceinline__ void raycast_body(WG g, const MapView& m, const RaycastArgs& a) {
-    PoseF pose = a.pose;
-    if (a.st && !pose_from_state(a.st, pose)) return;
-    const int x = g.x * 16 + (threadIdx.x & 15);
-    const int y = g.y * 16 + (threadIdx.x >> 4);
-    if (x >= a.cols || y >= a.rows) return;
+// The per-pixel start of Rendering::operator() (RenderScene.cu:354-385): false when the
+// pixel's outputs are already written (nothing rendered: the frame's own maps are kept, or an
+// empty z-range tile: NaN).
+__device__ __forceinline__ bool ray_setup(const MapView& m, const RaycastArgs& a, int x, int y, float& zr0, float& zr1) {
     const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
     const auto vout = FU_G(f4, a.vmap) + y * a.cols + x;
     const auto nout = FU_G(f4, a.nmap) + y * a.cols + x;
@@ -262,31 +346,37 @@ __device__ __forceinline__ void raycast_body(WG g, const MapView& m, const Rayca
         // render; the frame keeps its own level-0 maps
         *vout = FU_G(const f4, a.keep_v)[y * a.cols + x];
         *nout = FU_G(const f4, a.keep_n)[y * a.cols + x];
-        return;
+        return false;
     }
     const int tile = (y / 8) * a.zcols + (x / 8);
-    const float zr0 = __uint_as_float(FU_G(const unsigned, a.zmin)[tile]);
-    const float zr1 = __uint_as_float(FU_G(const unsigned, a.zmax)[tile]);
+    zr0 = __uint_as_float(FU_G(const unsigned, a.zmin)[tile]);
+    zr1 = __uint_as_float(FU_G(const unsigned, a.zmax)[tile]);
     if ((double)zr1 < 1e-3 || (double)zr0 < 1e-3 || isnanf_(zr0) || isnanf_(zr1)) {
         *vout = nan4;
         *nout = nan4;
-        return;
+        return false;
     }
+    return true;
+}
+
+// The march (RenderScene.cu:386-418): true on a hit, with the position in voxel units, the
+// sdf there (<= 0) and the ray direction.
+template <int F>
+__device__ __forceinline__ bool ray_march(const MapView& m, const RaycastArgs& a, const PoseF& pose, int x, int y,
+                                          float zr0, float zr1, Sampler<F>& smp, f3& result, float& sdf, f3& dir) {
     const float vinv = m.voxel_inv;
     const float step_scale = a.step_scale;
-    float sdf = 1.0f;
+    sdf = 1.0f;
     f3 p3 = mk3(zr0 * ((float)x - a.cx) * a.invfx, zr0 * ((float)y - a.cy) * a.invfy, zr0);
     float dist_s = norm(p3) * vinv;
     const f3 block_s = mul(add(mv(pose.R, p3), pose.t), vinv);
     p3 = mk3(zr1 * ((float)x - a.cx) * a.invfx, zr1 * ((float)y - a.cy) * a.invfy, zr1);
     const float dist_e = norm(p3) * vinv;
     const f3 block_e = mul(add(mv(pose.R, p3), pose.t), vinv);
-    const f3 dir = normalised(sub(block_e, block_s));
-    f3 result = block_s;
+    dir = normalised(sub(block_e, block_s));
+    result = block_s;
     bool valid = false;
-    bool found = false;
     float step;
-    Sampler<F> smp(m);
     while (dist_s < dist_e) {
         sdf = smp.read_sdf(result, valid);
         if (!valid) {
@@ -302,46 +392,80 @@ __device__ __forceinline__ void raycast_body(WG g, const MapView& m, const Rayca
         result = add(result, mul(step, dir));
         dist_s += step;
     }
-    if (sdf <= 0.0f) {
-        step = sdf * step_scale;
-        result = add(result, mul(step, dir));
-        sdf = smp.read_interp(result, valid);
-        step = sdf * step_scale;
-        result = add(result, mul(step, dir));
-        found = true;
+    return sdf <= 0.0f;
+}
+
+// Two half-steps of refinement (RenderScene.cu:419-428) and Rendering::readNormal
+// (RenderScene.cu:321-352): true with the camera-frame vertex and normal, false when a
+// normal sample is NaN, 1.0 or invalid.
+template <int F>
+__device__ __forceinline__ bool ray_shade(const MapView& m, const RaycastArgs& a, const PoseF& pose, Sampler<F>& smp,
+                                          f3 result, float sdf, f3 dir, f4& vo, f4& no) {
+    const float step_scale = a.step_scale;
+    bool valid;
+    float step = sdf * step_scale;
+    result = add(result, mul(step, dir));
+    sdf = smp.read_interp(result, valid);
+    step = sdf * step_scale;
+    result = add(result, mul(step, dir));
+    // the six samples' corners lie between make_int3(result - 1) and make_int3((result + 1) + 1)
+    // per axis (truncation is monotonic), five voxels: at most two blocks per axis
+    const i3 lo = mki3(f2i(result.x - 1.0f) >> 3, f2i(result.y - 1.0f) >> 3, f2i(result.z - 1.0f) >> 3);
+    int hb[8];
+    smp.resolve8(lo, hb);
+    float s[6];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const float d = (k & 1) ? -1.0f : 1.0f;
+        const f3 o = mk3(k / 2 == 0 ? d : 0.0f, k / 2 == 1 ? d : 0.0f, k / 2 == 2 ? d : 0.0f);
+        bool v2;
+        s[k] = smp.interp_nb(add(result, o), lo, hb, v2);
+        ok = ok && !(isnanf_(s[k]) || s[k] == 1.0f || !v2);
     }
-    if (found) {
-        // Rendering::readNormal (RenderScene.cu:321-352)
-        const f3 off[6] = {mk3(1, 0, 0), mk3(-1, 0, 0), mk3(0, 1, 0), mk3(0, -1, 0), mk3(0, 0, 1), mk3(0, 0, -1)};
-        float s[6];
-        bool ok = true;
-        for (int k = 0; k < 6 && ok; ++k) {
-            bool v2;
-            s[k] = smp.read_interp(add(result, off[k]), v2);
-            if (isnanf_(s[k]) || s[k] == 1.0f || !v2) ok = false;
-        }
-        if (ok) {
-            const f3 n = normalised(mv(pose.Rinv, mk3(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
-            const f3 vtx = mv(pose.Rinv, sub(mul(result, m.voxel_size), pose.t));
-            *vout = mk4(vtx.x, vtx.y, vtx.z, 1.0f);
-            *nout = mk4(n.x, n.y, n.z, 1.0f);
-            return;
-        }
+    if (!ok) return false;
+    const f3 n = normalised(mv(pose.Rinv, mk3(s[0] - s[1], s[2] - s[3], s[4] - s[5])));
+    const f3 vtx = mv(pose.Rinv, sub(mul(result, m.voxel_size), pose.t));
+    vo = mk4(vtx.x, vtx.y, vtx.z, 1.0f);
+    no = mk4(n.x, n.y, n.z, 1.0f);
+    return true;
+}
+
+// one lane per pixel: set-up, march, refinement and normal
+template <int F>
+__device__ __forceinline__ void raycast_body(WG g, const MapView& m, const RaycastArgs& a) {
+    PoseF pose = a.pose;
+    if (a.st && !pose_from_state(a.st, pose)) return;
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= a.cols || y >= a.rows) return;
+    float zr0, zr1;
+    if (!ray_setup(m, a, x, y, zr0, zr1)) return;
+    Sampler<F> smp(m);
+    f3 result, dir;
+    float sdf;
+    f4 vo = mk4(fnan(), fnan(), fnan(), fnan()), no = vo;
+    if (ray_march<F>(m, a, pose, x, y, zr0, zr1, smp, result, sdf, dir)) {
+        if (!ray_shade<F>(m, a, pose, smp, result, sdf, dir, vo, no)) vo = no = mk4(fnan(), fnan(), fnan(), fnan());
     }
-    *vout = nan4;
-    *nout = nan4;
+    FU_G(f4, a.vmap)[y * a.cols + x] = vo;
+    FU_G(f4, a.nmap)[y * a.cols + x] = no;
 }
 template <int F>
 __global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
     raycast_body<F>(hw_wg(), m, a);
 }
 
-void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
+template <int F>
+static void raycast_f(const MapView& m, const RaycastArgs& a, hipStream_t s) {
     const dim3 g((a.cols + 15) / 16, (a.rows + 15) / 16);
+    hipLaunchKernelGGL(k_raycast<F>, g, dim3(256), 0, s, m, a);
+}
+void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
     switch (m.fmt) {
-        case VF_F32_W8: hipLaunchKernelGGL(k_raycast<VF_F32_W8>, g, dim3(256), 0, s, m, a); break;
-        case VF_F32_W16: hipLaunchKernelGGL(k_raycast<VF_F32_W16>, g, dim3(256), 0, s, m, a); break;
-        default: hipLaunchKernelGGL(k_raycast<VF_F16_W8>, g, dim3(256), 0, s, m, a); break;
+        case VF_F32_W8: raycast_f<VF_F32_W8>(m, a, s); break;
+        case VF_F32_W16: raycast_f<VF_F32_W16>(m, a, s); break;
+        default: raycast_f<VF_F16_W8>(m, a, s); break;
     }
 }
 
@@ -382,12 +506,16 @@ void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
         hipLaunchKernelGGL(kb_project<false>, dim3(grid), dim3(256), 0, s, b);
     }
 }
-void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
+template <int F>
+static void b_raycast_f(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     const int grid = batch_layout(b, h, BT_TILES, 0, 0);
+    hipLaunchKernelGGL(kb_raycast<F>, dim3(grid), dim3(256), 0, s, b);
+}
+void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
     switch (fmt) {
-        case VF_F32_W8: hipLaunchKernelGGL(kb_raycast<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
-        case VF_F32_W16: hipLaunchKernelGGL(kb_raycast<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;
-        default: hipLaunchKernelGGL(kb_raycast<VF_F16_W8>, dim3(grid), dim3(256), 0, s, b); break;
+        case VF_F32_W8: b_raycast_f<VF_F32_W8>(b, h, s); break;
+        case VF_F32_W16: b_raycast_f<VF_F32_W16>(b, h, s); break;
+        default: b_raycast_f<VF_F16_W8>(b, h, s); break;
     }
 }
 
