@@ -128,7 +128,7 @@ static unsigned newset_size(const fu_map_params* p) {
 extern "C" size_t fu_map_bytes(const fu_map_params* p) {
     const size_t ne = (size_t)p->num_buckets + p->num_excess;
     const size_t chunks = (p->num_blocks + 2047) / 2048;
-    return ne * sizeof(HEntry) + 64 + (size_t)p->num_blocks * (8 + 4 + 4) + newset_size(p) * 8ull +
+    return ne * sizeof(HEntry) + 64 + (size_t)p->num_blocks * (8 + 4 + 8 + 4) + newset_size(p) * 8ull +
            (size_t)p->max_new_blocks * 12 + chunks * 4 + occ_bits(p) / 8 +
            (size_t)p->num_blocks * vf_block_bytes((int)p->voxel_format);
 }
@@ -180,7 +180,8 @@ extern "C" fu_status fu_map_create(const fu_map_params* p, fu_map** out) {
     const size_t chunks = (p->num_blocks + 2047) / 2048;
     if ((st = dalloc(m, &v.entries, (size_t)p->num_buckets + p->num_excess)) ||
         (st = dalloc(m, &v.ctr, (size_t)C_COUNT)) || (st = dalloc(m, &v.block_key, p->num_blocks)) ||
-        (st = dalloc(m, &v.visible, p->num_blocks)) || (st = dalloc(m, &v.newset, ns)) ||
+        (st = dalloc(m, &v.visible, p->num_blocks)) || (st = dalloc(m, &v.visible_key, p->num_blocks)) ||
+        (st = dalloc(m, &v.newset, ns)) ||
         (st = dalloc(m, &v.newkeys, p->max_new_blocks)) || (st = dalloc(m, &v.newslot, p->max_new_blocks)) ||
         (st = dalloc(m, &m->mesh_counts, p->num_blocks)) || (st = dalloc(m, &m->mesh_chunk, chunks)) ||
         (st = dalloc(m, &v.occ, occ_bits(p) / 32)) ||
@@ -265,6 +266,8 @@ static FuseArgs make_fuse(const fu_map* m, const float* depth, const uint8_t* rg
     a.depth = depth;
     a.nmap = (const f4*)nmap;
     a.rgb = rgb;
+    a.px = nullptr;
+    a.prgb = nullptr;
     (void)m;
     return a;
 }
@@ -826,6 +829,8 @@ struct fu_system {
     const uint8_t* graph_rgb = nullptr;
     fu_batch* batch = nullptr;  // batch context the system is attached to
     int batch_slot = -1;
+    f4* fuse_px = nullptr;      // packed integration inputs (depth, normal) per pixel
+    unsigned* fuse_rgb = nullptr;
 };
 
 extern "C" void fu_system_params_default(fu_system_params* p) {
@@ -896,7 +901,8 @@ extern "C" fu_status fu_system_create(const fu_system_params* p, fu_system** out
     }
     b.zcols = (p->cols + 7) / 8;
     b.zrows = (p->rows + 7) / 8;
-    if ((st = salloc(sys, &b.depth_in, n0)) || (st = salloc(sys, &b.rgb_in, n0 * 3)) || (st = salloc(sys, &b.raw, n0)) ||
+    if ((st = salloc(sys, &sys->fuse_px, n0)) || (st = salloc(sys, &sys->fuse_rgb, n0)) ||
+        (st = salloc(sys, &b.depth_in, n0)) || (st = salloc(sys, &b.rgb_in, n0 * 3)) || (st = salloc(sys, &b.raw, n0)) ||
         (st = salloc(sys, &b.zrange_min, (size_t)b.zcols * b.zrows)) ||
         (st = salloc(sys, &b.zrange_max, (size_t)b.zcols * b.zrows))) {
         fu_system_destroy(sys);
@@ -969,9 +975,12 @@ static void queue_fuse_raycast(fu_system* sys, const uint8_t* rgb, hipStream_t s
     fu_map* m = sys->map;
     const TrackState* st = sys->trk->st;
     PoseF dummy{};
-    const FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, st, p.fx, p.fy, p.cx, p.cy);
+    FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, st, p.fx, p.fy, p.cx, p.cy);
+    a.px = sys->fuse_px;
+    a.prgb = sys->fuse_rgb;
     const VisArgs v = make_vis(m, dummy, st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
     launch_begin_pass(m->v, 7, s);
+    launch_fuse_pack(a, s);
     launch_alloc(m->v, a, s);
     mark(sys, FU_STAGE_ALLOC + 1, s);
     launch_visible(m->v, v, s);
@@ -1060,9 +1069,12 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     // FuseMapColor with its synchronisations (FuseMap.cu:294-317)
     fu_map* m = sys->map;
     PoseF dummy{};
-    const FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
+    FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
+    a.px = sys->fuse_px;
+    a.prgb = sys->fuse_rgb;
     const VisArgs v = make_vis(m, dummy, t->st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
     launch_begin_pass(m->v, 7, s);
+    launch_fuse_pack(a, s);
     launch_alloc(m->v, a, s);
     HIPCHK(hipStreamSynchronize(s));
     launch_visible(m->v, v, s);
@@ -1231,6 +1243,7 @@ int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, in
             case BT_ONE: gx = cnt = 1; break;
             case BT_FIXED: gx = cnt = fixed; break;
             case BT_ZTILES: gx = cnt = (d.zr.zcols * d.zr.zrows + 255) / 256; break;
+            case BT_PIX256: gx = cnt = (d.cols * d.rows + 255) / 256; break;
         }
         b.pre[i] = total;
         b.gx[i] = (unsigned short)gx;
@@ -1341,6 +1354,8 @@ static void fill_desc(fu_system* sys, SessDesc& d) {
     }
     PoseF dummy{};
     d.fuse = make_fuse(m, b.raw, nullptr, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
+    d.fuse.px = sys->fuse_px;
+    d.fuse.prgb = sys->fuse_rgb;
     d.vis = make_vis(m, dummy, t->st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
     d.zr = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min, dummy, t->st, -1,
                    p.fx, p.fy, p.cx, p.cy);

@@ -69,6 +69,7 @@ struct MapView {
     unsigned* ctr;              // [C_COUNT]
     unsigned long long* block_key;  // [num_blocks], key of each allocated block (kEmptyKey = hole)
     unsigned* visible;          // [num_blocks] block ids of the visible list
+    unsigned long long* visible_key;  // [num_blocks] their keys (read without the id -> key hop)
     unsigned long long* newset; // [newset_mask + 1] per-pass set of new keys
     unsigned long long* newkeys;  // [max_new]
     unsigned* newslot;          // [max_new]

@@ -68,6 +68,11 @@ struct FuseArgs {
     const float* depth;      // raw depth (metres, NaN = invalid)
     const f4* nmap;          // level-0 normal map of the same frame
     const unsigned char* rgb;  // 3 bytes per pixel
+    // when set, integration gathers these per-frame packed inputs instead: (depth, n.x, n.y,
+    // n.z) and r | g << 8 | b << 16 per pixel, written by the pack pass from depth, nmap and
+    // rgb (nmap's w is 1 wherever x is a number, so it is implied)
+    f4* px;
+    unsigned* prgb;
 };
 
 struct VisArgs {
@@ -149,6 +154,7 @@ void launch_begin_pass(const MapView& m, int which, hipStream_t s);
 void launch_alloc(const MapView& m, const FuseArgs& a, hipStream_t s);
 void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s);
 void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s);
+void launch_fuse_pack(const FuseArgs& a, hipStream_t s);  // fills a.px / a.prgb
 // raycast.hip
 void launch_zrange(const MapView& m, const ZRangeArgs& a, hipStream_t s);
 void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s);

@@ -301,12 +301,16 @@ __device__ __forceinline__ void visible_body(WG g, const MapView& m, const VisAr
     for (unsigned b0 = base0; b0 < nalloc; b0 += stride) {
         const unsigned i = b0 + lane_id();
         bool vis = false;
+        unsigned long long key = kEmptyKey;
         if (i < nalloc) {
-            const unsigned long long key = FU_G(const unsigned long long, m.block_key)[i];
+            key = FU_G(const unsigned long long, m.block_key)[i];
             if (key != kEmptyKey) vis = block_visible(v, pose, unpack_key(key), scale);
         }
         const unsigned idx = wave_append(&m.ctr[C_VISIBLE], vis);
-        if (vis && idx < m.num_blocks) FU_G(unsigned, m.visible)[idx] = i;
+        if (vis && idx < m.num_blocks) {
+            FU_G(unsigned, m.visible)[idx] = i;
+            FU_G(unsigned long long, m.visible_key)[idx] = key;
+        }
     }
 }
 __global__ __launch_bounds__(256) void k_visible(MapView m, VisArgs v) { visible_body(hw_wg(), m, v); }
@@ -317,9 +321,15 @@ void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
 
 // ------------------------------------------------------------ integration (FuseMap.cu:140-191)
 
-template <int F>
+// Fusion::integrateColor (FuseMap.cu:140-191).  A work-group takes kIntBlocks visible blocks per
+// iteration; each lane owns voxels li and li + 256 (index z*64 + y*8 + x) of each, so
+// 2 * kIntBlocks voxels' gathers are in flight per lane.  The next iteration's block ids and
+// keys are loaded before the current blocks are processed.
+constexpr int kIntBlocks = 2;
+template <int F, bool kPacked>
 __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const FuseArgs& a) {
     using IO = VoxelIO<F>;
+    constexpr int H = 2 * kIntBlocks;
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
     const unsigned nvis = min(FU_G(const unsigned, m.ctr)[C_VISIBLE], m.num_blocks);
@@ -327,59 +337,96 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     const auto depth = FU_G(const float, a.depth);
     const auto nmap = FU_G(const f4, a.nmap);
     const auto rgb = FU_G(const unsigned char, a.rgb);
+    const auto px = FU_G(const f4, a.px);
+    const auto prgb = FU_G(const unsigned, a.prgb);
+    const auto vis = FU_G(const unsigned, m.visible);
+    const auto vkey = FU_G(const unsigned long long, m.visible_key);
     const float thresh = m.trunc;
-    for (unsigned vb = g.x; vb < nvis; vb += g.nx) {
-        const unsigned heap = FU_G(const unsigned, m.visible)[vb];
-        const i3 bp = unpack_key(FU_G(const unsigned long long, m.block_key)[heap]);
-        unsigned char* blk = IO::block(m, (int)heap);
-        // each lane owns voxels li and li + 256 (index z*64 + y*8 + x); the work is split in
-        // phases so that both voxels' gathers are in flight together
-        int pix[2];
-        bool inb[2];
-        float pz[2];
-        f3 pos[2];
+    const unsigned stride = (unsigned)g.nx * kIntBlocks;
+    unsigned vb = (unsigned)g.x * kIntBlocks;
+    unsigned hn[kIntBlocks];
+    unsigned long long kn[kIntBlocks];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int li = threadIdx.x + h * 256;
-            const i3 vp = mki3(bp.x * 8 + (li & 7), bp.y * 8 + ((li >> 3) & 7), bp.z * 8 + (li >> 6));
+    for (int b = 0; b < kIntBlocks; ++b) {
+        hn[b] = vb + b < nvis ? vis[vb + b] : 0u;
+        kn[b] = vb + b < nvis ? vkey[vb + b] : 0ull;
+    }
+    for (; vb < nvis; vb += stride) {
+        unsigned heap[kIntBlocks];
+        i3 bp[kIntBlocks];
+        bool live[kIntBlocks];
+#pragma unroll
+        for (int b = 0; b < kIntBlocks; ++b) {
+            heap[b] = hn[b];
+            bp[b] = unpack_key(kn[b]);
+            live[b] = vb + b < nvis;
+            const unsigned nb = vb + stride + b;
+            hn[b] = nb < nvis ? vis[nb] : 0u;
+            kn[b] = nb < nvis ? vkey[nb] : 0ull;
+        }
+        int pix[H];
+        bool upd[H];
+        float sdf[H];
+        f3 pos[H];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int b = h >> 1;
+            const int li = threadIdx.x + (h & 1) * 256;
+            const i3 vp = mki3(bp[b].x * 8 + (li & 7), bp[b].y * 8 + ((li >> 3) & 7), bp[b].z * 8 + (li >> 6));
             pos[h] = mv(pose.Rinv, sub(i3f(vp, m.voxel_size), pose.t));
             const int u = f2i(a.fx * pos[h].x / pos[h].z + a.cx);
             const int v = f2i(a.fy * pos[h].y / pos[h].z + a.cy);
-            inb[h] = !(u < 0 || v < 0 || u >= cols || v >= rows);
-            pix[h] = inb[h] ? v * cols + u : 0;
-            pz[h] = pos[h].z;
+            upd[h] = live[b] && !(u < 0 || v < 0 || u >= cols || v >= rows);
+            pix[h] = upd[h] ? v * cols + u : 0;
         }
-        float dp[2];
+        float dp[H];
+        f4 nl[H];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) dp[h] = depth[pix[h]];
-        bool upd[2];
-        float sdf[2];
+        for (int h = 0; h < H; ++h) {
+            if constexpr (kPacked) {
+                const f4 q = px[pix[h]];
+                dp[h] = q.x;
+                nl[h] = mk4(q.y, q.z, q.w, 1.0f);
+            } else {
+                dp[h] = depth[pix[h]];
+            }
+        }
+        bool any = false;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const bool dv = inb[h] && !(isnanf_(dp[h]) || dp[h] > m.depth_max || dp[h] < m.depth_min);
-            sdf[h] = dp[h] - pz[h];
+        for (int h = 0; h < H; ++h) {
+            const bool dv = upd[h] && !(isnanf_(dp[h]) || dp[h] > m.depth_max || dp[h] < m.depth_min);
+            sdf[h] = dp[h] - pos[h].z;
             upd[h] = dv && (sdf[h] >= -thresh);
             if (!upd[h]) pix[h] = 0;
+            any = any || upd[h];
         }
-        if (!(upd[0] || upd[1])) continue;
-        f4 nl[2];
-        unsigned char cr[2], cg[2], cb[2];
-        float sold[2];
-        VoxAux old[2];
+        if (!any) continue;
+        unsigned char cr[H], cg[H], cb[H];
+        float sold[H];
+        VoxAux old[H];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int li = threadIdx.x + h * 256;
-            nl[h] = nmap[pix[h]];
-            cr[h] = rgb[pix[h] * 3 + 0];
-            cg[h] = rgb[pix[h] * 3 + 1];
-            cb[h] = rgb[pix[h] * 3 + 2];
+        for (int h = 0; h < H; ++h) {
+            const int li = threadIdx.x + (h & 1) * 256;
+            const unsigned char* blk = IO::block(m, (int)heap[h >> 1]);
+            if constexpr (kPacked) {
+                const unsigned c = prgb[pix[h]];
+                cr[h] = (unsigned char)c;
+                cg[h] = (unsigned char)(c >> 8);
+                cb[h] = (unsigned char)(c >> 16);
+            } else {
+                nl[h] = nmap[pix[h]];
+                cr[h] = rgb[pix[h] * 3 + 0];
+                cg[h] = rgb[pix[h] * 3 + 1];
+                cb[h] = rgb[pix[h] * 3 + 2];
+            }
             sold[h] = IO::sdf(blk, li);
             old[h] = IO::aux(blk, li);
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < H; ++h) {
             if (!upd[h] || isnanf_(nl[h].x)) continue;
-            const int li = threadIdx.x + h * 256;
+            const int li = threadIdx.x + (h & 1) * 256;
+            unsigned char* blk = IO::block(m, (int)heap[h >> 1]);
             const float s = fminf(1.0f, sdf[h] / thresh);
             // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342)
             const float w = dot4(nl[h], normalised4(mk4(pos[h].x, pos[h].y, pos[h].z, 1.0f)));
@@ -407,17 +454,39 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
         }
     }
 }
-template <int F>
+template <int F, bool kPacked>
 __global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
-    integrate_body<F>(hw_wg(), m, a);
+    integrate_body<F, kPacked>(hw_wg(), m, a);
 }
 
-void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
+// the packed per-pixel inputs of integration (FuseArgs::px, prgb)
+__device__ __forceinline__ void fuse_pack_body(WG g, const FuseArgs& a) {
+    const int i = (g.y * g.nx + g.x) * 256 + threadIdx.x;
+    if (i >= a.cols * a.rows) return;
+    const f4 n = FU_G(const f4, a.nmap)[i];
+    const auto c = FU_G(const unsigned char, a.rgb) + 3 * i;
+    FU_G(f4, a.px)[i] = mk4(FU_G(const float, a.depth)[i], n.x, n.y, n.z);
+    FU_G(unsigned, a.prgb)[i] = (unsigned)c[0] | ((unsigned)c[1] << 8) | ((unsigned)c[2] << 16);
+}
+__global__ __launch_bounds__(256) void k_fuse_pack(FuseArgs a) { fuse_pack_body(hw_wg(), a); }
+
+template <bool kPacked>
+static void integrate_f(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
     switch (m.fmt) {
-        case VF_F32_W8: hipLaunchKernelGGL(k_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
-        case VF_F32_W16: hipLaunchKernelGGL(k_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, m, a); break;
-        default: hipLaunchKernelGGL(k_integrate<VF_F16_W8>, dim3(grid), dim3(256), 0, s, m, a); break;
+        case VF_F32_W8: hipLaunchKernelGGL((k_integrate<VF_F32_W8, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
+        case VF_F32_W16: hipLaunchKernelGGL((k_integrate<VF_F32_W16, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
+        default: hipLaunchKernelGGL((k_integrate<VF_F16_W8, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
     }
+}
+void launch_fuse_pack(const FuseArgs& a, hipStream_t s) {
+    const int n = a.cols * a.rows;
+    hipLaunchKernelGGL(k_fuse_pack, dim3((n + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_integrate(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
+    if (a.px)
+        integrate_f<true>(m, a, grid, s);
+    else
+        integrate_f<false>(m, a, grid, s);
 }
 
 // ------------------------------------------------------------------ batched launches
@@ -458,7 +527,13 @@ __global__ __launch_bounds__(256) void kb_integrate(BatchLaunch b) {
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
     const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
-    integrate_body<F>(g, m, slot_fuse(b, d, s));
+    integrate_body<F, true>(g, m, slot_fuse(b, d, s));
+}
+__global__ __launch_bounds__(256) void kb_fuse_pack(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    fuse_pack_body(g, slot_fuse(b, d, s));
 }
 
 // work-groups per session for the grid-stride stages
@@ -469,7 +544,9 @@ static int per_slot(int total, int n, int lo, int hi) {
 
 void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     hipLaunchKernelGGL(kb_begin_pass, dim3(b.n), dim3(64), 0, s, b);
-    int grid = batch_layout(b, h, BT_TILES, 0, 0);
+    int grid = batch_layout(b, h, BT_PIX256, 0, 0);
+    hipLaunchKernelGGL(kb_fuse_pack, dim3(grid), dim3(256), 0, s, b);
+    grid = batch_layout(b, h, BT_TILES, 0, 0);
     hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(256), 0, s, b);
     grid = batch_layout(b, h, BT_FIXED, 0, per_slot(2048, b.n, 16, 512));
     hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);

@@ -74,7 +74,7 @@ __device__ __forceinline__ void project_body(WG g, const MapView& m, const ZRang
         int ulx = zc, uly = zr, lrx = -1, lry = -1;
         float zlo = a.depth_max, zhi = a.depth_min;
         if (live) {
-            const i3 pos = unpack_key(FU_G(const unsigned long long, m.block_key)[FU_G(const unsigned, m.visible)[i]]);
+            const i3 pos = unpack_key(FU_G(const unsigned long long, m.visible_key)[i]);
             // projectBlock (RenderScene.cu:54-104)
             for (int c = 0; c < 8; ++c) {
                 const i3 q = mki3(pos.x + (c & 1), pos.y + ((c >> 1) & 1), pos.z + ((c >> 2) & 1));
