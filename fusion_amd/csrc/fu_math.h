@@ -70,10 +70,18 @@ FU_HD bool eqi(i3 a, i3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 // leaves out-of-range conversions undefined, so the rule is spelt out and shared with the
 // CPU oracle.
 FU_HD int f2i(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_cvt_i32_f32 truncates, saturates out-of-range values and converts NaN to 0: the rule
+    // below, in one instruction
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+#else
     if (!(v == v)) return 0;
     if (v >= 2147483648.0f) return 2147483647;
     if (v <= -2147483648.0f) return (-2147483647 - 1);
     return (int)v;
+#endif
 }
 FU_HD int d2i(double v) {
     if (!(v == v)) return 0;
@@ -86,6 +94,15 @@ FU_HD i3 f2i3(f3 a) { return i3{f2i(a.x), f2i(a.y), f2i(a.z)}; }
 FU_HD uint8_t f2u8(float v) { return (uint8_t)(unsigned)f2i(v); }
 
 FU_HD float fnan() { return __builtin_nanf(""); }
+
+// (float)i / 255.f for every byte value, evaluated at compile time with IEEE division (the
+// colour blend of integrateColor divides bytes by 255, FuseMap.cu:180-186)
+struct U8Div255 {
+    float v[256];
+    constexpr U8Div255() : v() {
+        for (int i = 0; i < 256; ++i) v[i] = (float)i / 255.f;
+    }
+};
 FU_HD bool isnanf_(float v) { return !(v == v); }
 
 // ---------------------------------------------------------------------------------------

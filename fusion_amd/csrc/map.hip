@@ -23,6 +23,9 @@
 
 namespace fu {
 
+__constant__ U8Div255 c_u8div255 = U8Div255();
+__device__ __forceinline__ float u8div255(unsigned c) { return FU_G(const float, c_u8div255.v)[c]; }
+
 // ---------------------------------------------------------------- reset (FuseMap.cu:321-364)
 __global__ void k_reset_entries(HEntry* e, unsigned n) {
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -440,9 +443,9 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
             } else {
                 const float aw = w * 0.2f;
                 const float bw = 1.0f - w * 0.2f;
-                const float fr = (aw * ((float)cr[h] / 255.f) + bw * ((float)old[h].r / 255.f)) * 255.f;
-                const float fg = (aw * ((float)cg[h] / 255.f) + bw * ((float)old[h].g / 255.f)) * 255.f;
-                const float fb = (aw * ((float)cb[h] / 255.f) + bw * ((float)old[h].b / 255.f)) * 255.f;
+                const float fr = (aw * u8div255(cr[h]) + bw * u8div255(old[h].r)) * 255.f;
+                const float fg = (aw * u8div255(cg[h]) + bw * u8div255(old[h].g)) * 255.f;
+                const float fb = (aw * u8div255(cb[h]) + bw * u8div255(old[h].b)) * 255.f;
                 const float fw = (float)old[h].w;
                 IO::set_sdf(blk, li, (sold[h] * fw + w * s) / (fw + w));
                 nv.w = min(m.max_weight, old[h].w + 1u);
