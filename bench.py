@@ -64,7 +64,42 @@ def parse():
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--probe-traffic", action="store_true",
+                    help="only the program the HBM-traffic counter runs profile: a 512 MiB copy (calibration) "
+                         "and 2 + 4 served steps of 64 sessions; prints no JSON")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_served_64x640x480_5mm.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per 32-frame launch of `kernel` from the committed counter summary (two
+    rocprofv3 --pmc runs of `bench.py --probe-traffic`, calibrated with a copy), or None"""
+    try:
+        text = open(TRAFFIC_FILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)[kernel])
+    except Exception:
+        return None
+
+
+def probe_traffic(args):
+    import types
+    import torch
+    from fusion_amd import api as A
+
+    a = torch.empty(128 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    sizes = [(640, 480)] * 64
+    frames = render_sessions(A, sizes, 6)
+    torch.cuda.synchronize()
+    pa = types.SimpleNamespace(max_batch=32, inflight=2, lanes=2, warmup=2, seed=args.seed)
+    r = served(A, torch, pa, sizes, frames, 4, 0, 0)
+    log("traffic probe: %d frames served" % r["frames"])
 
 
 def cpu_cores():
@@ -265,12 +300,16 @@ def rooflines(st, sizes, nsess):
     b_rc = per_step * (px * 32 + nvis * 512 * 8)
     # ICP: 64 B per pixel-iteration (current vertex+normal, last vertex+normal)
     b_icp = (st["icp_pixels"] / steps) * 64
-    for name, b, kern in (("integrate", b_int, "kb_integrate<VF_F32_W8>"), ("raycast", b_rc, "kb_raycast<VF_F32_W8>"),
-                          ("icp", b_icp, "kb_icp + kb_icp_final (stage)")):
+    for name, b, kern, tkey in (("integrate", b_int, "kb_integrate<VF_F32_W8>", "kb_integrate<0>"),
+                                ("raycast", b_rc, "kb_raycast<VF_F32_W8>", "kb_raycast<0>"),
+                                ("icp", b_icp, "kb_icp + kb_icp_final (stage)", None)):
         ms = st["ms"][name] / steps
         ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        # measured HBM bytes of a 32-frame launch (counter runs), scaled to this step size
+        tr = measured_traffic(tkey) if tkey else None
+        traffic = int(tr * per_step / 32.0) if tr else None
         out[name] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kern, bytes_per_launch=int(b),
+                         frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, kernel=kern, bytes_per_launch=int(b),
                          avg_launch_us=round(ms * 1e3, 2), frames_per_launch=round(per_step, 2),
                          n_visible_per_frame=round(nvis, 1))
     return out
@@ -308,6 +347,9 @@ def main():
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
               "(--gpus %d refused)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.probe_traffic:
+        probe_traffic(args)
+        return
     import torch
     from fusion_amd import api as A
 
