@@ -255,30 +255,45 @@ def plain(A, torch, args, frames):
 
 def single_stream(A, torch, args):
     """one session on a 12x8x3 m multi-room scene (2^21 buckets, 2M blocks), frames replayed
-    as one hipGraph each: the latency of a frame alone on the GPU"""
+    as one hipGraph each: the latency of a frame alone on the GPU, for each voxel format, and
+    a marching-cubes extraction of the final map"""
     scene = A.Scene(1, (12.0, 8.0, 3.0), args.seed)
     n = args.single_steps
     W = 10
     dev = [A.to_device(*scene.render(f, 640, 480, K640, 5000.0, True)) for f in range(W + n)]
-    s = A.System(A.system_params(use_graph=True, map=MAP_SINGLE))
-    for i in range(W):
-        s.GrabImage(*dev[i])
-    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(n):
-        s.GrabImage(*dev[W + i])
-        ev[i + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(n)]
-    r = s.result()
-    s.close()
-    torch.cuda.synchronize()
-    return dict(value=round(n / wall, 2), frames=n, latency_ms={"p50": pct(lat, 50), "p99": pct(lat, 99)},
-                last_track_status=r["track_status"],
+    out = {}
+    for fmt, name in ((0, "f32 sdf + u8 weight"), (1, "f32 sdf + u16 weight"), (2, "f16 sdf + u8 weight")):
+        s = A.System(A.system_params(use_graph=True, map=dict(MAP_SINGLE, voxel_format=fmt)))
+        for i in range(W):
+            s.GrabImage(*dev[i])
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        t0 = time.perf_counter()
+        ev[0].record(stream)
+        for i in range(n):
+            s.GrabImage(*dev[W + i])
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [ev[i].elapsed_time(ev[i + 1]) for i in range(n)]
+        r = s.result()
+        entry = dict(value=round(n / wall, 2), latency_ms={"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                     last_track_status=r["track_status"], n_allocated=r["n_allocated"])
+        if fmt == 0:
+            # MeshScene of the whole map (the configuration meshes every 100 frames)
+            m = s.mapping()
+            m.MeshScene(max_triangles=4_000_000)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            v, _, _, _ = m.MeshScene(max_triangles=4_000_000)
+            torch.cuda.synchronize()
+            entry["mesh"] = dict(ms=round((time.perf_counter() - a) * 1e3, 3), triangles=int(v.shape[0]))
+        out[name] = entry
+        s.close()
+        torch.cuda.synchronize()
+    first = out["f32 sdf + u8 weight"]
+    return dict(value=first["value"], frames=n, latency_ms=first["latency_ms"], formats=out,
                 workload="12x8x3 m multi-room, 640x480, 2^21 buckets, 2M blocks, hipGraph per frame")
 
 
