@@ -22,6 +22,14 @@
  *  D6 ResizeMap bounds its grid by the destination size (the reference checks the source
  *     size, Pyrdown.cu:256, which reads past the source at 1280x720).
  *  D7 invalid vertex-map pixels are all-NaN (the reference sets only x, Pyrdown.cu:75).
+ *
+ * Pinning (tests/test_oracle.py checks every function against closed forms, brute force or
+ * numpy/scipy; the reference has no tests or golden vectors of its own).  PARITY UNPINNED:
+ * the exact bits of the reference's third-party solve and exponential - Eigen's LDLT
+ * (Tracking.cc:388, Eigen >= 3.2, unversioned) and Sophus::SE3d::exp (Tracking.cc:389,
+ * unpinned) - which or_solve6 / or_se3_exp replace by a Cholesky solve and the closed-form
+ * Rodrigues / left-Jacobian exponential; the tests bound them against numpy.linalg.solve and
+ * scipy.linalg.expm to about 1e-15, not to the reference's own outputs.
  */
 #ifndef FU_ORACLE_H
 #define FU_ORACLE_H
