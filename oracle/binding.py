@@ -31,6 +31,13 @@ _SIG = {
     "or_compute_se3": (I, [C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), I, I, FP,
                            C.POINTER(I), C.POINTER(I), F, DP, DP, FP, C.POINTER(I)]),
     "or_pose_f": (None, [DP, FP, FP, FP]),
+    "or_image_to_intensity": (None, [P, P, I, I]),
+    "or_pyrdown_u8": (None, [P, I, I, P]),
+    "or_derivative_image": (None, [P, I, I, P, P]),
+    "or_rgb_sums": (None, [P, P, P, P, P, P, I, I, FP, FP, FP, FP, FP, FP, F, F, F, F, C.POINTER(C.c_int32), FP,
+                           FP, DP]),
+    "or_compute_se3_rgbd": (I, [C.POINTER(P)] * 8 + [I, I, FP, C.POINTER(I), C.POINTER(I), F, DP, DP, FP,
+                                                    C.POINTER(I), C.POINTER(I)]),
     "or_map_create": (P, [U, U, U, F, F, F, F, U, I]),
     "or_map_destroy": (None, [P]),
     "or_map_reset": (None, [P]),
@@ -53,6 +60,9 @@ _SIG = {
     "or_system_grab": (I, [P, P, P, DP]),
     "or_system_map": (P, [P]),
     "or_system_buffer": (FP, [P, I, I]),
+    "or_system_set_rgb": (None, [P, I]),
+    "or_system_image": (P, [P, I, I]),
+    "or_system_rgb_count": (I, [P]),
     "or_set_threads": (None, [I]),
 }
 
@@ -194,6 +204,71 @@ def compute_se3(next_v, next_n, last_v, last_n, K, last_pose, iters=(10, 5, 4), 
     return st, out.reshape(4, 4), err.value, cnt.value
 
 
+def image_to_intensity(rgb):
+    """ImageToIntensity (Pyrdown.cu:183-194): (rows, cols, 3) u8 -> (rows, cols) u8"""
+    c = np.ascontiguousarray(rgb, dtype=np.uint8)
+    rows, cols = c.shape[:2]
+    out = np.zeros((rows, cols), np.uint8)
+    lib().or_image_to_intensity(_ptr(c), _ptr(out), cols, rows)
+    return out
+
+
+def pyrdown_u8(src):
+    s = np.ascontiguousarray(src, dtype=np.uint8)
+    rows, cols = s.shape
+    dst = np.zeros((rows // 2, cols // 2), np.uint8)
+    lib().or_pyrdown_u8(_ptr(s), cols, rows, _ptr(dst))
+    return dst
+
+
+def derivative_image(img):
+    """ComputeDerivativeImage (Pyrdown.cu:220-249): (dIdx, dIdy) int16"""
+    s = np.ascontiguousarray(img, dtype=np.uint8)
+    rows, cols = s.shape
+    dx = np.zeros((rows, cols), np.int16)
+    dy = np.zeros((rows, cols), np.int16)
+    lib().or_derivative_image(_ptr(s), cols, rows, _ptr(dx), _ptr(dy))
+    return dx, dy
+
+
+def rgb_sums(ni, li, nv, lv, dx, dy, pose_curr, pose_last, K):
+    """RGBStep (Reduction.cu:537-627): ((count, sum diff^2), sigma, sums32, sums64)"""
+    rows, cols = ni.shape
+    Rc, Rci, tc = pose_f(pose_curr)
+    Rl, Rli, tl = pose_f(pose_last)
+    keep = [np.ascontiguousarray(ni, np.uint8), np.ascontiguousarray(li, np.uint8),
+            np.ascontiguousarray(nv, np.float32), np.ascontiguousarray(lv, np.float32),
+            np.ascontiguousarray(dx, np.int16), np.ascontiguousarray(dy, np.int16)]
+    corr = np.zeros(2, np.int32)
+    sigma = C.c_float()
+    s32 = np.zeros(29, np.float32)
+    s64 = np.zeros(29, np.float64)
+    fp = lambda a: a.ctypes.data_as(FP)
+    lib().or_rgb_sums(*[_ptr(a) for a in keep], cols, rows, fp(Rc), fp(Rci), fp(Rl), fp(Rli), fp(tc), fp(tl),
+                      K[0], K[1], K[2], K[3], corr.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(sigma), fp(s32),
+                      s64.ctypes.data_as(DP))
+    return (int(corr[0]), int(corr[1])), sigma.value, s32, s64
+
+
+def compute_se3_rgbd(next_v, next_n, last_v, last_n, next_i, last_i, dx, dy, K, last_pose, iters=(10, 5, 4),
+                     min_count=(2000, 1000, 100), thresh=1e-4):
+    """ComputeSE3(icpOnly=false): (status, pose, icp_error, icp_count, rgb_count)"""
+    groups = [[np.ascontiguousarray(a, dtype=np.float32) for a in g] for g in (next_v, next_n, last_v, last_n)]
+    groups += [[np.ascontiguousarray(a, dtype=np.uint8) for a in g] for g in (next_i, last_i)]
+    groups += [[np.ascontiguousarray(a, dtype=np.int16) for a in g] for g in (dx, dy)]
+    arrs = [(C.c_void_p * 3)(*[a.ctypes.data for a in g]) for g in groups]
+    rows, cols = groups[4][0].shape
+    lp = np.ascontiguousarray(last_pose, dtype=np.float64).ravel()
+    out = np.zeros(16)
+    err = C.c_float()
+    cnt = C.c_int()
+    rcnt = C.c_int()
+    st = lib().or_compute_se3_rgbd(*arrs, cols, rows, (C.c_float * 4)(*K), (C.c_int * 3)(*iters),
+                                   (C.c_int * 3)(*min_count), thresh, lp.ctypes.data_as(DP), out.ctypes.data_as(DP),
+                                   C.byref(err), C.byref(cnt), C.byref(rcnt))
+    return st, out.reshape(4, 4), err.value, cnt.value, rcnt.value
+
+
 class Map:
     def __init__(self, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
                  trunc_dist=None, depth_min=0.1, depth_max=3.0, max_weight=255, precision=0):
@@ -315,12 +390,12 @@ class Map:
 
 
 class System:
-    """System::GrabImage (ICP-only) on the CPU"""
+    """System::GrabImage on the CPU: ICP only, or with the photometric term (use_rgb)"""
 
     def __init__(self, cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), depth_scale=5000.0, depth_cutoff=3.0,
                  iters=(10, 5, 4), min_count=(2000, 1000, 100), icp_thresh=1e-4, num_buckets=1 << 18,
                  num_excess=1 << 17, num_blocks=200_000, voxel_size=0.01, trunc_dist=None, max_weight=255,
-                 precision=0):
+                 precision=0, use_rgb=False):
         if trunc_dist is None:
             trunc_dist = float(np.float32(voxel_size) * np.float32(3.0))
         self.cols, self.rows = cols, rows
@@ -331,6 +406,7 @@ class System:
         self.map.h = lib().or_system_map(self.h)
         self.map.owned = False
         self.map.depth_min, self.map.depth_max = 0.1, 3.0
+        lib().or_system_set_rgb(self.h, 1 if use_rgb else 0)
 
     def __del__(self):
         try:
@@ -356,6 +432,17 @@ class System:
         shape = (rows, cols) if which in (0, 5) else (rows, cols, 4)
         ptr = lib().or_system_buffer(self.h, which, level)
         return np.ctypeslib.as_array(ptr, shape=shape).copy()
+
+
+    def image(self, which, level=0):
+        """0 intensity, 1 last frame's intensity (u8); 2 dIdx, 3 dIdy (i16)"""
+        rows, cols = self.rows >> level, self.cols >> level
+        ct = C.c_uint8 if which < 2 else C.c_int16
+        ptr = C.cast(lib().or_system_image(self.h, which, level), C.POINTER(ct))
+        return np.ctypeslib.as_array(ptr, shape=(rows, cols)).copy()
+
+    def rgb_count(self):
+        return lib().or_system_rgb_count(self.h)
 
 
 def set_threads(n):

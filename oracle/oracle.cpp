@@ -413,49 +413,261 @@ extern "C" void or_pose_f(const double pose[16], float R[9], float Rinv[9], floa
     }
 }
 
-// Tracker::ComputeSE3(icpOnly = true) (Tracking.cc:296-409)
-extern "C" int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap[3],
-                              const float* const last_vmap[3], const float* const last_nmap[3], int cols, int rows,
-                              const float K[4], const int iters[3], const int min_count[3], float thresh,
-                              const double last_pose[16], double pose[16], float* err_out, int* count_out) {
+// ------------------------------------------------------------------ photometric term
+// ImageToIntensityKernel (Pyrdown.cu:183-194): (int)(0.2125 c1 + 0.7154 c0 + 0.0721 c2) in
+// double, evaluated as written (two products summed, then the third; no fused multiply-add)
+extern "C" void or_image_to_intensity(const uint8_t* rgb, uint8_t* img, int cols, int rows) {
+    const size_t n = (size_t)cols * rows;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; ++i) {
+        const uint8_t* c = rgb + 3 * i;
+        const double v = (0.2125 * (double)c[1] + 0.7154 * (double)c[0]) + 0.0721 * (double)c[2];
+        img[i] = (uint8_t)toid(v);
+    }
+}
+
+// PyrDownGaussKernel<uchar,uchar> (Pyrdown.cu:133-158, launched by :171-181): the float
+// kernel's window and mirrored weights, the mean truncated to a byte
+extern "C" void or_pyrdown_u8(const uint8_t* src, int cols, int rows, uint8_t* dst) {
+    static const float g[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
+    const int dc = cols / 2, dr = rows / 2;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < dr; ++y)
+        for (int x = 0; x < dc; ++x) {
+            const int tx = std::min(2 * x + 3, cols - 1);
+            const int ty = std::min(2 * y + 3, rows - 1);
+            float sum = 0;
+            int count = 0;
+            for (int cy = std::max(0, 2 * y - 2); cy < ty; ++cy)
+                for (int cx = std::max(0, 2 * x - 2); cx < tx; ++cx) {
+                    const float w = g[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                    sum += (float)src[cy * cols + cx] * w;
+                    count = (int)((float)count + w);
+                }
+            dst[y * dc + x] = (uint8_t)toi(sum / (float)count);
+        }
+}
+
+// ComputeDerivativeImageKernel (Pyrdown.cu:220-236): 3x3 Sobel over the window clipped to
+// the image, the weight index counting down from 8 across the clipped window (so at the
+// border the weights shift), float sums stored as shorts
+extern "C" void or_derivative_image(const uint8_t* img, int cols, int rows, int16_t* dx, int16_t* dy) {
+    static const float sx[9] = {1, 0, -1, 2, 0, -2, 1, 0, -1};
+    static const float sy[9] = {1, 2, 1, 0, 0, 0, -1, -2, -1};
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < rows; ++y)
+        for (int x = 0; x < cols; ++x) {
+            float a = 0, b = 0;
+            int k = 8;
+            for (int j = std::max(y - 1, 0); j <= std::min(y + 1, rows - 1); ++j)
+                for (int i = std::max(x - 1, 0); i <= std::min(x + 1, cols - 1); ++i) {
+                    const float v = (float)img[j * cols + i];
+                    a += v * sx[k];
+                    b += v * sy[k];
+                    --k;
+                }
+            dx[y * cols + x] = (int16_t)toi(a);
+            dy[y * cols + x] = (int16_t)toi(b);
+        }
+}
+
+namespace {
+struct RgbPose {
+    M3 Rc, Rci, Rl, Rli;
+    V3 tc, tl;
+};
+RgbPose rgb_pose(const float* Rcurr, const float* RcurrInv, const float* Rlast, const float* RlastInv,
+                 const float* tcurr, const float* tlast) {
+    return RgbPose{m3f(Rcurr), m3f(RcurrInv), m3f(Rlast), m3f(RlastInv), v3(tcurr[0], tcurr[1], tcurr[2]),
+                   v3(tlast[0], tlast[1], tlast[2])};
+}
+
+// RGBReduction::FindCorresp (Reduction.cu:414-467): the last-frame pixel j and the intensity
+// difference of pixel (x, y), or false
+inline bool rgb_corresp(const uint8_t* ni, const uint8_t* li, const float* nv, const float* lv, const int16_t* dx,
+                        const int16_t* dy, int cols, int rows, int x, int y, const RgbPose& P, float fx, float fy,
+                        float cx, float cy, int& j, int& diff) {
+    if (!(x >= 5 && x < cols - 5 && y >= 5 && y < rows - 5)) return false;
+    bool valid = true;  // the 4x4 window [y-2, y+2) x [x-2, x+2) is non-zero
+    for (int u = std::max(y - 2, 0); u < std::min(y + 2, rows); ++u)
+        for (int v = std::max(x - 2, 0); v < std::min(x + 2, cols); ++v) valid = valid && ni[u * cols + v] > 0;
+    if (!valid) return false;
+    const int i = y * cols + x;
+    const int gx = dx[i], gy = dy[i];
+    if (!(sqrtf((float)(gx * gx + gy * gy)) > 0)) return false;
+    V4 vc;
+    memcpy(&vc, &nv[(size_t)i * 4], 16);
+    if (isn(vc.x)) return false;
+    const V3 vg = mul(P.Rc, xyz(vc)) + P.tc;
+    const V3 vl = mul(P.Rli, vg - P.tl);
+    // __float2int_rd(fx * x / z + cx + 0.5): float quotient and sum, the 0.5 added in double,
+    // the double rounded to float, then floor
+    const int u = toi(floorf((float)((double)(fx * vl.x / vl.z + cx) + 0.5)));
+    const int v = toi(floorf((float)((double)(fy * vl.y / vl.z + cy) + 0.5)));
+    if (!(u >= 0 && v >= 0 && u < cols && v < rows)) return false;
+    const int k = v * cols + u;
+    V4 la;
+    memcpy(&la, &lv[(size_t)k * 4], 16);
+    const V4 d{la.x - vc.x, la.y - vc.y, la.z - vc.z, la.w - vc.w};
+    if (!(!isn(la.x) && (double)sqrtf(dot4(d, d)) < 0.1 && li[k] != 0)) return false;
+    j = k;
+    diff = (int)ni[i] - (int)li[k];
+    return true;
+}
+
+// RGBReduction::GetRow (Reduction.cu:485-520) of a valid residual
+inline void rgb_row(const float* lv, const int16_t* dx, const int16_t* dy, int i, int j, int diff, float sigma,
+                    const RgbPose& P, float fx, float fy, float row[7]) {
+    float w = sigma + (float)abs(diff);
+    w = (double)w < 1e-3 ? 1.0f : 1.0f / w;
+    if ((double)sigma < 1e-6) w = 1.0f;
+    V4 la;
+    memcpy(&la, &lv[(size_t)j * 4], 16);
+    const V3 point = mul(P.Rci, (mul(P.Rl, xyz(la)) + P.tl) - P.tc);
+    const float gx = (float)((double)dx[i] / 9.0);
+    const float gy = (float)((double)dy[i] / 9.0);
+    const float invz = 1.0f / point.z;
+    V3 left;
+    left.x = gx * fx * invz;
+    left.y = gy * fy * invz;
+    left.z = -(left.x * point.x + left.y * point.y) * invz;
+    const V3 a = w * left, b = w * cross(point, left);
+    row[0] = a.x;
+    row[1] = a.y;
+    row[2] = a.z;
+    row[3] = b.x;
+    row[4] = b.y;
+    row[5] = b.z;
+    row[6] = -w * (float)diff;
+}
+}  // namespace
+
+// RGBStep (Reduction.cu:537-627): residual pass with integer sums (count, sum of squared
+// differences; 32-bit, wrapping), sigma = sqrt(sum / count), then the weighted rows.
+// corresp = {count, sum diff^2}; sums32 in pixel order in FP32, sums64 the same products in FP64.
+extern "C" void or_rgb_sums(const uint8_t* next_img, const uint8_t* last_img, const float* next_vmap,
+                            const float* last_vmap, const int16_t* dx, const int16_t* dy, int cols, int rows,
+                            const float Rcurr[9], const float RcurrInv[9], const float Rlast[9],
+                            const float RlastInv[9], const float tcurr[3], const float tlast[3], float fx, float fy,
+                            float cx, float cy, int32_t corresp[2], float* sigma_out, float sums32[29],
+                            double sums64[29]) {
+    const RgbPose P = rgb_pose(Rcurr, RcurrInv, Rlast, RlastInv, tcurr, tlast);
+    const size_t n = (size_t)cols * rows;
+    std::vector<int> jj(n, -1), dd(n, 0);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < rows; ++y)
+        for (int x = 0; x < cols; ++x) {
+            int j, d;
+            if (rgb_corresp(next_img, last_img, next_vmap, last_vmap, dx, dy, cols, rows, x, y, P, fx, fy, cx, cy, j, d)) {
+                jj[(size_t)y * cols + x] = j;
+                dd[(size_t)y * cols + x] = d;
+            }
+        }
+    uint32_t cnt = 0, sq = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (jj[i] >= 0) {
+            cnt += 1u;
+            sq += (uint32_t)(dd[i] * dd[i]);
+        }
+    const int c = (int)cnt, q = (int)sq;
+    const float sigma = sqrtf((float)q / (float)(c == 0 ? 1 : c));
+    if (corresp) {
+        corresp[0] = c;
+        corresp[1] = q;
+    }
+    if (sigma_out) *sigma_out = sigma;
+    float s32[29] = {0};
+    double s64[29] = {0};
+    for (size_t i = 0; i < n; ++i) {
+        if (jj[i] < 0) continue;
+        float r[7];
+        rgb_row(last_vmap, dx, dy, (int)i, jj[i], dd[i], sigma, P, fx, fy, r);
+        int k = 0;
+        for (int a = 0; a < 7; ++a)
+            for (int b = a; b < 7; ++b) {
+                const float p = r[a] * r[b];
+                s32[k] += p;
+                s64[k] += (double)p;
+                ++k;
+            }
+        s32[28] += 1.0f;
+        s64[28] += 1.0;
+    }
+    if (sums32) memcpy(sums32, s32, sizeof(s32));
+    if (sums64) memcpy(sums64, s64, sizeof(s64));
+}
+
+namespace {
+// CreateMatrix<6,7> (Reduction.cu:6-16)
+void unpack29(const double* s, double* A, double* b) {
+    int shift = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int k = i; k < 7; ++k) {
+            const double v = s[shift++];
+            if (k == 6)
+                b[i] = v;
+            else
+                A[k * 6 + i] = A[i * 6 + k] = v;
+        }
+}
+
+struct RgbImages {
+    const uint8_t* const* next_img;
+    const uint8_t* const* last_img;
+    const int16_t* const* dx;
+    const int16_t* const* dy;
+};
+
+// Tracker::ComputeSE3 (Tracking.cc:296-409); rgb == nullptr is icpOnly = true
+int compute_se3(const float* const next_vmap[3], const float* const next_nmap[3], const float* const last_vmap[3],
+                const float* const last_nmap[3], const RgbImages* rgb, int cols, int rows, const float K[4],
+                const int iters[3], const int min_count[3], float thresh, const double last_pose[16], double pose[16],
+                float* err_out, int* count_out, int* rgb_count_out) {
     double next[16], delta[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     memcpy(next, last_pose, sizeof(next));
     float Rl[9], Rli[9], tl[3];
     or_pose_f(last_pose, Rl, Rli, tl);
     float icp_error = 0;
     int icp_count = 0;
+    auto fail = [&]() {
+        memcpy(pose, last_pose, 16 * sizeof(double));
+        return 2;
+    };
     for (int l = 2; l >= 0; --l) {
         const int c = cols >> l, r = rows >> l;
         const float sc = (float)(1 << l);  // Intrinsics::operator() (Utility/Intrinsics.h:13-16)
+        const float fx = K[0] / sc, fy = K[1] / sc, cx = K[2] / sc, cy = K[3] / sc;
         for (int j = 0; j < iters[l]; ++j) {
             float Rc[9], Rci[9], tc[3];
             or_pose_f(next, Rc, Rci, tc);
+            double A_rgb[36], b_rgb[6];
+            if (rgb) {  // RGBStep and its failure test (Tracking.cc:318-346)
+                double r64[29];
+                or_rgb_sums(rgb->next_img[l], rgb->last_img[l], next_vmap[l], last_vmap[l], rgb->dx[l], rgb->dy[l], c,
+                            r, Rc, Rci, Rl, Rli, tc, tl, fx, fy, cx, cy, nullptr, nullptr, nullptr, r64);
+                unpack29(r64, A_rgb, b_rgb);
+                const float rgb_error = (float)(sqrt(r64[27]) / r64[28]);
+                const int rgb_count = (int)r64[28];
+                if (rgb_count_out) *rgb_count_out = rgb_count;
+                if (isn(rgb_error) || rgb_count < min_count[l]) return fail();
+            }
             double s64[29];
-            or_icp_sums(next_vmap[l], last_vmap[l], next_nmap[l], last_nmap[l], c, r, Rc, tc, Rl, Rli, tl, K[0] / sc,
-                        K[1] / sc, K[2] / sc, K[3] / sc, nullptr, s64);
+            or_icp_sums(next_vmap[l], last_vmap[l], next_nmap[l], last_nmap[l], c, r, Rc, tc, Rl, Rli, tl, fx, fy, cx,
+                        cy, nullptr, s64);
             double A[36], b[6];
-            int shift = 0;  // CreateMatrix<6,7> (Reduction.cu:6-16)
-            for (int i = 0; i < 6; ++i)
-                for (int k = i; k < 7; ++k) {
-                    const double v = s64[shift++];
-                    if (k == 6)
-                        b[i] = v;
-                    else
-                        A[k * 6 + i] = A[i * 6 + k] = v;
-                }
+            unpack29(s64, A, b);
             icp_error = (float)(sqrt(s64[27]) / s64[28]);
             icp_count = (int)s64[28];
             if (err_out) *err_out = icp_error;
             if (count_out) *count_out = icp_count;
-            if (isn(icp_error) || icp_count < min_count[l]) {
-                memcpy(pose, last_pose, 16 * sizeof(double));
-                return 2;
+            if (isn(icp_error) || icp_count < min_count[l]) return fail();
+            if (rgb) {  // matA = matA_rgb * w + matA_icp with float w = 1e-4 (Tracking.cc:379-383)
+                const double w = (double)1e-4f;
+                for (int k = 0; k < 36; ++k) A[k] = A_rgb[k] * w + A[k];
+                for (int k = 0; k < 6; ++k) b[k] = b_rgb[k] * w + b[k];
             }
             double x[6], dT[16], dinv[16];
-            if (or_solve6(A, b, x)) {
-                memcpy(pose, last_pose, 16 * sizeof(double));
-                return 2;
-            }
+            if (or_solve6(A, b, x)) return fail();
             or_se3_exp(x, dT);
             mm4(dT, delta, delta);
             inv4(delta, dinv);
@@ -477,6 +689,29 @@ extern "C" int or_compute_se3(const float* const next_vmap[3], const float* cons
     }
     memcpy(pose, last_pose, 16 * sizeof(double));
     return 3;
+}
+}  // namespace
+
+// Tracker::ComputeSE3(icpOnly = true) (Tracking.cc:296-409)
+extern "C" int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap[3],
+                              const float* const last_vmap[3], const float* const last_nmap[3], int cols, int rows,
+                              const float K[4], const int iters[3], const int min_count[3], float thresh,
+                              const double last_pose[16], double pose[16], float* err_out, int* count_out) {
+    return compute_se3(next_vmap, next_nmap, last_vmap, last_nmap, nullptr, cols, rows, K, iters, min_count, thresh,
+                       last_pose, pose, err_out, count_out, nullptr);
+}
+
+// Tracker::ComputeSE3(icpOnly = false): RGBStep blended into every ICP iteration
+extern "C" int or_compute_se3_rgbd(const float* const next_vmap[3], const float* const next_nmap[3],
+                                   const float* const last_vmap[3], const float* const last_nmap[3],
+                                   const uint8_t* const next_img[3], const uint8_t* const last_img[3],
+                                   const int16_t* const dx[3], const int16_t* const dy[3], int cols, int rows,
+                                   const float K[4], const int iters[3], const int min_count[3], float thresh,
+                                   const double last_pose[16], double pose[16], float* err_out, int* count_out,
+                                   int* rgb_count_out) {
+    const RgbImages im{next_img, last_img, dx, dy};
+    return compute_se3(next_vmap, next_nmap, last_vmap, last_nmap, &im, cols, rows, K, iters, min_count, thresh,
+                       last_pose, pose, err_out, count_out, rgb_count_out);
 }
 
 // ------------------------------------------------------------------ voxel-hash map
@@ -1227,6 +1462,12 @@ struct or_system {
     int zcols, zrows;
     double last_pose[16];
     int frame = 0;
+    // photometric term (ComputeSE3 with icpOnly = false): this frame's intensity pyramid and
+    // derivatives, and the pyramid of the last tracked frame (LastFrame->image)
+    int use_rgb = 0;
+    std::vector<uint8_t> img[3], last_img[3];
+    std::vector<int16_t> dx[3], dy[3];
+    int last_rgb_count = 0;
 };
 
 extern "C" or_system* or_system_create(int cols, int rows, float fx, float fy, float cx, float cy, float depth_scale,
@@ -1257,6 +1498,10 @@ extern "C" or_system* or_system_create(int cols, int rows, float fx, float fy, f
         s->nmap[l].resize(n * 4);
         s->mvmap[l].resize(n * 4);
         s->mnmap[l].resize(n * 4);
+        s->img[l].resize(n);
+        s->last_img[l].resize(n);
+        s->dx[l].resize(n);
+        s->dy[l].resize(n);
     }
     s->zcols = (cols + 7) / 8;
     s->zrows = (rows + 7) / 8;
@@ -1284,6 +1529,18 @@ extern "C" const float* or_system_buffer(or_system* s, int which, int level) {
     }
 }
 
+extern "C" void or_system_set_rgb(or_system* s, int use_rgb) { s->use_rgb = use_rgb; }
+extern "C" const void* or_system_image(or_system* s, int which, int level) {
+    switch (which) {
+        case 0: return s->img[level].data();
+        case 1: return s->last_img[level].data();
+        case 2: return s->dx[level].data();
+        case 3: return s->dy[level].data();
+        default: return nullptr;
+    }
+}
+extern "C" int or_system_rgb_count(const or_system* s) { return s->last_rgb_count; }
+
 extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t* rgb, double pose_out[16]) {
     const int c0 = s->cols, r0 = s->rows;
     const bool first = s->frame == 0;
@@ -1301,6 +1558,12 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
                         s->K[2] / sc, s->K[3] / sc, s->cutoff);
         or_compute_nmap(s->vmap[l].data(), s->nmap[l].data(), c0 >> l, r0 >> l);
     }
+    if (s->use_rgb) {  // ImageToIntensity, PyrDownGauss<uchar>, ComputeDerivativeImage
+        or_image_to_intensity(rgb, s->img[0].data(), c0, r0);
+        for (int l = 1; l < 3; ++l)
+            or_pyrdown_u8(s->img[l - 1].data(), c0 >> (l - 1), r0 >> (l - 1), s->img[l].data());
+        for (int l = 0; l < 3; ++l) or_derivative_image(s->img[l].data(), c0 >> l, r0 >> l, s->dx[l].data(), s->dy[l].data());
+    }
     s->frame++;
     double pose[16];
     int status = 4;
@@ -1313,11 +1576,25 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
         const float* ln[3] = {s->mnmap[0].data(), s->mnmap[1].data(), s->mnmap[2].data()};
         float err;
         int cnt;
-        status = or_compute_se3(nv, nn, lv, ln, c0, r0, s->K, s->iters, s->minc, s->thresh, s->last_pose, pose, &err, &cnt);
+        if (s->use_rgb) {
+            const uint8_t* ni[3] = {s->img[0].data(), s->img[1].data(), s->img[2].data()};
+            const uint8_t* li[3] = {s->last_img[0].data(), s->last_img[1].data(), s->last_img[2].data()};
+            const int16_t* gx[3] = {s->dx[0].data(), s->dx[1].data(), s->dx[2].data()};
+            const int16_t* gy[3] = {s->dy[0].data(), s->dy[1].data(), s->dy[2].data()};
+            status = or_compute_se3_rgbd(nv, nn, lv, ln, ni, li, gx, gy, c0, r0, s->K, s->iters, s->minc, s->thresh,
+                                         s->last_pose, pose, &err, &cnt, &s->last_rgb_count);
+        } else {
+            status = or_compute_se3(nv, nn, lv, ln, c0, r0, s->K, s->iters, s->minc, s->thresh, s->last_pose, pose,
+                                    &err, &cnt);
+        }
     }
     if (pose_out) memcpy(pose_out, pose, sizeof(pose));
     if (status != 1 && status != 4) return status;
     memcpy(s->last_pose, pose, sizeof(pose));
+    // Tracker::SwapFrame (Tracking.cc:245-247) after a tracked (or first) frame: its images
+    // become LastFrame's
+    if (s->use_rgb)
+        for (int l = 0; l < 3; ++l) s->last_img[l] = s->img[l];
     float R[9], Rinv[9], t[3];
     or_pose_f(pose, R, Rinv, t);
     // Mapping::FuseColor + Mapping::RayTrace (Mapping.cc:76-109)

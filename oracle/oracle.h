@@ -22,6 +22,10 @@
  *  D6 ResizeMap bounds its grid by the destination size (the reference checks the source
  *     size, Pyrdown.cu:256, which reads past the source at 1280x720).
  *  D7 invalid vertex-map pixels are all-NaN (the reference sets only x, Pyrdown.cu:75).
+ *  D11 ImageToIntensity's double expression (Pyrdown.cu:191) is evaluated as written, with no
+ *     fused multiply-add.  For 63 of the 2^24 colours (those whose exact luma
+ *     (2125 c1 + 7154 c0 + 721 c2) / 10000 is an integer) the truncated value depends on that
+ *     choice, so a contracting compile of the reference may differ there by one level.
  *
  * Pinning (tests/test_oracle.py checks every function against closed forms, brute force or
  * numpy/scipy; the reference has no tests or golden vectors of its own).  PARITY UNPINNED:
@@ -72,6 +76,24 @@ int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap
                    const float* const last_nmap[3], int cols, int rows, const float K[4], const int iters[3],
                    const int min_count[3], float thresh, const double last_pose[16], double pose[16],
                    float* err, int* count);
+/* photometric term (Tracking/Pyrdown.cu:127-249, Tracking/Reduction.cu:377-627) */
+void or_image_to_intensity(const uint8_t* rgb, uint8_t* img, int cols, int rows);
+void or_pyrdown_u8(const uint8_t* src, int cols, int rows, uint8_t* dst);
+void or_derivative_image(const uint8_t* img, int cols, int rows, int16_t* dx, int16_t* dy);
+/* RGBStep: corresp = {count, sum diff^2} of the residual pass, sigma, and the 29 sums of the
+ * weighted rows (FP32 in pixel order; the same FP32 products accumulated in FP64) */
+void or_rgb_sums(const uint8_t* next_img, const uint8_t* last_img, const float* next_vmap, const float* last_vmap,
+                 const int16_t* dx, const int16_t* dy, int cols, int rows, const float Rcurr[9],
+                 const float RcurrInv[9], const float Rlast[9], const float RlastInv[9], const float tcurr[3],
+                 const float tlast[3], float fx, float fy, float cx, float cy, int32_t corresp[2], float* sigma,
+                 float sums32[29], double sums64[29]);
+/* Tracker::ComputeSE3(icpOnly = false): RGBStep + ICPStep per iteration, A = A_rgb * 1e-4f + A_icp */
+int or_compute_se3_rgbd(const float* const next_vmap[3], const float* const next_nmap[3],
+                        const float* const last_vmap[3], const float* const last_nmap[3],
+                        const uint8_t* const next_img[3], const uint8_t* const last_img[3],
+                        const int16_t* const dx[3], const int16_t* const dy[3], int cols, int rows, const float K[4],
+                        const int iters[3], const int min_count[3], float thresh, const double last_pose[16],
+                        double pose[16], float* err, int* count, int* rgb_count);
 /* float views of a 4x4 pose: R, Rinv (transpose in double, then float), t */
 void or_pose_f(const double pose[16], float R[9], float Rinv[9], float t[3]);
 
@@ -120,6 +142,12 @@ void or_system_destroy(or_system* s);
 /* returns the track status (1 ok, 2 failed, 3 rejected, 4 first frame) */
 int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t* rgb, double pose_out[16]);
 or_map* or_system_map(or_system* s);
+/* 1: track with the photometric term (ComputeSE3 icpOnly = false), building the intensity
+ * pyramid and derivatives each frame */
+void or_system_set_rgb(or_system* s, int use_rgb);
+/* which: 0 intensity (u8), 1 last frame's intensity, 2 dIdx (i16), 3 dIdy */
+const void* or_system_image(or_system* s, int which, int level);
+int or_system_rgb_count(const or_system* s);
 /* level-0 maps: 0 raw depth, 1 vmap, 2 nmap, 3 model vmap, 4 model nmap */
 const float* or_system_buffer(or_system* s, int which, int level);
 void or_set_threads(int n);

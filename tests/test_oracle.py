@@ -373,3 +373,186 @@ def test_marching_cubes_sphere_is_a_closed_surface():
     radial = (vert.reshape(-1, 3) - np.array(c)) / d[:, None]
     assert np.min(np.sum(nrm * radial, axis=1)) > 0.9
     assert np.all(col.reshape(-1, 3) == [100, 150, 200])
+
+
+# ----------------------------------------------------------------------- photometric term
+def test_intensity_is_the_truncated_luma_of_every_colour():
+    """all 2^24 colours against exact rational arithmetic; where the exact luma is an integer
+    (63 colours) the double expression may land just below it, and there the oracle must
+    equal numpy's uncontracted double evaluation in the same order"""
+    v = np.arange(1 << 24, dtype=np.uint32)
+    rgb = np.stack([(v >> 16) & 255, (v >> 8) & 255, v & 255], axis=1).astype(np.uint8).reshape(4096, 4096, 3)
+    out = O.image_to_intensity(rgb).ravel().astype(np.int64)
+    c = rgb.reshape(-1, 3).astype(np.int64)
+    num = 2125 * c[:, 1] + 7154 * c[:, 0] + 721 * c[:, 2]  # 0.2125 G + 0.7154 R + 0.0721 B (channel 0 = R)
+    exact = num // 10000
+    tie = (num % 10000) == 0
+    assert np.array_equal(out[~tie], exact[~tie])
+    cf = c[tie].astype(np.float64)
+    dbl = (0.2125 * cf[:, 1] + 0.7154 * cf[:, 0]) + 0.0721 * cf[:, 2]
+    assert np.array_equal(out[tie], np.trunc(dbl).astype(np.int64))
+    assert ((out[tie] == exact[tie]) | (out[tie] == exact[tie] - 1)).all()
+    assert int(tie.sum()) > 0 and int((out[tie] != exact[tie]).sum()) == 63
+
+
+def test_pyrdown_u8_is_the_truncated_float_pyramid():
+    """interior: the binomial 5x5 mean (scipy), truncated; everywhere, including the mirrored
+    border weights: the float pyramid (pinned above) of the same bytes, truncated"""
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(4)
+    src = rng.integers(0, 256, size=(36, 52), dtype=np.uint8)
+    dst = O.pyrdown_u8(src)
+    assert dst.shape == (18, 26) and dst.dtype == np.uint8
+    assert np.array_equal(dst, np.trunc(O.pyrdown_gauss(src.astype(np.float32))).astype(np.uint8))
+    g = np.array([1, 4, 6, 4, 1], np.float64)
+    full = correlate(src.astype(np.float64), np.outer(g, g) / 256.0, mode="constant")
+    ys, xs = np.mgrid[1:17, 1:25]
+    ref = np.floor(full[2 * ys, 2 * xs] + 1e-9).astype(np.uint8)
+    assert np.array_equal(dst[1:17, 1:25], ref)
+    assert np.array_equal(O.pyrdown_u8(np.full((20, 20), 77, np.uint8)), np.full((10, 10), 77, np.uint8))
+
+
+def test_derivative_image_sobel_interior_and_clipped_border():
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, size=(17, 23), dtype=np.uint8)
+    dx, dy = O.derivative_image(img)
+    assert dx.dtype == np.int16 and dy.dtype == np.int16
+    kx = np.array([[-1, 0, 1], [-2, 0, 2], [-1, 0, 1]], np.float64)  # +x to the right
+    ix = correlate(img.astype(np.float64), kx, mode="constant")
+    iy = correlate(img.astype(np.float64), kx.T, mode="constant")
+    assert np.array_equal(dx[1:-1, 1:-1], ix[1:-1, 1:-1].astype(np.int16))
+    assert np.array_equal(dy[1:-1, 1:-1], iy[1:-1, 1:-1].astype(np.int16))
+    # a ramp a x + b y: 8a, 8b inside
+    yy, xx = np.mgrid[0:12, 0:16]
+    rdx, rdy = O.derivative_image((3 * xx + 5 * yy).astype(np.uint8))
+    assert (rdx[1:-1, 1:-1] == 24).all() and (rdy[1:-1, 1:-1] == 40).all()
+    # top-left corner: the 2x2 window takes kernel entries 8, 7, 6, 5 (counted down in window
+    # order), i.e. dx = -v00 + v10 - 2 v11 and dy = -v00 - 2 v01 - v10 (Pyrdown.cu:226-233)
+    v = img.astype(np.int64)
+    assert dx[0, 0] == -v[0, 0] + v[1, 0] - 2 * v[1, 1]
+    assert dy[0, 0] == -v[0, 0] - 2 * v[0, 1] - v[1, 0]
+
+
+def _plane_frame(t, cols=640, rows=480, Kq=K, Z=1.5):
+    """a fronto-parallel plane z = Z (world) with a smooth grey texture, seen by a camera at
+    translation t (no rotation): depth u16 at 5000/m and the colour image"""
+    v, u = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    X = t[0] + (Z - t[2]) * (u - Kq[2]) / Kq[0]
+    Y = t[1] + (Z - t[2]) * (v - Kq[3]) / Kq[1]
+    inten = 128.0 + 50.0 * np.sin(2 * np.pi * X / 0.15) * np.cos(2 * np.pi * Y / 0.2)
+    g = np.clip(np.floor(inten + 0.5), 1, 255).astype(np.uint8)
+    d = np.full((rows, cols), int(round((Z - t[2]) * 5000)), np.uint16)
+    return d, np.repeat(g[:, :, None], 3, axis=2)
+
+
+def _plane_frame_pose(T, cols=640, rows=480, Kq=K, Z=1.5):
+    """the textured plane z = Z seen by a camera at any pose T (ray-plane intersection)"""
+    v, u = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    dc = np.stack([(u - Kq[2]) / Kq[0], (v - Kq[3]) / Kq[1], np.ones_like(u)], -1)
+    d = dc @ T[:3, :3].T
+    o = T[:3, 3]
+    s = (Z - o[2]) / d[..., 2]  # the camera-frame ray has z = 1, so s is the depth
+    X = o[0] + s * d[..., 0]
+    Y = o[1] + s * d[..., 1]
+    inten = 128.0 + 50.0 * np.sin(2 * np.pi * X / 0.15) * np.cos(2 * np.pi * Y / 0.2)
+    g = np.clip(np.floor(inten + 0.5), 1, 255).astype(np.uint8)
+    return np.floor(s * 5000 + 0.5).astype(np.uint16), np.repeat(g[:, :, None], 3, axis=2)
+
+
+def _images(rgb):
+    im = [O.image_to_intensity(rgb)]
+    im += [O.pyrdown_u8(im[0])]
+    im += [O.pyrdown_u8(im[1])]
+    der = [O.derivative_image(i) for i in im]
+    return im, [d[0] for d in der], [d[1] for d in der]
+
+
+def test_rgb_residuals_of_a_frame_against_itself():
+    """identity pose and the same frame twice: every eligible pixel matches itself with zero
+    difference, so sigma = 0, every weight is 1 and b = 0"""
+    d, rgb = _plane_frame((0.0, 0.0, 0.0))
+    vs, ns = _pyr(d, K)
+    im, dx, dy = _images(rgb)
+    (cnt, sq), sigma, s32, s64 = O.rgb_sums(im[0], im[0], vs[0], vs[0], dx[0], dy[0], np.eye(4), np.eye(4), K)
+    rows, cols = im[0].shape
+    yy, xx = np.mgrid[0:rows, 0:cols]
+    inner = (xx >= 5) & (xx < cols - 5) & (yy >= 5) & (yy < rows - 5)
+    win = np.ones_like(inner)
+    for a in range(-2, 2):  # the 4x4 window [y-2, y+2) x [x-2, x+2)
+        for b in range(-2, 2):
+            win &= np.roll(np.roll(im[0], -a, 0), -b, 1) > 0
+    grad = (dx[0] != 0) | (dy[0] != 0)
+    assert cnt == int((inner & win & grad & ~np.isnan(vs[0][..., 0])).sum()) > 200000
+    assert sq == 0 and sigma == 0.0
+    assert s64[28] == cnt
+    A, b = O.unpack_sums(s64)
+    assert np.all(b == 0) and np.all(np.linalg.eigvalsh(A) >= -1e-6)
+
+
+def test_rgb_term_recovers_in_plane_motion_that_icp_cannot():
+    """a textured plane: point-to-plane ICP leaves the in-plane translation free (its normal
+    matrix is singular there), the photometric term pins it; the blended tracker must recover
+    a known sideways motion.  The residual pairs pixels by nearest-pixel projection, so every
+    pose within half a level-0 pixel of the truth (1.43 mm here) has the same residuals: the
+    bound is 0.6 pixel."""
+    t = np.array([0.012, -0.009, 0.0])
+    d0, c0 = _plane_frame((0.0, 0.0, 0.0))
+    d1, c1 = _plane_frame(t)
+    v0, n0 = _pyr(d0, K)
+    v1, n1 = _pyr(d1, K)
+    i0, _, _ = _images(c0)
+    i1, dx1, dy1 = _images(c1)
+    s32, s64 = O.icp_sums(v1[0], v0[0], n1[0], n0[0], np.eye(4), np.eye(4), K)
+    A, _ = O.unpack_sums(s64)
+    ev = np.linalg.eigvalsh(A)
+    assert ev[2] < 1e-6 * ev[-1]  # tx, ty and the roll are unobservable by ICP
+    st, pose, err, cnt, rcnt = O.compute_se3_rgbd(v1, n1, v0, n0, i1, i0, dx1, dy1, K, np.eye(4))
+    px = 1.5 / K[0]
+    assert st == 1 and rcnt > 100
+    assert np.abs(pose[:3, 3] - t).max() < 0.6 * px, pose[:3, 3]
+    assert np.abs(pose[:3, :3] - np.eye(3)).max() < 2e-3
+    # the sign of the photometric rows matters: the reversed motion is recovered as well
+    st, pose, _, _, _ = O.compute_se3_rgbd(v0, n0, v1, n1, i0, i1, *_images(c0)[1:], K, np.eye(4))
+    assert st == 1 and np.abs(pose[:3, 3] + t).max() < 0.6 * px, pose[:3, 3]
+
+
+def test_rgb_rows_alone_recover_a_6dof_motion():
+    """Gauss-Newton on the photometric sums alone (numpy solve, the oracle's exponential),
+    coarse to fine: every column of the Jacobian rows and the sign of the residual take part"""
+    xi = np.array([0.006, -0.004, 0.005, np.radians(0.6), np.radians(-0.5), np.radians(0.8)])
+    T = O.se3_exp(xi)
+    d0, c0 = _plane_frame_pose(np.eye(4))
+    d1, c1 = _plane_frame_pose(T)
+    v0, _ = _pyr(d0, K)
+    v1, _ = _pyr(d1, K)
+    i0, _, _ = _images(c0)
+    i1, dx1, dy1 = _images(c1)
+    last, delta, nxt = np.eye(4), np.eye(4), np.eye(4)
+    for l, iters in ((2, 10), (1, 5), (0, 4)):
+        Kl = [float(np.float32(k) / np.float32(1 << l)) for k in K]
+        for _ in range(iters):
+            _, _, _, s64 = O.rgb_sums(i1[l], i0[l], v1[l], v0[l], dx1[l], dy1[l], nxt, last, Kl)
+            A, b = O.unpack_sums(s64)
+            delta = O.se3_exp(np.linalg.solve(A, b)) @ delta
+            nxt = last @ np.linalg.inv(delta)
+    assert np.abs(nxt[:3, 3] - T[:3, 3]).max() < 5e-4
+    assert np.degrees(np.abs(nxt[:3, :3] - T[:3, :3]).max()) < 0.03
+
+
+def test_rgb_sums_fp32_close_to_fp64_and_failure_on_no_texture():
+    d0, c0 = _plane_frame((0.0, 0.0, 0.0))
+    d1, c1 = _plane_frame((0.01, 0.0, 0.0))
+    v0, _ = _pyr(d0, K)
+    v1, _ = _pyr(d1, K)
+    i0, _, _ = _images(c0)
+    i1, dx1, dy1 = _images(c1)
+    (cnt, sq), sigma, s32, s64 = O.rgb_sums(i1[0], i0[0], v1[0], v0[0], dx1[0], dy1[0], np.eye(4), np.eye(4), K)
+    assert cnt > 100000 and sq > 0 and sigma == np.float32(np.sqrt(np.float32(sq) / np.float32(cnt)))
+    assert np.all(np.abs(s32 - s64) <= 1e-3 * np.abs(s64).max())
+    # a flat grey image has no gradient: no residuals, RGBStep fails the frame
+    flat = np.full_like(c1, 90)
+    fi, fdx, fdy = _images(flat)
+    v0n, n0n = _pyr(d0, K)
+    st, pose, _, _, rcnt = O.compute_se3_rgbd(v0n, n0n, v0n, n0n, fi, fi, fdx, fdy, K, np.eye(4))
+    assert st == 2 and rcnt == 0 and np.array_equal(pose, np.eye(4))
