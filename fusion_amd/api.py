@@ -104,6 +104,49 @@ def ICPStep(next_vmap, last_vmap, next_nmap, last_nmap, pose_curr, pose_last, K)
     return np.array(A[:]).reshape(6, 6), np.array(b[:]), np.array(res[:], dtype=np.float32)
 
 
+# --------------------------------------------------------------------- photometric term
+def ImageToIntensity(rgb):
+    """ImageToIntensity (Pyrdown.cu:196-206): device uint8 (H, W, 3) -> uint8 (H, W)"""
+    rows, cols = rgb.shape[:2]
+    img = torch.empty((rows, cols), dtype=torch.uint8, device=rgb.device)
+    check(N.lib().fu_image_to_intensity(_p(rgb), _p(img), cols, rows, _s()), "fu_image_to_intensity")
+    return img
+
+
+def PyrDownGaussU8(src):
+    """PyrDownGauss on bytes (Pyrdown.cu:171-181)"""
+    rows, cols = src.shape
+    dst = torch.empty((rows // 2, cols // 2), dtype=torch.uint8, device=src.device)
+    check(N.lib().fu_pyrdown_gauss_u8(_p(src), cols, rows, _p(dst), _s()), "fu_pyrdown_gauss_u8")
+    return dst
+
+
+def ComputeDerivativeImage(image):
+    """ComputeDerivativeImage (Pyrdown.cu:238-249): (dIdx, dIdy) int16"""
+    rows, cols = image.shape
+    dx = torch.empty((rows, cols), dtype=torch.int16, device=image.device)
+    dy = torch.empty_like(dx)
+    check(N.lib().fu_compute_derivative_image(_p(image), _p(dx), _p(dy), cols, rows, _s()),
+          "fu_compute_derivative_image")
+    return dx, dy
+
+
+def RGBStep(next_image, last_image, next_vmap, last_vmap, dIdx, dIdy, pose_curr, pose_last, K):
+    """One photometric reduction (Reduction.cu:537-627): (A 6x6, b 6, residual[2],
+    (count, sum diff^2))"""
+    rows, cols = next_image.shape
+    pc, pl = pose_f(pose_curr), pose_f(pose_last)
+    res = (C.c_float * 2)()
+    A = (C.c_double * 36)()
+    b = (C.c_double * 6)()
+    corr = (C.c_int32 * 2)()
+    check(N.lib().fu_rgb_step(_p(next_image), _p(last_image), _p(next_vmap), _p(last_vmap), _p(dIdx), _p(dIdy),
+                              cols, rows, pc.R, pc.Rinv, pl.R, pl.Rinv, pc.t, pl.t, K[0], K[1], K[2], K[3], res, A, b,
+                              corr, _s()), "fu_rgb_step")
+    return (np.array(A[:]).reshape(6, 6), np.array(b[:]), np.array(res[:], dtype=np.float32),
+            (int(corr[0]), int(corr[1])))
+
+
 # --------------------------------------------------------------------- map (Mapping)
 def map_params(**kw):
     p = N.MapParams()
@@ -259,12 +302,23 @@ class Tracker:
                                      (C.c_int32 * 3)(*min_count), thresh, _s()), "fu_compute_se3")
         return self.result()
 
+    def ComputeSE3RGBD(self, next_vmaps, next_nmaps, last_vmaps, last_nmaps, next_images, last_images, dIdx, dIdy, K,
+                       iters=(10, 5, 4), min_count=(2000, 1000, 100), thresh=1e-4):
+        """ComputeSE3(icpOnly=false): the photometric term blended into every iteration"""
+        arr = lambda ts: (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+        check(N.lib().fu_compute_se3_rgbd(self.handle, arr(next_vmaps), arr(next_nmaps), arr(last_vmaps),
+                                          arr(last_nmaps), arr(next_images), arr(last_images), arr(dIdx), arr(dIdy),
+                                          (C.c_float * 4)(*K), (C.c_int32 * 3)(*iters), (C.c_int32 * 3)(*min_count),
+                                          thresh, _s()), "fu_compute_se3_rgbd")
+        return self.result()
+
     def result(self):
         r = N.TrackResult()
         check(N.lib().fu_tracker_result(self.handle, C.byref(r), _s()), "fu_tracker_result")
         return {"pose": np.array(r.pose[:]).reshape(4, 4), "last_pose": np.array(r.last_pose[:]).reshape(4, 4),
                 "x": np.array(r.x[:]), "A": np.array(r.A[:]).reshape(6, 6), "b": np.array(r.b[:]),
-                "icp_error": r.icp_error, "count": r.count, "status": r.status, "iterations": r.iterations}
+                "icp_error": r.icp_error, "count": r.count, "status": r.status, "iterations": r.iterations,
+                "rgb_error": r.rgb_error, "rgb_count": r.rgb_count}
 
 
 class IcpBatch:
@@ -318,7 +372,8 @@ def system_params(cols=640, rows=480, K=(525.0, 525.0, 319.5, 239.5), plain=Fals
 
 
 class System:
-    """System::GrabImage (Core/System.cc:92-138) in ICP-only mode."""
+    """System::GrabImage (Core/System.cc:92-138): ICP only, or with the photometric term
+    (params.use_rgb, the reference's default TrackFrame)."""
 
     def __init__(self, params):
         _dev()
@@ -375,6 +430,11 @@ class System:
             addr = ptr
         else:
             addr = ptr[level]
+        if name in ("image", "last_image", "dIdx", "dIdy"):
+            if not addr:
+                raise N.FusionError("%s: the system tracks without the photometric term" % name)
+            typestr = "|u1" if name in ("image", "last_image") else "<i2"
+            return torch.as_tensor(_DevBuf(addr, (rows, cols), typestr), device="cuda")
         shape = {"raw": (rows, cols), "depth": (rows, cols)}.get(name, (rows, cols, 4))
         n = int(np.prod(shape))
         return _wrap(addr, n, shape)

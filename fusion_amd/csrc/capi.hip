@@ -613,7 +613,9 @@ extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int col
 struct fu_tracker {
     int cols, rows;
     TrackState* st = nullptr;
-    float* partials = nullptr;
+    float* partials = nullptr;  // 64 floats per ICP work-group
+    int* rgb_rec = nullptr;     // photometric residual records, cols * rows
+    int* rgb_corr = nullptr;    // 2 ints per ICP work-group
     int nblocks[3];
 };
 
@@ -646,10 +648,14 @@ extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
     t->cols = cols;
     t->rows = rows;
     for (int l = 0; l < 3; ++l) t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
-    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 32 * 4) != hipSuccess) {
+    if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 64 * 4) != hipSuccess ||
+        hipMalloc(&t->rgb_corr, kIcpMaxBlocks * 2 * 4) != hipSuccess ||
+        hipMalloc(&t->rgb_rec, (size_t)cols * rows * 4) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(t->st);
         (void)hipFree(t->partials);
+        (void)hipFree(t->rgb_corr);
+        (void)hipFree(t->rgb_rec);
         delete t;
         return set_err(FU_ERR_NOMEM, "tracker allocation failed");
     }
@@ -664,6 +670,8 @@ extern "C" fu_status fu_tracker_destroy(fu_tracker* t) {
     if (!t) return FU_OK;
     (void)hipFree(t->st);
     (void)hipFree(t->partials);
+    (void)hipFree(t->rgb_corr);
+    (void)hipFree(t->rgb_rec);
     delete t;
     return FU_OK;
 }
@@ -700,13 +708,35 @@ static IcpArgs make_icp(fu_tracker* t, int level, const float* nv, const float* 
     a.min_count = min_count;
     a.final_iter = final_iter;
     a.thresh = thresh;
+    a.use_rgb = 0;
+    memset(&a.rgb, 0, sizeof(a.rgb));
     return a;
 }
 
+// the photometric inputs of one level (RGBStep's images, Tracking.cc:319-325)
+static void set_rgb(IcpArgs& a, fu_tracker* t, const uint8_t* inext, const uint8_t* ilast, const int16_t* dx,
+                    const int16_t* dy) {
+    a.use_rgb = 1;
+    a.rgb.inext = inext;
+    a.rgb.ilast = ilast;
+    a.rgb.dx = dx;
+    a.rgb.dy = dy;
+    a.rgb.rec = t->rgb_rec;
+    a.rgb.corr = t->rgb_corr;
+}
+
+// the intensity pyramids and derivatives of ComputeSE3(icpOnly = false); null for ICP only
+struct RgbLevels {
+    const uint8_t* const* next_image;
+    const uint8_t* const* last_image;
+    const int16_t* const* dx;
+    const int16_t* const* dy;
+};
+
 // queue the whole coarse-to-fine loop (device-resident)
 static void queue_se3(fu_tracker* t, const float* const nv[3], const float* const nn[3], const float* const lv[3],
-                      const float* const ln[3], const float K[4], const int32_t iters[3], const int32_t minc[3],
-                      float thresh, hipStream_t s) {
+                      const float* const ln[3], const RgbLevels* rgb, const float K[4], const int32_t iters[3],
+                      const int32_t minc[3], float thresh, hipStream_t s) {
     launch_track_begin(t->st, s);
     int last_level = -1;
     for (int l = 2; l >= 0; --l)
@@ -714,7 +744,9 @@ static void queue_se3(fu_tracker* t, const float* const nv[3], const float* cons
     for (int l = 2; l >= 0; --l)
         for (int j = 0; j < iters[l]; ++j) {
             const int fin = (l == last_level && j == iters[l] - 1);
-            launch_icp(make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], fin, thresh), t->nblocks[l], s);
+            IcpArgs a = make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], fin, thresh);
+            if (rgb) set_rgb(a, t, rgb->next_image[l], rgb->last_image[l], rgb->dx[l], rgb->dy[l]);
+            launch_icp(a, t->nblocks[l], s);
         }
 }
 
@@ -726,7 +758,28 @@ extern "C" fu_status fu_compute_se3(fu_tracker* t, const float* const next_vmap[
     for (int l = 0; l < 3; ++l)
         if (!next_vmap[l] || !next_nmap[l] || !last_vmap[l] || !last_nmap[l] || iters[l] < 0)
             return set_err(FU_ERR_ARG, "fu_compute_se3: bad level %d", l);
-    queue_se3(t, next_vmap, next_nmap, last_vmap, last_nmap, K, iters, min_count, thresh, S(s));
+    queue_se3(t, next_vmap, next_nmap, last_vmap, last_nmap, nullptr, K, iters, min_count, thresh, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_compute_se3_rgbd(fu_tracker* t, const float* const next_vmap[3],
+                                         const float* const next_nmap[3], const float* const last_vmap[3],
+                                         const float* const last_nmap[3], const uint8_t* const next_image[3],
+                                         const uint8_t* const last_image[3], const int16_t* const dIdx[3],
+                                         const int16_t* const dIdy[3], const float K[4], const int32_t iters[3],
+                                         const int32_t min_count[3], float thresh, fu_stream s) {
+    if (!t || !next_vmap || !next_nmap || !last_vmap || !last_nmap || !next_image || !last_image || !dIdx || !dIdy ||
+        !K || !iters || !min_count)
+        return set_err(FU_ERR_ARG, "fu_compute_se3_rgbd: null argument");
+    if ((long)t->cols * t->rows > kRgbMaxPixels)
+        return set_err(FU_ERR_ARG, "fu_compute_se3_rgbd: %dx%d exceeds %d pixels", t->cols, t->rows, kRgbMaxPixels);
+    for (int l = 0; l < 3; ++l)
+        if (!next_vmap[l] || !next_nmap[l] || !last_vmap[l] || !last_nmap[l] || !next_image[l] || !last_image[l] ||
+            !dIdx[l] || !dIdy[l] || iters[l] < 0)
+            return set_err(FU_ERR_ARG, "fu_compute_se3_rgbd: bad level %d", l);
+    const RgbLevels rgb{next_image, last_image, dIdx, dIdy};
+    queue_se3(t, next_vmap, next_nmap, last_vmap, last_nmap, &rgb, K, iters, min_count, thresh, S(s));
     LAUNCHCHK();
     return FU_OK;
 }
@@ -741,6 +794,8 @@ static void fill_result(const TrackState& h, fu_track_result* out) {
     out->count = h.count;
     out->status = h.status;
     out->iterations = h.iters_done;
+    out->rgb_error = h.rgb_error;
+    out->rgb_count = h.rgb_count;
 }
 
 extern "C" fu_status fu_tracker_result(fu_tracker* t, fu_track_result* out, fu_stream s) {
@@ -809,6 +864,109 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
         residual[0] = h.sums[27];
         residual[1] = h.sums[28];
     }
+    return FU_OK;
+}
+
+// RGBStep (Reduction.h:74-84, Reduction.cu:537-627): the photometric reduction of one level.
+// residual = {sum of the weighted squared differences, count}, A and b in FP64 on the host,
+// corresp = {count, sum of squared differences} of the residual pass (may be null).
+// Synchronises.
+extern "C" fu_status fu_rgb_step(const uint8_t* next_image, const uint8_t* last_image, const float* next_vmap,
+                                 const float* last_vmap, const int16_t* dIdx, const int16_t* dIdy, int cols, int rows,
+                                 const float Rcurr[9], const float RcurrInv[9], const float Rlast[9],
+                                 const float RlastInv[9], const float tcurr[3], const float tlast[3], float fx,
+                                 float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                                 int32_t corresp[2], fu_stream s) {
+    if (!next_image || !last_image || !next_vmap || !last_vmap || !dIdx || !dIdy || !Rcurr || !RcurrInv || !Rlast ||
+        !RlastInv || !tcurr || !tlast)
+        return set_err(FU_ERR_ARG, "fu_rgb_step: null argument");
+    if (cols <= 0 || rows <= 0 || (long)cols * rows > kRgbMaxPixels)
+        return set_err(FU_ERR_ARG, "fu_rgb_step: bad size %dx%d", cols, rows);
+    // scratch per thread, grown to the largest image seen (the reference allocates its
+    // residual array on every call, Reduction.cu:572)
+    static thread_local fu_tracker* scratch = nullptr;
+    if (!scratch || (long)scratch->cols * scratch->rows < (long)cols * rows) {
+        fu_tracker_destroy(scratch);
+        scratch = nullptr;
+        fu_status st = fu_tracker_create(cols < 4 ? 4 : cols, rows < 4 ? 4 : rows, &scratch);
+        if (st) return st;
+    }
+    TrackState h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.Rcurr, Rcurr, 36);
+    memcpy(h.RcurrInv, RcurrInv, 36);
+    memcpy(h.tcurr, tcurr, 12);
+    memcpy(h.Rlast, Rlast, 36);
+    memcpy(h.RlastInv, RlastInv, 36);
+    memcpy(h.tlast, tlast, 12);
+    h.status = TS_RUNNING;
+    HIPCHK(hipMemcpyAsync(scratch->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    IcpArgs a;
+    memset(&a, 0, sizeof(a));
+    // no normal maps: the combined pass computes only the photometric rows
+    a.vcurr = (const f4*)next_vmap;
+    a.vlast = (const f4*)last_vmap;
+    a.ncurr = nullptr;
+    a.nlast = nullptr;
+    a.cols = cols;
+    a.rows = rows;
+    a.fx = fx;
+    a.fy = fy;
+    a.cx = cx;
+    a.cy = cy;
+    a.st = scratch->st;
+    a.partials = scratch->partials;
+    a.mode = 0;
+    set_rgb(a, scratch, next_image, last_image, dIdx, dIdy);
+    launch_icp(a, icp_blocks(cols * rows), S(s));
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(&h, scratch->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    double Ah[36], bh[6];
+    unpack_sums29(h.rgb_sums, Ah, bh);
+    if (A) memcpy(A, Ah, sizeof(Ah));
+    if (b) memcpy(b, bh, sizeof(bh));
+    if (residual) {
+        residual[0] = h.rgb_sums[27];
+        residual[1] = h.rgb_sums[28];
+    }
+    if (corresp) {
+        corresp[0] = h.rgb_corresp[0];
+        corresp[1] = h.rgb_corresp[1];
+    }
+    return FU_OK;
+}
+
+static fu_status check_u8(const void* a, int cols, int rows, const char* what) { return check_img(a, cols, rows, what); }
+
+// ImageToIntensity (Reduction.h:34-35, Pyrdown.cu:196-206)
+extern "C" fu_status fu_image_to_intensity(const uint8_t* rgb, uint8_t* image, int cols, int rows, fu_stream s) {
+    fu_status st = check_u8(rgb, cols, rows, "fu_image_to_intensity");
+    if (st) return st;
+    if (!image) return set_err(FU_ERR_ARG, "fu_image_to_intensity: null output");
+    launch_intensity(rgb, image, cols, rows, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+// PyrDownGauss on bytes (Reduction.h:31-32, Pyrdown.cu:171-181); dst is (cols/2) x (rows/2)
+extern "C" fu_status fu_pyrdown_gauss_u8(const uint8_t* src, int cols, int rows, uint8_t* dst, fu_stream s) {
+    fu_status st = check_u8(src, cols, rows, "fu_pyrdown_gauss_u8");
+    if (st) return st;
+    if (!dst || cols < 4 || rows < 4) return set_err(FU_ERR_ARG, "fu_pyrdown_gauss_u8: bad destination");
+    launch_pyrdown_u8(src, cols, rows, dst, cols / 2, rows / 2, S(s));
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+// ComputeDerivativeImage (Reduction.h:37-38, Pyrdown.cu:238-249)
+extern "C" fu_status fu_compute_derivative_image(const uint8_t* image, int16_t* dIdx, int16_t* dIdy, int cols,
+                                                 int rows, fu_stream s) {
+    fu_status st = check_u8(image, cols, rows, "fu_compute_derivative_image");
+    if (st) return st;
+    if (!dIdx || !dIdy) return set_err(FU_ERR_ARG, "fu_compute_derivative_image: null output");
+    launch_derivative(image, dIdx, dIdy, cols, rows, S(s));
+    LAUNCHCHK();
     return FU_OK;
 }
 
@@ -885,6 +1043,8 @@ extern "C" fu_status fu_system_create(const fu_system_params* p, fu_system** out
     if (p->cols < 32 || p->rows < 32 || (p->cols % 4) || (p->rows % 4))
         return set_err(FU_ERR_ARG, "frame size %dx%d must be >= 32 and a multiple of 4", p->cols, p->rows);
     if (!(p->depth_scale > 0) || !(p->fx > 0) || !(p->fy > 0)) return set_err(FU_ERR_ARG, "bad camera parameters");
+    if (p->use_rgb && (long)p->cols * p->rows > kRgbMaxPixels)
+        return set_err(FU_ERR_ARG, "photometric tracking is limited to %d pixels", kRgbMaxPixels);
     fu_system* sys = new fu_system();
     sys->p = *p;
     fu_status st;
@@ -916,6 +1076,11 @@ extern "C" fu_status fu_system_create(const fu_system_params* p, fu_system** out
             fu_system_destroy(sys);
             return st;
         }
+        if (p->use_rgb && ((st = salloc(sys, &b.image[l], n)) || (st = salloc(sys, &b.last_image[l], n)) ||
+                           (st = salloc(sys, &b.dIdx[l], n)) || (st = salloc(sys, &b.dIdy[l], n)))) {
+            fu_system_destroy(sys);
+            return st;
+        }
     }
     *out = sys;
     return fu_system_reset(sys, nullptr);
@@ -941,7 +1106,24 @@ extern "C" fu_status fu_system_get_buffers(fu_system* sys, fu_system_buffers* ou
 
 extern "C" fu_map* fu_system_map(fu_system* sys) { return sys ? sys->map : nullptr; }
 
-// Frame::FillImages (Core/Frame.cc:59-78) without the intensity/derivative images
+// ImageToIntensity, PyrDownGauss of the intensity, ComputeDerivativeImage (Frame.cc:63-76)
+static void queue_images(fu_system* sys, const uint8_t* rgb, hipStream_t s) {
+    fu_system_buffers& b = sys->b;
+    launch_intensity(rgb, b.image[0], sys->lc[0], sys->lr[0], s);
+    for (int l = 1; l < 3; ++l) launch_pyrdown_u8(b.image[l - 1], sys->lc[l - 1], sys->lr[l - 1], b.image[l], sys->lc[l], sys->lr[l], s);
+    for (int l = 0; l < 3; ++l) launch_derivative(b.image[l], b.dIdx[l], b.dIdy[l], sys->lc[l], sys->lr[l], s);
+}
+
+// Tracker::SwapFrame of the intensity pyramid: on the device, conditioned on the tracking state
+static void queue_image_commit(fu_system* sys, hipStream_t s) {
+    fu_system_buffers& b = sys->b;
+    const unsigned char* src[3] = {b.image[0], b.image[1], b.image[2]};
+    unsigned char* dst[3] = {b.last_image[0], b.last_image[1], b.last_image[2]};
+    const int n[3] = {sys->lc[0] * sys->lr[0], sys->lc[1] * sys->lr[1], sys->lc[2] * sys->lr[2]};
+    launch_image_commit(sys->trk->st, src, dst, n, s);
+}
+
+// Frame::FillImages (Core/Frame.cc:59-78); the intensity images only with the photometric term
 static void queue_pyramid(fu_system* sys, const uint16_t* depth, hipStream_t s) {
     const fu_system_params& p = sys->p;
     fu_system_buffers& b = sys->b;
@@ -1006,6 +1188,7 @@ static void queue_frame_device(fu_system* sys, const uint16_t* depth, const uint
     if (!first) queue_resize_model(sys, s);
     mark(sys, FU_STAGE_RESIZE + 1, s);
     queue_pyramid(sys, depth, s);
+    if (p.use_rgb) queue_images(sys, rgb, s);
     mark(sys, FU_STAGE_PYRAMID + 1, s);
     if (!first) {
         const float* nv[3] = {b.vmap[0], b.vmap[1], b.vmap[2]};
@@ -1013,9 +1196,15 @@ static void queue_frame_device(fu_system* sys, const uint16_t* depth, const uint
         const float* lv[3] = {b.model_vmap[0], b.model_vmap[1], b.model_vmap[2]};
         const float* ln[3] = {b.model_nmap[0], b.model_nmap[1], b.model_nmap[2]};
         const float K[4] = {p.fx, p.fy, p.cx, p.cy};
-        queue_se3(sys->trk, nv, nn, lv, ln, K, p.iters, p.min_count, p.icp_thresh, s);
+        const uint8_t* ni[3] = {b.image[0], b.image[1], b.image[2]};
+        const uint8_t* li[3] = {b.last_image[0], b.last_image[1], b.last_image[2]};
+        const int16_t* gx[3] = {b.dIdx[0], b.dIdx[1], b.dIdx[2]};
+        const int16_t* gy[3] = {b.dIdy[0], b.dIdy[1], b.dIdy[2]};
+        const RgbLevels rgbl{ni, li, gx, gy};
+        queue_se3(sys->trk, nv, nn, lv, ln, p.use_rgb ? &rgbl : nullptr, K, p.iters, p.min_count, p.icp_thresh, s);
     }
     launch_track_commit(sys->trk->st, s);
+    if (p.use_rgb) queue_image_commit(sys, s);
     mark(sys, FU_STAGE_ICP + 1, s);
     queue_fuse_raycast(sys, rgb, s);
 }
@@ -1029,6 +1218,7 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     fu_tracker* t = sys->trk;
     if (!first) queue_resize_model(sys, s);
     queue_pyramid(sys, depth, s);
+    if (p.use_rgb) queue_images(sys, rgb, s);
     HIPCHK(hipStreamSynchronize(s));
     TrackState h;
     HIPCHK(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
@@ -1047,21 +1237,26 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
         for (int l = 2; l >= 0 && h.status == TS_RUNNING; --l)
             for (int j = 0; j < p.iters[l] && h.status == TS_RUNNING; ++j) {
                 HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
-                launch_icp(make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0),
-                           t->nblocks[l], s);
+                IcpArgs a = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0);
+                if (p.use_rgb) set_rgb(a, t, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
+                launch_icp(a, t->nblocks[l], s);
                 LAUNCHCHK();
                 HIPCHK(hipMemcpyAsync(&h, t->st, sizeof(h), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
-                float sums[32];
+                float sums[32], rsums[32];
                 memcpy(sums, h.sums, sizeof(sums));
+                memcpy(rsums, h.rgb_sums, sizeof(rsums));
                 const int fin = (l == last_level && j == p.iters[l] - 1);
-                icp_update(&h, sums, p.min_count[l], fin, p.icp_thresh);
+                icp_update(&h, sums, p.use_rgb ? rsums : nullptr, p.min_count[l], fin, p.icp_thresh);
             }
     }
     if (h.status == TS_OK || h.status == TS_FIRST)
         for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i];
     refresh_float_pose(&h);
     HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    if (p.use_rgb && (h.status == TS_OK || h.status == TS_FIRST))  // Tracker::SwapFrame
+        for (int l = 0; l < 3; ++l)
+            HIPCHK(hipMemcpyAsync(b.last_image[l], b.image[l], (size_t)sys->lc[l] * sys->lr[l], hipMemcpyDeviceToDevice, s));
     if (h.status != TS_OK && h.status != TS_FIRST) {
         HIPCHK(hipStreamSynchronize(s));
         return FU_OK;
@@ -1219,6 +1414,7 @@ extern "C" fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_s
     out->n_fragments = c[C_RENDER];
     out->flags = c[C_FLAGS];
     out->frame_index = sys->frame;
+    out->rgb_count = sys->p.use_rgb ? h.rgb_count : 0;
     return FU_OK;
 }
 
@@ -1259,6 +1455,7 @@ int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, in
 struct fu_batch {
     int cap = 0;
     int fmt = -1;
+    int rgb = -1;  // tracking mode shared by the attached systems (use_rgb)
     SessDesc* d_desc = nullptr;
     std::vector<SessDesc> h_desc;
     std::vector<fu_system*> owner;
@@ -1328,6 +1525,7 @@ static void fill_desc(fu_system* sys, SessDesc& d) {
     d.depth_scale_inv = (float)(1.0 / (double)p.depth_scale);
     d.cutoff = p.depth_cutoff;
     d.raw = b.raw;
+    d.use_rgb = p.use_rgb ? 1 : 0;
     const float K[4] = {p.fx, p.fy, p.cx, p.cy};
     d.last_level = -1;
     for (int l = 2; l >= 0; --l)
@@ -1349,6 +1547,13 @@ static void fill_desc(fu_system* sys, SessDesc& d) {
         d.model_n[l] = (f4*)b.model_nmap[l];
         d.icp[l] = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 1, p.min_count[l], 0,
                             p.icp_thresh);
+        if (p.use_rgb) {
+            set_rgb(d.icp[l], t, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
+            d.image[l] = b.image[l];
+            d.last_image[l] = b.last_image[l];
+            d.didx[l] = b.dIdx[l];
+            d.didy[l] = b.dIdy[l];
+        }
         d.icp_blocks[l] = t->nblocks[l];
         d.iters[l] = p.iters[l];
     }
@@ -1370,6 +1575,8 @@ extern "C" fu_status fu_batch_attach(fu_batch* bt, fu_system* sys) {
     if (sys->batch) return set_err(FU_ERR_STATE, "system is already attached to a batch context");
     if (bt->fmt >= 0 && bt->fmt != sys->p.map.voxel_format)
         return set_err(FU_ERR_ARG, "voxel format %d differs from the batch context's %d", sys->p.map.voxel_format, bt->fmt);
+    if (bt->rgb >= 0 && bt->rgb != (sys->p.use_rgb ? 1 : 0))
+        return set_err(FU_ERR_ARG, "tracking mode (use_rgb %d) differs from the batch context's", sys->p.use_rgb);
     int tiles = 0;
     for (int l = 0; l < 3; ++l) tiles += ((sys->lc[l] + 15) / 16) * ((sys->lr[l] + 15) / 16);
     if (tiles > 65535) return set_err(FU_ERR_ARG, "frame %dx%d too large for batched launches", sys->p.cols, sys->p.rows);
@@ -1381,6 +1588,7 @@ extern "C" fu_status fu_batch_attach(fu_batch* bt, fu_system* sys) {
     HIPCHK(hipMemcpy(bt->d_desc + slot, &bt->h_desc[slot], sizeof(SessDesc), hipMemcpyHostToDevice));
     bt->owner[slot] = sys;
     bt->fmt = sys->p.map.voxel_format;
+    bt->rgb = sys->p.use_rgb ? 1 : 0;
     bt->attached++;
     sys->batch = bt;
     sys->batch_slot = slot;
@@ -1391,7 +1599,7 @@ extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
     if (!bt || !sys) return set_err(FU_ERR_ARG, "null argument");
     if (sys->batch != bt) return set_err(FU_ERR_STATE, "system is not attached to this batch context");
     bt->owner[sys->batch_slot] = nullptr;
-    if (--bt->attached == 0) bt->fmt = -1;
+    if (--bt->attached == 0) bt->fmt = bt->rgb = -1;
     sys->batch = nullptr;
     sys->batch_slot = -1;
     return FU_OK;
@@ -1441,6 +1649,10 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     launch_b_pyrdown(b, h, 2, hs);
     launch_b_vmap(b, h, hs);
     launch_b_nmap(b, h, hs);
+    if (bt->rgb == 1) {
+        for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
+        launch_b_images(b, h, hs);
+    }
     mark(FU_STAGE_PYRAMID + 1);
     if (track) {
         launch_b_track_begin(b, h, hs);
@@ -1448,6 +1660,7 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
             for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
     }
     launch_b_track_commit(b, h, hs);
+    if (bt->rgb == 1) launch_b_image_commit(b, h, hs);
     mark(FU_STAGE_ICP + 1);
     for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
     launch_b_alloc(b, h, hs);

@@ -38,6 +38,13 @@ struct SessDesc {
     f4* nmap[3];
     f4* model_v[3];
     f4* model_n[3];
+    // photometric term (null unless the session tracks with it): intensity pyramid of this
+    // frame and of the last tracked frame, Sobel derivatives
+    unsigned char* image[3];
+    unsigned char* last_image[3];
+    short* didx[3];
+    short* didy[3];
+    int use_rgb;
     IcpArgs icp[3];
     int icp_blocks[3];
     int iters[3];
@@ -127,7 +134,8 @@ struct FrameBrief {
     unsigned n_visible;
     unsigned n_alloc;
     unsigned flags;
-    unsigned pad[2];
+    int rgb_count;  // photometric residuals of the last iteration (0 without the term)
+    unsigned pad;
 };
 
 // pyramid.hip
@@ -136,6 +144,10 @@ void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
 void launch_b_vmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+// intensity pyramid and derivatives (sessions with the photometric term only), and the
+// post-tracking copy of the pyramid into the last frame's
+void launch_b_images(BatchLaunch& b, const SessDesc* h, hipStream_t s);
+void launch_b_image_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 // icp.hip
 void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStream_t s);

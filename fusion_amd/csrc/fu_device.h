@@ -270,4 +270,48 @@ __device__ __forceinline__ float wave_reduce_scatter32(float (&v)[32]) {
     return v[0] + dpp_f<0xB1>(v[0]);  // quad_perm [1,0,3,2]
 }
 
+// The same reduce-scatter for 64 values: one more halving step, so that afterwards lane k
+// holds the wave total of value k.  `v` is clobbered.
+__device__ __forceinline__ float wave_reduce_scatter64(float (&v)[64]) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {  // lanes 0-31 keep values 0-31, lanes 32-63 values 32-63
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 32]), false, false);
+        v[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {  // even rows keep the lower 16 of their 32, odd rows the upper 16
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[j]), __float_as_uint(v[j + 16]), false, false);
+        v[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const int lane = lane_id();
+    bool hi = (lane & 8) != 0;  // partner lane ^ 8
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float keep = hi ? v[j + 8] : v[j], send = hi ? v[j] : v[j + 8];
+        v[j] = keep + dpp_f<0x128>(send);  // row_ror:8
+    }
+    hi = (lane & 4) != 0;  // partner 7 - q within each half row
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float keep = hi ? v[j + 4] : v[j], send = hi ? v[j] : v[j + 4];
+        v[j] = keep + dpp_f<0x141>(send);  // row_half_mirror
+    }
+    hi = (lane & 2) != 0;  // partner lane ^ 2
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float keep = hi ? v[j + 2] : v[j], send = hi ? v[j] : v[j + 2];
+        v[j] = keep + dpp_f<0x4E>(send);  // quad_perm [2,3,0,1]
+    }
+    hi = (lane & 1) != 0;  // partner lane ^ 1
+    const float keep = hi ? v[1] : v[0], send = hi ? v[0] : v[1];
+    return keep + dpp_f<0xB1>(send);  // quad_perm [1,0,3,2]
+}
+
+// wave total of an unsigned (wrapping) sum; every lane gets it
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) v += (unsigned)__shfl_xor((int)v, m);
+    return v;
+}
+
 }  // namespace fu

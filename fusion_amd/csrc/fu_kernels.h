@@ -33,6 +33,14 @@ struct TrackState {
     int status;
     int iters_done;
     float sums[32];
+    // photometric term of the last iteration (RGBStep): the weighted-row sums, the residual
+    // pass's {count, sum diff^2} and sigma, and sqrt(sum r^2) / count with its count
+    float rgb_sums[32];
+    int rgb_corresp[2];
+    float sigma;
+    float rgb_error;
+    int rgb_count;
+    int pad_;
 };
 
 // camera pose as the kernels use it: R (camera->world), Rinv = R^T, t
@@ -124,6 +132,16 @@ struct MeshOut {
 constexpr int kIcpThreads = 1024;
 constexpr int kIcpFinalThreads = 256;
 
+// Inputs of the photometric term at one level (RGBReduction, Reduction.cu:377-407)
+struct RgbArgs {
+    const unsigned char* inext;  // intensity of this frame
+    const unsigned char* ilast;  // intensity of the last tracked frame
+    const short* dx;             // Sobel derivatives of this frame's intensity
+    const short* dy;
+    int* rec;   // per pixel, written by the residual pass: (j << 9) | (diff + 256), or -1
+    int* corr;  // 2 ints per residual-pass work-group: count, sum of diff^2 (32-bit, wrapping)
+};
+
 struct IcpArgs {
     const f4* vcurr;
     const f4* ncurr;
@@ -132,12 +150,18 @@ struct IcpArgs {
     int cols, rows;
     float fx, fy, cx, cy;
     TrackState* st;
-    float* partials;  // 32 floats per work-group
+    float* partials;  // 32 floats per work-group (64 with the photometric term)
     int mode;         // 0: sums only; 1: sums, solve and pose update
     int min_count;
     int final_iter;
     float thresh;
+    int use_rgb;      // 1: RGBStep is blended into every iteration (ComputeSE3 icpOnly = false)
+    RgbArgs rgb;
 };
+
+// frames with the photometric term are limited to 2^22 pixels (the residual record packs
+// the last-frame pixel index in 22 bits)
+constexpr int kRgbMaxPixels = 1 << 22;
 
 // pyramid.hip
 void launch_filter_depth(const uint16_t* depth, float* raw, float* filt, int cols, int rows, float scale_inv,
@@ -148,6 +172,14 @@ void launch_vmap(const float* depth, f4* vmap, int cols, int rows, float invfx, 
 void launch_nmap(const f4* vmap, f4* nmap, int cols, int rows, hipStream_t s);
 void launch_resize(const f4* vsrc, const f4* nsrc, int scols, f4* vdst, f4* ndst, int dcols, int drows,
                    hipStream_t s);
+void launch_intensity(const unsigned char* rgb, unsigned char* img, int cols, int rows, hipStream_t s);
+void launch_pyrdown_u8(const unsigned char* src, int scols, int srows, unsigned char* dst, int dcols, int drows,
+                       hipStream_t s);
+void launch_derivative(const unsigned char* img, short* dx, short* dy, int cols, int rows, hipStream_t s);
+// copies `n` bytes of each of up to three images when the frame was tracked (or is the first):
+// Tracker::SwapFrame of the intensity pyramid
+void launch_image_commit(const TrackState* st, const unsigned char* const src[3], unsigned char* const dst[3],
+                         const int n[3], hipStream_t s);
 // map.hip
 void launch_reset_map(const MapView& m, hipStream_t s);
 void launch_begin_pass(const MapView& m, int which, hipStream_t s);
@@ -162,7 +194,7 @@ void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s);
 void upload_mc_tables();
 void launch_mesh(const MapView& m, const MeshOut& o, unsigned* counts, unsigned* chunk, hipStream_t s);
 // icp.hip
-void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s);
+void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s);  // with a.use_rgb: RGBStep too
 void launch_track_begin(TrackState* st, hipStream_t s);
 void launch_track_commit(TrackState* st, hipStream_t s);
 

@@ -43,6 +43,7 @@ FU_HD f3 mul(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 FU_HD f3 mul(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
 FU_HD f3 divs(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 FU_HD f4 add4(f4 a, f4 b) { return f4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+FU_HD f4 sub4(f4 a, f4 b) { return f4{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
 FU_HD f4 divs4(f4 a, float s) { return f4{a.x / s, a.y / s, a.z / s, a.w / s}; }
 // float3 . float3 (VectorMath.h:191-193): ((x*x' + y*y') + z*z')
 FU_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }

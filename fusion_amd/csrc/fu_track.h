@@ -135,10 +135,8 @@ FU_HD double sin_euler_norm(const double* p) {
     return sqrt(sa * sa + sb * sb + sc * sc);
 }
 
-// One ComputeSE3 iteration after the reduction (Tracking.cc:368-394) and, on the last
-// iteration, the acceptance test (:398-408).
-FU_HD void icp_update(TrackState* st, const float* sums, int min_count, int final_iter, float thresh) {
-    double A[36], b[6];
+// CreateMatrix<6,7> (Reduction.cu:6-16): the 29 sums -> A (6x6, symmetric) and b (6)
+FU_HD void unpack_sums29(const float* sums, double* A, double* b) {
     int shift = 0;
     FU_UNROLL for (int i = 0; i < 6; ++i) FU_UNROLL for (int j = i; j < 7; ++j) {
         const double v = (double)sums[shift++];
@@ -146,6 +144,29 @@ FU_HD void icp_update(TrackState* st, const float* sums, int min_count, int fina
             b[i] = v;
         else
             A[j * 6 + i] = A[i * 6 + j] = v;
+    }
+}
+
+// One ComputeSE3 iteration after the reduction (Tracking.cc:368-394) and, on the last
+// iteration, the acceptance test (:398-408).  `rgb` holds the photometric sums of the same
+// iteration, or is null (icpOnly): their failure test (:341-346) fails the frame like ICP's,
+// and the systems are blended as matA = matA_rgb * w + matA_icp with float w = 1e-4 (:379-383).
+FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int min_count, int final_iter,
+                      float thresh) {
+    double A[36], b[6];
+    unpack_sums29(sums, A, b);
+    bool rgb_ok = true;
+    if (rgb) {
+        double Ar[36], br[6];
+        unpack_sums29(rgb, Ar, br);
+        const float rerr = sqrtf(rgb[27]) / rgb[28];
+        const int rcnt = f2i(rgb[28]);
+        st->rgb_error = rerr;
+        st->rgb_count = rcnt;
+        rgb_ok = !(isnanf_(rerr) || rcnt < min_count);
+        const double w = (double)1e-4f;
+        FU_UNROLL for (int i = 0; i < 36; ++i) A[i] = Ar[i] * w + A[i];
+        FU_UNROLL for (int i = 0; i < 6; ++i) b[i] = br[i] * w + b[i];
     }
     FU_UNROLL for (int i = 0; i < 36; ++i) st->A[i] = A[i];
     FU_UNROLL for (int i = 0; i < 6; ++i) st->b[i] = b[i];
@@ -157,7 +178,7 @@ FU_HD void icp_update(TrackState* st, const float* sums, int min_count, int fina
     st->count = cnt;
     st->iters_done += 1;
     double x[6];
-    const bool solved = !(isnanf_(err) || cnt < min_count) && chol_solve6(A, b, x);
+    const bool solved = rgb_ok && !(isnanf_(err) || cnt < min_count) && chol_solve6(A, b, x);
     if (!solved) {
         FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = st->last_pose[i];
         st->status = TS_FAILED;

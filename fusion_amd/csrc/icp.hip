@@ -10,6 +10,13 @@
 // A one-group kernel then sums the rows with the same compensated tree and (mode 1) unpacks
 // A (6x6) and b (6) in FP64, solves by Cholesky, takes the SE(3) exponential and composes
 // the pose on one lane: the only FP64 steps of the pipeline.
+//
+// With the photometric term (ComputeSE3 icpOnly = false) every iteration is three launches:
+// the RGB residual pass (RGBReduction::ComputeResidual, Reduction.cu:469-483: one residual
+// record per pixel and the integer sums of each work-group), one pass that computes both the
+// ICP and the weighted RGB rows of each pixel (each work-group first adds the integer sums
+// up to sigma itself) and reduces 64 values per lane, and the final group, which blends
+// A_rgb * 1e-4f + A_icp in FP64 before the solve.
 #include "fu_device.h"
 #include "fu_kernels.h"
 #include "fu_track.h"
@@ -112,6 +119,16 @@ __device__ __forceinline__ void icp_products(const float (&row)[7], float (&v)[3
         for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
     v[28] = 1.0f;
 }
+// the same 29 values into v[O .. O+28]
+template <int O, int NV>
+__device__ __forceinline__ void row_products(const float (&row)[7], float (&v)[NV]) {
+    int k = O;
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+#pragma unroll
+        for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
+    v[O + 28] = 1.0f;
+}
 
 // kOne: the grid has a lane for every pixel (one pixel per lane, no per-lane sums); otherwise
 // lanes loop over pixels with compensated sums seeded exactly by their first pixel.  Both
@@ -204,12 +221,244 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
-        if (a.mode == 1) icp_update(st, total, a.min_count, a.final_iter, a.thresh);
+        if (a.mode == 1) icp_update(st, total, nullptr, a.min_count, a.final_iter, a.thresh);
     }
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
 
+// ------------------------------------------------------------------ photometric term
+struct RgbPose {
+    m3 Rc, Rci, Rl, Rli;
+    f3 tc, tl;
+};
+__device__ __forceinline__ RgbPose rgb_pose(const TrackState* st_) {
+    const auto st = FU_G(const TrackState, st_);
+    RgbPose P;
+    P.Rc = load_m3(st->Rcurr);
+    P.Rci = load_m3(st->RcurrInv);
+    P.Rl = load_m3(st->Rlast);
+    P.Rli = load_m3(st->RlastInv);
+    P.tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
+    P.tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
+    return P;
+}
+
+// RGBReduction::FindCorresp (Reduction.cu:414-467) of pixel (x, y): the last-frame pixel j
+// and the intensity difference, or false
+__device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, int x, int y, int& j, int& diff) {
+    const int cols = a.cols, rows = a.rows;
+    if (!(x >= 5 && x < cols - 5 && y >= 5 && y < rows - 5)) return false;
+    const auto inext = FU_G(const unsigned char, a.rgb.inext);
+    // the 4x4 window [y-2, y+2) x [x-2, x+2) is non-zero; the margin keeps it in the image,
+    // so the reference's clamps are inactive
+    bool valid = true;
+#pragma unroll
+    for (int u = -2; u < 2; ++u)
+#pragma unroll
+        for (int v = -2; v < 2; ++v) valid &= inext[(y + u) * cols + x + v] > 0;
+    if (!valid) return false;
+    const int i = y * cols + x;
+    const int gx = FU_G(const short, a.rgb.dx)[i], gy = FU_G(const short, a.rgb.dy)[i];
+    // sqrtf(gx * gx + gy * gy) > 0: the integer sum is positive unless both are zero
+    if (gx == 0 && gy == 0) return false;
+    const f4 vc = FU_G(const f4, a.vcurr)[i];
+    if (isnanf_(vc.x)) return false;
+    const f3 vg = add(mv(P.Rc, xyz(vc)), P.tc);
+    const f3 vl = mv(P.Rli, sub(vg, P.tl));
+    // __float2int_rd(fx * x / z + cx + 0.5): float quotient and sum, 0.5 added in double,
+    // the double rounded to float, then floor
+    const int u = f2i(floorf((float)((double)(a.fx * vl.x / vl.z + a.cx) + 0.5)));
+    const int v = f2i(floorf((float)((double)(a.fy * vl.y / vl.z + a.cy) + 0.5)));
+    if (!(u >= 0 && v >= 0 && u < cols && v < rows)) return false;
+    const int k = v * cols + u;
+    const f4 la = FU_G(const f4, a.vlast)[k];
+    const int lk = FU_G(const unsigned char, a.rgb.ilast)[k];
+    if (!(!isnanf_(la.x) && (double)norm4(sub4(la, vc)) < 0.1 && lk != 0)) return false;
+    j = k;
+    diff = (int)inext[i] - lk;
+    return true;
+}
+
+// RGBReduction::GetRow (Reduction.cu:485-520) of a valid residual of pixel i
+__device__ __forceinline__ void rgb_row(const IcpArgs& a, const RgbPose& P, int i, int j, int diff, float sigma,
+                                        float (&row)[7]) {
+    float w = sigma + (float)abs(diff);
+    w = (double)w < 1e-3 ? 1.0f : 1.0f / w;
+    if ((double)sigma < 1e-6) w = 1.0f;
+    const f4 la = FU_G(const f4, a.vlast)[j];
+    const f3 point = mv(P.Rci, sub(add(mv(P.Rl, xyz(la)), P.tl), P.tc));
+    const float gx = (float)((double)FU_G(const short, a.rgb.dx)[i] / 9.0);
+    const float gy = (float)((double)FU_G(const short, a.rgb.dy)[i] / 9.0);
+    const float invz = 1.0f / point.z;
+    f3 left;
+    left.x = gx * a.fx * invz;
+    left.y = gy * a.fy * invz;
+    left.z = -(left.x * point.x + left.y * point.y) * invz;
+    const f3 c = cross(point, left);
+    row[0] = w * left.x;
+    row[1] = w * left.y;
+    row[2] = w * left.z;
+    row[3] = w * c.x;
+    row[4] = w * c.y;
+    row[5] = w * c.z;
+    row[6] = -w * (float)diff;
+}
+
+// residual pass: record (j << 9) | (diff + 256) or -1 per pixel, and the work-group's
+// {count, sum diff^2}; one pixel per lane
+__device__ __forceinline__ void rgb_resid_body(WG g, const IcpArgs& a) {
+    const auto st = FU_G(const TrackState, a.st);
+    if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
+    const RgbPose P = rgb_pose(a.st);
+    const int N = a.cols * a.rows;
+    const int i = g.x * kIcpThreads + threadIdx.x;
+    int rec = -1, diff = 0;
+    if (i < N) {
+        const int y = i / a.cols, x = i - y * a.cols;
+        int j;
+        if (rgb_corresp(a, P, x, y, j, diff)) rec = (j << 9) | (diff + 256);
+        FU_G(int, a.rgb.rec)[i] = rec;
+    }
+    const bool ok = rec >= 0;
+    const unsigned c = (unsigned)__popcll(ballot(ok));
+    const unsigned q = wave_sum_u32(ok ? (unsigned)(diff * diff) : 0u);
+    __shared__ unsigned part[kIcpThreads / 64][2];
+    if (lane_id() == 0) {
+        part[threadIdx.x >> 6][0] = c;
+        part[threadIdx.x >> 6][1] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        unsigned t = 0;
+        for (int w = 0; w < kIcpThreads / 64; ++w) t += part[w][threadIdx.x];
+        FU_G(int, a.rgb.corr)[g.x * 2 + threadIdx.x] = (int)t;
+    }
+}
+__global__ __launch_bounds__(kIcpThreads) void k_rgb_resid(IcpArgs a) { rgb_resid_body(hw_wg(), a); }
+
+// totals of the residual pass's {count, sum diff^2} over its `nparts` work-groups (32-bit,
+// wrapping, as the reference's int sums), and sigma = sqrt(sum / count) (Reduction.cu:605)
+__device__ __forceinline__ float rgb_sigma(const IcpArgs& a, int nparts, int* corr_out) {
+    __shared__ unsigned part[kIcpThreads / 64][2];
+    const auto corr = FU_G(const int, a.rgb.corr);
+    unsigned c = 0, q = 0;
+    for (int t = threadIdx.x; t < nparts; t += blockDim.x) {
+        c += (unsigned)corr[2 * t];
+        q += (unsigned)corr[2 * t + 1];
+    }
+    c = wave_sum_u32(c);
+    q = wave_sum_u32(q);
+    if (lane_id() == 0) {
+        part[threadIdx.x >> 6][0] = c;
+        part[threadIdx.x >> 6][1] = q;
+    }
+    __syncthreads();
+    unsigned tc = 0, tq = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        tc += part[w][0];
+        tq += part[w][1];
+    }
+    const int ci = (int)tc, qi = (int)tq;
+    if (corr_out && threadIdx.x == 0) {
+        corr_out[0] = ci;
+        corr_out[1] = qi;
+    }
+    return sqrtf((float)qi / (float)(ci == 0 ? 1 : ci));
+}
+
+// ICP and photometric rows of one pixel per lane: values 0-28 are the ICP sums, 32-60 the RGB
+// sums; the work-group's 64 totals go to partials[g.x * 64 ..]
+__device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
+    const auto st = FU_G(const TrackState, a.st);
+    if (a.mode != 0 && st->status != TS_RUNNING) return;
+    IcpPose P;
+    P.Rc = load_m3(st->Rcurr);
+    P.Rl = load_m3(st->Rlast);
+    P.Rli = load_m3(st->RlastInv);
+    P.tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
+    P.tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
+    const RgbPose Q = rgb_pose(a.st);
+    const float sigma = rgb_sigma(a, g.nx, nullptr);
+    const int N = a.cols * a.rows;
+    __shared__ float wsum[kIcpThreads / 64][64];
+    float v[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) v[j] = 0.f;
+    const int i = g.x * kIcpThreads + threadIdx.x;
+    if (i < N) {
+        float row[7];
+        if (a.ncurr && icp_row(a, P, i, row)) row_products<0>(row, v);  // no normals: RGB rows only
+        const int rec = FU_G(const int, a.rgb.rec)[i];
+        if (rec >= 0) {
+            rgb_row(a, Q, i, rec >> 9, (rec & 511) - 256, sigma, row);
+            row_products<32>(row, v);
+        }
+    }
+    const float t = wave_reduce_scatter64(v);
+    wsum[threadIdx.x >> 6][lane_id()] = t;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int j = threadIdx.x;
+        float w[kIcpThreads / 64];
+#pragma unroll
+        for (int k = 0; k < kIcpThreads / 64; ++k) w[k] = wsum[k][j];
+#pragma unroll
+        for (int h = kIcpThreads / 128; h >= 1; h /= 2)
+#pragma unroll
+            for (int k = 0; k < h; ++k) w[k] = w[2 * k] + w[2 * k + 1];
+        FU_G(float, a.partials)[g.x * 64 + j] = w[0];
+    }
+}
+__global__ __launch_bounds__(kIcpThreads) void k_icp_rgb(IcpArgs a) { icp_rgb_body(hw_wg(), a); }
+
+// final group with the photometric term: each wave adds every fourth partial row
+// (compensated, one value per lane), the waves are added pairwise, then the FP64 update
+__device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts) {
+    TrackState* st = a.st;
+    if (a.mode != 0 && st->status != TS_RUNNING) return;
+    constexpr int NW = kIcpFinalThreads / 64;
+    __shared__ float wsum[NW][64];
+    __shared__ float total[64];
+    __shared__ int corr[2];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const float sigma = rgb_sigma(a, nparts, corr);
+    Kahan k{0.f, 0.f};
+    for (int g = w; g < nparts; g += NW) k.add(FU_G(const float, a.partials)[g * 64 + lane]);
+    wsum[w][lane] = k.val();
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float x[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) x[q] = wsum[q][threadIdx.x];
+#pragma unroll
+        for (int h = NW / 2; h >= 1; h /= 2)
+#pragma unroll
+            for (int q = 0; q < h; ++q) x[q] = x[2 * q] + x[2 * q + 1];
+        total[threadIdx.x] = x[0];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < 29; ++j) {
+            st->sums[j] = total[j];
+            st->rgb_sums[j] = total[32 + j];
+        }
+        st->rgb_corresp[0] = corr[0];
+        st->rgb_corresp[1] = corr[1];
+        st->sigma = sigma;
+        if (a.mode == 1) icp_update(st, total, total + 32, a.min_count, a.final_iter, a.thresh);
+    }
+}
+__global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts) {
+    icp_rgb_final_body(a, nparts);
+}
+
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
+    if (a.use_rgb) {  // one pixel per lane (the caller sizes nblocks to cover the level)
+        hipLaunchKernelGGL(k_rgb_resid, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
+        hipLaunchKernelGGL(k_icp_rgb, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
+        hipLaunchKernelGGL(k_icp_rgb_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, nblocks);
+        return;
+    }
     if ((long long)nblocks * kIcpThreads >= (long long)a.cols * a.rows)
         hipLaunchKernelGGL(k_icp<true>, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
     else
@@ -273,6 +522,29 @@ __global__ __launch_bounds__(kIcpFinalThreads) void kb_icp_final(BatchLaunch b) 
     icp_final_body(a, nb);
 }
 
+__global__ __launch_bounds__(kIcpThreads) void kb_rgb_resid(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, s, a, nb)) return;
+    rgb_resid_body(g, a);
+}
+__global__ __launch_bounds__(kIcpThreads) void kb_icp_rgb(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, s, a, nb)) return;
+    icp_rgb_body(g, a);
+}
+__global__ __launch_bounds__(kIcpFinalThreads) void kb_icp_rgb_final(BatchLaunch b) {
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, blockIdx.x, a, nb)) return;
+    icp_rgb_final_body(a, nb);
+}
+
 __global__ void kb_track_begin(BatchLaunch b) {
     const int s = blockIdx.x;
     if (slot_first(b, s)) return;
@@ -294,6 +566,7 @@ __global__ void kb_results(BatchLaunch b, FrameBrief* out) {
     if (t == 19) o->n_visible = d.m.ctr[C_VISIBLE];
     if (t == 20) o->n_alloc = min(d.m.ctr[C_ALLOC], d.m.num_blocks);
     if (t == 21) o->flags = d.m.ctr[C_FLAGS];
+    if (t == 22) o->rgb_count = d.use_rgb ? st->rgb_count : 0;
 }
 void launch_b_results(BatchLaunch& b, FrameBrief* out, hipStream_t s) {
     hipLaunchKernelGGL(kb_results, dim3(b.n), dim3(64), 0, s, b, out);
@@ -307,6 +580,12 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
     b.level = level;
     b.iter = iter;
     const int grid = batch_layout(b, h, BT_ICP, level, 0);
+    if (h[b.sess[0]].use_rgb) {  // the sessions of a batch share the tracking mode
+        hipLaunchKernelGGL(kb_rgb_resid, dim3(grid), dim3(kIcpThreads), 0, s, b);
+        hipLaunchKernelGGL(kb_icp_rgb, dim3(grid), dim3(kIcpThreads), 0, s, b);
+        hipLaunchKernelGGL(kb_icp_rgb_final, dim3(b.n), dim3(kIcpFinalThreads), 0, s, b);
+        return;
+    }
     bool one = true;
     for (int i = 0; i < b.n; ++i) {
         const SessDesc& d = h[b.sess[i]];

@@ -162,7 +162,136 @@ __global__ __launch_bounds__(256) void k_resize(const f4* __restrict__ vsrc, con
     resize_body(hw_wg(), vsrc, nsrc, scols, vdst, ndst, dcols, drows);
 }
 
+// ------------------------------------------------------------------ intensity pyramid
+// ImageToIntensityKernel (Pyrdown.cu:183-194): (int)(0.2125 c1 + 0.7154 c0 + 0.0721 c2) in
+// double, evaluated as written (no fused multiply-add), stored as a byte
+__device__ __forceinline__ void intensity_body(WG g, const unsigned char* __restrict__ rgb_,
+                                               unsigned char* __restrict__ img_, int cols, int rows) {
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    const auto rgb = FU_G(const unsigned char, rgb_);
+    const auto img = FU_G(unsigned char, img_);
+    const int i = y * cols + x;
+    const double v = (0.2125 * (double)rgb[3 * i + 1] + 0.7154 * (double)rgb[3 * i]) + 0.0721 * (double)rgb[3 * i + 2];
+    img[i] = (unsigned char)(int)v;  // 0 <= v < 256
+}
+__global__ __launch_bounds__(256) void k_intensity(const unsigned char* __restrict__ rgb, unsigned char* __restrict__ img,
+                                                  int cols, int rows) {
+    intensity_body(hw_wg(), rgb, img, cols, rows);
+}
+
+// PyrDownGaussKernel<uchar, uchar> (Pyrdown.cu:133-158): the float kernel's window and
+// mirrored weights over bytes (no byte is NaN), the float mean truncated to a byte.  Every
+// product and partial sum is an integer below 2^24, so the float sums are exact.
+__device__ __forceinline__ void pyrdown_u8_body(WG g, const unsigned char* __restrict__ src_, int scols, int srows,
+                                                unsigned char* __restrict__ dst_, int dcols, int drows) {
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= dcols || y >= drows) return;
+    const auto src = FU_G(const unsigned char, src_);
+    const auto dst = FU_G(unsigned char, dst_);
+    const int tx = min(2 * x + 3, scols - 1);
+    const int ty = min(2 * y + 3, srows - 1);
+    float sum = 0.f;
+    int count = 0;
+    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
+        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+            const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
+            sum += (float)src[cy * scols + cx] * g;
+            count += (int)g;
+        }
+    }
+    dst[y * dcols + x] = (unsigned char)f2i(sum / (float)count);
+}
+__global__ __launch_bounds__(256) void k_pyrdown_u8(const unsigned char* __restrict__ src, int scols, int srows,
+                                                   unsigned char* __restrict__ dst, int dcols, int drows) {
+    pyrdown_u8_body(hw_wg(), src, scols, srows, dst, dcols, drows);
+}
+
+// ComputeDerivativeImageKernel (Pyrdown.cu:210-236): 3x3 Sobel over the window clipped to
+// the image, the weight index counting down from 8 across the clipped window (at the border
+// the weights shift), float sums stored as shorts
+__constant__ float c_sobel_x[9] = {1, 0, -1, 2, 0, -2, 1, 0, -1};
+__constant__ float c_sobel_y[9] = {1, 2, 1, 0, 0, 0, -1, -2, -1};
+__device__ __forceinline__ void derivative_body(WG g, const unsigned char* __restrict__ img_, short* __restrict__ dx_,
+                                                short* __restrict__ dy_, int cols, int rows) {
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    const auto img = FU_G(const unsigned char, img_);
+    float a = 0.f, b = 0.f;
+    int k = 8;
+    for (int j = max(y - 1, 0); j <= min(y + 1, rows - 1); ++j) {
+        for (int i = max(x - 1, 0); i <= min(x + 1, cols - 1); ++i) {
+            const float v = (float)img[j * cols + i];
+            a += v * c_sobel_x[k];
+            b += v * c_sobel_y[k];
+            --k;
+        }
+    }
+    FU_G(short, dx_)[y * cols + x] = (short)f2i(a);
+    FU_G(short, dy_)[y * cols + x] = (short)f2i(b);
+}
+__global__ __launch_bounds__(256) void k_derivative(const unsigned char* __restrict__ img, short* __restrict__ dx,
+                                                   short* __restrict__ dy, int cols, int rows) {
+    derivative_body(hw_wg(), img, dx, dy, cols, rows);
+}
+
+// Tracker::SwapFrame for the intensity pyramid: after a tracked (or first) frame its images
+// become the last frame's.  16 bytes per lane where both ends are 16-byte aligned.
+__device__ __forceinline__ void image_commit_body(int lane, const TrackState* st_, const unsigned char* src_,
+                                                  unsigned char* dst_, int n) {
+    const auto st = FU_G(const TrackState, st_);
+    const int s = st->status;
+    if (s != TS_OK && s != TS_FIRST) return;
+    const auto src = FU_G(const unsigned char, src_);
+    const auto dst = FU_G(unsigned char, dst_);
+    const int i = lane * 16;
+    if (i + 16 <= n) {
+        *(FU_GPTR(uint4))(dst + i) = *(FU_GPTR(const uint4))(src + i);
+    } else {
+        for (int k = i; k < n; ++k) dst[k] = src[k];
+    }
+}
+struct ImageCommit {
+    const TrackState* st;
+    const unsigned char* src[3];
+    unsigned char* dst[3];
+    int n[3];
+    int pre[4];  // prefix of the levels' 256-lane groups
+};
+__global__ __launch_bounds__(256) void k_image_commit(ImageCommit c) {
+    int l = 0;
+    while (l < 2 && (int)blockIdx.x >= c.pre[l + 1]) ++l;
+    image_commit_body(((int)blockIdx.x - c.pre[l]) * 256 + threadIdx.x, c.st, c.src[l], c.dst[l], c.n[l]);
+}
+
 static inline dim3 grid16(int cols, int rows) { return dim3((cols + 15) / 16, (rows + 15) / 16); }
+
+void launch_intensity(const unsigned char* rgb, unsigned char* img, int cols, int rows, hipStream_t s) {
+    hipLaunchKernelGGL(k_intensity, grid16(cols, rows), dim3(256), 0, s, rgb, img, cols, rows);
+}
+void launch_pyrdown_u8(const unsigned char* src, int scols, int srows, unsigned char* dst, int dcols, int drows,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_pyrdown_u8, grid16(dcols, drows), dim3(256), 0, s, src, scols, srows, dst, dcols, drows);
+}
+void launch_derivative(const unsigned char* img, short* dx, short* dy, int cols, int rows, hipStream_t s) {
+    hipLaunchKernelGGL(k_derivative, grid16(cols, rows), dim3(256), 0, s, img, dx, dy, cols, rows);
+}
+void launch_image_commit(const TrackState* st, const unsigned char* const src[3], unsigned char* const dst[3],
+                         const int n[3], hipStream_t s) {
+    ImageCommit c;
+    c.st = st;
+    c.pre[0] = 0;
+    for (int l = 0; l < 3; ++l) {
+        c.src[l] = src[l];
+        c.dst[l] = dst[l];
+        c.n[l] = n[l];
+        c.pre[l + 1] = c.pre[l] + (n[l] + 16 * 256 - 1) / (16 * 256);
+    }
+    hipLaunchKernelGGL(k_image_commit, dim3(c.pre[3]), dim3(256), 0, s, c);
+}
 
 void launch_filter_depth(const uint16_t* depth, float* raw, float* filt, int cols, int rows, float scale_inv,
                          float cutoff, hipStream_t s) {
@@ -246,6 +375,39 @@ __global__ __launch_bounds__(256) void kb_nmap(BatchLaunch b) {
     nmap_body(g, d.vmap[l], d.nmap[l], d.lc[l], d.lr[l]);
 }
 
+// intensity pyramid of the sessions with the photometric term (the others have no share)
+__global__ __launch_bounds__(256) void kb_intensity(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    intensity_body(g, (const unsigned char*)b.in[s], d.image[0], d.cols, d.rows);
+}
+__global__ __launch_bounds__(256) void kb_pyrdown_u8(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const int l = b.level;
+    pyrdown_u8_body(g, d.image[l - 1], d.lc[l - 1], d.lr[l - 1], d.image[l], d.lc[l], d.lr[l]);
+}
+__global__ __launch_bounds__(256) void kb_derivative(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const int l = pyr3_level(d, g.x, g);
+    derivative_body(g, d.image[l], d.didx[l], d.didy[l], d.lc[l], d.lr[l]);
+}
+// after tracking: one 256-lane group per 4 KB of the three levels (BT_PYR3 counts 16x16
+// tiles, 256 pixels each, and the lanes of a tile copy 16 bytes: enough groups for 4096
+// bytes per tile; the extra groups of a level find nothing to copy)
+__global__ __launch_bounds__(256) void kb_image_commit(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    const SessDesc& d = b.desc[b.sess[s]];
+    const int l = pyr3_level(d, g.x, g);
+    const int tile = g.y * g.nx + g.x;
+    image_commit_body(tile * 256 + threadIdx.x, d.st, d.image[l], d.last_image[l], d.lc[l] * d.lr[l]);
+}
+
 static void blaunch(void (*k)(BatchLaunch), int grid, const BatchLaunch& b, hipStream_t s) {
     if (grid > 0) hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, b);
 }
@@ -266,6 +428,18 @@ void launch_b_vmap(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
 }
 void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     blaunch(kb_nmap, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
+}
+void launch_b_images(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    b.level = 0;
+    blaunch(kb_intensity, batch_layout(b, h, BT_TILES, 0, 0), b, s);
+    for (int l = 1; l < 3; ++l) {
+        b.level = l;
+        blaunch(kb_pyrdown_u8, batch_layout(b, h, BT_TILES, l, 0), b, s);
+    }
+    blaunch(kb_derivative, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
+}
+void launch_b_image_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    blaunch(kb_image_commit, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
 }
 
 }  // namespace fu

@@ -41,6 +41,7 @@ struct fu_scene {
     std::vector<Box> solids;  // interior walls and boxes
     std::vector<Sphere> spheres;
     unsigned char face_alb[6][3];  // room faces: -x +x -y +y floor ceiling
+    double light[3];               // point light fixed in the world (view-independent shading)
     std::vector<std::array<double, 3>> path;  // dense closed path samples
     std::vector<double> cum;                   // cumulative arc length
     double total = 0;
@@ -287,6 +288,9 @@ extern "C" int fu_scene_create(int kind, float sx, float sy, float sz, uint64_t 
     s->hi[0] = sx;
     s->hi[1] = sy;
     s->hi[2] = sz;
+    s->light[0] = 0.5 * sx;
+    s->light[1] = 0.5 * sy;
+    s->light[2] = sz - 0.3;
     for (int f = 0; f < 6; ++f)
         for (int c = 0; c < 3; ++c) s->face_alb[f][c] = (unsigned char)urange(rng, 120, 235);
     const double camz = std::min(1.5, (double)sz - 0.7);
@@ -416,9 +420,14 @@ extern "C" int fu_scene_render_pose(const fu_scene* s, const double pose[16], in
             }
             if (depth) depth[pix] = q;
             if (rgb) {
-                const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                double c = -(n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) / dl;
-                if (!isfinite(t)) c = 0;
+                // Lambertian under a point light fixed in the world: a surface point has the
+                // same colour from every viewpoint (brightness constancy), with smooth shading
+                // gradients across flat faces
+                double l[3];
+                for (int i = 0; i < 3; ++i) l[i] = s->light[i] - (o[i] + t * d[i]);
+                const double ll = sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+                double c = (n[0] * l[0] + n[1] * l[1] + n[2] * l[2]) / ll;
+                if (!isfinite(t) || !(ll > 0)) c = 0;
                 const double shade = 0.25 + 0.75 * (c > 0 ? c : 0.0);
                 for (int k = 0; k < 3; ++k) rgb[pix * 3 + k] = (uint8_t)floor(alb[k] * shade + 0.5);
             }

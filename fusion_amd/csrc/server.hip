@@ -160,11 +160,19 @@ extern "C" fu_status fu_server_open(fu_server* sv, const fu_system_params* p, in
     Session s;
     fu_status st = fu_system_create(p, &s.sys);
     if (st) return st;
-    // the lane with the least pixel load per step
-    int best = 0;
-    for (int i = 1; i < (int)sv->lanes.size(); ++i)
-        if (sv->lanes[i].pixels < sv->lanes[best].pixels) best = i;
-    if ((st = fu_batch_attach(sv->lanes[best].batch, s.sys))) {
+    // the lane with the least pixel load per step among those whose batch context takes the
+    // session (a context holds one voxel format and one tracking mode)
+    std::vector<int> order(sv->lanes.size());
+    for (int i = 0; i < (int)order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return sv->lanes[a].pixels < sv->lanes[b].pixels; });
+    int best = -1;
+    for (int i : order)
+        if ((st = fu_batch_attach(sv->lanes[i].batch, s.sys)) == FU_OK) {
+            best = i;
+            break;
+        }
+    if (best < 0) {
         fu_system_destroy(s.sys);
         return st;
     }

@@ -42,28 +42,31 @@ class PoseF(C.Structure):
 class TrackResult(C.Structure):
     _fields_ = [("pose", C.c_double * 16), ("last_pose", C.c_double * 16), ("x", C.c_double * 6),
                 ("A", C.c_double * 36), ("b", C.c_double * 6), ("icp_error", C.c_float), ("count", C.c_int32),
-                ("status", C.c_int32), ("iterations", C.c_int32)]
+                ("status", C.c_int32), ("iterations", C.c_int32), ("rgb_error", C.c_float),
+                ("rgb_count", C.c_int32)]
 
 
 class SystemParams(C.Structure):
     _fields_ = [("cols", C.c_int32), ("rows", C.c_int32), ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float),
                 ("cy", C.c_float), ("depth_scale", C.c_float), ("depth_cutoff", C.c_float),
                 ("iters", C.c_int32 * 3), ("min_count", C.c_int32 * 3), ("icp_thresh", C.c_float),
-                ("plain", C.c_int32), ("use_graph", C.c_int32), ("map", MapParams)]
+                ("plain", C.c_int32), ("use_graph", C.c_int32), ("use_rgb", C.c_int32), ("map", MapParams)]
 
 
 class FrameResult(C.Structure):
     _fields_ = [("pose", C.c_double * 16), ("track_status", C.c_int32), ("fused", C.c_int32),
                 ("icp_error", C.c_float), ("icp_count", C.c_int32), ("n_visible", C.c_uint32),
                 ("n_allocated", C.c_uint32), ("n_fragments", C.c_uint32), ("flags", C.c_uint32),
-                ("frame_index", C.c_uint32)]
+                ("frame_index", C.c_uint32), ("rgb_count", C.c_int32)]
 
 
 class SystemBuffers(C.Structure):
     _fields_ = [("depth_in", C.c_void_p), ("rgb_in", C.c_void_p), ("raw", C.c_void_p),
                 ("depth", C.c_void_p * 3), ("vmap", C.c_void_p * 3), ("nmap", C.c_void_p * 3),
                 ("model_vmap", C.c_void_p * 3), ("model_nmap", C.c_void_p * 3), ("zrange_min", C.c_void_p),
-                ("zrange_max", C.c_void_p), ("zcols", C.c_int32), ("zrows", C.c_int32)]
+                ("zrange_max", C.c_void_p), ("zcols", C.c_int32), ("zrows", C.c_int32),
+                ("image", C.c_void_p * 3), ("last_image", C.c_void_p * 3), ("dIdx", C.c_void_p * 3),
+                ("dIdy", C.c_void_p * 3)]
 
 
 class StageTimes(C.Structure):
@@ -132,6 +135,12 @@ SIGNATURES = {
     "fu_tracker_set_last_pose": (I, [P, DP, P]),
     "fu_compute_se3": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), FP, IP, IP, F, P]),
     "fu_tracker_result": (I, [P, C.POINTER(TrackResult), P]),
+    "fu_image_to_intensity": (I, [P, P, I, I, P]),
+    "fu_pyrdown_gauss_u8": (I, [P, I, I, P, P]),
+    "fu_compute_derivative_image": (I, [P, P, P, I, I, P]),
+    "fu_rgb_step": (I, [P, P, P, P, P, P, I, I, FP, FP, FP, FP, FP, FP, F, F, F, F, FP, DP, DP,
+                        C.POINTER(C.c_int32), P]),
+    "fu_compute_se3_rgbd": (I, [P] + [C.POINTER(P)] * 8 + [FP, IP, IP, F, P]),
     "fu_system_params_default": (None, [C.POINTER(SystemParams)]),
     "fu_system_create": (I, [C.POINTER(SystemParams), C.POINTER(P)]),
     "fu_system_destroy": (I, [P]),

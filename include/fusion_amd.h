@@ -170,6 +170,25 @@ fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const floa
                       const float RlastInv[9], const float tlast[3], float fx, float fy, float cx, float cy,
                       float residual[2], double A[36], double b[6], fu_stream s);
 
+/* ---------------------------------------------------------------- photometric term */
+/* ImageToIntensity (Reduction.h:34-35, Pyrdown.cu:196-206): (int)(0.2125 c1 + 0.7154 c0 +
+ * 0.0721 c2) in double per pixel of a 3-byte image */
+fu_status fu_image_to_intensity(const uint8_t* rgb, uint8_t* image, int cols, int rows, fu_stream s);
+/* PyrDownGauss on bytes (Reduction.h:31-32, Pyrdown.cu:171-181); dst is (cols/2) x (rows/2) */
+fu_status fu_pyrdown_gauss_u8(const uint8_t* src, int cols, int rows, uint8_t* dst, fu_stream s);
+/* ComputeDerivativeImage (Reduction.h:37-38, Pyrdown.cu:238-249): 3x3 Sobel as shorts */
+fu_status fu_compute_derivative_image(const uint8_t* image, int16_t* dIdx, int16_t* dIdy, int cols, int rows,
+                                      fu_stream s);
+/* RGBStep (Reduction.h:74-84, Reduction.cu:537-627): one photometric reduction.
+ * residual[2] = {sum of weighted squared differences, count}, A (6x6 row-major) and b (6) in
+ * FP64 on the host; corresp[2] (may be NULL) = {count, sum of squared differences} of the
+ * residual pass, from which sigma = sqrt(sum / count).  At most 2^22 pixels.  Synchronises. */
+fu_status fu_rgb_step(const uint8_t* next_image, const uint8_t* last_image, const float* next_vmap,
+                      const float* last_vmap, const int16_t* dIdx, const int16_t* dIdy, int cols, int rows,
+                      const float Rcurr[9], const float RcurrInv[9], const float Rlast[9], const float RlastInv[9],
+                      const float tcurr[3], const float tlast[3], float fx, float fy, float cx, float cy,
+                      float residual[2], double A[36], double b[6], int32_t corresp[2], fu_stream s);
+
 /* ---------------------------------------------------------------- tracker (Tracker::ComputeSE3) */
 typedef struct fu_track_result {
     double pose[16];       /* NextFrame->pose after tracking (rolled back on failure) */
@@ -180,6 +199,8 @@ typedef struct fu_track_result {
     int32_t count;         /* correspondences of the last iteration */
     int32_t status;        /* FU_TRACK_* */
     int32_t iterations;    /* iterations run */
+    float rgb_error;       /* photometric term of the last iteration: sqrt(sum r^2) / count */
+    int32_t rgb_count;
 } fu_track_result;
 
 #define FU_TRACK_RUNNING 0
@@ -200,6 +221,16 @@ fu_status fu_tracker_set_last_pose(fu_tracker* t, const double pose[16], fu_stre
 fu_status fu_compute_se3(fu_tracker* t, const float* const next_vmap[3], const float* const next_nmap[3],
                          const float* const last_vmap[3], const float* const last_nmap[3], const float K[4],
                          const int32_t iters[3], const int32_t min_count[3], float thresh, fu_stream s);
+/* Tracker::ComputeSE3(icpOnly=false, iter, thresh) (Tracking.cc:296-409), the reference's
+ * default tracking: every iteration runs RGBStep (Reduction.cu:537-627) on the intensity
+ * pyramids (next/last, level l is (cols>>l) x (rows>>l) bytes) and the next frame's Sobel
+ * derivatives, fails the frame when its residual count is below min_count or its error is
+ * NaN (:341-346), and solves matA_rgb * 1e-4f + matA_icp (:379-383).  At most 2^22 pixels. */
+fu_status fu_compute_se3_rgbd(fu_tracker* t, const float* const next_vmap[3], const float* const next_nmap[3],
+                              const float* const last_vmap[3], const float* const last_nmap[3],
+                              const uint8_t* const next_image[3], const uint8_t* const last_image[3],
+                              const int16_t* const dIdx[3], const int16_t* const dIdy[3], const float K[4],
+                              const int32_t iters[3], const int32_t min_count[3], float thresh, fu_stream s);
 /* the result of the last compute_se3; synchronises */
 fu_status fu_tracker_result(fu_tracker* t, fu_track_result* out, fu_stream s);
 
@@ -215,6 +246,9 @@ typedef struct fu_system_params {
     int32_t plain;       /* 1: reference-style control (host solve, one synchronisation per
                             stage); 0: device-resident (on-device solve, no host round trip) */
     int32_t use_graph;   /* device-resident mode only: replay the frame as one hipGraph */
+    int32_t use_rgb;     /* 1: track with the photometric term blended into ICP, as the
+                            reference's default TrackFrame does (ComputeSE3(icpOnly=false),
+                            Tracking.cc:162, 379-383); 0: ICP only (Tracking.cc:642-682) */
     fu_map_params map;
 } fu_system_params;
 
@@ -229,6 +263,7 @@ typedef struct fu_frame_result {
     uint32_t n_fragments;
     uint32_t flags;
     uint32_t frame_index;
+    int32_t rgb_count;      /* photometric residuals of the last iteration (use_rgb) */
 } fu_frame_result;
 
 /* maps owned by a system (device pointers) */
@@ -244,6 +279,12 @@ typedef struct fu_system_buffers {
     float* zrange_min;
     float* zrange_max;
     int32_t zcols, zrows;
+    /* photometric term (use_rgb only, else null): this frame's intensity pyramid and its
+     * Sobel derivatives, and the pyramid of the last tracked frame (LastFrame->image) */
+    uint8_t* image[3];
+    uint8_t* last_image[3];
+    int16_t* dIdx[3];
+    int16_t* dIdy[3];
 } fu_system_buffers;
 
 void fu_system_params_default(fu_system_params* p);

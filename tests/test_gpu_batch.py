@@ -37,9 +37,9 @@ def _frames():
     return out
 
 
-def _system(i, K):
+def _system(i, K, use_rgb=0):
     kind, size, seed, cols, rows, start, skip = SESSIONS[i]
-    return A.System(A.system_params(cols=cols, rows=rows, K=K, map=MAP))
+    return A.System(A.system_params(cols=cols, rows=rows, K=K, map=MAP, use_rgb=use_rgb))
 
 
 def _state(s):
@@ -50,15 +50,19 @@ def _state(s):
     sdf, w, rgb = m.read_blocks(xyz)
     vm = s.view("model_vmap").cpu().numpy()
     nm = s.view("model_nmap").cpu().numpy()
+    if s.params.use_rgb:  # the last tracked frame's intensity pyramid
+        vm = [vm] + [s.view("last_image", l).cpu().numpy().astype(np.float32) for l in range(3)]
+        vm = np.concatenate([v.ravel() for v in vm])
     return xyz, sdf, w, rgb, vm, nm
 
 
-def test_batch_equals_alone(gpu):
+@pytest.mark.parametrize("use_rgb", [0, 1])
+def test_batch_equals_alone(gpu, use_rgb):
     frames = _frames()
     alone_res, alone_state = [], []
     for i, (K, fr) in enumerate(frames):
-        print("alone session", i, flush=True)
-        s = _system(i, K)
+        print("alone session", i, "use_rgb", use_rgb, flush=True)
+        s = _system(i, K, use_rgb)
         res = []
         for d, c in fr:
             s.GrabImage(d, c)
@@ -70,7 +74,7 @@ def test_batch_equals_alone(gpu):
 
     print("batched", flush=True)
     b = A.Batch(8)
-    systems = [_system(i, K) for i, (K, _) in enumerate(frames)]
+    systems = [_system(i, K, use_rgb) for i, (K, _) in enumerate(frames)]
     for s in systems:
         b.attach(s)
     nxt = [0] * len(systems)
@@ -89,6 +93,7 @@ def test_batch_equals_alone(gpu):
             assert ra["track_status"] == rb["track_status"], (i, f)
             assert np.array_equal(ra["pose"], rb["pose"]), (i, f, ra["pose"], rb["pose"])
             assert ra["n_visible"] == rb["n_visible"] and ra["n_allocated"] == rb["n_allocated"], (i, f)
+            assert ra["rgb_count"] == rb["rgb_count"], (i, f)
         xa, sa, wa, ca, va, na = alone_state[i]
         xb, sb, wb, cb, vb, nb = _state(s)
         assert np.array_equal(xa, xb), i

@@ -13,10 +13,10 @@ from tests.test_gpu_batch import MAP, SESSIONS, _frames, _state, _system
 pytestmark = pytest.mark.gpu
 
 
-def _alone(frames):
+def _alone(frames, use_rgb=0):
     res, state = [], []
     for i, (K, fr) in enumerate(frames):
-        s = _system(i, K)
+        s = _system(i, K, use_rgb)
         r = []
         for d, c in fr:
             s.GrabImage(d, c)
@@ -27,17 +27,17 @@ def _alone(frames):
     return res, state
 
 
-def _params(i, K):
+def _params(i, K, use_rgb=0):
     kind, size, seed, cols, rows, start, skip = SESSIONS[i]
-    return A.system_params(cols=cols, rows=rows, K=K, map=MAP)
+    return A.system_params(cols=cols, rows=rows, K=K, map=MAP, use_rgb=use_rgb)
 
 
-@pytest.mark.parametrize("closed_loop", [False, True])
-def test_server_same_alone_and_in_mix(gpu, closed_loop):
+@pytest.mark.parametrize("closed_loop,use_rgb", [(False, 0), (True, 0), (False, 1)])
+def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb):
     frames = _frames()
-    alone_res, alone_state = _alone(frames)
+    alone_res, alone_state = _alone(frames, use_rgb)
     sv = A.Server(max_batch=3, max_inflight=2, lanes=2)
-    sids = [sv.open(_params(i, K)) for i, (K, _) in enumerate(frames)]
+    sids = [sv.open(_params(i, K, use_rgb)) for i, (K, _) in enumerate(frames)]
     trace = []
     rng = np.random.default_rng(5)
     for i, (K, fr) in enumerate(frames):
