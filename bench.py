@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=12)
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed steps of the mixed-resolution run (0: skip)")
     ap.add_argument("--single-steps", type=int, default=200, help="frames of the single-stream run (0: skip)")
+    ap.add_argument("--rgb-steps", type=int, default=30,
+                    help="timed steps of the served run with the photometric term (0: skip)")
     ap.add_argument("--lanes", type=int, default=2)
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--inflight", type=int, default=2)
@@ -160,13 +162,14 @@ def poisson_arrivals(rng, n_sessions, n, fps=30.0):
     return out
 
 
-def served(A, torch, args, sizes, frames, steps, realtime, profile):
+def served(A, torch, args, sizes, frames, steps, realtime, profile, use_rgb=0):
     """open the sessions, warm up, then the backlog (capacity) run, the 30 fps run and the
     stage profile; returns a dict of results"""
     sv = A.Server(max_batch=args.max_batch, max_inflight=args.inflight, lanes=args.lanes, block_budget=BUDGET)
     sids = []
     for cols, rows in sizes:
-        sids.append(sv.open(A.system_params(cols=cols, rows=rows, K=kmat(cols, rows), map=MAP_SESSION)))
+        sids.append(sv.open(A.system_params(cols=cols, rows=rows, K=kmat(cols, rows), map=MAP_SESSION,
+                                            use_rgb=use_rgb)))
     W = args.warmup
     sv.run_trace(trace(sids, frames, 0, W), closed_loop=True)
     torch.cuda.synchronize()
@@ -221,6 +224,7 @@ def served(A, torch, args, sizes, frames, steps, realtime, profile):
         sv.run_trace(trace(sids, frames, lo, lo + profile))
         res["stages"] = sv.stage_times()
         sv.set_profiling(False)
+    res["rgb_count_mean"] = float(np.mean([c.get("rgb_count", 0) for c in done if c.get("kind", 0) == 0]))
     sv.close()
     torch.cuda.synchronize()
     return res
@@ -263,8 +267,9 @@ def single_stream(A, torch, args):
     dev = [A.to_device(*scene.render(f, 640, 480, K640, 5000.0, True)) for f in range(W + n)]
     stream = torch.cuda.current_stream()
     out = {}
-    for fmt, name in ((0, "f32 sdf + u8 weight"), (1, "f32 sdf + u16 weight"), (2, "f16 sdf + u8 weight")):
-        s = A.System(A.system_params(use_graph=True, map=dict(MAP_SINGLE, voxel_format=fmt)))
+    for fmt, name, rgbm in ((0, "f32 sdf + u8 weight", 0), (1, "f32 sdf + u16 weight", 0), (2, "f16 sdf + u8 weight", 0),
+                            (0, "f32 sdf + u8 weight, RGB+ICP tracking", 1)):
+        s = A.System(A.system_params(use_graph=True, use_rgb=rgbm, map=dict(MAP_SINGLE, voxel_format=fmt)))
         for i in range(W):
             s.GrabImage(*dev[i])
         torch.cuda.synchronize()
@@ -280,7 +285,7 @@ def single_stream(A, torch, args):
         r = s.result()
         entry = dict(value=round(n / wall, 2), latency_ms={"p50": pct(lat, 50), "p99": pct(lat, 99)},
                      last_track_status=r["track_status"], n_allocated=r["n_allocated"])
-        if fmt == 0:
+        if fmt == 0 and not rgbm:
             # MeshScene of the whole map (the configuration meshes every 100 frames)
             m = s.mapping()
             m.MeshScene(max_triangles=4_000_000)
@@ -328,6 +333,20 @@ def rooflines(st, sizes, nsess):
                          avg_launch_us=round(ms * 1e3, 2), frames_per_launch=round(per_step, 2),
                          n_visible_per_frame=round(nvis, 1))
     return out
+
+
+def rgb_tracking_roofline(st):
+    """the tracking stage with the photometric term: per pixel-iteration the ICP inputs (64 B)
+    plus this frame's intensity, the last frame's intensity at the match, the two derivatives
+    and the residual record written and read back (1 + 1 + 4 + 8 B) - 78 B as the algorithmic
+    lower bound, over the stage's HIP-event time (three launches per iteration)"""
+    steps = max(1.0, st["steps"])
+    b = (st["icp_pixels"] / steps) * 78
+    ms = st["ms"]["icp"] / steps
+    ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                bytes_per_step=int(b), stage_ms_per_step=round(ms, 4),
+                kernels="kb_rgb_resid + kb_icp_rgb + kb_icp_rgb_final per iteration")
 
 
 def cpu_baseline(A, seconds):
@@ -379,6 +398,20 @@ def main():
     log("served: %.0f frames/s" % (r["tracked"] / r["wall"]))
     p = plain(A, torch, args, frames)
     log("plain: %.0f frames/s" % p["value"])
+    rgbd = None
+    if args.rgb_steps > 0:
+        # the reference's default tracking: the photometric term blended into every ICP iteration
+        steps_rgb = min(args.rgb_steps, args.steps)
+        g = served(A, torch, args, sizes, frames, steps_rgb, args.realtime_frames, args.profile_steps, use_rgb=1)
+        gst = g.get("stages")
+        gsteps = max(1.0, gst["steps"]) if gst else 1.0
+        rgbd = dict(value=round(g["tracked"] / g["wall"], 2), frames=g["frames"],
+                    tracked_fraction=round(g["tracked"] / max(1, g["frames"]), 4),
+                    latency_ms={"p50": pct(g.get("lat_rt", []), 50), "p99": pct(g.get("lat_rt", []), 99)},
+                    rgb_residuals_per_frame=round(g["rgb_count_mean"], 1),
+                    stage_ms_per_step=({k: round(v / gsteps, 4) for k, v in gst["ms"].items()} if gst else None),
+                    tracking=rgb_tracking_roofline(gst) if gst else None)
+        log("served RGB+ICP: %.0f frames/s" % rgbd["value"])
     del frames
     torch.cuda.empty_cache()
 
@@ -437,6 +470,7 @@ def main():
         "served_over_plain": round(fps / p["value"], 3),
         "mixed_resolution": mixed,
         "single_stream": single,
+        "rgbd_tracking": rgbd,
         "stage_ms_per_step": ({k: round(v / steps_p, 4) for k, v in st["ms"].items()} if st else None),
         "map": {"n_allocated_max": r["n_allocated_max"], "n_allocated_mean": round(r["n_allocated_mean"], 1),
                 "flags": r["flags"]},

@@ -288,6 +288,7 @@ static fu_status retire(fu_server* sv, Lane& L) {
             c.n_visible = r[i].n_visible;
             c.n_allocated = r[i].n_alloc;
             c.flags = r[i].flags;
+            c.rgb_count = r[i].rgb_count;
             c.t_submit_ms = st.req[i].t_submit;
             c.t_done_ms = now;
             sv->done.push_back(c);

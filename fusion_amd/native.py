@@ -96,7 +96,7 @@ class Completion(C.Structure):
                 ("pose", C.c_double * 16), ("icp_error", C.c_float), ("n_visible", C.c_uint32),
                 ("n_allocated", C.c_uint32), ("flags", C.c_uint32), ("t_submit_ms", C.c_double),
                 ("t_done_ms", C.c_double), ("kind", C.c_int32), ("n_triangles", C.c_uint32),
-                ("n_triangles_total", C.c_uint32), ("n_occupied", C.c_uint32)]
+                ("n_triangles_total", C.c_uint32), ("n_occupied", C.c_uint32), ("rgb_count", C.c_int32)]
 
 
 class TraceFrame(C.Structure):

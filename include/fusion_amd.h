@@ -400,6 +400,7 @@ typedef struct {
     uint32_t n_triangles;  /* mesh requests: triangles written (min(total, max_triangles)) */
     uint32_t n_triangles_total;
     uint32_t n_occupied;   /* mesh requests: blocks with at least one triangle */
+    int32_t rgb_count;     /* frames of sessions with use_rgb: photometric residuals of the last iteration */
 } fu_completion;
 #define FU_REQ_FRAME 0
 #define FU_REQ_MESH 1
