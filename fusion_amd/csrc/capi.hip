@@ -614,8 +614,12 @@ struct fu_tracker {
     int cols, rows;
     TrackState* st = nullptr;
     float* partials = nullptr;  // 64 floats per ICP work-group
-    int* rgb_rec = nullptr;     // photometric residual records, cols * rows
+    int* rgb_rec = nullptr;     // photometric residual records, cols * rows (and the eligibility
+                                // words while a level's list is built: 8 B per 64 pixels)
     int* rgb_corr = nullptr;    // 2 ints per ICP work-group
+    int* rgb_list = nullptr;    // eligible pixels of the three levels (level l at list_off[l])
+    int* rgb_aux = nullptr;     // per level: the list length, then kIcpMaxBlocks work-group counts
+    size_t list_off[3];
     int nblocks[3];
 };
 
@@ -647,15 +651,24 @@ extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
     fu_tracker* t = new fu_tracker();
     t->cols = cols;
     t->rows = rows;
-    for (int l = 0; l < 3; ++l) t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
+    size_t nlist = 0;
+    for (int l = 0; l < 3; ++l) {
+        t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
+        t->list_off[l] = nlist;
+        nlist += (size_t)(cols >> l) * (rows >> l);
+    }
     if (hipMalloc(&t->st, sizeof(TrackState)) != hipSuccess || hipMalloc(&t->partials, kIcpMaxBlocks * 64 * 4) != hipSuccess ||
         hipMalloc(&t->rgb_corr, kIcpMaxBlocks * 2 * 4) != hipSuccess ||
-        hipMalloc(&t->rgb_rec, (size_t)cols * rows * 4) != hipSuccess) {
+        hipMalloc(&t->rgb_rec, (size_t)cols * rows * 4 + 256) != hipSuccess ||
+        hipMalloc(&t->rgb_list, nlist * 4 + 16) != hipSuccess ||
+        hipMalloc(&t->rgb_aux, (size_t)3 * (kIcpMaxBlocks + 64) * 4) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(t->st);
         (void)hipFree(t->partials);
         (void)hipFree(t->rgb_corr);
         (void)hipFree(t->rgb_rec);
+        (void)hipFree(t->rgb_list);
+        (void)hipFree(t->rgb_aux);
         delete t;
         return set_err(FU_ERR_NOMEM, "tracker allocation failed");
     }
@@ -672,6 +685,8 @@ extern "C" fu_status fu_tracker_destroy(fu_tracker* t) {
     (void)hipFree(t->partials);
     (void)hipFree(t->rgb_corr);
     (void)hipFree(t->rgb_rec);
+    (void)hipFree(t->rgb_list);
+    (void)hipFree(t->rgb_aux);
     delete t;
     return FU_OK;
 }
@@ -714,13 +729,16 @@ static IcpArgs make_icp(fu_tracker* t, int level, const float* nv, const float* 
 }
 
 // the photometric inputs of one level (RGBStep's images, Tracking.cc:319-325)
-static void set_rgb(IcpArgs& a, fu_tracker* t, const uint8_t* inext, const uint8_t* ilast, const int16_t* dx,
-                    const int16_t* dy) {
+static void set_rgb(IcpArgs& a, fu_tracker* t, int level, const uint8_t* inext, const uint8_t* ilast,
+                    const int16_t* dx, const int16_t* dy) {
     a.use_rgb = 1;
     a.rgb.inext = inext;
     a.rgb.ilast = ilast;
     a.rgb.dx = dx;
     a.rgb.dy = dy;
+    a.rgb.list = t->rgb_list + t->list_off[level];
+    a.rgb.count = t->rgb_aux + (size_t)level * (kIcpMaxBlocks + 64);
+    a.rgb.wgcnt = a.rgb.count + 64;
     a.rgb.rec = t->rgb_rec;
     a.rgb.corr = t->rgb_corr;
 }
@@ -741,11 +759,20 @@ static void queue_se3(fu_tracker* t, const float* const nv[3], const float* cons
     int last_level = -1;
     for (int l = 2; l >= 0; --l)
         if (iters[l] > 0) last_level = l;
+    if (rgb) {  // the eligible pixels of each level, once per frame
+        IcpArgs la[3];
+        for (int l = 0; l < 3; ++l) {
+            la[l] = make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], 0, thresh);
+            set_rgb(la[l], t, l, rgb->next_image[l], rgb->last_image[l], rgb->dx[l], rgb->dy[l]);
+            la[l].use_rgb = iters[l] > 0;
+        }
+        launch_rgb_lists(la, t->nblocks, 3, s);
+    }
     for (int l = 2; l >= 0; --l)
         for (int j = 0; j < iters[l]; ++j) {
             const int fin = (l == last_level && j == iters[l] - 1);
             IcpArgs a = make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], fin, thresh);
-            if (rgb) set_rgb(a, t, rgb->next_image[l], rgb->last_image[l], rgb->dx[l], rgb->dy[l]);
+            if (rgb) set_rgb(a, t, l, rgb->next_image[l], rgb->last_image[l], rgb->dx[l], rgb->dy[l]);
             launch_icp(a, t->nblocks[l], s);
         }
 }
@@ -917,8 +944,10 @@ extern "C" fu_status fu_rgb_step(const uint8_t* next_image, const uint8_t* last_
     a.st = scratch->st;
     a.partials = scratch->partials;
     a.mode = 0;
-    set_rgb(a, scratch, next_image, last_image, dIdx, dIdy);
-    launch_icp(a, icp_blocks(cols * rows), S(s));
+    set_rgb(a, scratch, 0, next_image, last_image, dIdx, dIdy);
+    const int nb = icp_blocks(cols * rows);
+    launch_rgb_lists(&a, &nb, 1, S(s));
+    launch_icp(a, nb, S(s));
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(&h, scratch->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
     HIPCHK(hipStreamSynchronize(S(s)));
@@ -1234,11 +1263,20 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
         int last_level = -1;
         for (int l = 2; l >= 0; --l)
             if (p.iters[l] > 0) last_level = l;
+        if (p.use_rgb) {
+            IcpArgs la[3];
+            for (int l = 0; l < 3; ++l) {
+                la[l] = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0);
+                set_rgb(la[l], t, l, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
+                la[l].use_rgb = p.iters[l] > 0;
+            }
+            launch_rgb_lists(la, t->nblocks, 3, s);
+        }
         for (int l = 2; l >= 0 && h.status == TS_RUNNING; --l)
             for (int j = 0; j < p.iters[l] && h.status == TS_RUNNING; ++j) {
                 HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
                 IcpArgs a = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0);
-                if (p.use_rgb) set_rgb(a, t, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
+                if (p.use_rgb) set_rgb(a, t, l, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
                 launch_icp(a, t->nblocks[l], s);
                 LAUNCHCHK();
                 HIPCHK(hipMemcpyAsync(&h, t->st, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1548,7 +1586,7 @@ static void fill_desc(fu_system* sys, SessDesc& d) {
         d.icp[l] = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 1, p.min_count[l], 0,
                             p.icp_thresh);
         if (p.use_rgb) {
-            set_rgb(d.icp[l], t, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
+            set_rgb(d.icp[l], t, l, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
             d.image[l] = b.image[l];
             d.last_image[l] = b.last_image[l];
             d.didx[l] = b.dIdx[l];
@@ -1656,6 +1694,7 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     mark(FU_STAGE_PYRAMID + 1);
     if (track) {
         launch_b_track_begin(b, h, hs);
+        if (bt->rgb == 1) launch_b_rgb_lists(b, h, hs);
         for (int l = 2; l >= 0; --l)
             for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
     }

@@ -151,6 +151,7 @@ void launch_b_image_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 // icp.hip
 void launch_b_track_begin(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStream_t s);
+void launch_b_rgb_lists(BatchLaunch& b, const SessDesc* h, hipStream_t s);  // once per frame
 void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_results(BatchLaunch& b, FrameBrief* out, hipStream_t s);
 // map.hip

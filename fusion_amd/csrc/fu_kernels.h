@@ -133,13 +133,19 @@ constexpr int kIcpThreads = 1024;
 constexpr int kIcpFinalThreads = 256;
 
 // Inputs of the photometric term at one level (RGBReduction, Reduction.cu:377-407)
+// The pose-independent tests of FindCorresp (Reduction.cu:420-433: the 5-pixel margin, a
+// non-zero 4x4 window, a non-zero gradient, a valid vertex) are run once per frame and level;
+// the pixels that pass form an ordered list, and the per-iteration passes run over the list.
 struct RgbArgs {
     const unsigned char* inext;  // intensity of this frame
     const unsigned char* ilast;  // intensity of the last tracked frame
     const short* dx;             // Sobel derivatives of this frame's intensity
     const short* dy;
-    int* rec;   // per pixel, written by the residual pass: (j << 9) | (diff + 256), or -1
-    int* corr;  // 2 ints per residual-pass work-group: count, sum of diff^2 (32-bit, wrapping)
+    int* list;    // eligible pixels of the level in pixel order
+    int* count;   // their number
+    int* wgcnt;   // per list-building work-group: eligible pixels of its 1024
+    int* rec;     // per list entry, written by the residual pass: (j << 9) | (diff + 256), or -1
+    int* corr;    // 2 ints per residual-pass work-group: count, sum of diff^2 (32-bit, wrapping)
 };
 
 struct IcpArgs {
@@ -195,6 +201,8 @@ void upload_mc_tables();
 void launch_mesh(const MapView& m, const MeshOut& o, unsigned* counts, unsigned* chunk, hipStream_t s);
 // icp.hip
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s);  // with a.use_rgb: RGBStep too
+// the list of eligible pixels of the photometric term for up to three levels (a[l].use_rgb)
+void launch_rgb_lists(const IcpArgs* a, const int* nblocks, int nlevels, hipStream_t s);
 void launch_track_begin(TrackState* st, hipStream_t s);
 void launch_track_commit(TrackState* st, hipStream_t s);
 

@@ -243,14 +243,15 @@ __device__ __forceinline__ RgbPose rgb_pose(const TrackState* st_) {
     return P;
 }
 
-// RGBReduction::FindCorresp (Reduction.cu:414-467) of pixel (x, y): the last-frame pixel j
-// and the intensity difference, or false
-__device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, int x, int y, int& j, int& diff) {
+// The pose-independent part of RGBReduction::FindCorresp (Reduction.cu:420-433) for pixel
+// (x, y): inside the 5-pixel margin, the 4x4 window [y-2, y+2) x [x-2, x+2) non-zero (the
+// margin keeps it in the image, so the reference's clamps are inactive), a non-zero gradient
+// (sqrtf(gx * gx + gy * gy) > 0: the integer sum is positive unless both are zero) and a valid
+// vertex
+__device__ __forceinline__ bool rgb_eligible(const IcpArgs& a, int x, int y) {
     const int cols = a.cols, rows = a.rows;
     if (!(x >= 5 && x < cols - 5 && y >= 5 && y < rows - 5)) return false;
     const auto inext = FU_G(const unsigned char, a.rgb.inext);
-    // the 4x4 window [y-2, y+2) x [x-2, x+2) is non-zero; the margin keeps it in the image,
-    // so the reference's clamps are inactive
     bool valid = true;
 #pragma unroll
     for (int u = -2; u < 2; ++u)
@@ -259,10 +260,15 @@ __device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, 
     if (!valid) return false;
     const int i = y * cols + x;
     const int gx = FU_G(const short, a.rgb.dx)[i], gy = FU_G(const short, a.rgb.dy)[i];
-    // sqrtf(gx * gx + gy * gy) > 0: the integer sum is positive unless both are zero
     if (gx == 0 && gy == 0) return false;
+    return !isnanf_(FU_G(const f4, a.vcurr)[i].x);
+}
+
+// the rest of FindCorresp (Reduction.cu:434-463) for an eligible pixel i: the last-frame
+// pixel j and the intensity difference, or false
+__device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, int i, int& j, int& diff) {
+    const int cols = a.cols, rows = a.rows;
     const f4 vc = FU_G(const f4, a.vcurr)[i];
-    if (isnanf_(vc.x)) return false;
     const f3 vg = add(mv(P.Rc, xyz(vc)), P.tc);
     const f3 vl = mv(P.Rli, sub(vg, P.tl));
     // __float2int_rd(fx * x / z + cx + 0.5): float quotient and sum, 0.5 added in double,
@@ -275,7 +281,7 @@ __device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, 
     const int lk = FU_G(const unsigned char, a.rgb.ilast)[k];
     if (!(!isnanf_(la.x) && (double)norm4(sub4(la, vc)) < 0.1 && lk != 0)) return false;
     j = k;
-    diff = (int)inext[i] - lk;
+    diff = (int)FU_G(const unsigned char, a.rgb.inext)[i] - lk;
     return true;
 }
 
@@ -287,8 +293,11 @@ __device__ __forceinline__ void rgb_row(const IcpArgs& a, const RgbPose& P, int 
     if ((double)sigma < 1e-6) w = 1.0f;
     const f4 la = FU_G(const f4, a.vlast)[j];
     const f3 point = mv(P.Rci, sub(add(mv(P.Rl, xyz(la)), P.tl), P.tc));
-    const float gx = (float)((double)FU_G(const short, a.rgb.dx)[i] / 9.0);
-    const float gy = (float)((double)FU_G(const short, a.rgb.dy)[i] / 9.0);
+    // (float)((double)s / 9.0) in the reference: for every short s this equals the correctly
+    // rounded float quotient (s / 9 has a period-6 binary expansion, so rounding to double
+    // first never lands on a float rounding boundary; checked for all 65536 values)
+    const float gx = (float)FU_G(const short, a.rgb.dx)[i] / 9.0f;
+    const float gy = (float)FU_G(const short, a.rgb.dy)[i] / 9.0f;
     const float invz = 1.0f / point.z;
     f3 left;
     left.x = gx * a.fx * invz;
@@ -304,20 +313,80 @@ __device__ __forceinline__ void rgb_row(const IcpArgs& a, const RgbPose& P, int 
     row[6] = -w * (float)diff;
 }
 
-// residual pass: record (j << 9) | (diff + 256) or -1 per pixel, and the work-group's
-// {count, sum diff^2}; one pixel per lane
+// sum over the work-group of one int per lane (all lanes get it); `part` holds a slot per wave
+__device__ __forceinline__ unsigned block_sum_u32(unsigned v, unsigned* part) {
+    v = wave_sum_u32(v);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    return t;
+}
+
+// list building, pass 1: the eligibility of each pixel as one 64-bit word per wave (kept in
+// the record buffer until pass 2) and the eligible pixels of each 1024-pixel work-group
+__device__ __forceinline__ void rgb_count_body(WG g, const IcpArgs& a) {
+    const int N = a.cols * a.rows;
+    const int i = g.x * kIcpThreads + threadIdx.x;
+    bool e = false;
+    if (i < N) {
+        const int y = i / a.cols;
+        e = rgb_eligible(a, i - y * a.cols, y);
+    }
+    const unsigned long long m = ballot(e);
+    __shared__ unsigned part[kIcpThreads / 64];
+    if (lane_id() == 0) {
+        part[threadIdx.x >> 6] = (unsigned)__popcll(m);
+        FU_G(unsigned long long, a.rgb.rec)[g.x * (kIcpThreads / 64) + (threadIdx.x >> 6)] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int w = 0; w < kIcpThreads / 64; ++w) t += part[w];
+        FU_G(int, a.rgb.wgcnt)[g.x] = (int)t;
+    }
+}
+// pass 2: each work-group writes its eligible pixels at the prefix of the counts before it
+// (pixel order); the last work-group stores the total
+__device__ __forceinline__ void rgb_scatter_body(WG g, const IcpArgs& a) {
+    __shared__ unsigned part[kIcpThreads / 64];
+    __shared__ unsigned wbase[kIcpThreads / 64];
+    const auto wgcnt = FU_G(const int, a.rgb.wgcnt);
+    unsigned before = 0;
+    for (int t = threadIdx.x; t < g.x; t += blockDim.x) before += (unsigned)wgcnt[t];
+    const unsigned base = block_sum_u32(before, part);
+    const int wave = threadIdx.x >> 6;
+    const unsigned long long m = FU_G(const unsigned long long, a.rgb.rec)[g.x * (kIcpThreads / 64) + wave];
+    __syncthreads();  // part is reused
+    if (lane_id() == 0) part[wave] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = base;
+        for (int w = 0; w < kIcpThreads / 64; ++w) {
+            wbase[w] = t;
+            t += part[w];
+        }
+        if (g.x == g.nx - 1) *FU_G(int, a.rgb.count) = (int)t;
+    }
+    __syncthreads();
+    if ((m >> lane_id()) & 1ull) FU_G(int, a.rgb.list)[wbase[wave] + lane_rank(m)] = g.x * kIcpThreads + threadIdx.x;
+}
+
+// residual pass over the list: record (j << 9) | (diff + 256) or -1 per entry, and the
+// work-group's {count, sum diff^2}; one entry per lane
 __device__ __forceinline__ void rgb_resid_body(WG g, const IcpArgs& a) {
     const auto st = FU_G(const TrackState, a.st);
     if (a.mode != 0 && st->status != TS_RUNNING) return;  // uniform: a failed frame skips
-    const RgbPose P = rgb_pose(a.st);
-    const int N = a.cols * a.rows;
-    const int i = g.x * kIcpThreads + threadIdx.x;
+    const int n = *FU_G(const int, a.rgb.count);
+    const int k = g.x * kIcpThreads + threadIdx.x;
     int rec = -1, diff = 0;
-    if (i < N) {
-        const int y = i / a.cols, x = i - y * a.cols;
-        int j;
-        if (rgb_corresp(a, P, x, y, j, diff)) rec = (j << 9) | (diff + 256);
-        FU_G(int, a.rgb.rec)[i] = rec;
+    if (g.x * kIcpThreads < n) {  // uniform
+        const RgbPose P = rgb_pose(a.st);
+        if (k < n) {
+            int j;
+            if (rgb_corresp(a, P, FU_G(const int, a.rgb.list)[k], j, diff)) rec = (j << 9) | (diff + 256);
+            FU_G(int, a.rgb.rec)[k] = rec;
+        }
     }
     const bool ok = rec >= 0;
     const unsigned c = (unsigned)__popcll(ballot(ok));
@@ -384,13 +453,15 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
     float v[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) v[j] = 0.f;
+    // lane i: the ICP row of pixel i and the photometric row of list entry i
     const int i = g.x * kIcpThreads + threadIdx.x;
-    if (i < N) {
-        float row[7];
-        if (a.ncurr && icp_row(a, P, i, row)) row_products<0>(row, v);  // no normals: RGB rows only
+    float row[7];
+    if (i < N && a.ncurr && icp_row(a, P, i, row)) row_products<0>(row, v);  // no normals: RGB rows only
+    const int n = *FU_G(const int, a.rgb.count);
+    if (g.x * kIcpThreads < n && i < n) {
         const int rec = FU_G(const int, a.rgb.rec)[i];
         if (rec >= 0) {
-            rgb_row(a, Q, i, rec >> 9, (rec & 511) - 256, sigma, row);
+            rgb_row(a, Q, FU_G(const int, a.rgb.list)[i], rec >> 9, (rec & 511) - 256, sigma, row);
             row_products<32>(row, v);
         }
     }
@@ -422,8 +493,17 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts)
     __shared__ int corr[2];
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const float sigma = rgb_sigma(a, nparts, corr);
+    // rows w, w + NW, w + 2 NW, ... in that order; eight loads in flight ahead of the adds
     Kahan k{0.f, 0.f};
-    for (int g = w; g < nparts; g += NW) k.add(FU_G(const float, a.partials)[g * 64 + lane]);
+    const auto parts = FU_G(const float, a.partials);
+    for (int g = w; g < nparts; g += 8 * NW) {
+        float x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = (g + u * NW < nparts) ? parts[(g + u * NW) * 64 + lane] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (g + u * NW < nparts) k.add(x[u]);
+    }
     wsum[w][lane] = k.val();
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -450,6 +530,16 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts)
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts) {
     icp_rgb_final_body(a, nparts);
+}
+
+__global__ __launch_bounds__(kIcpThreads) void k_rgb_count(IcpArgs a) { rgb_count_body(hw_wg(), a); }
+__global__ __launch_bounds__(kIcpThreads) void k_rgb_scatter(IcpArgs a) { rgb_scatter_body(hw_wg(), a); }
+void launch_rgb_lists(const IcpArgs* a, const int* nblocks, int nlevels, hipStream_t s) {
+    for (int l = 0; l < nlevels; ++l)
+        if (a[l].use_rgb) {
+            hipLaunchKernelGGL(k_rgb_count, dim3(nblocks[l]), dim3(kIcpThreads), 0, s, a[l]);
+            hipLaunchKernelGGL(k_rgb_scatter, dim3(nblocks[l]), dim3(kIcpThreads), 0, s, a[l]);
+        }
 }
 
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
@@ -522,6 +612,22 @@ __global__ __launch_bounds__(kIcpFinalThreads) void kb_icp_final(BatchLaunch b) 
     icp_final_body(a, nb);
 }
 
+__global__ __launch_bounds__(kIcpThreads) void kb_rgb_count(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, s, a, nb)) return;
+    rgb_count_body(g, a);
+}
+__global__ __launch_bounds__(kIcpThreads) void kb_rgb_scatter(BatchLaunch b) {
+    WG g;
+    const int s = batch_slot(b, blockIdx.x, g);
+    IcpArgs a;
+    int nb;
+    if (!icp_slot_args(b, s, a, nb)) return;
+    rgb_scatter_body(g, a);
+}
 __global__ __launch_bounds__(kIcpThreads) void kb_rgb_resid(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
@@ -597,6 +703,18 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
         hipLaunchKernelGGL(kb_icp<false>, dim3(grid), dim3(kIcpThreads), 0, s, b);
     hipLaunchKernelGGL(kb_icp_final, dim3(b.n), dim3(kIcpFinalThreads), 0, s, b);
 }
+// the eligible-pixel lists of every level (iteration 0 of each level: a session without
+// iterations at a level builds none)
+void launch_b_rgb_lists(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    for (int l = 0; l < 3; ++l) {
+        b.level = l;
+        b.iter = 0;
+        const int grid = batch_layout(b, h, BT_ICP, l, 0);
+        hipLaunchKernelGGL(kb_rgb_count, dim3(grid), dim3(kIcpThreads), 0, s, b);
+        hipLaunchKernelGGL(kb_rgb_scatter, dim3(grid), dim3(kIcpThreads), 0, s, b);
+    }
+}
+
 void launch_b_track_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     (void)h;
     hipLaunchKernelGGL(kb_track_commit, dim3(b.n), dim3(64), 0, s, b);

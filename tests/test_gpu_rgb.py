@@ -136,7 +136,9 @@ def test_compute_se3_rgbd_vs_oracle(gpu, case):
     cu = lambda xs: [_cu(x) for x in xs]
     r = t.ComputeSE3RGBD(cu(v1), cu(n1), cu(v0), cu(n0), cu(i1), cu(i0), cu(dx1), cu(dy1), Kc)
     assert r["status"] == st == 1, (r["status"], st)
-    assert abs(r["rgb_count"] - rcnt_o) <= 2 and abs(r["count"] - cnt_o) <= 2
+    # the last iteration's counts come from poses that differ by rounding: a handful of
+    # pixels near an edge can change sides (exact on identical inputs: test_rgb_step_matches_oracle)
+    assert abs(r["rgb_count"] - rcnt_o) <= 1e-3 * rcnt_o and abs(r["count"] - cnt_o) <= 1e-3 * cnt_o
     assert _rot_err(r["pose"][:3, :3], pose_o[:3, :3]) <= 1e-5
     assert np.linalg.norm(r["pose"][:3, 3] - pose_o[:3, 3]) <= 1e-5
     if case == "plane":  # the in-plane motion ICP alone cannot see (0.6 pixel resolution)

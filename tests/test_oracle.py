@@ -556,3 +556,12 @@ def test_rgb_sums_fp32_close_to_fp64_and_failure_on_no_texture():
     v0n, n0n = _pyr(d0, K)
     st, pose, _, _, rcnt = O.compute_se3_rgbd(v0n, n0n, v0n, n0n, fi, fi, fdx, fdy, K, np.eye(4))
     assert st == 2 and rcnt == 0 and np.array_equal(pose, np.eye(4))
+
+
+def test_sobel_scale_float_division_equals_double_rounding():
+    """the GPU divides the Sobel responses by 9 in float; the reference computes
+    (float)((double)s / 9.0): the two agree for every short s"""
+    s = np.arange(-32768, 32768, dtype=np.int64)
+    ref = (s.astype(np.float64) / 9.0).astype(np.float32)
+    gpu = s.astype(np.float32) / np.float32(9.0)
+    assert np.array_equal(ref.view(np.uint32), gpu.view(np.uint32))
