@@ -26,6 +26,8 @@
  *     fused multiply-add.  For 63 of the 2^24 colours (those whose exact luma
  *     (2125 c1 + 7154 c0 + 721 c2) / 10000 is an integer) the truncated value depends on that
  *     choice, so a contracting compile of the reference may differ there by one level.
+ *  D12 RGBStep's integer sums (count, sum of squared differences) wrap at 32 bits instead of
+ *     overflowing (undefined in C++).
  *
  * Pinning (tests/test_oracle.py checks every function against closed forms, brute force or
  * numpy/scipy; the reference has no tests or golden vectors of its own).  PARITY UNPINNED:
