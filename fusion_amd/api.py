@@ -10,6 +10,7 @@ int16 (H, W), ranges float32 (H, W), vertex/normal maps float32 (H, W, 4), colou
 (H, W, 3).  Poses are 4x4 camera-to-world numpy float64 arrays.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import torch
@@ -147,6 +148,14 @@ def RGBStep(next_image, last_image, next_vmap, last_vmap, dIdx, dIdy, pose_curr,
             (int(corr[0]), int(corr[1])))
 
 
+def WriteMeshToDisk(path, vertex, normal, color, binary=False):
+    """System::WriteMeshToDisk (Core/System.cc:140-193): a PLY triangle soup of MeshScene's
+    output (device tensors (n, 3, 3) float, float, uint8)"""
+    n = int(vertex.shape[0])
+    check(N.lib().fu_mesh_write_ply(os.fsencode(path), _p(vertex) if n else None, _p(normal) if n else None,
+                                    _p(color) if n else None, n, int(binary), _s()), "fu_mesh_write_ply")
+
+
 # --------------------------------------------------------------------- map (Mapping)
 def map_params(**kw):
     p = N.MapParams()
@@ -244,6 +253,14 @@ class Mapping:
         k = n.value
         out = (vert[:k].reshape(k, 3, 3), norm[:k].reshape(k, 3, 3), col[:k].reshape(k, 3, 3))
         return out + ((meta[:k] if meta is not None else None),)
+
+    def save(self, path):
+        """System::WriteMapToDisk (Core/System.cc:195-234) as a compact versioned snapshot"""
+        check(N.lib().fu_map_save(self.handle, os.fsencode(path), _s()), "fu_map_save")
+
+    def load(self, path):
+        """System::ReadMapFromDisk (Core/System.cc:236-271) into a map of the same parameters"""
+        check(N.lib().fu_map_load(self.handle, os.fsencode(path), _s()), "fu_map_load")
 
     def export_blocks(self):
         n = C.c_uint32()
@@ -412,6 +429,13 @@ class System:
         d = {f: getattr(r, f) for f, _ in N.FrameResult._fields_ if f != "pose"}
         d["pose"] = np.array(r.pose[:]).reshape(4, 4)
         return d
+
+    def save(self, path):
+        """session snapshot: map, tracking state, model maps (and last intensity pyramid)"""
+        check(N.lib().fu_system_save(self.handle, os.fsencode(path), _s()), "fu_system_save")
+
+    def load(self, path):
+        check(N.lib().fu_system_load(self.handle, os.fsencode(path), _s()), "fu_system_load")
 
     def set_profiling(self, enable=True):
         check(N.lib().fu_system_set_profiling(self.handle, int(enable)), "fu_system_set_profiling")

@@ -15,6 +15,7 @@
 #include "fu_track.h"
 #include "fu_batch.h"
 #include "mc_tables.h"
+#include "fu_api.h"
 
 using namespace fu;
 
@@ -89,13 +90,6 @@ static PoseF posef_from(const fu_pose_f* p) {
 }
 
 // ------------------------------------------------------------------------------ map
-struct fu_map {
-    fu_map_params p;
-    MapView v;
-    unsigned* mesh_counts = nullptr;
-    unsigned* mesh_chunk = nullptr;
-    std::vector<void*> allocs;
-};
 
 extern "C" void fu_map_params_default(fu_map_params* p) {
     // Mapping/DeviceMap.h:135-151 with the voxel size and truncation of the 5 mm configuration
@@ -610,18 +604,6 @@ extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int col
 }
 
 // ------------------------------------------------------------------------------ tracker
-struct fu_tracker {
-    int cols, rows;
-    TrackState* st = nullptr;
-    float* partials = nullptr;  // 64 floats per ICP work-group
-    int* rgb_rec = nullptr;     // photometric residual records, cols * rows (and the eligibility
-                                // words while a level's list is built: 8 B per 64 pixels)
-    int* rgb_corr = nullptr;    // 2 ints per ICP work-group
-    int* rgb_list = nullptr;    // eligible pixels of the three levels (level l at list_off[l])
-    int* rgb_aux = nullptr;     // per level: the list length, then kIcpMaxBlocks work-group counts
-    size_t list_off[3];
-    int nblocks[3];
-};
 
 // One pixel per lane (a kIcpThreads-pixel work-group each) up to 4096 work-groups (4M
 // pixels): every pixel's association is a dependent gather, so latency is hidden across waves
@@ -1000,25 +982,6 @@ extern "C" fu_status fu_compute_derivative_image(const uint8_t* image, int16_t* 
 }
 
 // ------------------------------------------------------------------------------ system
-struct fu_system {
-    fu_system_params p;
-    fu_map* map = nullptr;
-    fu_tracker* trk = nullptr;
-    std::vector<void*> allocs;
-    fu_system_buffers b;
-    int lc[3], lr[3];
-    unsigned frame = 0;
-    hipGraphExec_t graph = nullptr;
-    bool prof = false;
-    hipEvent_t ev[FU_NUM_STAGES + 1] = {};
-    fu_stage_times times{};
-    const uint16_t* graph_depth = nullptr;
-    const uint8_t* graph_rgb = nullptr;
-    fu_batch* batch = nullptr;  // batch context the system is attached to
-    int batch_slot = -1;
-    f4* fuse_px = nullptr;      // packed integration inputs (depth, normal) per pixel
-    unsigned* fuse_rgb = nullptr;
-};
 
 extern "C" void fu_system_params_default(fu_system_params* p) {
     memset(p, 0, sizeof(*p));

@@ -136,6 +136,11 @@ SIGNATURES = {
     "fu_compute_se3": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), FP, IP, IP, F, P]),
     "fu_tracker_result": (I, [P, C.POINTER(TrackResult), P]),
     "fu_image_to_intensity": (I, [P, P, I, I, P]),
+    "fu_map_save": (I, [P, C.c_char_p, P]),
+    "fu_map_load": (I, [P, C.c_char_p, P]),
+    "fu_system_save": (I, [P, C.c_char_p, P]),
+    "fu_system_load": (I, [P, C.c_char_p, P]),
+    "fu_mesh_write_ply": (I, [C.c_char_p, P, P, P, C.c_uint32, I, P]),
     "fu_pyrdown_gauss_u8": (I, [P, I, I, P, P]),
     "fu_compute_derivative_image": (I, [P, P, P, I, I, P]),
     "fu_rgb_step": (I, [P, P, P, P, P, P, I, I, FP, FP, FP, FP, FP, FP, F, F, F, F, FP, DP, DP,

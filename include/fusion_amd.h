@@ -303,6 +303,32 @@ fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const
 /* outcome of the last frame; synchronises */
 fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
 
+/* ---------------------------------------------------------------- on-disk formats */
+/* Map snapshot: System::WriteMapToDisk / ReadMapFromDisk (Core/System.cc:195-271).  The
+ * reference dumps the whole device map raw; the snapshot holds the used hash entries (at
+ * their indices), the allocated blocks' keys and voxels and the map counters, in a versioned
+ * little-endian file with an FNV-1a checksum.  Loading needs a map of the same parameters
+ * (buckets, excess, blocks, voxel size/format, truncation, depth range, weight cap) and
+ * rebuilds the same entries and pool slots, so the map continues exactly as before.  A
+ * file of other parameters is refused with the map untouched; a truncated or corrupt one
+ * is refused and leaves the map empty.  Both synchronise `s`. */
+fu_status fu_map_save(fu_map* m, const char* path, fu_stream s);
+fu_status fu_map_load(fu_map* m, const char* path, fu_stream s);
+/* Session snapshot (no reference counterpart; eviction and resume of a served session): the
+ * map snapshot plus the tracking state, the frame count, the level-0 model maps the next frame
+ * is tracked against and, with use_rgb, the last frame's intensity pyramid.  Loading needs a
+ * system of the same frame size, intrinsics, depth scale/cut-off, tracking mode and map
+ * parameters; the next frame then tracks and fuses exactly as it would have without the
+ * round trip.  After a failed load, reset the session (fu_system_reset) before using it. */
+fu_status fu_system_save(fu_system* sys, const char* path, fu_stream s);
+fu_status fu_system_load(fu_system* sys, const char* path, fu_stream s);
+/* System::WriteMeshToDisk (Core/System.cc:140-193): a triangle soup of 3 vertices per
+ * triangle (x y z nx ny nz red green blue) and faces "3 i i+1 i+2", as the output of
+ * fu_mesh_scene (device or host arrays).  binary = 0: the reference's ASCII text (floats as
+ * std::ostream prints them, i.e. %g); 1: the same elements as binary_little_endian. */
+fu_status fu_mesh_write_ply(const char* path, const float* vertex, const float* normal, const uint8_t* color,
+                            uint32_t n_triangles, int binary, fu_stream s);
+
 /* Per-stage timing with HIP events on the frame's stream (the reference defines a StopWatch,
  * Utility/StopWatch.h:8-28, but never calls it).  While enabled, every grab launches its
  * kernels directly (no graph), records an event at each stage boundary, synchronises at the
