@@ -291,15 +291,45 @@ def single_stream(A, torch, args):
             m.MeshScene(max_triangles=4_000_000)
             torch.cuda.synchronize()
             a = time.perf_counter()
-            v, _, _, _ = m.MeshScene(max_triangles=4_000_000)
+            v, nrm, col, _ = m.MeshScene(max_triangles=4_000_000)
             torch.cuda.synchronize()
             entry["mesh"] = dict(ms=round((time.perf_counter() - a) * 1e3, 3), triangles=int(v.shape[0]))
+            entry["on_disk"] = on_disk(A, torch, m, v, nrm, col)
         out[name] = entry
         s.close()
         torch.cuda.synchronize()
     first = out["f32 sdf + u8 weight"]
     return dict(value=first["value"], frames=n, latency_ms=first["latency_ms"], formats=out,
                 workload="12x8x3 m multi-room, 640x480, 2^21 buckets, 2M blocks, hipGraph per frame")
+
+
+def on_disk(A, torch, m, v, nrm, col):
+    """the map snapshot (save, then load into a second map of the same parameters) and the
+    binary PLY of the mesh, through a temporary directory of the host"""
+    import tempfile
+
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "map.fusn")
+        a = time.perf_counter()
+        m.save(path)
+        t_save = time.perf_counter() - a
+        size = os.path.getsize(path)
+        m2 = A.Mapping(**dict(MAP_SINGLE, voxel_format=0))
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        m2.load(path)
+        t_load = time.perf_counter() - a
+        same = bool(np.array_equal(m.export_blocks(), m2.export_blocks()))
+        m2.close()
+        ply = os.path.join(td, "mesh.ply")
+        a = time.perf_counter()
+        A.WriteMeshToDisk(ply, v, nrm, col, binary=True)
+        t_ply = time.perf_counter() - a
+        out = dict(snapshot_bytes=size, save_ms=round(t_save * 1e3, 1), load_ms=round(t_load * 1e3, 1),
+                   save_GBps=round(size / t_save / 1e9, 2), load_GBps=round(size / t_load / 1e9, 2),
+                   loaded_blocks_equal=same, ply_binary_bytes=os.path.getsize(ply), ply_binary_ms=round(t_ply * 1e3, 1))
+    return out
 
 
 def rooflines(st, sizes, nsess):
