@@ -191,3 +191,33 @@ def test_served_session_evicted_and_resumed(gpu, tmp_path):
         assert np.array_equal(c["pose"], ref_poses[c["tag"]]), c["tag"]
     sv.close()
     ref.close()
+
+
+def test_tum_replay_matches_in_memory(gpu, tmp_path):
+    """a sequence written in the TUM layout and read back through TumSource tracks exactly as
+    the in-memory frames do; its trajectory file and ATE against the scene's ground truth"""
+    from fusion_amd import ingest as I
+    src = I.SceneSource(frames=6)
+    frames = list(src)
+    stamps = [src.timestamp(i) for i in range(len(src))]
+    I.write_tum_sequence(str(tmp_path / "seq"), frames, stamps)
+    tum = I.TumSource(str(tmp_path / "seq"), K=src.K, depth_scale=5000.0)
+    assert len(tum) == 6
+    a = _system()
+    b = _system()
+    pa, pb = [], []
+    for f, g in zip(frames, tum):
+        a.GrabImage(*A.to_device(*f))
+        pa.append(a.result()["pose"])
+        b.GrabImage(*A.to_device(*g))
+        pb.append(b.result()["pose"])
+    assert all(np.array_equal(x, y) for x, y in zip(pa, pb))
+    traj = str(tmp_path / "est.txt")
+    I.write_tum_trajectory(traj, stamps, pb)
+    ts, back = I.read_tum_trajectory(traj)
+    assert len(back) == 6 and max(np.abs(x - y).max() for x, y in zip(back, pb)) < 1e-7
+    gt = [np.linalg.inv(src.ground_truth(0)) @ src.ground_truth(i) for i in range(6)]
+    rmse, _, _ = I.ate_rmse(pb, gt)
+    assert rmse < 0.02, rmse
+    a.close()
+    b.close()
