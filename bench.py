@@ -303,6 +303,38 @@ def single_stream(A, torch, args):
                 workload="12x8x3 m multi-room, 640x480, 2^21 buckets, 2M blocks, hipGraph per frame")
 
 
+def ingest(A, torch, n=120):
+    """recorded-sequence replay: 640x480 frames of room 0 written in the TUM layout (16-bit
+    depth PNG + RGB PNG), then decoded on the host (8 threads ahead of the GPU), copied to
+    HBM and tracked+fused by one session (hipGraph per frame)"""
+    import tempfile
+    from fusion_amd import ingest as I
+
+    src = I.SceneSource(frames=n)
+    with tempfile.TemporaryDirectory() as td:
+        I.write_tum_sequence(td, list(src), [src.timestamp(i) for i in range(n)])
+        tum = I.TumSource(td, K=src.K, depth_scale=5000.0)
+        a = time.perf_counter()
+        for i in range(n):
+            tum.frame(i)
+        t_dec1 = time.perf_counter() - a
+        a = time.perf_counter()
+        for _ in I.prefetch(tum):
+            pass
+        t_dec8 = time.perf_counter() - a
+        s = A.System(A.system_params(use_graph=True, map=MAP_SESSION))
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for d, c in I.prefetch(tum):
+            s.GrabImage(*A.to_device(d, c))
+        r = s.result()
+        t_all = time.perf_counter() - a
+        s.close()
+    return dict(frames=n, decode_fps_1thread=round(n / t_dec1, 1), decode_fps_8threads=round(n / t_dec8, 1),
+                replay_fps=round(n / t_all, 1), last_track_status=r["track_status"],
+                note="host PNG decode + pageable host-to-device copy + one session's frame; decode-bound")
+
+
 def on_disk(A, torch, m, v, nrm, col):
     """the map snapshot (save, then load into a second map of the same parameters) and the
     binary PLY of the mesh, through a temporary directory of the host"""
@@ -464,6 +496,7 @@ def main():
         log("mixed: %.0f frames/s" % mixed["value"])
 
     single = single_stream(A, torch, args) if args.single_steps > 0 else None
+    replay = ingest(A, torch) if args.single_steps > 0 else None
     cpu = cpu_baseline(A, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     fps = r["tracked"] / r["wall"]
@@ -501,6 +534,7 @@ def main():
         "mixed_resolution": mixed,
         "single_stream": single,
         "rgbd_tracking": rgbd,
+        "recorded_replay": replay,
         "stage_ms_per_step": ({k: round(v / steps_p, 4) for k, v in st["ms"].items()} if st else None),
         "map": {"n_allocated_max": r["n_allocated_max"], "n_allocated_mean": round(r["n_allocated_mean"], 1),
                 "flags": r["flags"]},

@@ -185,6 +185,22 @@ class StreamSource:
             yield np.ascontiguousarray(d, dtype=np.uint16), np.ascontiguousarray(c, dtype=np.uint8)
 
 
+def prefetch(source, workers=8, depth=16):
+    """the frames of an indexable source (`frame(i)`, `len`), decoded ahead by a thread pool
+    (PNG and JPEG decoding release the interpreter lock), in order"""
+    from concurrent.futures import ThreadPoolExecutor
+
+    n = len(source)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(source.frame, i) for i in range(min(depth, n))]
+        for i in range(n):
+            f = futs[i].result()
+            if i + depth < n:
+                futs.append(ex.submit(source.frame, i + depth))
+            futs[i] = None
+            yield f
+
+
 def write_tum_sequence(path, frames, stamps):
     """a TUM-layout directory (depth.txt, rgb.txt with three comment lines, depth/ and rgb/
     PNGs) of host frames"""

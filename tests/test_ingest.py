@@ -120,3 +120,11 @@ def test_ate_is_alignment_invariant_and_measures_noise():
         noisy.append(Q)
     rmse, _, _ = I.ate_rmse(noisy, gt)
     assert rmse <= np.sqrt((off ** 2).sum(1).mean()) + 1e-12 and rmse > 0.5 * 0.01 * np.sqrt(3)
+
+
+def test_prefetch_keeps_order(tmp_path):
+    fr = _frames(n=7)
+    I.write_tum_sequence(str(tmp_path / "seq"), fr, [float(i) for i in range(7)])
+    src = I.TumSource(str(tmp_path / "seq"))
+    got = list(I.prefetch(src, workers=3, depth=2))
+    assert len(got) == 7 and all(np.array_equal(g[0], f[0]) and np.array_equal(g[1], f[1]) for g, f in zip(got, fr))
