@@ -307,6 +307,44 @@ __device__ __forceinline__ float wave_reduce_scatter64(float (&v)[64]) {
     return keep + dpp_f<0xB1>(send);  // quad_perm [1,0,3,2]
 }
 
+// IEEE float division a / b without the range scaling and special-case steps.  HIP compiles
+// `a / b` to v_div_scale (x2), v_rcp, a Newton refinement of the reciprocal, two corrections
+// of the quotient, v_div_fmas and v_div_fixup; for finite operands with |a| and |b| in
+// [2^-60, 2^30] (or a = +0), whose quotient is normal, the scale steps are the identity and
+// the fixup returns its input, so the core below gives the same bits.  Several quotients
+// with one denominator share its refined reciprocal (fu_rcp_refined).
+__device__ __forceinline__ float fu_rcp_refined(float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = fmaf(-b, y0, 1.0f);
+    return fmaf(e, y0, y0);
+}
+__device__ __forceinline__ float fu_div_refined(float a, float b, float y) {
+    const float m = a * y;
+    const float f2 = fmaf(-b, m, a);
+    const float f3 = fmaf(f2, y, m);
+    const float f4 = fmaf(-b, f3, a);
+    return fmaf(f4, y, f3);
+}
+// IEEE sqrtf without the range scaling and special-value steps: HIP compiles sqrtf to a
+// scale for tiny inputs, v_sqrt, a one-ulp correction by the signs of s - y'y for the two
+// neighbours y' of y, the unscale and a class test for 0 and inf; for finite s in
+// [2^-90, 2^120] the scale, unscale and class steps are the identity.
+__device__ __forceinline__ float fu_sqrt_normal(float s) {
+    const float y = __builtin_amdgcn_sqrtf(s);
+    const float dn = __uint_as_float(__float_as_uint(y) - 1u);
+    const float up = __uint_as_float(__float_as_uint(y) + 1u);
+    const float r = fmaf(-dn, y, s) <= 0.0f ? dn : y;
+    return fmaf(-up, y, s) > 0.0f ? up : r;
+}
+
+// whether every component of (x, y, z) lies in the range of fu_div_refined by a denominator
+// of at most 2^31 (the 4-D norm of (x, y, z, 1)): 2^-60 <= |c| <= 2^30
+__device__ __forceinline__ bool fu_div_range3(float x, float y, float z) {
+    const float lo = fminf(fminf(fabsf(x), fabsf(y)), fabsf(z));
+    const float hi = fmaxf(fmaxf(fabsf(x), fabsf(y)), fabsf(z));
+    return lo >= 0x1p-60f && hi <= 0x1p30f;
+}
+
 // wave total of an unsigned (wrapping) sum; every lane gets it
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
 #pragma unroll

@@ -334,7 +334,11 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     using IO = VoxelIO<F>;
     constexpr int H = 2 * kIntBlocks;
     PoseF pose = a.pose;
-    if (a.st && !pose_from_state(a.st, pose)) return;
+    if (a.st && !pose_from_state(a.st, pose)) return;  // uniform: the frame was not tracked
+    // the byte / 255 quotients in LDS: a gather there instead of six global loads per voxel
+    __shared__ float div255[256];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) div255[i] = u8div255(i);
+    __syncthreads();
     const unsigned nvis = min(FU_G(const unsigned, m.ctr)[C_VISIBLE], m.num_blocks);
     const int cols = a.cols, rows = a.rows;
     const auto depth = FU_G(const float, a.depth);
@@ -345,6 +349,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     const auto vis = FU_G(const unsigned, m.visible);
     const auto vkey = FU_G(const unsigned long long, m.visible_key);
     const float thresh = m.trunc;
+    const float rthresh = fu_rcp_refined(thresh);
     const unsigned stride = (unsigned)g.nx * kIntBlocks;
     unsigned vb = (unsigned)g.x * kIntBlocks;
     unsigned hn[kIntBlocks];
@@ -377,8 +382,21 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
             const int li = threadIdx.x + (h & 1) * 256;
             const i3 vp = mki3(bp[b].x * 8 + (li & 7), bp[b].y * 8 + ((li >> 3) & 7), bp[b].z * 8 + (li >> 6));
             pos[h] = mv(pose.Rinv, sub(i3f(vp, m.voxel_size), pose.t));
-            const int u = f2i(a.fx * pos[h].x / pos[h].z + a.cx);
-            const int v = f2i(a.fy * pos[h].y / pos[h].z + a.cy);
+            // f2i(fx * x / z + cx): both quotients share z's reciprocal in the range where
+            // the division core is exact; below |a| = 2^-100, where it may not be, the
+            // quotient is under 2^-60 and q + cx truncates to the same integer either way
+            const float nu = a.fx * pos[h].x, nw = a.fy * pos[h].y, z = pos[h].z;
+            float qu, qv;
+            if (fabsf(z) >= 0x1p-60f && fmaxf(fmaxf(fabsf(nu), fabsf(nw)), fabsf(z)) <= 0x1p30f) {
+                const float yz = fu_rcp_refined(z);
+                qu = fu_div_refined(nu, z, yz);
+                qv = fu_div_refined(nw, z, yz);
+            } else {
+                qu = nu / z;
+                qv = nw / z;
+            }
+            const int u = f2i(qu + a.cx);
+            const int v = f2i(qv + a.cy);
             upd[h] = live[b] && !(u < 0 || v < 0 || u >= cols || v >= rows);
             pix[h] = upd[h] ? v * cols + u : 0;
         }
@@ -430,9 +448,21 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
             if (!upd[h] || isnanf_(nl[h].x)) continue;
             const int li = threadIdx.x + (h & 1) * 256;
             unsigned char* blk = IO::block(m, (int)heap[h >> 1]);
-            const float s = fminf(1.0f, sdf[h] / thresh);
-            // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342)
-            const float w = dot4(nl[h], normalised4(mk4(pos[h].x, pos[h].y, pos[h].z, 1.0f)));
+            // sdf / thresh: |sdf| is 0 or at least ulp(depth_min), thresh a positive normal
+            const float s = fminf(1.0f, fu_div_refined(sdf[h], thresh, rthresh));
+            // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342); the
+            // four quotients by the norm share its reciprocal when the components allow
+            const f4 p4 = mk4(pos[h].x, pos[h].y, pos[h].z, 1.0f);
+            f4 pn;
+            if (fu_div_range3(p4.x, p4.y, p4.z)) {
+                const float n = fu_sqrt_normal(dot4(p4, p4));  // norm4(p4): the sum is >= 1
+                const float y = fu_rcp_refined(n);
+                pn = mk4(fu_div_refined(p4.x, n, y), fu_div_refined(p4.y, n, y), fu_div_refined(p4.z, n, y),
+                         fu_div_refined(1.0f, n, y));
+            } else {
+                pn = normalised4(p4);
+            }
+            const float w = dot4(nl[h], pn);
             VoxAux nv;
             if (old[h].w == 0) {
                 IO::set_sdf(blk, li, s);
@@ -443,9 +473,9 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
             } else {
                 const float aw = w * 0.2f;
                 const float bw = 1.0f - w * 0.2f;
-                const float fr = (aw * u8div255(cr[h]) + bw * u8div255(old[h].r)) * 255.f;
-                const float fg = (aw * u8div255(cg[h]) + bw * u8div255(old[h].g)) * 255.f;
-                const float fb = (aw * u8div255(cb[h]) + bw * u8div255(old[h].b)) * 255.f;
+                const float fr = (aw * div255[cr[h]] + bw * div255[old[h].r]) * 255.f;
+                const float fg = (aw * div255[cg[h]] + bw * div255[old[h].g]) * 255.f;
+                const float fb = (aw * div255[cb[h]] + bw * div255[old[h].b]) * 255.f;
                 const float fw = (float)old[h].w;
                 IO::set_sdf(blk, li, (sold[h] * fw + w * s) / (fw + w));
                 nv.w = min(m.max_weight, old[h].w + 1u);

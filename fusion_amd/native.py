@@ -136,6 +136,7 @@ SIGNATURES = {
     "fu_compute_se3": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), FP, IP, IP, F, P]),
     "fu_tracker_result": (I, [P, C.POINTER(TrackResult), P]),
     "fu_image_to_intensity": (I, [P, P, I, I, P]),
+    "fu_selftest_arith": (I, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32), P]),
     "fu_map_save": (I, [P, C.c_char_p, P]),
     "fu_map_load": (I, [P, C.c_char_p, P]),
     "fu_system_save": (I, [P, C.c_char_p, P]),

@@ -177,3 +177,12 @@ def test_precision_vs_fp64_twin(gpu, fmt, bound):
     print("format %d: %d voxels, max |sdf - sdf64| = %.3g T, mean %.3g T, 99.9%% %.3g T" % (
         fmt, int(m.sum()), err.max(), err.mean(), np.percentile(err, 99.9)), flush=True)
     assert err.max() <= bound
+
+
+def test_range_specialised_arithmetic_equals_ieee(gpu):
+    """the integration kernel's shared-reciprocal divisions and range-specialised square root
+    give the compiler's IEEE results inside their ranges (2^28 random cases of each)"""
+    import ctypes as C
+    bad = (C.c_uint32 * 4)()
+    A.check(A.N.lib().fu_selftest_arith(12345, 1 << 28, bad, None), "fu_selftest_arith")
+    assert list(bad) == [0, 0, 0, 0], list(bad)
