@@ -42,6 +42,14 @@ __device__ __forceinline__ m3 load_m3(FU_GPTR(const float) r) {
     return m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
 }
 
+// smallest float s with sqrtf(s) >= 0.6f, largest d with sqrtf(d) <= 0.1f, largest e with
+// (double)sqrtf(e) < 0.1 (found by bisection over the floats with the host's IEEE sqrtf;
+// tests/test_oracle.py checks all three)
+constexpr float kSine2 = 0x1.70a3d8p-2f;
+constexpr float kDist2 = 0x1.47ae16p-7f;
+constexpr float kRgbDist2 = 0x1.47ae12p-7f;
+constexpr bool angle_thresh_ok() { return kSine2 > 0.35f && kSine2 < 0.37f && kDist2 > 0.0099f && kDist2 < 0.0101f; }
+
 struct IcpPose {
     m3 Rc, Rl, Rli;
     f3 tc, tl;
@@ -51,7 +59,7 @@ struct IcpPose {
 // [-n, n x v_l, -n.(v_c - v_l)] in the last camera frame, or false
 __device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int i, float (&row)[7]) {
     const int cols = a.cols, rows = a.rows;
-    const float angle_thresh = 0.6f, dist_thresh = 0.1f;  // Reduction.cu:352-353
+    // angle 0.6f, distance 0.1f (Reduction.cu:352-353) as bounds on the squared norms
     const auto vcurr = FU_G(const f4, a.vcurr);
     const auto ncurr = FU_G(const f4, a.ncurr);
     const auto vlast = FU_G(const f4, a.vlast);
@@ -61,7 +69,7 @@ __device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int 
     if (isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
     const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
     const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
-    const double invz = 1.0 / (double)vcurr_p.z;
+    const double invz = fu_finite_nonzero(vcurr_p.z) ? fu_drcp_from_float((double)vcurr_p.z) : 1.0 / (double)vcurr_p.z;
     const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
     const int v = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
     if (u < 0 || v < 0 || u >= cols || v >= rows) return false;
@@ -71,9 +79,15 @@ __device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int 
     const f3 ncurr_g = mv(P.Rc, ncurr_c);
     const f3 nlast_c = xyz(nlast[j]);
     const f3 nlast_g = mv(P.Rl, nlast_c);
-    const float dist = norm(sub(vlast_g, vcurr_g));
-    const float sine = norm(cross(ncurr_g, nlast_g));
-    if (!(sine < angle_thresh && dist <= dist_thresh && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x))) return false;
+    // sine = norm(cross), dist = norm(difference) are only compared with the thresholds:
+    // sqrtf is correctly rounded and monotonic, so sqrtf(s) < 0.6f <=> s < kSine2 and
+    // sqrtf(d) <= 0.1f <=> d <= kDist2 (the first / last floats of those ranges; a NaN fails
+    // both forms)
+    const f3 dv = sub(vlast_g, vcurr_g);
+    const f3 cr2 = cross(ncurr_g, nlast_g);
+    const float dist2 = dot(dv, dv), sine2 = dot(cr2, cr2);
+    static_assert(angle_thresh_ok(), "thresholds");
+    if (!(sine2 < kSine2 && dist2 <= kDist2 && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x))) return false;
     const f3 nl = mv(P.Rli, nlast_g);
     const f3 vcu = mv(P.Rli, sub(vcurr_g, P.tl));
     const f3 vla = mv(P.Rli, sub(vlast_g, P.tl));
@@ -279,7 +293,9 @@ __device__ __forceinline__ bool rgb_corresp(const IcpArgs& a, const RgbPose& P, 
     const int k = v * cols + u;
     const f4 la = FU_G(const f4, a.vlast)[k];
     const int lk = FU_G(const unsigned char, a.rgb.ilast)[k];
-    if (!(!isnanf_(la.x) && (double)norm4(sub4(la, vc)) < 0.1 && lk != 0)) return false;
+    // (double)norm4(...) < 0.1 on the squared norm: see kRgbDist2
+    const f4 d4 = sub4(la, vc);
+    if (!(!isnanf_(la.x) && dot4(d4, d4) <= kRgbDist2 && lk != 0)) return false;
     j = k;
     diff = (int)FU_G(const unsigned char, a.rgb.inext)[i] - lk;
     return true;

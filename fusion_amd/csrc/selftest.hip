@@ -1,5 +1,5 @@
 // Self-test of the range-specialised IEEE arithmetic (fu_device.h: fu_rcp_refined,
-// fu_div_refined, fu_sqrt_normal) against the compiler's IEEE division and square root on
+// fu_div_refined, fu_sqrt_normal, fu_drcp_from_float) against the compiler's IEEE division and square root on
 // the device itself: the kernels rely on them being bit-identical inside their stated ranges.
 #include <hip/hip_runtime.h>
 
@@ -30,7 +30,7 @@ __device__ __forceinline__ float rnd(uint64_t& s, int lo, int hi, bool sign) {
 }
 
 __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
-    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t s = mix(seed ^ ((uint64_t)i * 0x2545f4914f6cdd1dull));
         // 1: quotients by a 4-D norm: numerators in [2^-60, 2^30], denominators in [1, 2^31]
@@ -60,26 +60,36 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             const float x = rnd(s, -90, 119, false);
             b3 += __float_as_uint(fu_sqrt_normal(x)) != __float_as_uint(sqrtf(x));
         }
+        // 5: FP64 reciprocals of nonzero finite floats (denormal ones included)
+        {
+            float z = rnd(s, -126, 127, true);
+            if (((s >> 52) & 15) == 1) z = __uint_as_float((unsigned)(s >> 9) & 0x807fffffu);
+            if (fu_finite_nonzero(z)) {
+                const double r = fu_drcp_from_float((double)z), q = 1.0 / (double)z;
+                b4 += __double_as_longlong(r) != __double_as_longlong(q);
+            }
+        }
     }
     if (b0) atomicAdd(&bad[0], b0);
     if (b1) atomicAdd(&bad[1], b1);
     if (b2) atomicAdd(&bad[2], b2);
     if (b3) atomicAdd(&bad[3], b3);
+    if (b4) atomicAdd(&bad[4], b4);
 }
 
 }  // namespace
 
-extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[4], fu_stream s) {
+extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[5], fu_stream s) {
     if (!mismatches) return api_error(FU_ERR_ARG, "fu_selftest_arith: null output");
     unsigned* d = nullptr;
-    if (hipMalloc(&d, 16) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
+    if (hipMalloc(&d, 20) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
     hipStream_t hs = (hipStream_t)s;
-    hipError_t e = hipMemsetAsync(d, 0, 16, hs);
+    hipError_t e = hipMemsetAsync(d, 0, 20, hs);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, hs, seed, n, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 16, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 20, hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
     (void)hipFree(d);
     return e == hipSuccess ? FU_OK : api_error(FU_ERR_HIP, "fu_selftest_arith: %s", hipGetErrorString(e));

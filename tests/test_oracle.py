@@ -565,3 +565,27 @@ def test_sobel_scale_float_division_equals_double_rounding():
     ref = (s.astype(np.float64) / 9.0).astype(np.float32)
     gpu = s.astype(np.float32) / np.float32(9.0)
     assert np.array_equal(ref.view(np.uint32), gpu.view(np.uint32))
+
+
+def test_squared_norm_thresholds_of_the_gpu_association():
+    """the GPU compares squared norms with these bounds instead of taking square roots:
+    sqrtf(s) < 0.6f <=> s < T1, sqrtf(d) <= 0.1f <=> d <= T2, (double)sqrtf(e) < 0.1 <=> e <= T3
+    (square roots in float are correctly rounded and monotonic, as numpy's are)"""
+    T1 = np.float32(float.fromhex("0x1.70a3d8p-2"))
+    T2 = np.float32(float.fromhex("0x1.47ae16p-7"))
+    T3 = np.float32(float.fromhex("0x1.47ae12p-7"))
+    up = lambda x: np.nextafter(x, np.float32(1))
+    dn = lambda x: np.nextafter(x, np.float32(0))
+    assert np.sqrt(T1) >= np.float32(0.6) and np.sqrt(dn(T1)) < np.float32(0.6)
+    assert np.sqrt(T2) <= np.float32(0.1) and np.sqrt(up(T2)) > np.float32(0.1)
+    assert float(np.sqrt(T3)) < 0.1 and float(np.sqrt(up(T3))) >= 0.1
+    # and on random values around and away from the bounds
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(0, 1, 100000).astype(np.float32),
+                        (T1.view(np.int32) + np.arange(-500, 500)).astype(np.int32).view(np.float32),
+                        (T2.view(np.int32) + np.arange(-500, 500)).astype(np.int32).view(np.float32),
+                        (T3.view(np.int32) + np.arange(-500, 500)).astype(np.int32).view(np.float32)])
+    r = np.sqrt(x)
+    assert np.array_equal(r < np.float32(0.6), x < T1)
+    assert np.array_equal(r <= np.float32(0.1), x <= T2)
+    assert np.array_equal(r.astype(np.float64) < 0.1, x <= T3)
