@@ -66,6 +66,10 @@ def parse():
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--autotune", choices=["run", "cache", "off"], default="run",
+                    help="launch geometry of the batched stages: tune on the served workload after its warm-up "
+                         "(untimed), take the cached tuning of this GPU and workload, or keep the defaults")
+    ap.add_argument("--tuning-cache", default=None, help="JSON cache to store the tuning in (with --autotune run)")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only the program the HBM-traffic counter runs profile: a 512 MiB copy (calibration) "
                          "and 2 + 4 served steps of 64 sessions; prints no JSON")
@@ -162,9 +166,10 @@ def poisson_arrivals(rng, n_sessions, n, fps=30.0):
     return out
 
 
-def served(A, torch, args, sizes, frames, steps, realtime, profile, use_rgb=0):
-    """open the sessions, warm up, then the backlog (capacity) run, the 30 fps run and the
-    stage profile; returns a dict of results"""
+def served(A, torch, args, sizes, frames, steps, realtime, profile, use_rgb=0, tune=False):
+    """open the sessions, warm up (and with `tune`, autotune the launch geometry on the
+    warm-up frames), then the backlog (capacity) run, the 30 fps run and the stage profile;
+    returns a dict of results"""
     sv = A.Server(max_batch=args.max_batch, max_inflight=args.inflight, lanes=args.lanes, block_budget=BUDGET)
     sids = []
     for cols, rows in sizes:
@@ -173,6 +178,16 @@ def served(A, torch, args, sizes, frames, steps, realtime, profile, use_rgb=0):
     W = args.warmup
     sv.run_trace(trace(sids, frames, 0, W), closed_loop=True)
     torch.cuda.synchronize()
+    tuned = None
+    if tune:
+        from fusion_amd import autotune as T
+        t0 = time.perf_counter()
+        best, report = T.tune_server(A, sv, sids, [f[:W] for f in frames])
+        tuned = dict(tuning=best, report=report, seconds=round(time.perf_counter() - t0, 2))
+        log("autotuned in %.1f s: %s" % (tuned["seconds"], best))
+        # back to the end of the warm-up window, so the timed frames continue the sequence
+        sv.run_trace(trace(sids, frames, 0, W), closed_loop=True)
+        torch.cuda.synchronize()
     # capacity: every session's timed frames queued at t = 0
     tr = trace(sids, frames, W, W + steps)
     torch.cuda.synchronize()
@@ -185,7 +200,7 @@ def served(A, torch, args, sizes, frames, steps, realtime, profile, use_rgb=0):
     flags = int(np.bitwise_or.reduce([c["flags"] for c in done])) if done else 0
     nalloc = [c["n_allocated"] for c in done]
     res = dict(wall=wall, frames=len(done), tracked=int(np.isin(st, TRACKED).sum()), lat_backlog=lat_b, flags=flags,
-               n_allocated_max=int(max(nalloc)), n_allocated_mean=float(np.mean(nalloc)))
+               n_allocated_max=int(max(nalloc)), n_allocated_mean=float(np.mean(nalloc)), tuned=tuned)
     # latency: 30 fps Poisson arrivals per session
     lo = W + steps
     if realtime > 0:
@@ -456,7 +471,18 @@ def main():
     frames = render_sessions(A, sizes, per)
     torch.cuda.synchronize()
     log("rendered %d frames in %.1f s" % (ns * per, time.time() - t))
-    r = served(A, torch, args, sizes, frames, args.steps, args.realtime_frames, args.profile_steps)
+    from fusion_amd import autotune as T
+    key = T.cache_key(T.device_arch(torch), 640, 480, args.max_batch, 0, 0)
+    if args.autotune == "cache":
+        cached = T.load(key)
+        if cached is None:
+            log("no cached tuning for %s: defaults" % key)
+        else:
+            A.set_tuning(**cached)
+    r = served(A, torch, args, sizes, frames, args.steps, args.realtime_frames, args.profile_steps,
+               tune=args.autotune == "run")
+    if r["tuned"] and args.tuning_cache:
+        T.save(key, r["tuned"]["tuning"], r["tuned"]["report"], path=args.tuning_cache)
     log("served: %.0f frames/s" % (r["tracked"] / r["wall"]))
     p = plain(A, torch, args, frames)
     log("plain: %.0f frames/s" % p["value"])
@@ -521,7 +547,8 @@ def main():
         "config": {"workload": "served reconstruction: %d sessions of 640x480 at 5 mm, trunc 8 voxels, 2^19 buckets "
                                "and 500k blocks per session, ICP 10/5/4, 200 GB block budget" % ns,
                    "sessions": ns, "frames_per_session": args.steps, "voxel_format": "f32 sdf + u8 weight + rgb",
-                   "lanes": args.lanes, "max_batch": args.max_batch, "max_inflight": args.inflight},
+                   "lanes": args.lanes, "max_batch": args.max_batch, "max_inflight": args.inflight,
+                   "tuning": A.get_tuning(), "tuning_source": args.autotune},
         "frames_per_s_all": round(r["frames"] / r["wall"], 2),
         "tracked_fraction": round(r["tracked"] / max(1, r["frames"]), 4),
         "latency_ms": {"p50": pct(r.get("lat_rt", []), 50), "p99": pct(r.get("lat_rt", []), 99),
@@ -541,6 +568,7 @@ def main():
         "roofline": rl[dominant] if rl else None,
         "kernels": rl,
         "cpu_baseline": cpu,
+        "autotune": r["tuned"],
     }
     print(json.dumps(out), flush=True)
 

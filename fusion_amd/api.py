@@ -472,6 +472,35 @@ class System:
         return m
 
 
+TUNING_FIELDS = [f for f, _ in N.Tuning._fields_]
+
+
+def default_tuning():
+    """the library's default launch geometry of the batched stages, as a dict (fu_tuning)"""
+    t = N.Tuning()
+    N.lib().fu_tuning_default(C.byref(t))
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def get_tuning():
+    t = N.Tuning()
+    check(N.lib().fu_get_tuning(C.byref(t)), "fu_get_tuning")
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def set_tuning(**kw):
+    """change fields of the process-wide launch geometry (fu_set_tuning); returns the new one.
+    Results do not depend on it, only the speed of batched steps."""
+    cur = get_tuning()
+    for k, v in kw.items():
+        if k not in cur:
+            raise KeyError("no tuning field %r (fields: %s)" % (k, ", ".join(TUNING_FIELDS)))
+        cur[k] = int(v)
+    t = N.Tuning(**cur)
+    check(N.lib().fu_set_tuning(C.byref(t)), "fu_set_tuning")
+    return cur
+
+
 class Batch:
     """A batch context: one frame of each of many attached Systems per call, every stage one
     launch for all of them (no reference counterpart; the reference runs one System)."""

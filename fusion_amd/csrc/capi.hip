@@ -1512,6 +1512,31 @@ extern "C" fu_status fu_batch_get_stage_times(fu_batch* bt, fu_stage_times* out)
     return FU_OK;
 }
 
+// ------------------------------------------------------------------------------ tuning
+static fu_tuning g_tuning = {FU_WAVE_8X8, 0, 16384, 8192, 2048};
+const fu_tuning& fu::tuning() { return g_tuning; }
+
+extern "C" void fu_tuning_default(fu_tuning* t) {
+    if (t) *t = fu_tuning{FU_WAVE_8X8, 0, 16384, 8192, 2048};
+}
+extern "C" fu_status fu_set_tuning(const fu_tuning* t) {
+    if (!t) return set_err(FU_ERR_ARG, "null tuning");
+    if (t->raycast_wave < FU_WAVE_16X4 || t->raycast_wave > FU_WAVE_4X16)
+        return set_err(FU_ERR_ARG, "fu_set_tuning: raycast_wave %d (0..2)", t->raycast_wave);
+    if (t->xcd_order & ~((1 << FU_NUM_STAGES) - 1))
+        return set_err(FU_ERR_ARG, "fu_set_tuning: xcd_order 0x%x has bits beyond the stages", t->xcd_order);
+    const int32_t g[3] = {t->integrate_groups, t->visible_groups, t->alloc_groups};
+    for (int i = 0; i < 3; ++i)
+        if (g[i] < 256 || g[i] > 262144) return set_err(FU_ERR_ARG, "fu_set_tuning: work-group count %d (256..262144)", g[i]);
+    g_tuning = *t;
+    return FU_OK;
+}
+extern "C" fu_status fu_get_tuning(fu_tuning* t) {
+    if (!t) return set_err(FU_ERR_ARG, "null tuning");
+    *t = g_tuning;
+    return FU_OK;
+}
+
 static void fill_desc(fu_system* sys, SessDesc& d) {
     const fu_system_params& p = sys->p;
     fu_system_buffers& b = sys->b;

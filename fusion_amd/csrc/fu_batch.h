@@ -62,12 +62,28 @@ struct BatchLaunch {
     int level;                // pyramid / ICP level of the launch
     int iter;                 // ICP iteration of the launch
     int uniform;              // > 0: every share has this many work-groups
+    int xcd;                  // != 0: hardware work-groups map to logical ones XCD by XCD (batch_wg)
+    int wave;                 // FU_WAVE_*: pixel layout of a raycast wave
     unsigned long long first;  // bit i: slot i is its session's first frame
     unsigned short sess[kMaxBatch];  // slot -> session descriptor index
     unsigned short gx[kMaxBatch];    // work-groups in x of the slot's share (2-D tilings)
     int pre[kMaxBatch + 1];          // prefix of the shares' work-group counts
     const void* in[kMaxBatch];       // per-slot input of the step (depth or colour)
 };
+
+// Logical work-group of this hardware work-group.  The dispatcher hands consecutive
+// work-groups to the 8 XCDs in turn (work-group i runs on XCD i mod 8), and each XCD has its
+// own L2.  With b.xcd set, XCD x runs one contiguous range of logical work-groups instead
+// (its i-th work-group is logical x*q + min(x, r) + i, q = G / 8, r = G mod 8: a
+// bijection), so the work-groups of one session share an L2.  Logical ids, and so every
+// result, are the same either way.
+__device__ __forceinline__ int batch_wg(const BatchLaunch& b) {
+    const int i = (int)blockIdx.x;
+    if (!b.xcd) return i;
+    const int G = (int)gridDim.x;
+    const int x = i & 7, q = G >> 3, r = G & 7;
+    return x * q + (x < r ? x : r) + (i >> 3);
+}
 
 // slot of hardware work-group `wg` and the work-group ids within its share
 __device__ __forceinline__ int batch_slot(const BatchLaunch& b, int wg, WG& g) {
@@ -124,6 +140,9 @@ enum BatchTiling : int {
 };
 // fills b.pre, b.gx and b.uniform for one launch; returns the total work-group count
 int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, int fixed);
+// the process-wide launch geometry (fu_set_tuning)
+const fu_tuning& tuning();
+inline int xcd_order(int stage) { return (tuning().xcd_order >> stage) & 1; }
 
 // per-frame result of a batched step, gathered on the device and copied back in one transfer
 struct FrameBrief {

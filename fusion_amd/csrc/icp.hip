@@ -614,7 +614,7 @@ __device__ __forceinline__ bool icp_slot_args(const BatchLaunch& b, int s, IcpAr
 template <bool kOne>
 __global__ __launch_bounds__(kIcpThreads) void kb_icp(BatchLaunch b) {
     WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
+    const int s = batch_slot(b, batch_wg(b), g);
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, s, a, nb)) return;
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(kIcpThreads) void kb_rgb_scatter(BatchLaunch b) {
 }
 __global__ __launch_bounds__(kIcpThreads) void kb_rgb_resid(BatchLaunch b) {
     WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
+    const int s = batch_slot(b, batch_wg(b), g);
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, s, a, nb)) return;
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(kIcpThreads) void kb_rgb_resid(BatchLaunch b) {
 }
 __global__ __launch_bounds__(kIcpThreads) void kb_icp_rgb(BatchLaunch b) {
     WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
+    const int s = batch_slot(b, batch_wg(b), g);
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, s, a, nb)) return;
@@ -702,6 +702,7 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
     b.level = level;
     b.iter = iter;
     const int grid = batch_layout(b, h, BT_ICP, level, 0);
+    b.xcd = xcd_order(FU_STAGE_ICP);
     if (h[b.sess[0]].use_rgb) {  // the sessions of a batch share the tracking mode
         hipLaunchKernelGGL(kb_rgb_resid, dim3(grid), dim3(kIcpThreads), 0, s, b);
         hipLaunchKernelGGL(kb_icp_rgb, dim3(grid), dim3(kIcpThreads), 0, s, b);

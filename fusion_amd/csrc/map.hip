@@ -325,14 +325,17 @@ void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
 // ------------------------------------------------------------ integration (FuseMap.cu:140-191)
 
 // Fusion::integrateColor (FuseMap.cu:140-191).  A work-group takes kIntBlocks visible blocks per
-// iteration; each lane owns voxels li and li + 256 (index z*64 + y*8 + x) of each, so
-// 2 * kIntBlocks voxels' gathers are in flight per lane.  The next iteration's block ids and
-// keys are loaded before the current blocks are processed.
+// iteration, 128 lanes (two waves) per block.  Lane t of a block owns the voxels t + 128k,
+// k = 0..3 (index z*64 + y*8 + x): one column (x, y) at z = t/64 + 2k, so the x and y terms of
+// the voxel's position (VectorMath.h:364-366, evaluated ((r.x*X + r.y*Y) + r.z*Z) as the
+// reference's float3 dot product) are computed once for its four voxels, and each wave's
+// loads and stores cover 64 consecutive voxels.  The next iteration's block ids and keys are
+// loaded before the current blocks are processed.
 constexpr int kIntBlocks = 2;
 template <int F, bool kPacked>
 __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const FuseArgs& a) {
     using IO = VoxelIO<F>;
-    constexpr int H = 2 * kIntBlocks;
+    constexpr int H = 4;  // voxels per lane per iteration
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;  // uniform: the frame was not tracked
     // the byte / 255 quotients in LDS: a gather there instead of six global loads per voxel
@@ -350,38 +353,36 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     const auto vkey = FU_G(const unsigned long long, m.visible_key);
     const float thresh = m.trunc;
     const float rthresh = fu_rcp_refined(thresh);
+    const float vs = m.voxel_size;
     const unsigned stride = (unsigned)g.nx * kIntBlocks;
-    unsigned vb = (unsigned)g.x * kIntBlocks;
-    unsigned hn[kIntBlocks];
-    unsigned long long kn[kIntBlocks];
-#pragma unroll
-    for (int b = 0; b < kIntBlocks; ++b) {
-        hn[b] = vb + b < nvis ? vis[vb + b] : 0u;
-        kn[b] = vb + b < nvis ? vkey[vb + b] : 0ull;
-    }
+    // this lane's block of the iteration: uniform per wave, so its id, key and voxel pointer
+    // live in scalar registers
+    const unsigned mine = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
+    const int t = threadIdx.x & 127;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    unsigned vb = (unsigned)g.x * kIntBlocks + mine;
+    unsigned hn = vb < nvis ? vis[vb] : 0u;
+    unsigned long long kn = vb < nvis ? vkey[vb] : 0ull;
     for (; vb < nvis; vb += stride) {
-        unsigned heap[kIntBlocks];
-        i3 bp[kIntBlocks];
-        bool live[kIntBlocks];
-#pragma unroll
-        for (int b = 0; b < kIntBlocks; ++b) {
-            heap[b] = hn[b];
-            bp[b] = unpack_key(kn[b]);
-            live[b] = vb + b < nvis;
-            const unsigned nb = vb + stride + b;
-            hn[b] = nb < nvis ? vis[nb] : 0u;
-            kn[b] = nb < nvis ? vkey[nb] : 0ull;
-        }
+        const unsigned heap = hn;
+        const i3 bp = unpack_key(kn);
+        hn = vb + stride < nvis ? vis[vb + stride] : 0u;
+        kn = vb + stride < nvis ? vkey[vb + stride] : 0ull;
+        // the voxel column's x and y terms: (float)vp * voxel_size - t, times the first two
+        // columns of Rinv, summed (the first addition of each row's dot product)
+        const float X = (float)(bp.x * 8 + lx) * vs - pose.t.x;
+        const float Y = (float)(bp.y * 8 + ly) * vs - pose.t.y;
+        const float px0 = pose.Rinv.r0.x * X + pose.Rinv.r0.y * Y;
+        const float py0 = pose.Rinv.r1.x * X + pose.Rinv.r1.y * Y;
+        const float pz0 = pose.Rinv.r2.x * X + pose.Rinv.r2.y * Y;
         int pix[H];
         bool upd[H];
         float sdf[H];
         f3 pos[H];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int b = h >> 1;
-            const int li = threadIdx.x + (h & 1) * 256;
-            const i3 vp = mki3(bp[b].x * 8 + (li & 7), bp[b].y * 8 + ((li >> 3) & 7), bp[b].z * 8 + (li >> 6));
-            pos[h] = mv(pose.Rinv, sub(i3f(vp, m.voxel_size), pose.t));
+            const float Z = (float)(bp.z * 8 + lz + 2 * h) * vs - pose.t.z;
+            pos[h] = mk3(px0 + pose.Rinv.r0.z * Z, py0 + pose.Rinv.r1.z * Z, pz0 + pose.Rinv.r2.z * Z);
             // f2i(fx * x / z + cx): both quotients share z's reciprocal in the range where
             // the division core is exact; below |a| = 2^-100, where it may not be, the
             // quotient is under 2^-60 and q + cx truncates to the same integer either way
@@ -397,7 +398,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
             }
             const int u = f2i(qu + a.cx);
             const int v = f2i(qv + a.cy);
-            upd[h] = live[b] && !(u < 0 || v < 0 || u >= cols || v >= rows);
+            upd[h] = !(u < 0 || v < 0 || u >= cols || v >= rows);
             pix[h] = upd[h] ? v * cols + u : 0;
         }
         float dp[H];
@@ -425,10 +426,10 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
         unsigned char cr[H], cg[H], cb[H];
         float sold[H];
         VoxAux old[H];
+        unsigned char* blk = IO::block(m, (int)heap);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-            const int li = threadIdx.x + (h & 1) * 256;
-            const unsigned char* blk = IO::block(m, (int)heap[h >> 1]);
+            const int li = t + 128 * h;
             if constexpr (kPacked) {
                 const unsigned c = prgb[pix[h]];
                 cr[h] = (unsigned char)c;
@@ -446,8 +447,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
 #pragma unroll
         for (int h = 0; h < H; ++h) {
             if (!upd[h] || isnanf_(nl[h].x)) continue;
-            const int li = threadIdx.x + (h & 1) * 256;
-            unsigned char* blk = IO::block(m, (int)heap[h >> 1]);
+            const int li = t + 128 * h;
             // sdf / thresh: |sdf| is 0 or at least ulp(depth_min), thresh a positive normal
             const float s = fminf(1.0f, fu_div_refined(sdf[h], thresh, rthresh));
             // 4-D dot with w = 1 on both sides (VectorMath.h:85-87,203-205,340-342); the
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(256) void kb_visible(BatchLaunch b) {
 template <int F>
 __global__ __launch_bounds__(256) void kb_integrate(BatchLaunch b) {
     WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
+    const int s = batch_slot(b, batch_wg(b), g);
     const SessDesc& d = b.desc[b.sess[s]];
     const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
     integrate_body<F, true>(g, m, slot_fuse(b, d, s));
@@ -581,15 +581,16 @@ void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     hipLaunchKernelGGL(kb_fuse_pack, dim3(grid), dim3(256), 0, s, b);
     grid = batch_layout(b, h, BT_TILES, 0, 0);
     hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(256), 0, s, b);
-    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(2048, b.n, 16, 512));
+    grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().alloc_groups, b.n, 16, 512));
     hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);
 }
 void launch_b_visible(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
-    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(8192, b.n, 64, 1024));
+    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().visible_groups, b.n, 64, 1024));
     hipLaunchKernelGGL(kb_visible, dim3(grid), dim3(256), 0, s, b);
 }
 void launch_b_integrate(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {
-    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(16384, b.n, 256, 8192));
+    const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().integrate_groups, b.n, 256, 8192));
+    b.xcd = xcd_order(FU_STAGE_INTEGRATE);
     switch (fmt) {
         case VF_F32_W8: hipLaunchKernelGGL(kb_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
         case VF_F32_W16: hipLaunchKernelGGL(kb_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;

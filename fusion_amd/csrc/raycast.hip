@@ -431,13 +431,26 @@ __device__ __forceinline__ bool ray_shade(const MapView& m, const RaycastArgs& a
     return true;
 }
 
-// one lane per pixel: set-up, march, refinement and normal
+// one lane per pixel of a 16x16 tile: set-up, march, refinement and normal.  `wave` picks the
+// pixels of each 64-lane wave (FU_WAVE_*): 16x4 rows, 8x8 quadrants or 4x16 columns.
 template <int F>
-__device__ __forceinline__ void raycast_body(WG g, const MapView& m, const RaycastArgs& a) {
+__device__ __forceinline__ void raycast_body(WG g, const MapView& m, const RaycastArgs& a, int wave) {
     PoseF pose = a.pose;
     if (a.st && !pose_from_state(a.st, pose)) return;
-    const int x = g.x * 16 + (threadIdx.x & 15);
-    const int y = g.y * 16 + (threadIdx.x >> 4);
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    int px, py;
+    if (wave == FU_WAVE_8X8) {
+        px = ((w & 1) << 3) | (l & 7);
+        py = ((w >> 1) << 3) | (l >> 3);
+    } else if (wave == FU_WAVE_4X16) {
+        px = (w << 2) | (l & 3);
+        py = l >> 2;
+    } else {
+        px = t & 15;
+        py = t >> 4;
+    }
+    const int x = g.x * 16 + px;
+    const int y = g.y * 16 + py;
     if (x >= a.cols || y >= a.rows) return;
     float zr0, zr1;
     if (!ray_setup(m, a, x, y, zr0, zr1)) return;
@@ -452,14 +465,14 @@ __device__ __forceinline__ void raycast_body(WG g, const MapView& m, const Rayca
     FU_G(f4, a.nmap)[y * a.cols + x] = no;
 }
 template <int F>
-__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a) {
-    raycast_body<F>(hw_wg(), m, a);
+__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a, int wave) {
+    raycast_body<F>(hw_wg(), m, a, wave);
 }
 
 template <int F>
 static void raycast_f(const MapView& m, const RaycastArgs& a, hipStream_t s) {
     const dim3 g((a.cols + 15) / 16, (a.rows + 15) / 16);
-    hipLaunchKernelGGL(k_raycast<F>, g, dim3(256), 0, s, m, a);
+    hipLaunchKernelGGL(k_raycast<F>, g, dim3(256), 0, s, m, a, (int)tuning().raycast_wave);
 }
 void launch_raycast(const MapView& m, const RaycastArgs& a, hipStream_t s) {
     switch (m.fmt) {
@@ -487,9 +500,9 @@ __global__ __launch_bounds__(256) void kb_project(BatchLaunch b) {
 template <int F>
 __global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
     WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
+    const int s = batch_slot(b, batch_wg(b), g);
     const SessDesc& d = b.desc[b.sess[s]];
-    raycast_body<F>(g, cload(d.m), cload(d.rc));
+    raycast_body<F>(g, cload(d.m), cload(d.rc), b.wave);
 }
 
 void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
@@ -509,6 +522,8 @@ void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
 template <int F>
 static void b_raycast_f(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     const int grid = batch_layout(b, h, BT_TILES, 0, 0);
+    b.xcd = xcd_order(FU_STAGE_RAYCAST);
+    b.wave = tuning().raycast_wave;
     hipLaunchKernelGGL(kb_raycast<F>, dim3(grid), dim3(256), 0, s, b);
 }
 void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s) {

@@ -76,6 +76,11 @@ class StageTimes(C.Structure):
 
 STAGES = ["resize", "pyramid", "icp", "alloc", "visible", "integrate", "zrange", "raycast"]
 
+
+class Tuning(C.Structure):
+    _fields_ = [("raycast_wave", C.c_int32), ("xcd_order", C.c_int32), ("integrate_groups", C.c_int32),
+                ("visible_groups", C.c_int32), ("alloc_groups", C.c_int32)]
+
 P = C.c_void_p
 I = C.c_int
 U = C.c_uint32
@@ -169,6 +174,9 @@ SIGNATURES = {
     "fu_batch_grab": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
     "fu_batch_set_profiling": (I, [P, I]),
     "fu_batch_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_tuning_default": (None, [C.POINTER(Tuning)]),
+    "fu_set_tuning": (I, [C.POINTER(Tuning)]),
+    "fu_get_tuning": (I, [C.POINTER(Tuning)]),
     "fu_server_set_profiling": (I, [P, I]),
     "fu_server_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
     "fu_server_params_default": (None, [C.POINTER(ServerParams)]),

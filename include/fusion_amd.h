@@ -401,6 +401,30 @@ fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_
 fu_status fu_batch_set_profiling(fu_batch* b, int enable);
 fu_status fu_batch_get_stage_times(fu_batch* b, fu_stage_times* out);
 
+/* ---- launch geometry of the batched stages ------------------------------------------------
+ * Work-group counts, the pixel layout of a raycast wave and the placement of work-groups on
+ * the XCDs for batched steps (fu_batch_grab and the server; the wave layout also for the raycast
+ * of one system, where a captured frame graph keeps the layout of its capture).  The defaults
+ * are FU_WAVE_8X8, no XCD ordering, 16384 / 8192 / 2048 work-groups.  None of them changes a result:
+ * per-pixel and per-block work does not depend on them, and the reductions keep their fixed
+ * decomposition (XCD placement only permutes which hardware work-group runs which logical
+ * one).  Process-wide; applies to steps launched afterwards.  fusion_amd/autotune.py measures
+ * candidates on a workload and caches the fastest per device.  No reference counterpart. */
+#define FU_WAVE_16X4 0 /* a 64-lane wave covers 16x4 pixels of its 16x16 tile */
+#define FU_WAVE_8X8 1  /* 8x8 pixels */
+#define FU_WAVE_4X16 2 /* 4x16 pixels */
+typedef struct fu_tuning {
+    int32_t raycast_wave;     /* FU_WAVE_*: lane layout of the ray-march waves */
+    int32_t xcd_order;        /* bit (1 << FU_STAGE_x): consecutive work-groups of the stage run on one XCD
+                                 (applies to ICP, integrate, raycast) */
+    int32_t integrate_groups; /* work-groups of one integration launch, split over its sessions, 256..262144 */
+    int32_t visible_groups;   /* the same for the visible-block list, 256..262144 */
+    int32_t alloc_groups;     /* the same for the allocation's second pass, 256..262144 */
+} fu_tuning;
+void fu_tuning_default(fu_tuning* t);
+fu_status fu_set_tuning(const fu_tuning* t); /* FU_ERR_ARG for a value out of range */
+fu_status fu_get_tuning(fu_tuning* t);
+
 /* ---- serving many reconstruction sessions -------------------------------------------------
  * A server keeps every session's map, pose, pyramid and last raycast resident in HBM and
  * processes frames as they arrive (continuous batching): sessions are spread over `lanes`

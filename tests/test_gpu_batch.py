@@ -56,6 +56,28 @@ def _state(s):
     return xyz, sdf, w, rgb, vm, nm
 
 
+def _run_batched(frames, use_rgb):
+    """the sessions' frames through one batch context, with the step membership of SESSIONS"""
+    print("batched", flush=True)
+    b = A.Batch(8)
+    systems = [_system(i, K, use_rgb) for i, (K, _) in enumerate(frames)]
+    for s in systems:
+        b.attach(s)
+    nxt = [0] * len(systems)
+    batch_res = [[] for _ in systems]
+    for step in range(STEPS):
+        members = [i for i, (_, _, _, _, _, start, skip) in enumerate(SESSIONS) if step >= start and step not in skip]
+        print("step", step, "sessions", members, flush=True)
+        b.grab([systems[i] for i in members], [frames[i][1][nxt[i]][0] for i in members],
+               [frames[i][1][nxt[i]][1] for i in members])
+        for i in members:
+            batch_res[i].append(systems[i].result())
+            nxt[i] += 1
+    for i in range(len(systems)):
+        assert nxt[i] == len(frames[i][1])
+    return batch_res, systems, b
+
+
 @pytest.mark.parametrize("use_rgb", [0, 1])
 def test_batch_equals_alone(gpu, use_rgb):
     frames = _frames()
@@ -72,23 +94,8 @@ def test_batch_equals_alone(gpu, use_rgb):
         s.close()
     torch.cuda.synchronize()
 
-    print("batched", flush=True)
-    b = A.Batch(8)
-    systems = [_system(i, K, use_rgb) for i, (K, _) in enumerate(frames)]
-    for s in systems:
-        b.attach(s)
-    nxt = [0] * len(systems)
-    batch_res = [[] for _ in systems]
-    for step in range(STEPS):
-        members = [i for i, (_, _, _, _, _, start, skip) in enumerate(SESSIONS) if step >= start and step not in skip]
-        print("step", step, "sessions", members, flush=True)
-        b.grab([systems[i] for i in members], [frames[i][1][nxt[i]][0] for i in members],
-               [frames[i][1][nxt[i]][1] for i in members])
-        for i in members:
-            batch_res[i].append(systems[i].result())
-            nxt[i] += 1
+    batch_res, systems, b = _run_batched(frames, use_rgb)
     for i, s in enumerate(systems):
-        assert nxt[i] == len(frames[i][1])
         for f, (ra, rb) in enumerate(zip(alone_res[i], batch_res[i])):
             assert ra["track_status"] == rb["track_status"], (i, f)
             assert np.array_equal(ra["pose"], rb["pose"]), (i, f, ra["pose"], rb["pose"])
@@ -101,6 +108,46 @@ def test_batch_equals_alone(gpu, use_rgb):
         assert same_bits(va, vb) and same_bits(na, nb), i
         assert alone_res[i][-1]["track_status"] in (1, 4), i  # the sequences do track
     b.close()
+
+
+# launch geometries far from the default: other wave layouts, every XCD-ordered stage, odd
+# work-group counts (a grid that is not a multiple of 8 exercises the XCD mapping's remainder)
+TUNINGS = [
+    dict(raycast_wave=1, xcd_order=0xFF, integrate_groups=3001, visible_groups=777, alloc_groups=259),
+    dict(raycast_wave=2, xcd_order=(1 << 2) | (1 << 7), integrate_groups=65536, visible_groups=20000,
+         alloc_groups=9000),
+]
+
+
+@pytest.mark.parametrize("use_rgb", [0, 1])
+def test_launch_geometry_does_not_change_results(gpu, use_rgb):
+    frames = _frames()
+    base = A.default_tuning()
+    runs = []
+    try:
+        for t in [base] + TUNINGS:
+            print("tuning", t, "use_rgb", use_rgb, flush=True)
+            A.set_tuning(**t)
+            res, systems, b = _run_batched(frames, use_rgb)
+            runs.append((res, [_state(s) for s in systems]))
+            for s in systems:
+                s.close()
+            b.close()
+    finally:
+        A.set_tuning(**base)
+    (r0, s0) = runs[0]
+    for k, (r1, s1) in enumerate(runs[1:]):
+        for i in range(len(SESSIONS)):
+            for f, (ra, rb) in enumerate(zip(r0[i], r1[i])):
+                assert ra["track_status"] == rb["track_status"], (k, i, f)
+                assert np.array_equal(ra["pose"], rb["pose"]), (k, i, f)
+                assert ra["n_visible"] == rb["n_visible"] and ra["n_allocated"] == rb["n_allocated"], (k, i, f)
+                assert ra["rgb_count"] == rb["rgb_count"], (k, i, f)
+            xa, sa, wa, ca, va, na = s0[i]
+            xb, sb, wb, cb, vb, nb = s1[i]
+            assert np.array_equal(xa, xb), (k, i)
+            assert same_bits(sa, sb) and np.array_equal(wa, wb) and np.array_equal(ca, cb), (k, i)
+            assert same_bits(va, vb) and same_bits(na, nb), (k, i)
 
 
 def test_batch_argument_errors(gpu):
