@@ -103,6 +103,57 @@ __device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int 
     return true;
 }
 
+// The 29 row products of pixel i (in: the lane has a pixel) into v[O .. O+28]: those of
+// icp_row when the pixel has a correspondence, else zeros.  The same tests and arithmetic as
+// icp_row, as selects instead of early returns (the gathers read pixel 0 for lanes that fail
+// the projection), so a wave runs one path; a wave without any valid vertex skips the rest.
+template <int O, int NV>
+__device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& P, int i, bool in, float (&v)[NV]) {
+    const int cols = a.cols, rows = a.rows;
+    const auto vcurr = FU_G(const f4, a.vcurr);
+    const auto ncurr = FU_G(const f4, a.ncurr);
+    const auto vlast = FU_G(const f4, a.vlast);
+    const auto nlast = FU_G(const f4, a.nlast);
+    const int ic = in ? i : 0;
+    const f3 vcurr_c = xyz(vcurr[ic]);
+    bool ok = in && !(isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3);
+    if (ballot(ok) == 0ull) {
+#pragma unroll
+        for (int k = 0; k < 29; ++k) v[O + k] = 0.f;
+        return;
+    }
+    const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
+    const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
+    const double invz = fu_finite_nonzero(vcurr_p.z) ? fu_drcp_from_float((double)vcurr_p.z) : 1.0 / (double)vcurr_p.z;
+    const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
+    const int vv = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
+    ok = ok && !(u < 0 || vv < 0 || u >= cols || vv >= rows);
+    const int j = ok ? vv * cols + u : 0;
+    const f3 vlast_g = add(mv(P.Rl, xyz(vlast[j])), P.tl);
+    const f3 ncurr_c = xyz(ncurr[ic]);
+    const f3 ncurr_g = mv(P.Rc, ncurr_c);
+    const f3 nlast_c = xyz(nlast[j]);
+    const f3 nlast_g = mv(P.Rl, nlast_c);
+    const f3 dv = sub(vlast_g, vcurr_g);
+    const f3 cr2 = cross(ncurr_g, nlast_g);
+    const float dist2 = dot(dv, dv), sine2 = dot(cr2, cr2);
+    ok = ok && sine2 < kSine2 && dist2 <= kDist2 && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x);
+    const f3 nl = mv(P.Rli, nlast_g);
+    const f3 vcu = mv(P.Rli, sub(vcurr_g, P.tl));
+    const f3 vla = mv(P.Rli, sub(vlast_g, P.tl));
+    const f3 mn = neg(nl);
+    const f3 cr = cross(nl, vla);
+    // a lane without a correspondence contributes +0 to every sum, as the products of a zero row
+    const float row[7] = {ok ? mn.x : 0.f, ok ? mn.y : 0.f, ok ? mn.z : 0.f, ok ? cr.x : 0.f,
+                          ok ? cr.y : 0.f,  ok ? cr.z : 0.f, ok ? dot(mn, sub(vcu, vla)) : 0.f};
+    int k = O;
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+#pragma unroll
+        for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
+    v[O + 28] = ok ? 1.0f : 0.f;
+}
+
 // sums of a work-group of NW waves: each wave's totals into LDS, then one 29-float row
 // (v[29..31] must be 0).  Value j of a wave's totals is held by lanes 2j and 2j+1 after the
 // reduce-scatter; the waves are added as a pairwise tree.
@@ -160,13 +211,13 @@ __device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
     const int N = a.cols * a.rows;
     __shared__ float wsum[kIcpThreads / 64][32];
     float v[32];
-#pragma unroll
-    for (int j = 0; j < 32; ++j) v[j] = 0.f;
+    v[29] = v[30] = v[31] = 0.f;
     int i = g.x * blockDim.x + threadIdx.x;
     if constexpr (kOne) {
-        float row[7];
-        if (i < N && icp_row(a, P, i, row)) icp_products(row, v);
+        icp_row_values<0>(a, P, i, i < N, v);
     } else {
+#pragma unroll
+        for (int j = 0; j < 29; ++j) v[j] = 0.f;
         Kahan acc[29];
 #pragma unroll
         for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
@@ -467,19 +518,32 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
     const int N = a.cols * a.rows;
     __shared__ float wsum[kIcpThreads / 64][64];
     float v[64];
-#pragma unroll
-    for (int j = 0; j < 64; ++j) v[j] = 0.f;
+    v[29] = v[30] = v[31] = 0.f;
+    v[61] = v[62] = v[63] = 0.f;
     // lane i: the ICP row of pixel i and the photometric row of list entry i
     const int i = g.x * kIcpThreads + threadIdx.x;
-    float row[7];
-    if (i < N && a.ncurr && icp_row(a, P, i, row)) row_products<0>(row, v);  // no normals: RGB rows only
+    if (a.ncurr) {
+        icp_row_values<0>(a, P, i, i < N, v);
+    } else {  // no normals: RGB rows only
+#pragma unroll
+        for (int k = 0; k < 29; ++k) v[k] = 0.f;
+    }
+    // a lane without a residual record contributes zeros, as the products of a zero row (the
+    // record of an entry outside the list is never read: rgb_row then reads entry 0's pixel)
     const int n = *FU_G(const int, a.rgb.count);
-    if (g.x * kIcpThreads < n && i < n) {
-        const int rec = FU_G(const int, a.rgb.rec)[i];
-        if (rec >= 0) {
-            rgb_row(a, Q, FU_G(const int, a.rgb.list)[i], rec >> 9, (rec & 511) - 256, sigma, row);
-            row_products<32>(row, v);
-        }
+    const int rec = (g.x * kIcpThreads < n && i < n) ? FU_G(const int, a.rgb.rec)[i] : -1;
+    const bool okr = rec >= 0;
+    if (ballot(okr) == 0ull) {
+#pragma unroll
+        for (int k = 32; k < 61; ++k) v[k] = 0.f;
+    } else {
+        float row[7];
+        rgb_row(a, Q, okr ? FU_G(const int, a.rgb.list)[i] : 0, okr ? rec >> 9 : 0, okr ? (rec & 511) - 256 : 0,
+                sigma, row);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) row[k] = okr ? row[k] : 0.f;
+        row_products<32>(row, v);
+        v[60] = okr ? 1.0f : 0.f;
     }
     const float t = wave_reduce_scatter64(v);
     wsum[threadIdx.x >> 6][lane_id()] = t;
