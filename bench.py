@@ -61,8 +61,8 @@ def parse():
     ap.add_argument("--single-steps", type=int, default=200, help="frames of the single-stream run (0: skip)")
     ap.add_argument("--rgb-steps", type=int, default=30,
                     help="timed steps of the served run with the photometric term (0: skip)")
-    ap.add_argument("--lanes", type=int, default=2)
-    ap.add_argument("--max-batch", type=int, default=32)
+    ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--max-batch", type=int, default=22)
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--seed", type=int, default=1)

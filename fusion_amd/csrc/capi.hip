@@ -605,16 +605,19 @@ extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int col
 
 // ------------------------------------------------------------------------------ tracker
 
-// One pixel per lane (a kIcpThreads-pixel work-group each) up to 4096 work-groups (4M
-// pixels): every pixel's association is a dependent gather, so latency is hidden across waves
-// rather than by a per-lane loop, and lanes keep no running sums.  Larger frames loop.
+// Work-groups of the ICP reductions, at most 4096.  The photometric kernels take one pixel
+// (and one list entry) per lane, a kIcpThreads-pixel work-group each (up to 4M pixels); the
+// ICP-only reduction two pixels per lane, a kIcpPixels chunk each (up to 8M pixels).  Every
+// pixel's association is a dependent gather, so latency is hidden across waves rather than
+// by a per-lane loop.  Larger frames loop over chunks.
 static constexpr int kIcpMaxBlocks = 4096;
-static int icp_blocks(int npix) {
-    int nb = (npix + kIcpThreads - 1) / kIcpThreads;
+static int icp_blocks(int npix, int per_group = kIcpThreads) {
+    int nb = (npix + per_group - 1) / per_group;
     if (nb < 1) nb = 1;
     if (nb > kIcpMaxBlocks) nb = kIcpMaxBlocks;
     return nb;
 }
+static int icp_groups(int npix) { return icp_blocks(npix, kIcpPixels); }
 
 static void init_state(TrackState* h) {
     memset(h, 0, sizeof(*h));
@@ -636,6 +639,7 @@ extern "C" fu_status fu_tracker_create(int cols, int rows, fu_tracker** out) {
     size_t nlist = 0;
     for (int l = 0; l < 3; ++l) {
         t->nblocks[l] = icp_blocks((cols >> l) * (rows >> l));
+        t->ngroups[l] = icp_groups((cols >> l) * (rows >> l));
         t->list_off[l] = nlist;
         nlist += (size_t)(cols >> l) * (rows >> l);
     }
@@ -755,7 +759,7 @@ static void queue_se3(fu_tracker* t, const float* const nv[3], const float* cons
             const int fin = (l == last_level && j == iters[l] - 1);
             IcpArgs a = make_icp(t, l, nv[l], nn[l], lv[l], ln[l], K, 1, minc[l], fin, thresh);
             if (rgb) set_rgb(a, t, l, rgb->next_image[l], rgb->last_image[l], rgb->dx[l], rgb->dy[l]);
-            launch_icp(a, t->nblocks[l], s);
+            launch_icp(a, a.use_rgb ? t->nblocks[l] : t->ngroups[l], s);
         }
 }
 
@@ -854,7 +858,7 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
     a.st = scratch->st;
     a.partials = scratch->partials;
     a.mode = 0;
-    launch_icp(a, icp_blocks(cols * rows), S(s));
+    launch_icp(a, icp_groups(cols * rows), S(s));
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(&h, scratch->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
     HIPCHK(hipStreamSynchronize(S(s)));
@@ -1240,7 +1244,7 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
                 HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
                 IcpArgs a = make_icp(t, l, b.vmap[l], b.nmap[l], b.model_vmap[l], b.model_nmap[l], K, 0, 0, 0, 0);
                 if (p.use_rgb) set_rgb(a, t, l, b.image[l], b.last_image[l], b.dIdx[l], b.dIdy[l]);
-                launch_icp(a, t->nblocks[l], s);
+                launch_icp(a, a.use_rgb ? t->nblocks[l] : t->ngroups[l], s);
                 LAUNCHCHK();
                 HIPCHK(hipMemcpyAsync(&h, t->st, sizeof(h), hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
@@ -1580,7 +1584,7 @@ static void fill_desc(fu_system* sys, SessDesc& d) {
             d.didx[l] = b.dIdx[l];
             d.didy[l] = b.dIdy[l];
         }
-        d.icp_blocks[l] = t->nblocks[l];
+        d.icp_blocks[l] = p.use_rgb ? t->nblocks[l] : t->ngroups[l];
         d.iters[l] = p.iters[l];
     }
     PoseF dummy{};
@@ -1817,7 +1821,7 @@ extern "C" fu_status fu_compute_se3_batch(fu_icp_batch* ib, int n, fu_tracker* c
             d.lr[l] = t[i]->rows >> l;
             d.icp[l] = make_icp(t[i], l, next_vmap[k], next_nmap[k], last_vmap[k], last_nmap[k], K + 4 * i, 1,
                                 min_count[l], 0, thresh);
-            d.icp_blocks[l] = t[i]->nblocks[l];
+            d.icp_blocks[l] = t[i]->ngroups[l];
             d.iters[l] = iters[l];
         }
     }

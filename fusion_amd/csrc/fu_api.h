@@ -27,7 +27,8 @@ struct fu_tracker {
     int* rgb_list = nullptr;    // eligible pixels of the three levels (level l at list_off[l])
     int* rgb_aux = nullptr;     // per level: the list length, then kIcpMaxBlocks work-group counts
     size_t list_off[3];
-    int nblocks[3];
+    int nblocks[3];  // work-groups per level of the photometric kernels (one pixel per lane)
+    int ngroups[3];  // work-groups per level of the ICP-only reduction (two pixels per lane)
 };
 
 struct fu_system {

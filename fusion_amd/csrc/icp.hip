@@ -5,8 +5,10 @@
 // operator() (:288-312), Reduce<float,29> (:49-64), CreateMatrix<6,7> (:6-16), ICPStep
 // (:319-375); Tracking/Tracking.cc Tracker::ComputeSE3 (:296-409).
 //
-// Each lane sums its pixels with Neumaier-compensated FP32 adds; lanes are combined by a
-// DPP pairwise tree (wave_sum) and waves through LDS into one partial row per work-group.
+// Each lane adds the products of its two pixels; a wave reduces its 29 sums with one
+// reduce-scatter butterfly (wave_reduce_scatter32) and the waves are added through LDS into
+// one partial row per work-group (frames with more pixel chunks than work-groups sum their
+// chunks per lane with Neumaier-compensated FP32 adds first).
 // A one-group kernel then sums the rows with the same compensated tree and (mode 1) unpacks
 // A (6x6) and b (6) in FP64, solves by Cholesky, takes the SE(3) exponential and composes
 // the pose on one lane: the only FP64 steps of the pipeline.
@@ -55,59 +57,13 @@ struct IcpPose {
     f3 tc, tl;
 };
 
-// ICPReduce::searchPoint + getRow (Reduction.cu:227-286) for pixel i: the 7-vector
-// [-n, n x v_l, -n.(v_c - v_l)] in the last camera frame, or false
-__device__ __forceinline__ bool icp_row(const IcpArgs& a, const IcpPose& P, int i, float (&row)[7]) {
-    const int cols = a.cols, rows = a.rows;
-    // angle 0.6f, distance 0.1f (Reduction.cu:352-353) as bounds on the squared norms
-    const auto vcurr = FU_G(const f4, a.vcurr);
-    const auto ncurr = FU_G(const f4, a.ncurr);
-    const auto vlast = FU_G(const f4, a.vlast);
-    const auto nlast = FU_G(const f4, a.nlast);
-    const f4 vc4 = vcurr[i];
-    const f3 vcurr_c = xyz(vc4);
-    if (isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
-    const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
-    const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
-    const double invz = fu_finite_nonzero(vcurr_p.z) ? fu_drcp_from_float((double)vcurr_p.z) : 1.0 / (double)vcurr_p.z;
-    const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
-    const int v = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
-    if (u < 0 || v < 0 || u >= cols || v >= rows) return false;
-    const int j = v * cols + u;
-    const f3 vlast_g = add(mv(P.Rl, xyz(vlast[j])), P.tl);
-    const f3 ncurr_c = xyz(ncurr[i]);
-    const f3 ncurr_g = mv(P.Rc, ncurr_c);
-    const f3 nlast_c = xyz(nlast[j]);
-    const f3 nlast_g = mv(P.Rl, nlast_c);
-    // sine = norm(cross), dist = norm(difference) are only compared with the thresholds:
-    // sqrtf is correctly rounded and monotonic, so sqrtf(s) < 0.6f <=> s < kSine2 and
-    // sqrtf(d) <= 0.1f <=> d <= kDist2 (the first / last floats of those ranges; a NaN fails
-    // both forms)
-    const f3 dv = sub(vlast_g, vcurr_g);
-    const f3 cr2 = cross(ncurr_g, nlast_g);
-    const float dist2 = dot(dv, dv), sine2 = dot(cr2, cr2);
-    static_assert(angle_thresh_ok(), "thresholds");
-    if (!(sine2 < kSine2 && dist2 <= kDist2 && !isnanf_(ncurr_c.x) && !isnanf_(nlast_c.x))) return false;
-    const f3 nl = mv(P.Rli, nlast_g);
-    const f3 vcu = mv(P.Rli, sub(vcurr_g, P.tl));
-    const f3 vla = mv(P.Rli, sub(vlast_g, P.tl));
-    const f3 mn = neg(nl);
-    const f3 cr = cross(nl, vla);
-    row[0] = mn.x;
-    row[1] = mn.y;
-    row[2] = mn.z;
-    row[3] = cr.x;
-    row[4] = cr.y;
-    row[5] = cr.z;
-    row[6] = dot(mn, sub(vcu, vla));
-    return true;
-}
-
-// The 29 row products of pixel i (in: the lane has a pixel) into v[O .. O+28]: those of
-// icp_row when the pixel has a correspondence, else zeros.  The same tests and arithmetic as
-// icp_row, as selects instead of early returns (the gathers read pixel 0 for lanes that fail
-// the projection), so a wave runs one path; a wave without any valid vertex skips the rest.
-template <int O, int NV>
+// ICPReduce::searchPoint + getRow (Reduction.cu:227-286) for pixel i (in: the lane has a
+// pixel): the 29 products of the row [-n, n x v_l, -n.(v_c - v_l)] (last camera frame) and
+// the count 1 into v[O .. O+28] when the pixel has a correspondence, else zeros (kAdd: added
+// to v instead).  The reference's tests and arithmetic, as selects instead of early returns
+// (the gathers read pixel 0 for lanes that fail the projection), so a wave runs one path; a
+// wave without any valid vertex skips the rest.
+template <int O, bool kAdd = false, int NV>
 __device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& P, int i, bool in, float (&v)[NV]) {
     const int cols = a.cols, rows = a.rows;
     const auto vcurr = FU_G(const f4, a.vcurr);
@@ -118,8 +74,10 @@ __device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& 
     const f3 vcurr_c = xyz(vcurr[ic]);
     bool ok = in && !(isnanf_(vcurr_c.x) || (double)vcurr_c.z < 1e-3);
     if (ballot(ok) == 0ull) {
+        if constexpr (!kAdd) {
 #pragma unroll
-        for (int k = 0; k < 29; ++k) v[O + k] = 0.f;
+            for (int k = 0; k < 29; ++k) v[O + k] = 0.f;
+        }
         return;
     }
     const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
@@ -150,8 +108,17 @@ __device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& 
 #pragma unroll
     for (int r = 0; r < 7; ++r)
 #pragma unroll
-        for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
-    v[O + 28] = ok ? 1.0f : 0.f;
+        for (int c = r; c < 7; ++c) {
+            if constexpr (kAdd)
+                v[k] += row[r] * row[c];
+            else
+                v[k] = row[r] * row[c];
+            ++k;
+        }
+    if constexpr (kAdd)
+        v[O + 28] += ok ? 1.0f : 0.f;
+    else
+        v[O + 28] = ok ? 1.0f : 0.f;
 }
 
 // sums of a work-group of NW waves: each wave's totals into LDS, then one 29-float row
@@ -176,14 +143,6 @@ __device__ __forceinline__ void block_rows29(float (&v)[32], float (&wsum)[NW][3
     }
 }
 
-__device__ __forceinline__ void icp_products(const float (&row)[7], float (&v)[32]) {
-    int k = 0;
-#pragma unroll
-    for (int r = 0; r < 7; ++r)
-#pragma unroll
-        for (int c = r; c < 7; ++c) v[k++] = row[r] * row[c];
-    v[28] = 1.0f;
-}
 // the same 29 values into v[O .. O+28]
 template <int O, int NV>
 __device__ __forceinline__ void row_products(const float (&row)[7], float (&v)[NV]) {
@@ -212,37 +171,28 @@ __device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
     __shared__ float wsum[kIcpThreads / 64][32];
     float v[32];
     v[29] = v[30] = v[31] = 0.f;
-    int i = g.x * blockDim.x + threadIdx.x;
-    if constexpr (kOne) {
-        icp_row_values<0>(a, P, i, i < N, v);
-    } else {
-#pragma unroll
-        for (int j = 0; j < 29; ++j) v[j] = 0.f;
+    // a work-group takes chunks of kIcpPixels = 2 * kIcpThreads pixels: lane t adds the
+    // products of pixels t and t + kIcpThreads of the chunk (one addition per value) before
+    // the wave reduction
+    const int i0 = g.x * kIcpPixels + threadIdx.x;
+    icp_row_values<0>(a, P, i0, i0 < N, v);
+    icp_row_values<0, true>(a, P, i0 + kIcpThreads, i0 + kIcpThreads < N, v);
+    if constexpr (!kOne) {
+        // more chunks than work-groups: compensated per-lane sums over the chunks, seeded
+        // exactly by the first (so both variants agree when every group has one chunk)
         Kahan acc[29];
 #pragma unroll
-        for (int j = 0; j < 29; ++j) acc[j] = Kahan{0.f, 0.f};
-        const int stride = g.nx * blockDim.x;
-        if (i < N) {
-            // a lane's first pixel seeds its sums: a compensated add into zero is exact
-            float row[7];
-            if (icp_row(a, P, i, row)) {
-                icp_products(row, v);
+        for (int j = 0; j < 29; ++j) acc[j] = Kahan{v[j], 0.f};
+        const int stride = g.nx * kIcpPixels;
+        for (int c = g.x * kIcpPixels + stride; c < N; c += stride) {
+            const int i = c + threadIdx.x;
+            icp_row_values<0>(a, P, i, i < N, v);
+            icp_row_values<0, true>(a, P, i + kIcpThreads, i + kIcpThreads < N, v);
 #pragma unroll
-                for (int j = 0; j < 29; ++j) acc[j].s = v[j];
-            }
-            i += stride;
-        }
-        for (; i < N; i += stride) {
-            float row[7];
-            if (icp_row(a, P, i, row)) {
-                icp_products(row, v);
-#pragma unroll
-                for (int j = 0; j < 29; ++j) acc[j].add(v[j]);
-            }
+            for (int j = 0; j < 29; ++j) acc[j].add(v[j]);
         }
 #pragma unroll
         for (int j = 0; j < 29; ++j) v[j] = acc[j].val();
-        v[29] = v[30] = v[31] = 0.f;
     }
     block_rows29(v, wsum, FU_G(float, a.partials) + g.x * 32);
 }
@@ -629,7 +579,7 @@ void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
         hipLaunchKernelGGL(k_icp_rgb_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, nblocks);
         return;
     }
-    if ((long long)nblocks * kIcpThreads >= (long long)a.cols * a.rows)
+    if ((long long)nblocks * kIcpPixels >= (long long)a.cols * a.rows)
         hipLaunchKernelGGL(k_icp<true>, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
     else
         hipLaunchKernelGGL(k_icp<false>, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
@@ -776,7 +726,7 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
     bool one = true;
     for (int i = 0; i < b.n; ++i) {
         const SessDesc& d = h[b.sess[i]];
-        one = one && (long long)d.icp_blocks[level] * kIcpThreads >= (long long)d.lc[level] * d.lr[level];
+        one = one && (long long)d.icp_blocks[level] * kIcpPixels >= (long long)d.lc[level] * d.lr[level];
     }
     if (one)
         hipLaunchKernelGGL(kb_icp<true>, dim3(grid), dim3(kIcpThreads), 0, s, b);
