@@ -452,9 +452,48 @@ __device__ __forceinline__ float rgb_sigma(const IcpArgs& a, int nparts, int* co
     return sqrtf((float)qi / (float)(ci == 0 ? 1 : ci));
 }
 
-// ICP and photometric rows of one pixel per lane: values 0-28 are the ICP sums, 32-60 the RGB
-// sums; the work-group's 64 totals go to partials[g.x * 64 ..]
-__device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
+// The 29 products of the photometric row of list entry i (n entries) into v[O .. O+28]
+// (kAdd: added), zeros for a lane without a residual record.  The record of an entry outside
+// the list is never read; such lanes evaluate rgb_row on entry 0's pixel and discard it.
+template <int O, bool kAdd = false>
+__device__ __forceinline__ void rgb_row_values(const IcpArgs& a, const RgbPose& Q, int i, int n, float sigma,
+                                               float (&v)[64]) {
+    const int rec = i < n ? FU_G(const int, a.rgb.rec)[i] : -1;
+    const bool okr = rec >= 0;
+    if (ballot(okr) == 0ull) {
+        if constexpr (!kAdd) {
+#pragma unroll
+            for (int k = 0; k < 29; ++k) v[O + k] = 0.f;
+        }
+        return;
+    }
+    float row[7];
+    rgb_row(a, Q, okr ? FU_G(const int, a.rgb.list)[i] : 0, okr ? rec >> 9 : 0, okr ? (rec & 511) - 256 : 0, sigma,
+            row);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) row[k] = okr ? row[k] : 0.f;
+    int k = O;
+#pragma unroll
+    for (int r = 0; r < 7; ++r)
+#pragma unroll
+        for (int c = r; c < 7; ++c) {
+            if constexpr (kAdd)
+                v[k] += row[r] * row[c];
+            else
+                v[k] = row[r] * row[c];
+            ++k;
+        }
+    if constexpr (kAdd)
+        v[O + 28] += okr ? 1.0f : 0.f;
+    else
+        v[O + 28] = okr ? 1.0f : 0.f;
+}
+
+// ICP and photometric rows of a kIcpPixels chunk (two pixels and two list entries per lane,
+// added before the wave reduction): values 0-28 are the ICP sums, 32-60 the RGB sums; the
+// work-group's 64 totals go to partials[g.x * 64 ..].  nresid: work-groups of the residual
+// pass, whose integer sums give sigma.
+__device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a, int nresid) {
     const auto st = FU_G(const TrackState, a.st);
     if (a.mode != 0 && st->status != TS_RUNNING) return;
     IcpPose P;
@@ -464,36 +503,27 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
     P.tc = mk3(st->tcurr[0], st->tcurr[1], st->tcurr[2]);
     P.tl = mk3(st->tlast[0], st->tlast[1], st->tlast[2]);
     const RgbPose Q = rgb_pose(a.st);
-    const float sigma = rgb_sigma(a, g.nx, nullptr);
+    const float sigma = rgb_sigma(a, nresid, nullptr);
     const int N = a.cols * a.rows;
     __shared__ float wsum[kIcpThreads / 64][64];
     float v[64];
     v[29] = v[30] = v[31] = 0.f;
     v[61] = v[62] = v[63] = 0.f;
-    // lane i: the ICP row of pixel i and the photometric row of list entry i
-    const int i = g.x * kIcpThreads + threadIdx.x;
+    const int i0 = g.x * kIcpPixels + threadIdx.x, i1 = i0 + kIcpThreads;
     if (a.ncurr) {
-        icp_row_values<0>(a, P, i, i < N, v);
+        icp_row_values<0>(a, P, i0, i0 < N, v);
+        icp_row_values<0, true>(a, P, i1, i1 < N, v);
     } else {  // no normals: RGB rows only
 #pragma unroll
         for (int k = 0; k < 29; ++k) v[k] = 0.f;
     }
-    // a lane without a residual record contributes zeros, as the products of a zero row (the
-    // record of an entry outside the list is never read: rgb_row then reads entry 0's pixel)
     const int n = *FU_G(const int, a.rgb.count);
-    const int rec = (g.x * kIcpThreads < n && i < n) ? FU_G(const int, a.rgb.rec)[i] : -1;
-    const bool okr = rec >= 0;
-    if (ballot(okr) == 0ull) {
+    if (g.x * kIcpPixels < n) {
+        rgb_row_values<32>(a, Q, i0, n, sigma, v);
+        rgb_row_values<32, true>(a, Q, i1, n, sigma, v);
+    } else {
 #pragma unroll
         for (int k = 32; k < 61; ++k) v[k] = 0.f;
-    } else {
-        float row[7];
-        rgb_row(a, Q, okr ? FU_G(const int, a.rgb.list)[i] : 0, okr ? rec >> 9 : 0, okr ? (rec & 511) - 256 : 0,
-                sigma, row);
-#pragma unroll
-        for (int k = 0; k < 7; ++k) row[k] = okr ? row[k] : 0.f;
-        row_products<32>(row, v);
-        v[60] = okr ? 1.0f : 0.f;
     }
     const float t = wave_reduce_scatter64(v);
     wsum[threadIdx.x >> 6][lane_id()] = t;
@@ -510,11 +540,12 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a) {
         FU_G(float, a.partials)[g.x * 64 + j] = w[0];
     }
 }
-__global__ __launch_bounds__(kIcpThreads) void k_icp_rgb(IcpArgs a) { icp_rgb_body(hw_wg(), a); }
+__global__ __launch_bounds__(kIcpThreads) void k_icp_rgb(IcpArgs a, int nresid) { icp_rgb_body(hw_wg(), a, nresid); }
 
-// final group with the photometric term: each wave adds every fourth partial row
-// (compensated, one value per lane), the waves are added pairwise, then the FP64 update
-__device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts) {
+// final group with the photometric term: each wave adds every fourth of the nparts partial
+// rows (compensated, one value per lane), the waves are added pairwise, then the FP64 update;
+// sigma and the correspondence counts from the nresid residual-pass partials
+__device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts, int nresid) {
     TrackState* st = a.st;
     if (a.mode != 0 && st->status != TS_RUNNING) return;
     constexpr int NW = kIcpFinalThreads / 64;
@@ -522,7 +553,7 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts)
     __shared__ float total[64];
     __shared__ int corr[2];
     const int lane = lane_id(), w = threadIdx.x >> 6;
-    const float sigma = rgb_sigma(a, nparts, corr);
+    const float sigma = rgb_sigma(a, nresid, corr);
     // rows w, w + NW, w + 2 NW, ... in that order; eight loads in flight ahead of the adds
     Kahan k{0.f, 0.f};
     const auto parts = FU_G(const float, a.partials);
@@ -558,8 +589,8 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts)
         if (a.mode == 1) icp_update(st, total, total + 32, a.min_count, a.final_iter, a.thresh);
     }
 }
-__global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts) {
-    icp_rgb_final_body(a, nparts);
+__global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts, int nresid) {
+    icp_rgb_final_body(a, nparts, nresid);
 }
 
 __global__ __launch_bounds__(kIcpThreads) void k_rgb_count(IcpArgs a) { rgb_count_body(hw_wg(), a); }
@@ -575,8 +606,9 @@ void launch_rgb_lists(const IcpArgs* a, const int* nblocks, int nlevels, hipStre
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
     if (a.use_rgb) {  // one pixel per lane (the caller sizes nblocks to cover the level)
         hipLaunchKernelGGL(k_rgb_resid, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
-        hipLaunchKernelGGL(k_icp_rgb, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
-        hipLaunchKernelGGL(k_icp_rgb_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, nblocks);
+        const int pairs = (nblocks + 1) / 2;  // kIcpPixels chunks of the combined pass
+        hipLaunchKernelGGL(k_icp_rgb, dim3(pairs), dim3(kIcpThreads), 0, s, a, nblocks);
+        hipLaunchKernelGGL(k_icp_rgb_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, pairs, nblocks);
         return;
     }
     if ((long long)nblocks * kIcpPixels >= (long long)a.cols * a.rows)
@@ -672,13 +704,13 @@ __global__ __launch_bounds__(kIcpThreads) void kb_icp_rgb(BatchLaunch b) {
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, s, a, nb)) return;
-    icp_rgb_body(g, a);
+    icp_rgb_body(g, a, nb);
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void kb_icp_rgb_final(BatchLaunch b) {
     IcpArgs a;
     int nb;
     if (!icp_slot_args(b, blockIdx.x, a, nb)) return;
-    icp_rgb_final_body(a, nb);
+    icp_rgb_final_body(a, (nb + 1) / 2, nb);
 }
 
 __global__ void kb_track_begin(BatchLaunch b) {
@@ -719,7 +751,8 @@ void launch_b_icp(BatchLaunch& b, const SessDesc* h, int level, int iter, hipStr
     b.xcd = xcd_order(FU_STAGE_ICP);
     if (h[b.sess[0]].use_rgb) {  // the sessions of a batch share the tracking mode
         hipLaunchKernelGGL(kb_rgb_resid, dim3(grid), dim3(kIcpThreads), 0, s, b);
-        hipLaunchKernelGGL(kb_icp_rgb, dim3(grid), dim3(kIcpThreads), 0, s, b);
+        const int pairs = batch_layout(b, h, BT_ICP_PAIRS, level, 0);
+        hipLaunchKernelGGL(kb_icp_rgb, dim3(pairs), dim3(kIcpThreads), 0, s, b);
         hipLaunchKernelGGL(kb_icp_rgb_final, dim3(b.n), dim3(kIcpFinalThreads), 0, s, b);
         return;
     }
