@@ -605,9 +605,10 @@ extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int col
 
 // ------------------------------------------------------------------------------ tracker
 
-// Work-groups of the ICP reductions, at most 4096.  The photometric kernels take one pixel
-// (and one list entry) per lane, a kIcpThreads-pixel work-group each (up to 4M pixels); the
-// ICP-only reduction two pixels per lane, a kIcpPixels chunk each (up to 8M pixels).  Every
+// Work-groups of the ICP reductions, at most 4096.  The photometric lists and residual pass
+// take one pixel (list entry) per lane, a kIcpThreads-pixel work-group each (up to 4M
+// pixels; the combined pass pairs these groups); the ICP-only reduction kIcpLanePixels
+// pixels per lane, a kIcpPixels chunk each.  Every
 // pixel's association is a dependent gather, so latency is hidden across waves rather than
 // by a per-lane loop.  Larger frames loop over chunks.
 static constexpr int kIcpMaxBlocks = 4096;

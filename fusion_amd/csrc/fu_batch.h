@@ -137,7 +137,7 @@ enum BatchTiling : int {
     BT_FIXED = 4,   // a fixed number of work-groups per session (grid-stride bodies)
     BT_ZTILES = 5,  // 256-thread groups over the z-range tiles
     BT_PIX256 = 6,  // 256-thread groups over the level-0 pixels
-    BT_ICP_PAIRS = 7,  // (icp_blocks[level] + 1) / 2 work-groups (kIcpPixels chunks)
+    BT_ICP_PAIRS = 7,  // (icp_blocks[level] + 1) / 2 work-groups (kRgbPixels chunks)
 };
 // fills b.pre, b.gx and b.uniform for one launch; returns the total work-group count
 int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, int fixed);

@@ -130,8 +130,11 @@ struct MeshOut {
 
 // work-group sizes of the ICP reduction (one pixel per lane) and of its one-group final sum
 constexpr int kIcpThreads = 1024;
-// pixels of one work-group chunk of the ICP-only reduction (two per lane)
-constexpr int kIcpPixels = 2 * kIcpThreads;
+// pixels per lane and per work-group chunk of the ICP-only reduction, and of the combined
+// ICP + photometric pass (two per lane)
+constexpr int kIcpLanePixels = 2;
+constexpr int kIcpPixels = kIcpLanePixels * kIcpThreads;
+constexpr int kRgbPixels = 2 * kIcpThreads;
 constexpr int kIcpFinalThreads = 256;
 
 // Inputs of the photometric term at one level (RGBReduction, Reduction.cu:377-407)

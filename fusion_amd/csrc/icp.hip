@@ -5,7 +5,7 @@
 // operator() (:288-312), Reduce<float,29> (:49-64), CreateMatrix<6,7> (:6-16), ICPStep
 // (:319-375); Tracking/Tracking.cc Tracker::ComputeSE3 (:296-409).
 //
-// Each lane adds the products of its two pixels; a wave reduces its 29 sums with one
+// Each lane adds the products of its kIcpLanePixels pixels; a wave reduces its 29 sums with one
 // reduce-scatter butterfly (wave_reduce_scatter32) and the waves are added through LDS into
 // one partial row per work-group (frames with more pixel chunks than work-groups sum their
 // chunks per lane with Neumaier-compensated FP32 adds first).
@@ -171,12 +171,13 @@ __device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
     __shared__ float wsum[kIcpThreads / 64][32];
     float v[32];
     v[29] = v[30] = v[31] = 0.f;
-    // a work-group takes chunks of kIcpPixels = 2 * kIcpThreads pixels: lane t adds the
-    // products of pixels t and t + kIcpThreads of the chunk (one addition per value) before
+    // a work-group takes chunks of kIcpPixels = kIcpLanePixels * kIcpThreads pixels: lane t
+    // adds the products of pixels t, t + kIcpThreads, ... of the chunk, in that order, before
     // the wave reduction
     const int i0 = g.x * kIcpPixels + threadIdx.x;
     icp_row_values<0>(a, P, i0, i0 < N, v);
-    icp_row_values<0, true>(a, P, i0 + kIcpThreads, i0 + kIcpThreads < N, v);
+#pragma unroll
+    for (int k = 1; k < kIcpLanePixels; ++k) icp_row_values<0, true>(a, P, i0 + k * kIcpThreads, i0 + k * kIcpThreads < N, v);
     if constexpr (!kOne) {
         // more chunks than work-groups: compensated per-lane sums over the chunks, seeded
         // exactly by the first (so both variants agree when every group has one chunk)
@@ -187,7 +188,8 @@ __device__ __forceinline__ void icp_body(WG g, const IcpArgs& a) {
         for (int c = g.x * kIcpPixels + stride; c < N; c += stride) {
             const int i = c + threadIdx.x;
             icp_row_values<0>(a, P, i, i < N, v);
-            icp_row_values<0, true>(a, P, i + kIcpThreads, i + kIcpThreads < N, v);
+#pragma unroll
+            for (int k = 1; k < kIcpLanePixels; ++k) icp_row_values<0, true>(a, P, i + k * kIcpThreads, i + k * kIcpThreads < N, v);
 #pragma unroll
             for (int j = 0; j < 29; ++j) acc[j].add(v[j]);
         }
@@ -489,7 +491,7 @@ __device__ __forceinline__ void rgb_row_values(const IcpArgs& a, const RgbPose& 
         v[O + 28] = okr ? 1.0f : 0.f;
 }
 
-// ICP and photometric rows of a kIcpPixels chunk (two pixels and two list entries per lane,
+// ICP and photometric rows of a kRgbPixels chunk (two pixels and two list entries per lane,
 // added before the wave reduction): values 0-28 are the ICP sums, 32-60 the RGB sums; the
 // work-group's 64 totals go to partials[g.x * 64 ..].  nresid: work-groups of the residual
 // pass, whose integer sums give sigma.
@@ -509,7 +511,7 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a, int nresid)
     float v[64];
     v[29] = v[30] = v[31] = 0.f;
     v[61] = v[62] = v[63] = 0.f;
-    const int i0 = g.x * kIcpPixels + threadIdx.x, i1 = i0 + kIcpThreads;
+    const int i0 = g.x * kRgbPixels + threadIdx.x, i1 = i0 + kIcpThreads;
     if (a.ncurr) {
         icp_row_values<0>(a, P, i0, i0 < N, v);
         icp_row_values<0, true>(a, P, i1, i1 < N, v);
@@ -518,7 +520,7 @@ __device__ __forceinline__ void icp_rgb_body(WG g, const IcpArgs& a, int nresid)
         for (int k = 0; k < 29; ++k) v[k] = 0.f;
     }
     const int n = *FU_G(const int, a.rgb.count);
-    if (g.x * kIcpPixels < n) {
+    if (g.x * kRgbPixels < n) {
         rgb_row_values<32>(a, Q, i0, n, sigma, v);
         rgb_row_values<32, true>(a, Q, i1, n, sigma, v);
     } else {
@@ -606,7 +608,7 @@ void launch_rgb_lists(const IcpArgs* a, const int* nblocks, int nlevels, hipStre
 void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
     if (a.use_rgb) {  // one pixel per lane (the caller sizes nblocks to cover the level)
         hipLaunchKernelGGL(k_rgb_resid, dim3(nblocks), dim3(kIcpThreads), 0, s, a);
-        const int pairs = (nblocks + 1) / 2;  // kIcpPixels chunks of the combined pass
+        const int pairs = (nblocks + 1) / 2;  // kRgbPixels chunks of the combined pass
         hipLaunchKernelGGL(k_icp_rgb, dim3(pairs), dim3(kIcpThreads), 0, s, a, nblocks);
         hipLaunchKernelGGL(k_icp_rgb_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, pairs, nblocks);
         return;
