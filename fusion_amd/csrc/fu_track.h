@@ -112,6 +112,20 @@ FU_HD void se3_exp(const double* xi, double* T) {
 
 // float views of the poses used by the kernels (Frame::GpuRotation/GpuInvRotation/
 // GpuTranslation, Core/Frame.cc:275-294)
+// the float poses of the kernels from a next (current) and last pose
+FU_HD void set_float_pose(TrackState* st, const double* next, const double* last) {
+    FU_UNROLL for (int i = 0; i < 3; ++i) {
+        FU_UNROLL for (int j = 0; j < 3; ++j) {
+            st->Rcurr[i * 3 + j] = (float)next[i * 4 + j];
+            st->Rlast[i * 3 + j] = (float)last[i * 4 + j];
+            st->RlastInv[i * 3 + j] = (float)last[j * 4 + i];
+            st->RcurrInv[i * 3 + j] = (float)next[j * 4 + i];
+        }
+        st->tcurr[i] = (float)next[i * 4 + 3];
+        st->tlast[i] = (float)last[i * 4 + 3];
+    }
+}
+
 FU_HD void refresh_float_pose(TrackState* st) {
     FU_UNROLL for (int i = 0; i < 3; ++i) {
         FU_UNROLL for (int j = 0; j < 3; ++j) {
@@ -151,8 +165,23 @@ FU_HD void unpack_sums29(const float* sums, double* A, double* b) {
 // iteration, the acceptance test (:398-408).  `rgb` holds the photometric sums of the same
 // iteration, or is null (icpOnly): their failure test (:341-346) fails the frame like ICP's,
 // and the systems are blended as matA = matA_rgb * w + matA_icp with float w = 1e-4 (:379-383).
-FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int min_count, int final_iter,
-                      float thresh) {
+// the poses an update reads: last, the accumulated delta and the initial guess
+struct TrackPoses {
+    double last[16], delta[16], init[16];
+};
+FU_HD void load_track_poses(const TrackState* st, TrackPoses& P) {
+    FU_UNROLL for (int i = 0; i < 16; ++i) {
+        P.last[i] = st->last_pose[i];
+        P.delta[i] = st->delta[i];
+        P.init[i] = st->init_pose[i];
+    }
+}
+
+// One ComputeSE3 iteration's update from the reduced sums (and the photometric sums when
+// rgb is set): unpack, blend, solve, compose, accept.  Reads the poses from P and writes the
+// state without reading it back.
+FU_HD void icp_update(TrackState* st, const TrackPoses& P, const float* sums, const float* rgb, int min_count,
+                      int final_iter, float thresh) {
     double A[36], b[6];
     unpack_sums29(sums, A, b);
     bool rgb_ok = true;
@@ -180,40 +209,39 @@ FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int m
     double x[6];
     const bool solved = rgb_ok && !(isnanf_(err) || cnt < min_count) && chol_solve6(A, b, x);
     if (!solved) {
-        FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = st->last_pose[i];
+        FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = P.last[i];
         st->status = TS_FAILED;
-        refresh_float_pose(st);
+        set_float_pose(st, P.last, P.last);
         return;
     }
-    double dT[16], delta[16], dinv[16], last[16], next[16];
-    FU_UNROLL for (int i = 0; i < 16; ++i) {
-        delta[i] = st->delta[i];
-        last[i] = st->last_pose[i];
-    }
+    double dT[16], delta[16], dinv[16], next[16];
     FU_UNROLL for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     se3_exp(x, dT);
-    mat4_mul(dT, delta, delta);
+    mat4_mul(dT, P.delta, delta);
     rigid_inv(delta, dinv);
-    mat4_mul(last, dinv, next);
-    FU_UNROLL for (int i = 0; i < 16; ++i) {
-        st->delta[i] = delta[i];
-        st->next_pose[i] = next[i];
-    }
+    mat4_mul(P.last, dinv, next);
+    FU_UNROLL for (int i = 0; i < 16; ++i) st->delta[i] = delta[i];
     if (final_iter) {
-        double init[16], pinv[16], p[16];
-        FU_UNROLL for (int i = 0; i < 16; ++i) init[i] = st->init_pose[i];
-        rigid_inv(init, pinv);
+        double pinv[16], p[16];
+        rigid_inv(P.init, pinv);
         mat4_mul(pinv, next, p);
         const double tn = sqrt(p[3] * p[3] + p[7] * p[7] + p[11] * p[11]);
         const double an = sin_euler_norm(p);
         if (err < thresh && an <= 0.1 && tn <= 0.1) {
             st->status = TS_OK;
         } else {
-            FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = last[i];
+            FU_UNROLL for (int i = 0; i < 16; ++i) next[i] = P.last[i];
             st->status = TS_REJECTED;
         }
     }
-    refresh_float_pose(st);
+    FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = next[i];
+    set_float_pose(st, next, P.last);
+}
+FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int min_count, int final_iter,
+                      float thresh) {
+    TrackPoses P;
+    load_track_poses(st, P);
+    icp_update(st, P, sums, rgb, min_count, final_iter, thresh);
 }
 
 }  // namespace fu

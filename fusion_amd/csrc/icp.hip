@@ -208,9 +208,25 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp(IcpArgs a) {
 // its own kernel makes the partials visible through the kernel boundary; a last-group ticket
 // inside k_icp would need a device-scope release per work-group, which on MI355X writes back
 // the work-group's L2 (one per XCD) every time.
+// The poses the update reads, loaded into LDS by 48 lanes at the start of a final kernel, so
+// their latency overlaps the partial-row loads instead of following the reduction.
+__device__ __forceinline__ void preload_track_poses(const TrackState* st, TrackPoses& P) {
+    const int t = threadIdx.x;
+    if (t < 16)
+        P.last[t] = st->last_pose[t];
+    else if (t < 32)
+        P.delta[t - 16] = st->delta[t - 16];
+    else if (t < 48)
+        P.init[t - 32] = st->init_pose[t - 32];
+}
+
 __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     TrackState* st = a.st;
-    if (a.mode != 0 && st->status != TS_RUNNING) return;
+    // the state's status, the partial rows and the poses are loaded together; a frame that is
+    // not running leaves (uniformly) before anything is written
+    const int status = st->status;
+    __shared__ TrackPoses poses;
+    if (a.mode == 1) preload_track_poses(st, poses);
     __shared__ float wsum[kIcpFinalThreads / 64][32];
     __shared__ float total[32];
     Kahan fin[29];
@@ -230,6 +246,7 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
 #pragma unroll
         for (int j = 0; j < 29; ++j) fin[j].add(v[j]);
     }
+    if (a.mode != 0 && status != TS_RUNNING) return;
     float v[32];
 #pragma unroll
     for (int j = 0; j < 29; ++j) v[j] = fin[j].val();
@@ -238,7 +255,7 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
-        if (a.mode == 1) icp_update(st, total, nullptr, a.min_count, a.final_iter, a.thresh);
+        if (a.mode == 1) icp_update(st, poses, total, nullptr, a.min_count, a.final_iter, a.thresh);
     }
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
@@ -549,7 +566,9 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_rgb(IcpArgs a, int nresid) 
 // sigma and the correspondence counts from the nresid residual-pass partials
 __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts, int nresid) {
     TrackState* st = a.st;
-    if (a.mode != 0 && st->status != TS_RUNNING) return;
+    const int status = st->status;
+    __shared__ TrackPoses poses;
+    if (a.mode == 1) preload_track_poses(st, poses);
     constexpr int NW = kIcpFinalThreads / 64;
     __shared__ float wsum[NW][64];
     __shared__ float total[64];
@@ -567,6 +586,7 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts,
         for (int u = 0; u < 8; ++u)
             if (g + u * NW < nparts) k.add(x[u]);
     }
+    if (a.mode != 0 && status != TS_RUNNING) return;  // uniform: nothing is written
     wsum[w][lane] = k.val();
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -588,7 +608,7 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts,
         st->rgb_corresp[0] = corr[0];
         st->rgb_corresp[1] = corr[1];
         st->sigma = sigma;
-        if (a.mode == 1) icp_update(st, total, total + 32, a.min_count, a.final_iter, a.thresh);
+        if (a.mode == 1) icp_update(st, poses, total, total + 32, a.min_count, a.final_iter, a.thresh);
     }
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts, int nresid) {
