@@ -83,10 +83,39 @@ __device__ __forceinline__ bool newset_claim(const MapView& m, unsigned long lon
 }
 
 
-// Fusion::CreateBlocks (FuseMap.cu:80-111), one lane per pixel, 16x16 pixels per group.
+// claim `key` (block bp) for allocation if it is not in the table (want: this lane has a
+// block; every lane of the wave calls, for the wave-wide append)
+__device__ __forceinline__ void claim_block(const MapView& m, bool want, unsigned long long key, i3 bp) {
+    bool won = false;
+    unsigned slot = 0;
+    if (want && find_block(m, bp) < 0) won = newset_claim(m, key, slot);
+    const unsigned idx = wave_append(&m.ctr[C_NEW], won);
+    if (won) {
+        if (idx < m.max_new) {
+            m.newkeys[idx] = key;
+            m.newslot[idx] = slot;
+        } else {
+            // list full: release the claim so a later pass can allocate this key
+            atomicExch(&m.newset[slot], kEmptyKey);
+            atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_NEWSET_FULL);
+        }
+    }
+}
+
+// Fusion::CreateBlocks (FuseMap.cu:80-111), one lane per pixel, 16x16 pixels per group.  Each
+// lane marches its ray's truncation band; the blocks its steps enter (a step in the block of
+// the previous step is a no-op for CreateBlock) go into a work-group set in LDS, so a block
+// the group's rays share is probed and claimed once, by the lanes that scan the set
+// afterwards.  A block that finds no free LDS slot within kAllocProbes is probed and claimed
+// at once by its lane; the global per-pass set de-duplicates across lanes and groups.
+constexpr int kAllocSlots = 2048;
+constexpr int kAllocProbes = 8;
 __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseArgs& p) {
+    __shared__ unsigned long long set[kAllocSlots];
     PoseF pose = p.pose;
-    if (p.st && !pose_from_state(p.st, pose)) return;
+    if (p.st && !pose_from_state(p.st, pose)) return;  // uniform: the frame was not tracked
+    for (int i = threadIdx.x; i < kAllocSlots; i += blockDim.x) set[i] = kEmptyKey;
+    __syncthreads();
     const auto depth = FU_G(const float, p.depth);
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
@@ -113,54 +142,30 @@ __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseA
             }
         }
     }
-    // Rounds: every lane advances its march to its next block that differs from the block
-    // of its previous step (consecutive repeats are no-ops for CreateBlock), so a wave runs
-    // as many rounds as its rays cross distinct blocks (about 3) instead of one per step.
-    int i = 0;
     i3 prev = mki3(0x7fffffff, 0, 0);
-    while (ballot(i < nsteps)) {
-        bool has = false;
-        i3 bp = prev;
-        while (i < nsteps) {
-            const i3 b = voxel_to_block(f2i3(pt));
-            pt = add(pt, dir);
-            ++i;
-            if (!eqi(b, prev)) {
-                prev = b;
-                bp = b;
-                has = true;
-                break;
-            }
-        }
-        if (has && !key_in_range(bp)) {
+    for (int i = 0; i < nsteps; ++i) {
+        const i3 b = voxel_to_block(f2i3(pt));
+        pt = add(pt, dir);
+        if (eqi(b, prev)) continue;
+        prev = b;
+        if (!key_in_range(b)) {
             atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_COORD_RANGE);
-            has = false;
+            continue;
         }
-        const unsigned long long key = has ? pack_key(bp) : kEmptyKey;
-        // wave de-duplication: one representative lane per distinct key
-        unsigned long long pending = ballot(has);
-        bool rep = false;
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const unsigned long long k = readlane64(key, leader);
-            const unsigned long long same = ballot(has && key == k);
-            pending &= ~same;
-            if (lane_id() == leader) rep = true;
+        const unsigned long long key = pack_key(b);
+        unsigned h = (unsigned)mix64(key) & (kAllocSlots - 1);
+        bool done = false;
+        for (int probe = 0; probe < kAllocProbes && !done; ++probe) {
+            const unsigned long long old = atomicCAS(&set[h], kEmptyKey, key);
+            done = old == kEmptyKey || old == key;
+            h = (h + 1) & (kAllocSlots - 1);
         }
-        bool won = false;
-        unsigned slot = 0;
-        if (rep && find_block(m, bp) < 0) won = newset_claim(m, key, slot);
-        const unsigned idx = wave_append(&m.ctr[C_NEW], won);
-        if (won) {
-            if (idx < m.max_new) {
-                m.newkeys[idx] = key;
-                m.newslot[idx] = slot;
-            } else {
-                // list full: release the claim so a later pass can allocate this key
-                atomicExch(&m.newset[slot], kEmptyKey);
-                atomicOr(&m.ctr[C_FLAGS], (unsigned)FLAG_NEWSET_FULL);
-            }
-        }
+        if (!done) claim_block(m, true, key, b);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAllocSlots; i += blockDim.x) {
+        const unsigned long long key = set[i];
+        claim_block(m, key != kEmptyKey, key, unpack_key(key));
     }
 }
 __global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) { alloc_a_body(hw_wg(), m, p); }
