@@ -85,10 +85,17 @@ FU_HD int f2i(float v) {
 #endif
 }
 FU_HD int d2i(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_cvt_i32_f64: truncation, saturation and NaN -> 0, as for f2i
+    int r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+#else
     if (!(v == v)) return 0;
     if (v >= 2147483648.0) return 2147483647;
     if (v <= -2147483648.0) return (-2147483647 - 1);
     return (int)v;
+#endif
 }
 FU_HD i3 f2i3(f3 a) { return i3{f2i(a.x), f2i(a.y), f2i(a.z)}; }
 // (unsigned char)(int)v : truncation then wrap modulo 256 (VectorMath.h:16-18)

@@ -82,7 +82,10 @@ __device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& 
     }
     const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
     const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
-    const double invz = fu_finite_nonzero(vcurr_p.z) ? fu_drcp_from_float((double)vcurr_p.z) : 1.0 / (double)vcurr_p.z;
+    // a lane without a valid vertex divides by 1 (its result is discarded), so only a valid
+    // vertex projecting to depth 0 or infinity takes the general division
+    const float zp = ok ? vcurr_p.z : 1.0f;
+    const double invz = fu_finite_nonzero(zp) ? fu_drcp_from_float((double)zp) : 1.0 / (double)zp;
     const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
     const int vv = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
     ok = ok && !(u < 0 || vv < 0 || u >= cols || vv >= rows);

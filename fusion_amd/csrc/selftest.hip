@@ -1,6 +1,8 @@
 // Self-test of the range-specialised IEEE arithmetic (fu_device.h: fu_rcp_refined,
 // fu_div_refined, fu_sqrt_normal, fu_drcp_from_float) against the compiler's IEEE division and square root on
 // the device itself: the kernels rely on them being bit-identical inside their stated ranges.
+// It also checks the single-instruction float/double -> int conversions (fu_math.h: f2i, d2i)
+// against the rule they implement (truncation, saturation, NaN -> 0).
 #include <hip/hip_runtime.h>
 
 #include "fu_api.h"
@@ -18,6 +20,14 @@ __device__ __forceinline__ uint64_t mix(uint64_t x) {
     x ^= x >> 33;
     return x;
 }
+// the conversion rule of f2i / d2i spelt out with comparisons
+template <class T>
+__device__ __forceinline__ int conv_rule(T v) {
+    if (!(v == v)) return 0;
+    if (v >= (T)2147483648.0) return 2147483647;
+    if (v <= (T)-2147483648.0) return (-2147483647 - 1);
+    return (int)v;
+}
 // a float with a uniformly random mantissa and an exponent uniform in [lo, hi], random sign
 __device__ __forceinline__ float rnd(uint64_t& s, int lo, int hi, bool sign) {
     s = mix(s + 0x9e3779b97f4a7c15ull);
@@ -30,7 +40,7 @@ __device__ __forceinline__ float rnd(uint64_t& s, int lo, int hi, bool sign) {
 }
 
 __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
-    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t s = mix(seed ^ ((uint64_t)i * 0x2545f4914f6cdd1dull));
         // 1: quotients by a 4-D norm: numerators in [2^-60, 2^30], denominators in [1, 2^31]
@@ -60,6 +70,15 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             const float x = rnd(s, -90, 119, false);
             b3 += __float_as_uint(fu_sqrt_normal(x)) != __float_as_uint(sqrtf(x));
         }
+        // 6: float and double -> int: any exponent (out of range, infinities and NaNs included)
+        {
+            s = mix(s + 0x9e3779b97f4a7c15ull);
+            const float f = __uint_as_float((unsigned)s);
+            const double d = __longlong_as_double((long long)mix(s));
+            b5 += f2i(f) != conv_rule(f);
+            b5 += d2i(d) != conv_rule(d);
+            b5 += d2i((double)f) != conv_rule((double)f);
+        }
         // 5: FP64 reciprocals of nonzero finite floats (denormal ones included)
         {
             float z = rnd(s, -126, 127, true);
@@ -70,6 +89,7 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             }
         }
     }
+    if (b5) atomicAdd(&bad[5], b5);
     if (b0) atomicAdd(&bad[0], b0);
     if (b1) atomicAdd(&bad[1], b1);
     if (b2) atomicAdd(&bad[2], b2);
@@ -79,17 +99,17 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
 
 }  // namespace
 
-extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[5], fu_stream s) {
+extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[6], fu_stream s) {
     if (!mismatches) return api_error(FU_ERR_ARG, "fu_selftest_arith: null output");
     unsigned* d = nullptr;
-    if (hipMalloc(&d, 20) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
+    if (hipMalloc(&d, 24) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
     hipStream_t hs = (hipStream_t)s;
-    hipError_t e = hipMemsetAsync(d, 0, 20, hs);
+    hipError_t e = hipMemsetAsync(d, 0, 24, hs);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, hs, seed, n, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 20, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 24, hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
     (void)hipFree(d);
     return e == hipSuccess ? FU_OK : api_error(FU_ERR_HIP, "fu_selftest_arith: %s", hipGetErrorString(e));

@@ -308,9 +308,10 @@ fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
  * of the compiler's IEEE sequences (no reference counterpart).  This runs n random cases of
  * each range on the device and counts the results that differ from the IEEE operation:
  * mismatches[0] quotients by a 4-D norm, [1] sdf / truncation, [2] projections (their
- * truncated pixel coordinate), [3] square roots, [4] FP64 reciprocals of floats.  All five
- * must be 0.  Synchronises. */
-fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[5], fu_stream s);
+ * truncated pixel coordinate), [3] square roots, [4] FP64 reciprocals of floats, [5] the
+ * single-instruction float and double -> int conversions against their rule (truncation,
+ * saturation, NaN -> 0).  All six must be 0.  Synchronises. */
+fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[6], fu_stream s);
 
 /* ---------------------------------------------------------------- on-disk formats */
 /* Map snapshot: System::WriteMapToDisk / ReadMapFromDisk (Core/System.cc:195-271).  The
