@@ -44,8 +44,11 @@ FU_HD void rigid_inv(const double* a, double* o) {
 // Returns false when A is not positive definite (a degenerate set of correspondences); the
 // tracker then fails the frame.  The reference solves with Eigen's LDLT, which agrees with
 // Cholesky on positive definite A to rounding.
+// Cholesky solve of the 6x6 normal equations; each diagonal's reciprocal is taken once and
+// multiplies the column and both substitutions (one division per column on the serial chain
+// instead of one per element)
 FU_HD bool chol_solve6(const double* A, const double* b, double* x) {
-    double L[6][6];
+    double L[6][6], r[6];
     bool ok = true;
     FU_UNROLL for (int j = 0; j < 6; ++j) {
         double s = A[j * 6 + j];
@@ -53,22 +56,23 @@ FU_HD bool chol_solve6(const double* A, const double* b, double* x) {
         ok = ok && (s > 0.0);
         const double d = sqrt(s > 0.0 ? s : 1.0);
         L[j][j] = d;
+        r[j] = 1.0 / d;
         FU_UNROLL for (int i = j + 1; i < 6; ++i) {
             double t = A[i * 6 + j];
             FU_UNROLL for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
-            L[i][j] = t / d;
+            L[i][j] = t * r[j];
         }
     }
     double y[6];
     FU_UNROLL for (int i = 0; i < 6; ++i) {
         double t = b[i];
         FU_UNROLL for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
-        y[i] = t / L[i][i];
+        y[i] = t * r[i];
     }
     FU_UNROLL for (int i = 5; i >= 0; --i) {
         double t = y[i];
         FU_UNROLL for (int k = i + 1; k < 6; ++k) t -= L[k][i] * x[k];
-        x[i] = t / L[i][i];
+        x[i] = t * r[i];
     }
     return ok;
 }

@@ -211,25 +211,29 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp(IcpArgs a) {
 // its own kernel makes the partials visible through the kernel boundary; a last-group ticket
 // inside k_icp would need a device-scope release per work-group, which on MI355X writes back
 // the work-group's L2 (one per XCD) every time.
-// The poses the update reads, loaded into LDS by 48 lanes at the start of a final kernel, so
-// their latency overlaps the partial-row loads instead of following the reduction.
-__device__ __forceinline__ void preload_track_poses(const TrackState* st, TrackPoses& P) {
-    const int t = threadIdx.x;
-    if (t < 16)
-        P.last[t] = st->last_pose[t];
-    else if (t < 32)
-        P.delta[t - 16] = st->delta[t - 16];
-    else if (t < 48)
-        P.init[t - 32] = st->init_pose[t - 32];
+// The tracking state of a final kernel is staged through LDS: every lane loads a part of it
+// at the start (overlapping the partial-row loads), lane 0 runs the serial FP64 update on the
+// LDS copy, and every lane writes a part back, instead of one lane's long chains of global
+// loads and stores.
+static_assert(sizeof(TrackState) % 4 == 0, "dword-sized state");
+constexpr int kStateWords = (int)(sizeof(TrackState) / 4);
+__device__ __forceinline__ void state_to_lds(const TrackState* st, TrackState* sst) {
+    const auto src = reinterpret_cast<const unsigned*>(st);
+    auto dst = reinterpret_cast<unsigned*>(sst);
+    for (int i = threadIdx.x; i < kStateWords; i += blockDim.x) dst[i] = src[i];
+}
+__device__ __forceinline__ void state_from_lds(const TrackState* sst, TrackState* st) {
+    const auto src = reinterpret_cast<const unsigned*>(sst);
+    auto dst = reinterpret_cast<unsigned*>(st);
+    for (int i = threadIdx.x; i < kStateWords; i += blockDim.x) dst[i] = src[i];
 }
 
 __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     TrackState* st = a.st;
-    // the state's status, the partial rows and the poses are loaded together; a frame that is
-    // not running leaves (uniformly) before anything is written
-    const int status = st->status;
-    __shared__ TrackPoses poses;
-    if (a.mode == 1) preload_track_poses(st, poses);
+    // the state and the partial rows are loaded together; a frame that is not running leaves
+    // (uniformly) before anything is written
+    __shared__ TrackState sst;
+    state_to_lds(st, &sst);
     __shared__ float wsum[kIcpFinalThreads / 64][32];
     __shared__ float total[32];
     Kahan fin[29];
@@ -249,7 +253,8 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
 #pragma unroll
         for (int j = 0; j < 29; ++j) fin[j].add(v[j]);
     }
-    if (a.mode != 0 && status != TS_RUNNING) return;
+    __syncthreads();
+    if (a.mode != 0 && sst.status != TS_RUNNING) return;
     float v[32];
 #pragma unroll
     for (int j = 0; j < 29; ++j) v[j] = fin[j].val();
@@ -257,9 +262,11 @@ __device__ __forceinline__ void icp_final_body(const IcpArgs& a, int nparts) {
     block_rows29(v, wsum, total);
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int j = 0; j < 29; ++j) st->sums[j] = total[j];
-        if (a.mode == 1) icp_update(st, poses, total, nullptr, a.min_count, a.final_iter, a.thresh);
+        for (int j = 0; j < 29; ++j) sst.sums[j] = total[j];
+        if (a.mode == 1) icp_update(&sst, total, nullptr, a.min_count, a.final_iter, a.thresh);
     }
+    __syncthreads();
+    state_from_lds(&sst, st);
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_final(IcpArgs a, int nparts) { icp_final_body(a, nparts); }
 
@@ -569,9 +576,8 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_rgb(IcpArgs a, int nresid) 
 // sigma and the correspondence counts from the nresid residual-pass partials
 __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts, int nresid) {
     TrackState* st = a.st;
-    const int status = st->status;
-    __shared__ TrackPoses poses;
-    if (a.mode == 1) preload_track_poses(st, poses);
+    __shared__ TrackState sst;
+    state_to_lds(st, &sst);
     constexpr int NW = kIcpFinalThreads / 64;
     __shared__ float wsum[NW][64];
     __shared__ float total[64];
@@ -589,7 +595,8 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts,
         for (int u = 0; u < 8; ++u)
             if (g + u * NW < nparts) k.add(x[u]);
     }
-    if (a.mode != 0 && status != TS_RUNNING) return;  // uniform: nothing is written
+    __syncthreads();
+    if (a.mode != 0 && sst.status != TS_RUNNING) return;  // uniform: nothing is written
     wsum[w][lane] = k.val();
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -605,14 +612,16 @@ __device__ __forceinline__ void icp_rgb_final_body(const IcpArgs& a, int nparts,
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int j = 0; j < 29; ++j) {
-            st->sums[j] = total[j];
-            st->rgb_sums[j] = total[32 + j];
+            sst.sums[j] = total[j];
+            sst.rgb_sums[j] = total[32 + j];
         }
-        st->rgb_corresp[0] = corr[0];
-        st->rgb_corresp[1] = corr[1];
-        st->sigma = sigma;
-        if (a.mode == 1) icp_update(st, poses, total, total + 32, a.min_count, a.final_iter, a.thresh);
+        sst.rgb_corresp[0] = corr[0];
+        sst.rgb_corresp[1] = corr[1];
+        sst.sigma = sigma;
+        if (a.mode == 1) icp_update(&sst, total, total + 32, a.min_count, a.final_iter, a.thresh);
     }
+    __syncthreads();
+    state_from_lds(&sst, st);
 }
 __global__ __launch_bounds__(kIcpFinalThreads) void k_icp_rgb_final(IcpArgs a, int nparts, int nresid) {
     icp_rgb_final_body(a, nparts, nresid);
