@@ -233,21 +233,35 @@ struct Sampler {
 #pragma unroll
             for (int k = 0; k < 8; ++k) hp[k] = h;
         } else {
-            // blocks resolved in corner order, each through the cache (as a corner-by-corner
-            // lookup would leave it)
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const i3 bp = mki3((k & 1) ? b1.x : b0.x, (k & 2) ? b1.y : b0.y, (k & 4) ? b1.z : b0.z);
-                if (cheap >= 0 && eqi(bp, cbp)) {
-                    hp[k] = cheap;
-                } else {
-                    hp[k] = find_block(m, bp);
-                    if (hp[k] >= 0) {
-                        cbp = bp;
-                        cheap = hp[k];
-                    }
+            // corners in two or more blocks: each axis spans b0 or b0 + 1 (mask bit set), so the
+            // corners' blocks are b0 + c for the subsets c of the mask.  Each distinct block is
+            // resolved once (b0 through the cache) and corner k takes block k & mask, instead
+            // of a lookup per corner, which alternates between the blocks and misses the
+            // one-block cache every time.
+            const int msk = (b1.x - b0.x) | ((b1.y - b0.y) << 1) | ((b1.z - b0.z) << 2);
+            int hb[8];
+            if (cheap >= 0 && eqi(b0, cbp)) {
+                hb[0] = cheap;
+            } else {
+                hb[0] = find_block(m, b0);
+                if (hb[0] >= 0) {
+                    cbp = b0;
+                    cheap = hb[0];
                 }
             }
+#pragma unroll
+            for (int c = 1; c < 8; ++c) {
+                hb[c] = -1;
+                if ((c & ~msk) == 0) hb[c] = find_block(m, mki3(b0.x + (c & 1), b0.y + ((c >> 1) & 1), b0.z + (c >> 2)));
+            }
+            hp[0] = hb[0];
+            hp[1] = (msk & 1) ? hb[1] : hb[0];
+            hp[2] = (msk & 2) ? hb[2] : hb[0];
+            hp[3] = (msk & 2) ? ((msk & 1) ? hb[3] : hb[2]) : hp[1];
+            hp[4] = (msk & 4) ? hb[4] : hb[0];
+            hp[5] = (msk & 4) ? ((msk & 1) ? hb[5] : hb[4]) : hp[1];
+            hp[6] = (msk & 4) ? ((msk & 2) ? hb[6] : hb[4]) : hp[2];
+            hp[7] = (msk & 4) ? ((msk & 2) ? ((msk & 1) ? hb[7] : hb[6]) : hp[5]) : hp[3];
         }
         // unconditional loads (an absent block reads pool block 0 and is masked to NaN), so
         // the eight loads issue back to back
