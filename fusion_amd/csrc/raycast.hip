@@ -265,11 +265,15 @@ struct Sampler {
         }
         // unconditional loads (an absent block reads pool block 0 and is masked to NaN), so
         // the eight loads issue back to back
+        // corner k's index within its block: the index of corner 0 plus per-axis steps (the
+        // local coordinate differences, negative where an axis crosses into the next block)
+        const int l0 = ((z0 & 7) << 6) | ((y0 & 7) << 3) | (x0 & 7);
+        const int ox = (x1 & 7) - (x0 & 7), oy = ((y1 & 7) - (y0 & 7)) << 3, oz = ((z1 & 7) - (z0 & 7)) << 6;
+        const int li[8] = {l0, l0 + ox, l0 + oy, l0 + oy + ox, l0 + oz, l0 + oz + ox, l0 + oz + oy, l0 + oz + oy + ox};
         float sv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int li = (((k & 4) ? z1 : z0) & 7) << 6 | (((k & 2) ? y1 : y0) & 7) << 3 | (((k & 1) ? x1 : x0) & 7);
-            const float raw = IO::sdf(IO::block(m, hp[k] >= 0 ? hp[k] : 0), li);
+            const float raw = IO::sdf(IO::block(m, hp[k] >= 0 ? hp[k] : 0), li[k]);
             sv[k] = hp[k] >= 0 ? raw : fnan();
         }
         valid = hp[7] >= 0;
@@ -283,17 +287,22 @@ struct Sampler {
         return (1.0f - f.z) * r2 + f.z * r3;
     }
 
-    // Heap indices of the 2x2x2 blocks lo + (dx, dy, dz), dx, dy, dz in {0, 1}, probed
-    // together: the occupancy words, then the bucket heads, as independent loads; only excess
-    // chains are walked one step at a time.
-    __device__ __forceinline__ void resolve8(i3 lo, int (&h)[8]) {
+    // Heap indices of the blocks lo + c, c = (dx, dy, dz) in {0, 1}^3 a subset of msk (bit 0:
+    // x, 1: y, 2: z; the other entries stay -1 and are not read), probed together: the
+    // occupancy words, then the bucket heads, as independent loads; only excess chains are
+    // walked one step at a time.
+    __device__ __forceinline__ void resolve8(i3 lo, int msk, int (&h)[8]) {
         uint32_t raw[8];
         bool maybe[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const i3 bp = mki3(lo.x + (k & 1), lo.y + ((k >> 1) & 1), lo.z + ((k >> 2) & 1));
-            raw[k] = block_hash_raw(bp);
-            maybe[k] = key_in_range(bp) && maybe_allocated(m, raw[k]);
+            maybe[k] = false;
+            raw[k] = 0;
+            if ((k & ~msk) == 0) {
+                const i3 bp = mki3(lo.x + (k & 1), lo.y + ((k >> 1) & 1), lo.z + ((k >> 2) & 1));
+                raw[k] = block_hash_raw(bp);
+                maybe[k] = key_in_range(bp) && maybe_allocated(m, raw[k]);
+            }
         }
         HEntry e[8];
 #pragma unroll
@@ -423,10 +432,13 @@ __device__ __forceinline__ bool ray_shade(const MapView& m, const RaycastArgs& a
     step = sdf * step_scale;
     result = add(result, mul(step, dir));
     // the six samples' corners lie between make_int3(result - 1) and make_int3((result + 1) + 1)
-    // per axis (truncation is monotonic), five voxels: at most two blocks per axis
+    // per axis (truncation is monotonic), at most four voxels: lo or lo + 1 per axis, and only
+    // the blocks of axes that do cross are probed
     const i3 lo = mki3(f2i(result.x - 1.0f) >> 3, f2i(result.y - 1.0f) >> 3, f2i(result.z - 1.0f) >> 3);
+    const i3 hi = mki3(f2i((result.x + 1.0f) + 1.0f) >> 3, f2i((result.y + 1.0f) + 1.0f) >> 3,
+                       f2i((result.z + 1.0f) + 1.0f) >> 3);
     int hb[8];
-    smp.resolve8(lo, hb);
+    smp.resolve8(lo, (hi.x - lo.x) | ((hi.y - lo.y) << 1) | ((hi.z - lo.z) << 2), hb);
     float s[6];
     bool ok = true;
 #pragma unroll
