@@ -337,6 +337,7 @@ void launch_visible(const MapView& m, const VisArgs& v, hipStream_t s) {
 // loads and stores cover 64 consecutive voxels.  The next iteration's block ids and keys are
 // loaded before the current blocks are processed.
 constexpr int kIntBlocks = 2;
+constexpr int kIntThreads = 128 * kIntBlocks;  // the launch size the lane mapping assumes
 template <int F, bool kPacked>
 __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const FuseArgs& a) {
     using IO = VoxelIO<F>;
@@ -493,7 +494,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     }
 }
 template <int F, bool kPacked>
-__global__ __launch_bounds__(256) void k_integrate(MapView m, FuseArgs a) {
+__global__ __launch_bounds__(kIntThreads) void k_integrate(MapView m, FuseArgs a) {
     integrate_body<F, kPacked>(hw_wg(), m, a);
 }
 
@@ -511,9 +512,9 @@ __global__ __launch_bounds__(256) void k_fuse_pack(FuseArgs a) { fuse_pack_body(
 template <bool kPacked>
 static void integrate_f(const MapView& m, const FuseArgs& a, unsigned grid, hipStream_t s) {
     switch (m.fmt) {
-        case VF_F32_W8: hipLaunchKernelGGL((k_integrate<VF_F32_W8, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
-        case VF_F32_W16: hipLaunchKernelGGL((k_integrate<VF_F32_W16, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
-        default: hipLaunchKernelGGL((k_integrate<VF_F16_W8, kPacked>), dim3(grid), dim3(256), 0, s, m, a); break;
+        case VF_F32_W8: hipLaunchKernelGGL((k_integrate<VF_F32_W8, kPacked>), dim3(grid), dim3(kIntThreads), 0, s, m, a); break;
+        case VF_F32_W16: hipLaunchKernelGGL((k_integrate<VF_F32_W16, kPacked>), dim3(grid), dim3(kIntThreads), 0, s, m, a); break;
+        default: hipLaunchKernelGGL((k_integrate<VF_F16_W8, kPacked>), dim3(grid), dim3(kIntThreads), 0, s, m, a); break;
     }
 }
 void launch_fuse_pack(const FuseArgs& a, hipStream_t s) {
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(256) void kb_visible(BatchLaunch b) {
     visible_body(g, m, d.vis);
 }
 template <int F>
-__global__ __launch_bounds__(256) void kb_integrate(BatchLaunch b) {
+__global__ __launch_bounds__(kIntThreads) void kb_integrate(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, batch_wg(b), g);
     const SessDesc& d = b.desc[b.sess[s]];
@@ -597,9 +598,9 @@ void launch_b_integrate(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t 
     const int grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().integrate_groups, b.n, 256, 8192));
     b.xcd = xcd_order(FU_STAGE_INTEGRATE);
     switch (fmt) {
-        case VF_F32_W8: hipLaunchKernelGGL(kb_integrate<VF_F32_W8>, dim3(grid), dim3(256), 0, s, b); break;
-        case VF_F32_W16: hipLaunchKernelGGL(kb_integrate<VF_F32_W16>, dim3(grid), dim3(256), 0, s, b); break;
-        default: hipLaunchKernelGGL(kb_integrate<VF_F16_W8>, dim3(grid), dim3(256), 0, s, b); break;
+        case VF_F32_W8: hipLaunchKernelGGL(kb_integrate<VF_F32_W8>, dim3(grid), dim3(kIntThreads), 0, s, b); break;
+        case VF_F32_W16: hipLaunchKernelGGL(kb_integrate<VF_F32_W16>, dim3(grid), dim3(kIntThreads), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_integrate<VF_F16_W8>, dim3(grid), dim3(kIntThreads), 0, s, b); break;
     }
 }
 
