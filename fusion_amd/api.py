@@ -19,8 +19,15 @@ from . import native as N
 from .native import check
 
 
+def _addr(t):
+    """device address of a dense tensor (the library reads images as dense row-major arrays)"""
+    if not t.is_contiguous():
+        raise N.FusionError("tensor of shape %s is not contiguous; pass .contiguous()" % (tuple(t.shape),))
+    return t.data_ptr()
+
+
 def _p(t):
-    return C.c_void_p(t.data_ptr()) if t is not None else None
+    return C.c_void_p(_addr(t)) if t is not None else None
 
 
 def _s():
@@ -313,7 +320,7 @@ class Tracker:
 
     def ComputeSE3(self, next_vmaps, next_nmaps, last_vmaps, last_nmaps, K, iters=(10, 5, 4),
                    min_count=(2000, 1000, 100), thresh=1e-4):
-        arr = lambda ts: (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+        arr = lambda ts: (C.c_void_p * 3)(*[_addr(t) for t in ts])
         check(N.lib().fu_compute_se3(self.handle, arr(next_vmaps), arr(next_nmaps), arr(last_vmaps),
                                      arr(last_nmaps), (C.c_float * 4)(*K), (C.c_int32 * 3)(*iters),
                                      (C.c_int32 * 3)(*min_count), thresh, _s()), "fu_compute_se3")
@@ -322,7 +329,7 @@ class Tracker:
     def ComputeSE3RGBD(self, next_vmaps, next_nmaps, last_vmaps, last_nmaps, next_images, last_images, dIdx, dIdy, K,
                        iters=(10, 5, 4), min_count=(2000, 1000, 100), thresh=1e-4):
         """ComputeSE3(icpOnly=false): the photometric term blended into every iteration"""
-        arr = lambda ts: (C.c_void_p * 3)(*[t.data_ptr() for t in ts])
+        arr = lambda ts: (C.c_void_p * 3)(*[_addr(t) for t in ts])
         check(N.lib().fu_compute_se3_rgbd(self.handle, arr(next_vmaps), arr(next_nmaps), arr(last_vmaps),
                                           arr(last_nmaps), arr(next_images), arr(last_images), arr(dIdx), arr(dIdy),
                                           (C.c_float * 4)(*K), (C.c_int32 * 3)(*iters), (C.c_int32 * 3)(*min_count),
@@ -358,7 +365,7 @@ class IcpBatch:
                    min_count=(2000, 1000, 100), thresh=1e-4):
         """per pair: 3-level lists of device maps and (fx, fy, cx, cy); returns each tracker's result"""
         n = len(trackers)
-        arr = lambda lists: (C.c_void_p * (3 * n))(*[t.data_ptr() for ts in lists for t in ts])
+        arr = lambda lists: (C.c_void_p * (3 * n))(*[_addr(t) for ts in lists for t in ts])
         Kf = (C.c_float * (4 * n))(*[float(v) for K in Ks for v in K])
         check(N.lib().fu_compute_se3_batch(self.handle, n, (C.c_void_p * n)(*[t.handle.value for t in trackers]),
                                            arr(next_vmaps), arr(next_nmaps), arr(last_vmaps), arr(last_nmaps), Kf,
@@ -546,8 +553,8 @@ class Batch:
         n = len(systems)
         arr = C.c_void_p * n
         sy = arr(*[s.handle.value for s in systems])
-        dp = arr(*[d.data_ptr() for d in depths])
-        cp = arr(*[c.data_ptr() for c in rgbs])
+        dp = arr(*[_addr(d) for d in depths])
+        cp = arr(*[_addr(c) for c in rgbs])
         check(N.lib().fu_batch_grab(self.handle, n, sy, dp, cp, _s()), "fu_batch_grab")
 
 
@@ -651,9 +658,9 @@ class Server:
             if c is None:
                 v, nm, col = d
                 tr[i].kind, tr[i].max_triangles = 1, v.shape[0]
-                tr[i].vertex, tr[i].normal, tr[i].color = v.data_ptr(), nm.data_ptr(), col.data_ptr()
+                tr[i].vertex, tr[i].normal, tr[i].color = _addr(v), _addr(nm), _addr(col)
             else:
-                tr[i].depth, tr[i].rgb = d.data_ptr(), c.data_ptr()
+                tr[i].depth, tr[i].rgb = _addr(d), _addr(c)
         out = (N.Completion * n)()
         check(N.lib().fu_server_run_trace(self.handle, tr, n, int(closed_loop), out), "fu_server_run_trace")
         return [_completion(out[i]) for i in range(n)]

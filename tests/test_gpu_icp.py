@@ -43,6 +43,41 @@ def test_icp_step_sums(gpu):
     assert np.abs(x_g - x_o).max() < 1e-6
 
 
+def _plane_maps(cols, rows, K, n=(0.1, -0.2, -1.0), c=-2.0, seed=0):
+    """vertex and normal maps of the plane n.p = c seen from the origin, with a mild ripple"""
+    fx, fy, cx, cy = K
+    n = np.asarray(n, np.float64)
+    n /= np.linalg.norm(n)
+    x = (np.arange(cols) - cx) / fx
+    y = (np.arange(rows) - cy) / fy
+    d = np.stack(np.broadcast_arrays(x[None, :], y[:, None], np.ones((1, 1))), -1)
+    t = c / (d @ n)
+    t *= 1.0 + 0.002 * np.sin(np.arange(cols) / 37.0)[None, :] * np.cos(np.arange(rows) / 23.0)[:, None]
+    v = np.concatenate([d * t[..., None], np.ones((rows, cols, 1))], -1)
+    nm = np.concatenate([np.broadcast_to(n, (rows, cols, 3)), np.ones((rows, cols, 1))], -1)
+    return np.ascontiguousarray(v, np.float32), np.ascontiguousarray(nm, np.float32)
+
+
+def test_icp_step_sums_chunk_loop(gpu):
+    """a frame of more pixels than 4096 work-groups of two pixels per lane cover (8.6M) takes
+    the per-lane chunk loop with compensated sums (k_icp<false>); its sums match FP64 and the
+    correspondence count is exact"""
+    cols, rows = 4096, 2112
+    K = (2000.0, 2000.0, cols / 2 - 0.5, rows / 2 - 0.5)
+    v, nm = _plane_maps(cols, rows, K)
+    pc = np.eye(4)
+    pc[:3, 3] = (0.002, -0.001, 0.003)
+    cu = lambda a: torch.from_numpy(a).cuda()
+    print("icp chunk loop %dx%d" % (cols, rows), flush=True)
+    A_g, b_g, r_g = A.ICPStep(cu(v), cu(v), cu(nm), cu(nm), pc, np.eye(4), K)
+    s32, s64 = O.icp_sums(v, v, nm, nm, pc, np.eye(4), K)
+    A_o, b_o = O.unpack_sums(s64)
+    assert r_g[1] == s64[28] and s64[28] > 0.9 * cols * rows
+    scale = np.abs(A_o).max()
+    assert np.abs(A_g - A_o).max() <= 1e-6 * scale
+    assert np.abs(b_g - b_o).max() <= 1e-6 * max(np.abs(b_o).max(), 1e-12) + 1e-9 * scale
+
+
 @pytest.mark.parametrize("cols,rows,f1", [(640, 480, 2), (320, 240, 3), (1280, 720, 1)])
 def test_compute_se3_vs_fp64(gpu, cols, rows, f1):
     print("icp %dx%d" % (cols, rows), flush=True)

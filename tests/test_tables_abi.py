@@ -107,3 +107,16 @@ def test_map_bytes_scales_with_format():
         sizes.append(lib.fu_map_bytes(C.byref(p)))
     # per block: 4096 B (f32+u8+rgb), 4608 B (f32+u16+rgb), 3072 B (f16+u8+rgb)
     assert sizes[1] - sizes[0] == 1000 * 512 and sizes[0] - sizes[2] == 1000 * 1024
+
+
+def test_api_rejects_non_contiguous_tensors():
+    """the library reads images as dense row-major arrays: a strided view is refused before any
+    call instead of being read with the wrong layout"""
+    import torch
+    from fusion_amd import api as A
+    t = torch.zeros((4, 6, 4), dtype=torch.float32)
+    assert A._addr(t) == t.data_ptr()
+    with pytest.raises(N.FusionError):
+        A._addr(t.transpose(0, 1))
+    with pytest.raises(N.FusionError):
+        A._p(t[:, ::2])
