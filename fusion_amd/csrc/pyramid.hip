@@ -221,13 +221,29 @@ __device__ __forceinline__ void derivative_body(WG g, const unsigned char* __res
     if (x >= cols || y >= rows) return;
     const auto img = FU_G(const unsigned char, img_);
     float a = 0.f, b = 0.f;
-    int k = 8;
-    for (int j = max(y - 1, 0); j <= min(y + 1, rows - 1); ++j) {
-        for (int i = max(x - 1, 0); i <= min(x + 1, cols - 1); ++i) {
-            const float v = (float)img[j * cols + i];
-            a += v * c_sobel_x[k];
-            b += v * c_sobel_y[k];
-            --k;
+    if (x >= 1 && y >= 1 && x + 1 < cols && y + 1 < rows) {
+        // the whole 3x3 window: its nine loads issue together, the same products and sums in
+        // the same order as the clipped loop below (k = 8 .. 0)
+        constexpr float sx[9] = {1, 0, -1, 2, 0, -2, 1, 0, -1};
+        constexpr float sy[9] = {1, 2, 1, 0, 0, 0, -1, -2, -1};
+        const auto p = img + (y - 1) * cols + (x - 1);
+        float v[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[q] = (float)p[(q / 3) * cols + q % 3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            a += v[q] * sx[8 - q];
+            b += v[q] * sy[8 - q];
+        }
+    } else {
+        int k = 8;
+        for (int j = max(y - 1, 0); j <= min(y + 1, rows - 1); ++j) {
+            for (int i = max(x - 1, 0); i <= min(x + 1, cols - 1); ++i) {
+                const float v = (float)img[j * cols + i];
+                a += v * c_sobel_x[k];
+                b += v * c_sobel_y[k];
+                --k;
+            }
         }
     }
     FU_G(short, dx_)[y * cols + x] = (short)f2i(a);
