@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void k_filter_depth(const uint16_t* __restrict
 // binomial [1 4 6 4 1] outer product, indexed (ty-cy-1)*5 + (tx-cx-1) as in Pyrdown.cu:151,
 // which mirrors the weights at the right/bottom border
 __constant__ float c_gauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
+constexpr float kGauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
 
 __device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src_, int scols, int srows,
                                              float* __restrict__ dst_, int dcols, int drows) {
@@ -65,13 +66,28 @@ __device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src
     const int ty = min(2 * y + 3, srows - 1);
     float sum = 0.f;
     int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const float v = src[cy * scols + cx];
-            if (!isnanf_(v)) {
-                const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
-                sum += v * g;
-                count += (int)g;
+    if (x >= 1 && y >= 1 && 2 * x + 3 < scols && 2 * y + 3 < srows) {
+        // the whole 5x5 window: its loads issue together; NaN samples are skipped as below
+        const auto p = src + (2 * y - 2) * scols + (2 * x - 2);
+        float v[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) v[q] = p[(q / 5) * scols + q % 5];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) {
+            const float g = kGauss[(4 - q / 5) * 5 + (4 - q % 5)];
+            const bool ok = !isnanf_(v[q]);
+            sum = ok ? sum + v[q] * g : sum;
+            count += ok ? (int)g : 0;
+        }
+    } else {
+        for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
+            for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+                const float v = src[cy * scols + cx];
+                if (!isnanf_(v)) {
+                    const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                    sum += v * g;
+                    count += (int)g;
+                }
             }
         }
     }
@@ -195,11 +211,26 @@ __device__ __forceinline__ void pyrdown_u8_body(WG g, const unsigned char* __res
     const int ty = min(2 * y + 3, srows - 1);
     float sum = 0.f;
     int count = 0;
-    for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
-        for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
-            const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
-            sum += (float)src[cy * scols + cx] * g;
+    if (x >= 1 && y >= 1 && 2 * x + 3 < scols && 2 * y + 3 < srows) {
+        // the whole 5x5 window: its loads issue together, the same products and sums in the
+        // same order as the clipped loop below
+        const auto p = src + (2 * y - 2) * scols + (2 * x - 2);
+        float v[25];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) v[q] = (float)p[(q / 5) * scols + q % 5];
+#pragma unroll
+        for (int q = 0; q < 25; ++q) {
+            const float g = kGauss[(4 - q / 5) * 5 + (4 - q % 5)];
+            sum += v[q] * g;
             count += (int)g;
+        }
+    } else {
+        for (int cy = max(0, 2 * y - 2); cy < ty; ++cy) {
+            for (int cx = max(0, 2 * x - 2); cx < tx; ++cx) {
+                const float g = c_gauss[(ty - cy - 1) * 5 + (tx - cx - 1)];
+                sum += (float)src[cy * scols + cx] * g;
+                count += (int)g;
+            }
         }
     }
     dst[y * dcols + x] = (unsigned char)f2i(sum / (float)count);
