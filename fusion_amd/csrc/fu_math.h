@@ -121,15 +121,13 @@ FU_HD bool isnanf_(float v) { return !(v == v); }
 // Taylor polynomial for e^r in Horner form (truncation error about 5e-9 relative on the
 // interval, below float rounding), and scaling by 2^k built in the exponent field.
 // Accurate to about 1 ulp against a correctly rounded exp on [-87, 88].
-FU_HD float fu_expf(float x) {
-    if (!(x == x)) return x;
-    if (x > 88.5f) return __builtin_inff();
-    if (x < -103.0f) return 0.0f;
+// the reduction and polynomial of fu_expf: x = k*ln2 + r, p = e^r in [2^-1, 2^1)
+FU_HD float fu_expf_poly(float x, int& k) {
     const float kLog2e = 1.44269502162933349609375f;
     const float kLn2Hi = 0.693145751953125f;         // 16 significant bits: k*kLn2Hi exact
     const float kLn2Lo = 1.428606765330187045e-06f;  // ln2 - kLn2Hi
     float kf = rintf(x * kLog2e);
-    int k = (int)kf;
+    k = (int)kf;
     float r = (x - kf * kLn2Hi) - kf * kLn2Lo;
     float p = 1.9841270e-4f;              // 1/5040
     p = p * r + 1.3888889e-3f;            // 1/720
@@ -139,12 +137,34 @@ FU_HD float fu_expf(float x) {
     p = p * r + 0.5f;
     p = p * r + 1.0f;
     p = p * r + 1.0f;
+    return p;
+}
+FU_HD float fu_expf(float x) {
+    if (!(x == x)) return x;
+    if (x > 88.5f) return __builtin_inff();
+    if (x < -103.0f) return 0.0f;
+    int k;
+    const float p = fu_expf_poly(x, k);
     // 2^k in two steps so that k down to -149 and up to 128 stay representable
     int k1 = k / 2, k2 = k - k1;
     union { uint32_t u; float f; } s1, s2;
     s1.u = (uint32_t)(k1 + 127) << 23;
     s2.u = (uint32_t)(k2 + 127) << 23;
     return (p * s1.f) * s2.f;
+}
+// fu_expf for x <= 0 that is not NaN (the bilateral weights), without the NaN and overflow
+// tests and with the scaling in one exponent-field add and one multiply: k is in [-149, 0],
+// p * 2^(k+64) is a normal float (exact) and the multiply by 2^-64 rounds once, as the
+// second multiply of fu_expf does, so both give the same bits
+FU_HD float fu_expf_nonpos(float x) {
+    int k;
+    const float p = fu_expf_poly(fmaxf(x, -103.0f), k);
+    union { uint32_t u; float f; } s;
+    union { float f; uint32_t u; } pb;
+    pb.f = p;
+    s.u = pb.u + ((uint32_t)(k + 64) << 23);
+    const float e = s.f * 0x1p-64f;
+    return x < -103.0f ? 0.0f : e;
 }
 
 // ---------------------------------------------------------------------------------------

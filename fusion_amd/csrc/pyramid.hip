@@ -36,7 +36,8 @@ __device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restri
             const float val = (float)depth[cy * cols + cx] * scale_inv;
             const float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
             const float color2 = (center - val) * (center - val);
-            const float weight = fu_expf(-(space2 * kSigSpace + color2 * kSigRange));
+            // the argument is never positive or NaN: depths are finite
+            const float weight = fu_expf_nonpos(-(space2 * kSigSpace + color2 * kSigRange));
             sum1 += val * weight;
             sum2 += weight;
         }

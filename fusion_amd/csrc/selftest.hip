@@ -2,7 +2,8 @@
 // fu_div_refined, fu_sqrt_normal, fu_drcp_from_float) against the compiler's IEEE division and square root on
 // the device itself: the kernels rely on them being bit-identical inside their stated ranges.
 // It also checks the single-instruction float/double -> int conversions (fu_math.h: f2i, d2i)
-// against the rule they implement (truncation, saturation, NaN -> 0).
+// against the rule they implement (truncation, saturation, NaN -> 0), and the bilateral
+// weights' exp (fu_expf_nonpos) against fu_expf.
 #include <hip/hip_runtime.h>
 
 #include "fu_api.h"
@@ -40,7 +41,7 @@ __device__ __forceinline__ float rnd(uint64_t& s, int lo, int hi, bool sign) {
 }
 
 __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
-    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0;
+    unsigned b0 = 0, b1 = 0, b2 = 0, b3 = 0, b4 = 0, b5 = 0, b6 = 0;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t s = mix(seed ^ ((uint64_t)i * 0x2545f4914f6cdd1dull));
         // 1: quotients by a 4-D norm: numerators in [2^-60, 2^30], denominators in [1, 2^31]
@@ -79,6 +80,15 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             b5 += d2i(d) != conv_rule(d);
             b5 += d2i((double)f) != conv_rule((double)f);
         }
+        // 7: exp of a non-positive argument, uniform over [-110, 0] (subnormal and zero results
+        // included) and over the exponents of small arguments
+        {
+            s = mix(s + 0x9e3779b97f4a7c15ull);
+            const float u = -110.0f * (float)((s >> 40) & 0xffffff) * 0x1p-24f;
+            const float t = -rnd(s, -30, 6, false);
+            b6 += __float_as_uint(fu_expf_nonpos(u)) != __float_as_uint(fu_expf(u));
+            b6 += __float_as_uint(fu_expf_nonpos(t)) != __float_as_uint(fu_expf(t));
+        }
         // 5: FP64 reciprocals of nonzero finite floats (denormal ones included)
         {
             float z = rnd(s, -126, 127, true);
@@ -89,6 +99,7 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             }
         }
     }
+    if (b6) atomicAdd(&bad[6], b6);
     if (b5) atomicAdd(&bad[5], b5);
     if (b0) atomicAdd(&bad[0], b0);
     if (b1) atomicAdd(&bad[1], b1);
@@ -99,17 +110,17 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
 
 }  // namespace
 
-extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[6], fu_stream s) {
+extern "C" fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[7], fu_stream s) {
     if (!mismatches) return api_error(FU_ERR_ARG, "fu_selftest_arith: null output");
     unsigned* d = nullptr;
-    if (hipMalloc(&d, 24) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
+    if (hipMalloc(&d, 28) != hipSuccess) return api_error(FU_ERR_NOMEM, "fu_selftest_arith: allocation failed");
     hipStream_t hs = (hipStream_t)s;
-    hipError_t e = hipMemsetAsync(d, 0, 24, hs);
+    hipError_t e = hipMemsetAsync(d, 0, 28, hs);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_selftest, dim3(4096), dim3(256), 0, hs, seed, n, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 24, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 28, hipMemcpyDeviceToHost, hs);
     if (e == hipSuccess) e = hipStreamSynchronize(hs);
     (void)hipFree(d);
     return e == hipSuccess ? FU_OK : api_error(FU_ERR_HIP, "fu_selftest_arith: %s", hipGetErrorString(e));

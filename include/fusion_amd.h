@@ -310,8 +310,9 @@ fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
  * mismatches[0] quotients by a 4-D norm, [1] sdf / truncation, [2] projections (their
  * truncated pixel coordinate), [3] square roots, [4] FP64 reciprocals of floats, [5] the
  * single-instruction float and double -> int conversions against their rule (truncation,
- * saturation, NaN -> 0).  All six must be 0.  Synchronises. */
-fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[6], fu_stream s);
+ * saturation, NaN -> 0), [6] the bilateral weights' exp against the general one.  All seven
+ * must be 0.  Synchronises. */
+fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[7], fu_stream s);
 
 /* ---------------------------------------------------------------- on-disk formats */
 /* Map snapshot: System::WriteMapToDisk / ReadMapFromDisk (Core/System.cc:195-271).  The
