@@ -8,7 +8,7 @@
  * built with -ffp-contract=off, so single-precision results are those of a strict IEEE
  * evaluation of the reference's expressions.
  *
- * Defined behaviour where the reference has none (DEVIATIONS, also listed in DESIGN.md):
+ * Defined behaviour where the reference has none (DEVIATIONS):
  *  D1 float->int conversions saturate and map NaN to 0 (the GPU conversion instruction).
  *  D2 the block hash reduces the wrapped int products modulo the *unsigned* bucket count
  *     (DeviceMap.cu:3-10: NumBuckets is `uint`).

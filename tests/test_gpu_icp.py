@@ -1,6 +1,6 @@
 """GPU ICP (Tracking/Reduction.cu, Tracking/Tracking.cc:296-409) against the FP64 oracle.
 
-Bar (north star): each ICP step's pose update within 1e-5 rad and 1e-5 m of the FP64
+Bar: each ICP step's pose update within 1e-5 rad and 1e-5 m of the FP64
 oracle on the same inputs; the reduced normal equations within FP32 summation error.
 """
 import numpy as np
