@@ -18,7 +18,7 @@ capacity); `latency_ms` is submit-to-result latency with 30 fps Poisson arrivals
 `plain` runs the same kind of session one frame at a time with
 the reference's host control (host solve per ICP iteration, synchronisation per stage).
 
-Prints one JSON line.  --gpus N > 1 is refused: this build targets one GPU.
+Prints one JSON line.  --gpus N > 1 is refused: the library and the benchmark run on one GPU.
 """
 import argparse
 import json
@@ -455,8 +455,8 @@ def cpu_baseline(A, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
-              "(--gpus %d refused)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (sessions would shard across "
+              "GPUs by session) (--gpus %d refused)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.probe_traffic:
         probe_traffic(args)
