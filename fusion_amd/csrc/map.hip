@@ -110,6 +110,8 @@ __device__ __forceinline__ void claim_block(const MapView& m, bool want, unsigne
 // at once by its lane; the global per-pass set de-duplicates across lanes and groups.
 constexpr int kAllocSlots = 2048;
 constexpr int kAllocProbes = 8;
+constexpr int kAllocThreads = 256;  // 16x16 pixels per group
+static_assert(kAllocSlots % kAllocThreads == 0, "whole slot rows per lane");
 __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseArgs& p) {
     __shared__ unsigned long long set[kAllocSlots];
     PoseF pose = p.pose;
@@ -163,12 +165,27 @@ __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseA
         if (!done) claim_block(m, true, key, b);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kAllocSlots; i += blockDim.x) {
-        const unsigned long long key = set[i];
+    // the group's keys (a few dozen in a 16x16 tile) move to the front of the set, so the
+    // table probes and claims run in as few wave passes as there are keys, not one per
+    // occupied 64-slot row; the order of the claims does not matter
+    __shared__ unsigned nkeys;
+    unsigned long long held[kAllocSlots / kAllocThreads];
+#pragma unroll
+    for (int k = 0; k < kAllocSlots / kAllocThreads; ++k) held[k] = set[threadIdx.x + k * kAllocThreads];
+    if (threadIdx.x == 0) nkeys = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kAllocSlots / kAllocThreads; ++k)
+        if (held[k] != kEmptyKey) set[atomicAdd(&nkeys, 1u)] = held[k];
+    __syncthreads();
+    const int n = (int)nkeys;  // uniform: every lane runs the same passes
+    for (int base = 0; base < n; base += kAllocThreads) {
+        const int i = base + (int)threadIdx.x;
+        const unsigned long long key = i < n ? set[i] : kEmptyKey;
         claim_block(m, key != kEmptyKey, key, unpack_key(key));
     }
 }
-__global__ __launch_bounds__(256) void k_alloc_pass_a(MapView m, FuseArgs p) { alloc_a_body(hw_wg(), m, p); }
+__global__ __launch_bounds__(kAllocThreads) void k_alloc_pass_a(MapView m, FuseArgs p) { alloc_a_body(hw_wg(), m, p); }
 
 // ------------------------------------------------------------ allocation pass B (per new key)
 // one wave per new block: lane 0 links the key, all lanes initialise the block's voxels
@@ -255,7 +272,7 @@ __device__ __forceinline__ void alloc_b_body(WG g, const MapView& m, const Track
 __global__ __launch_bounds__(256) void k_alloc_pass_b(MapView m, const TrackState* st) { alloc_b_body(hw_wg(), m, st); }
 
 void launch_alloc(const MapView& m, const FuseArgs& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_alloc_pass_a, dim3((p.cols + 15) / 16, (p.rows + 15) / 16), dim3(256), 0, s, m, p);
+    hipLaunchKernelGGL(k_alloc_pass_a, dim3((p.cols + 15) / 16, (p.rows + 15) / 16), dim3(kAllocThreads), 0, s, m, p);
     hipLaunchKernelGGL(k_alloc_pass_b, dim3(512), dim3(256), 0, s, m, p.st);
 }
 
@@ -539,7 +556,7 @@ __device__ __forceinline__ FuseArgs slot_fuse(const BatchLaunch& b, const SessDe
     return a;
 }
 
-__global__ __launch_bounds__(256) void kb_alloc_a(BatchLaunch b) {
+__global__ __launch_bounds__(kAllocThreads) void kb_alloc_a(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
@@ -586,7 +603,7 @@ void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     int grid = batch_layout(b, h, BT_PIX256, 0, 0);
     hipLaunchKernelGGL(kb_fuse_pack, dim3(grid), dim3(256), 0, s, b);
     grid = batch_layout(b, h, BT_TILES, 0, 0);
-    hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(kAllocThreads), 0, s, b);
     grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().alloc_groups, b.n, 16, 512));
     hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);
 }
