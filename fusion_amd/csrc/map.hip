@@ -108,7 +108,7 @@ __device__ __forceinline__ void claim_block(const MapView& m, bool want, unsigne
 // the group's rays share is probed and claimed once, by the lanes that scan the set
 // afterwards.  A block that finds no free LDS slot within kAllocProbes is probed and claimed
 // at once by its lane; the global per-pass set de-duplicates across lanes and groups.
-constexpr int kAllocSlots = 2048;
+constexpr int kAllocSlots = 512;  // a 16x16 tile holds a few dozen blocks
 constexpr int kAllocProbes = 8;
 constexpr int kAllocThreads = 256;  // 16x16 pixels per group
 static_assert(kAllocSlots % kAllocThreads == 0, "whole slot rows per lane");
