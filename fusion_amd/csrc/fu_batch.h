@@ -162,7 +162,6 @@ struct FrameBrief {
 void launch_b_resize(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
 void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
-void launch_b_vmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 // intensity pyramid and derivatives (sessions with the photometric term only), and the
 // post-tracking copy of the pyramid into the last frame's

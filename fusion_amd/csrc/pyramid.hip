@@ -15,9 +15,24 @@ namespace fu {
 constexpr float kSigSpace = 0.5f / 16.0f;
 constexpr float kSigRange = 2.0f;
 
+// ComputeVMapKernel's vertex of pixel (x, y) with depth z (Pyrdown.cu:56-78)
+__device__ __forceinline__ f4 vertex_of(float z, int x, int y, float invfx, float invfy, float cx, float cy,
+                                        float cutoff) {
+    // `z > 0.1` compares in double (Pyrdown.cu:69)
+    if (!isnanf_(z) && (double)z > 0.1 && z < cutoff)
+        return mk4(z * ((float)x - cx) * invfx, z * ((float)y - cy) * invfy, z, 1.0f);
+    // the reference sets only x = NaN; every consumer tests x alone
+    return mk4(fnan(), fnan(), fnan(), fnan());
+}
+// where a depth kernel also writes the vertex map of its level (null vmap: it does not)
+struct VertexOut {
+    f4* vmap;
+    float invfx, invfy, cx, cy;
+};
+
 __device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restrict__ depth_, float* __restrict__ raw_,
                                                   float* __restrict__ filt_, int cols, int rows, float scale_inv,
-                                                  float cutoff) {
+                                                  float cutoff, VertexOut vo = VertexOut{}) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
@@ -43,7 +58,9 @@ __device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restri
         }
     }
     const float fin = sum1 / sum2;
-    filt[y * cols + x] = fin < cutoff ? fin : fnan();
+    const float z = fin < cutoff ? fin : fnan();
+    filt[y * cols + x] = z;
+    if (vo.vmap) FU_G(f4, vo.vmap)[y * cols + x] = vertex_of(z, x, y, vo.invfx, vo.invfy, vo.cx, vo.cy, cutoff);
 }
 __global__ __launch_bounds__(256) void k_filter_depth(const uint16_t* __restrict__ depth, float* __restrict__ raw,
                                                      float* __restrict__ filt, int cols, int rows, float scale_inv,
@@ -57,7 +74,8 @@ __constant__ float c_gauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24
 constexpr float kGauss[25] = {1, 4, 6, 4, 1, 4, 16, 24, 16, 4, 6, 24, 36, 24, 6, 4, 16, 24, 16, 4, 1, 4, 6, 4, 1};
 
 __device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src_, int scols, int srows,
-                                             float* __restrict__ dst_, int dcols, int drows) {
+                                             float* __restrict__ dst_, int dcols, int drows,
+                                             VertexOut vo = VertexOut{}, float cutoff = 0.f) {
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= dcols || y >= drows) return;
@@ -92,7 +110,9 @@ __device__ __forceinline__ void pyrdown_body(WG g, const float* __restrict__ src
             }
         }
     }
-    dst[y * dcols + x] = sum / (float)count;
+    const float z = sum / (float)count;
+    dst[y * dcols + x] = z;
+    if (vo.vmap) FU_G(f4, vo.vmap)[y * dcols + x] = vertex_of(z, x, y, vo.invfx, vo.invfy, vo.cx, vo.cy, cutoff);
 }
 __global__ __launch_bounds__(256) void k_pyrdown(const float* __restrict__ src, int scols, int srows,
                                                 float* __restrict__ dst, int dcols, int drows) {
@@ -106,19 +126,7 @@ __device__ __forceinline__ void vmap_body(WG g, const float* __restrict__ depth_
     if (x >= cols || y >= rows) return;
     const auto depth = FU_G(const float, depth_);
     const auto vmap = FU_G(f4, vmap_);
-    const float z = depth[y * cols + x];
-    f4 v;
-    // `z > 0.1` compares in double (Pyrdown.cu:69)
-    if (!isnanf_(z) && (double)z > 0.1 && z < cutoff) {
-        v.x = z * ((float)x - cx) * invfx;
-        v.y = z * ((float)y - cy) * invfy;
-        v.z = z;
-        v.w = 1.0f;
-    } else {
-        // the reference sets only x = NaN; every consumer tests x alone
-        v = mk4(fnan(), fnan(), fnan(), fnan());
-    }
-    vmap[y * cols + x] = v;
+    vmap[y * cols + x] = vertex_of(depth[y * cols + x], x, y, invfx, invfy, cx, cy, cutoff);
 }
 __global__ __launch_bounds__(256) void k_vmap(const float* __restrict__ depth, f4* __restrict__ vmap, int cols,
                                              int rows, float invfx, float invfy, float cx, float cy, float cutoff) {
@@ -379,7 +387,8 @@ __global__ __launch_bounds__(256) void kb_filter(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
-    filter_depth_body(g, (const uint16_t*)b.in[s], d.raw, d.depth[0], d.cols, d.rows, d.depth_scale_inv, d.cutoff);
+    filter_depth_body(g, (const uint16_t*)b.in[s], d.raw, d.depth[0], d.cols, d.rows, d.depth_scale_inv, d.cutoff,
+                      VertexOut{d.vmap[0], d.invf[0][0], d.invf[0][1], d.Kl[0][2], d.Kl[0][3]});
 }
 
 __global__ __launch_bounds__(256) void kb_pyrdown(BatchLaunch b) {
@@ -387,7 +396,8 @@ __global__ __launch_bounds__(256) void kb_pyrdown(BatchLaunch b) {
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
     const int l = b.level;
-    pyrdown_body(g, d.depth[l - 1], d.lc[l - 1], d.lr[l - 1], d.depth[l], d.lc[l], d.lr[l]);
+    pyrdown_body(g, d.depth[l - 1], d.lc[l - 1], d.lr[l - 1], d.depth[l], d.lc[l], d.lr[l],
+                 VertexOut{d.vmap[l], d.invf[l][0], d.invf[l][1], d.Kl[l][2], d.Kl[l][3]}, d.cutoff);
 }
 
 // one launch over the tiles of all three levels; `local` -> (level, tile)
@@ -405,14 +415,6 @@ __device__ __forceinline__ int pyr3_level(const SessDesc& d, int local, WG& g) {
     g.nx = tx;
     g.ny = (d.lr[l] + 15) / 16;
     return l;
-}
-
-__global__ __launch_bounds__(256) void kb_vmap(BatchLaunch b) {
-    WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
-    const SessDesc& d = b.desc[b.sess[s]];
-    const int l = pyr3_level(d, g.x, g);
-    vmap_body(g, d.depth[l], d.vmap[l], d.lc[l], d.lr[l], d.invf[l][0], d.invf[l][1], d.Kl[l][2], d.Kl[l][3], d.cutoff);
 }
 
 __global__ __launch_bounds__(256) void kb_nmap(BatchLaunch b) {
@@ -470,9 +472,6 @@ void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
 void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s) {
     b.level = level;
     blaunch(kb_pyrdown, batch_layout(b, h, BT_TILES, level, 0), b, s);
-}
-void launch_b_vmap(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
-    blaunch(kb_vmap, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
 }
 void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     blaunch(kb_nmap, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
