@@ -1672,8 +1672,7 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     mark(0);
     for (int i = 0; i < n; ++i) b.in[i] = depth[i];
     if (track) {
-        launch_b_resize(b, h, 1, hs);
-        launch_b_resize(b, h, 2, hs);
+        launch_b_resize(b, h, hs);
     }
     mark(FU_STAGE_RESIZE + 1);
     launch_b_filter(b, h, hs);

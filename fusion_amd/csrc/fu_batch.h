@@ -159,7 +159,7 @@ struct FrameBrief {
 };
 
 // pyramid.hip
-void launch_b_resize(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
+void launch_b_resize(BatchLaunch& b, const SessDesc* h, hipStream_t s);  // levels 1 and 2
 void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);
 void launch_b_nmap(BatchLaunch& b, const SessDesc* h, hipStream_t s);

@@ -159,7 +159,20 @@ __global__ __launch_bounds__(256) void k_nmap(const f4* __restrict__ vmap, f4* _
 }
 
 // 2x2 mean of the vertex map, 2x2 mean of the normal map normalised in 4-D (w included,
-// VectorMath.h:340-342), NaN if any of the four is NaN
+// VectorMath.h:340-342), NaN if any of the four is NaN; sources in the order (2y, 2x),
+// (2y, 2x+1), (2y+1, 2x), (2y+1, 2x+1)
+__device__ __forceinline__ void resize_px(f4 v00, f4 v01, f4 v10, f4 v11, f4 n00, f4 n01, f4 n10, f4 n11, f4& vo,
+                                          f4& no) {
+    const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
+    if (isnanf_(v00.x) || isnanf_(v01.x) || isnanf_(v10.x) || isnanf_(v11.x))
+        vo = nan4;
+    else
+        vo = divs4(add4(add4(add4(v00, v01), v10), v11), 4.0f);
+    if (isnanf_(n00.x) || isnanf_(n01.x) || isnanf_(n10.x) || isnanf_(n11.x))
+        no = nan4;
+    else
+        no = normalised4(divs4(add4(add4(add4(n00, n01), n10), n11), 4.0f));
+}
 __device__ __forceinline__ void resize_body(WG g, const f4* __restrict__ vsrc_, const f4* __restrict__ nsrc_, int scols,
                                             f4* __restrict__ vdst_, f4* __restrict__ ndst_, int dcols, int drows) {
     const int x = g.x * 16 + (threadIdx.x & 15);
@@ -170,17 +183,10 @@ __device__ __forceinline__ void resize_body(WG g, const f4* __restrict__ vsrc_, 
     const auto vdst = FU_G(f4, vdst_);
     const auto ndst = FU_G(f4, ndst_);
     const int s0 = (2 * y) * scols + 2 * x, s1 = (2 * y + 1) * scols + 2 * x;
-    const f4 v00 = vsrc[s0], v01 = vsrc[s0 + 1], v10 = vsrc[s1], v11 = vsrc[s1 + 1];
-    const f4 n00 = nsrc[s0], n01 = nsrc[s0 + 1], n10 = nsrc[s1], n11 = nsrc[s1 + 1];
-    const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
-    if (isnanf_(v00.x) || isnanf_(v01.x) || isnanf_(v10.x) || isnanf_(v11.x))
-        vdst[y * dcols + x] = nan4;
-    else
-        vdst[y * dcols + x] = divs4(add4(add4(add4(v00, v01), v10), v11), 4.0f);
-    if (isnanf_(n00.x) || isnanf_(n01.x) || isnanf_(n10.x) || isnanf_(n11.x))
-        ndst[y * dcols + x] = nan4;
-    else
-        ndst[y * dcols + x] = normalised4(divs4(add4(add4(add4(n00, n01), n10), n11), 4.0f));
+    f4 vo, no;
+    resize_px(vsrc[s0], vsrc[s0 + 1], vsrc[s1], vsrc[s1 + 1], nsrc[s0], nsrc[s0 + 1], nsrc[s1], nsrc[s1 + 1], vo, no);
+    vdst[y * dcols + x] = vo;
+    ndst[y * dcols + x] = no;
 }
 __global__ __launch_bounds__(256) void k_resize(const f4* __restrict__ vsrc, const f4* __restrict__ nsrc, int scols,
                                                f4* __restrict__ vdst, f4* __restrict__ ndst, int dcols, int drows) {
@@ -374,13 +380,41 @@ void launch_resize(const f4* vsrc, const f4* nsrc, int scols, f4* vdst, f4* ndst
 // Each work-group finds its session's share (batch_slot) and runs the body the single-session
 // kernel runs, with the same work-group ids.
 
+// levels 1 and 2 of the model maps in one pass over 16x16 tiles of level 1: each lane makes
+// its level-1 pixel as resize_body does and keeps it in LDS; after the barrier the lanes of
+// even (x, y) make the level-2 pixel of their 2x2 quad from those four values, the ones a
+// level-2 resize would read back.  A level-2 pixel (x2, y2) exists only when 2*x2 + 1 and
+// 2*y2 + 1 are level-1 pixels, and its quad lies in the tile.
 __global__ __launch_bounds__(256) void kb_resize(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, blockIdx.x, g);
-    if (slot_first(b, s)) return;  // the first frame has no model maps
+    if (slot_first(b, s)) return;  // uniform: the first frame has no model maps
     const SessDesc& d = b.desc[b.sess[s]];
-    const int l = b.level;
-    resize_body(g, d.model_v[l - 1], d.model_n[l - 1], d.lc[l - 1], d.model_v[l], d.model_n[l], d.lc[l], d.lr[l]);
+    __shared__ f4 tv[16][16], tn[16][16];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x = g.x * 16 + tx, y = g.y * 16 + ty;
+    const int c0 = d.lc[0], c1 = d.lc[1];
+    if (x < c1 && y < d.lr[1]) {
+        const auto vsrc = FU_G(const f4, d.model_v[0]);
+        const auto nsrc = FU_G(const f4, d.model_n[0]);
+        const int s0 = (2 * y) * c0 + 2 * x, s1 = (2 * y + 1) * c0 + 2 * x;
+        f4 vo, no;
+        resize_px(vsrc[s0], vsrc[s0 + 1], vsrc[s1], vsrc[s1 + 1], nsrc[s0], nsrc[s0 + 1], nsrc[s1], nsrc[s1 + 1], vo,
+                  no);
+        FU_G(f4, d.model_v[1])[y * c1 + x] = vo;
+        FU_G(f4, d.model_n[1])[y * c1 + x] = no;
+        tv[ty][tx] = vo;
+        tn[ty][tx] = no;
+    }
+    __syncthreads();
+    const int x2 = x >> 1, y2 = y >> 1, c2 = d.lc[2];
+    if (!(tx & 1) && !(ty & 1) && x2 < c2 && y2 < d.lr[2]) {
+        f4 vo, no;
+        resize_px(tv[ty][tx], tv[ty][tx + 1], tv[ty + 1][tx], tv[ty + 1][tx + 1], tn[ty][tx], tn[ty][tx + 1],
+                  tn[ty + 1][tx], tn[ty + 1][tx + 1], vo, no);
+        FU_G(f4, d.model_v[2])[y2 * c2 + x2] = vo;
+        FU_G(f4, d.model_n[2])[y2 * c2 + x2] = no;
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_filter(BatchLaunch b) {
@@ -461,9 +495,9 @@ __global__ __launch_bounds__(256) void kb_image_commit(BatchLaunch b) {
 static void blaunch(void (*k)(BatchLaunch), int grid, const BatchLaunch& b, hipStream_t s) {
     if (grid > 0) hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, b);
 }
-void launch_b_resize(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s) {
-    b.level = level;
-    blaunch(kb_resize, batch_layout(b, h, BT_TILES, level, 0), b, s);
+void launch_b_resize(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
+    b.level = 1;
+    blaunch(kb_resize, batch_layout(b, h, BT_TILES, 1, 0), b, s);
 }
 void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     b.level = 0;
