@@ -1678,6 +1678,7 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     launch_b_filter(b, h, hs);
     launch_b_pyrdown(b, h, 1, hs);
     launch_b_pyrdown(b, h, 2, hs);
+    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];  // the normal pass packs the integration inputs
     launch_b_nmap(b, h, hs);
     if (bt->rgb == 1) {
         for (int i = 0; i < n; ++i) b.in[i] = rgb[i];

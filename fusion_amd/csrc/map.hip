@@ -585,12 +585,6 @@ __global__ __launch_bounds__(kIntThreads) void kb_integrate(BatchLaunch b) {
     const MapView m = d.m;  // private copy: the body keeps the fields in scalar registers
     integrate_body<F, true>(g, m, slot_fuse(b, d, s));
 }
-__global__ __launch_bounds__(256) void kb_fuse_pack(BatchLaunch b) {
-    WG g;
-    const int s = batch_slot(b, blockIdx.x, g);
-    const SessDesc& d = b.desc[b.sess[s]];
-    fuse_pack_body(g, slot_fuse(b, d, s));
-}
 
 // work-groups per session for the grid-stride stages
 static int per_slot(int total, int n, int lo, int hi) {
@@ -600,9 +594,8 @@ static int per_slot(int total, int n, int lo, int hi) {
 
 void launch_b_alloc(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     hipLaunchKernelGGL(kb_begin_pass, dim3(b.n), dim3(64), 0, s, b);
-    int grid = batch_layout(b, h, BT_PIX256, 0, 0);
-    hipLaunchKernelGGL(kb_fuse_pack, dim3(grid), dim3(256), 0, s, b);
-    grid = batch_layout(b, h, BT_TILES, 0, 0);
+    // the integration inputs were packed by the normal pass (kb_nmap)
+    int grid = batch_layout(b, h, BT_TILES, 0, 0);
     hipLaunchKernelGGL(kb_alloc_a, dim3(grid), dim3(kAllocThreads), 0, s, b);
     grid = batch_layout(b, h, BT_FIXED, 0, per_slot(tuning().alloc_groups, b.n, 16, 512));
     hipLaunchKernelGGL(kb_alloc_b, dim3(grid), dim3(256), 0, s, b);

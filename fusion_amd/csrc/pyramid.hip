@@ -133,26 +133,25 @@ __global__ __launch_bounds__(256) void k_vmap(const float* __restrict__ depth, f
     vmap_body(hw_wg(), depth, vmap, cols, rows, invfx, invfy, cx, cy, cutoff);
 }
 
-__device__ __forceinline__ void nmap_body(WG g, const f4* __restrict__ vmap_, f4* __restrict__ nmap_, int cols, int rows) {
-    const int x = g.x * 16 + (threadIdx.x & 15);
-    const int y = g.y * 16 + (threadIdx.x >> 4);
-    if (x >= cols || y >= rows) return;
+// ComputeNMapKernel's normal of pixel (x, y) inside the image
+__device__ __forceinline__ f4 normal_of(const f4* __restrict__ vmap_, int x, int y, int cols, int rows) {
     const auto vmap = FU_G(const f4, vmap_);
-    const auto nmap = FU_G(f4, nmap_);
     const f4 nan4 = mk4(fnan(), fnan(), fnan(), fnan());
-    if (x == cols - 1 || y == rows - 1) {
-        nmap[y * cols + x] = nan4;
-        return;
-    }
+    if (x == cols - 1 || y == rows - 1) return nan4;
     const f4 c = vmap[y * cols + x];
     const f4 r = vmap[y * cols + x + 1];
     const f4 d = vmap[(y + 1) * cols + x];
     if (!isnanf_(c.x) && !isnanf_(r.x) && !isnanf_(d.x)) {
         const f3 n = normalised(cross(sub(xyz(r), xyz(c)), sub(xyz(d), xyz(c))));
-        nmap[y * cols + x] = mk4(n.x, n.y, n.z, 1.0f);
-    } else {
-        nmap[y * cols + x] = nan4;
+        return mk4(n.x, n.y, n.z, 1.0f);
     }
+    return nan4;
+}
+__device__ __forceinline__ void nmap_body(WG g, const f4* __restrict__ vmap_, f4* __restrict__ nmap_, int cols, int rows) {
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= rows) return;
+    FU_G(f4, nmap_)[y * cols + x] = normal_of(vmap_, x, y, cols, rows);
 }
 __global__ __launch_bounds__(256) void k_nmap(const f4* __restrict__ vmap, f4* __restrict__ nmap, int cols, int rows) {
     nmap_body(hw_wg(), vmap, nmap, cols, rows);
@@ -456,7 +455,20 @@ __global__ __launch_bounds__(256) void kb_nmap(BatchLaunch b) {
     const int s = batch_slot(b, blockIdx.x, g);
     const SessDesc& d = b.desc[b.sess[s]];
     const int l = pyr3_level(d, g.x, g);
-    nmap_body(g, d.vmap[l], d.nmap[l], d.lc[l], d.lr[l]);
+    const int cols = d.lc[l];
+    const int x = g.x * 16 + (threadIdx.x & 15);
+    const int y = g.y * 16 + (threadIdx.x >> 4);
+    if (x >= cols || y >= d.lr[l]) return;
+    const int i = y * cols + x;
+    const f4 n = normal_of(d.vmap[l], x, y, cols, d.lr[l]);
+    FU_G(f4, d.nmap[l])[i] = n;
+    if (l == 0) {
+        // uniform: the integration inputs of the pixel (k_fuse_pack) from the normal above,
+        // the level-0 depth and the step's colour frame (b.in)
+        const auto c = FU_G(const unsigned char, (const unsigned char*)b.in[s]) + 3 * i;
+        FU_G(f4, d.fuse.px)[i] = mk4(FU_G(const float, d.fuse.depth)[i], n.x, n.y, n.z);
+        FU_G(unsigned, d.fuse.prgb)[i] = (unsigned)c[0] | ((unsigned)c[1] << 8) | ((unsigned)c[2] << 16);
+    }
 }
 
 // intensity pyramid of the sessions with the photometric term (the others have no share)
