@@ -1445,7 +1445,6 @@ int batch_layout(BatchLaunch& b, const SessDesc* h, BatchTiling t, int level, in
             case BT_ONE: gx = cnt = 1; break;
             case BT_FIXED: gx = cnt = fixed; break;
             case BT_ZTILES: gx = cnt = (d.zr.zcols * d.zr.zrows + 255) / 256; break;
-            case BT_PIX256: gx = cnt = (d.cols * d.rows + 255) / 256; break;
             case BT_ICP_PAIRS: gx = cnt = (d.icp_blocks[level] + 1) / 2; break;
         }
         b.pre[i] = total;

@@ -136,7 +136,6 @@ enum BatchTiling : int {
     BT_ONE = 3,     // one work-group per session
     BT_FIXED = 4,   // a fixed number of work-groups per session (grid-stride bodies)
     BT_ZTILES = 5,  // 256-thread groups over the z-range tiles
-    BT_PIX256 = 6,  // 256-thread groups over the level-0 pixels
     BT_ICP_PAIRS = 7,  // (icp_blocks[level] + 1) / 2 work-groups (kRgbPixels chunks)
 };
 // fills b.pre, b.gx and b.uniform for one launch; returns the total work-group count
