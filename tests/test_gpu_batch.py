@@ -24,6 +24,9 @@ SESSIONS = [
     (0, (3.0, 3.0, 2.5), 3, 320, 240, 0, (3,)),
     (0, (5.0, 4.0, 3.0), 7, 640, 480, 2, ()),
     (1, (8.0, 6.0, 3.0), 2, 1280, 720, 1, (4,)),
+    # partial 16x16 tiles at every level (202x154, 101x77): the ragged edges of the batched
+    # pyramid, resize and input-pack kernels
+    (0, (4.0, 4.0, 3.0), 5, 404, 308, 1, ()),
 ]
 STEPS = 6
 
