@@ -33,13 +33,24 @@ struct VertexOut {
 __device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restrict__ depth_, float* __restrict__ raw_,
                                                   float* __restrict__ filt_, int cols, int rows, float scale_inv,
                                                   float cutoff, VertexOut vo = VertexOut{}) {
+    // the group's 16x16 pixels and their 2-pixel margin, converted to metres once per sample
+    // ((float)d * scale_inv, the value every tap computes) and kept in LDS; samples outside
+    // the image are never read (the window bounds below stay inside it)
+    constexpr int T = 20;
+    __shared__ float tile[T * T];
+    const int x0 = g.x * 16 - 2, y0 = g.y * 16 - 2;
+    const auto depth = FU_G(const uint16_t, depth_);
+    for (int i = threadIdx.x; i < T * T; i += 256) {
+        const int gx = x0 + i % T, gy = y0 + i / T;
+        if (gx >= 0 && gy >= 0 && gx < cols && gy < rows) tile[i] = (float)depth[gy * cols + gx] * scale_inv;
+    }
+    __syncthreads();
     const int x = g.x * 16 + (threadIdx.x & 15);
     const int y = g.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
-    const auto depth = FU_G(const uint16_t, depth_);
     const auto raw = FU_G(float, raw_);
     const auto filt = FU_G(float, filt_);
-    const float center = (float)depth[y * cols + x] * scale_inv;
+    const float center = tile[(y - y0) * T + (x - x0)];
     raw[y * cols + x] = center < cutoff ? center : fnan();
     // the window is [max(p-2,0), min(p+3, n-1)) in each axis: the last row/column of the
     // image never contributes (Pyrdown.cu:23-29)
@@ -48,7 +59,7 @@ __device__ __forceinline__ void filter_depth_body(WG g, const uint16_t* __restri
     float sum1 = 0.f, sum2 = 0.f;
     for (int cy = max(y - 2, 0); cy < ty; ++cy) {
         for (int cx = max(x - 2, 0); cx < tx; ++cx) {
-            const float val = (float)depth[cy * cols + cx] * scale_inv;
+            const float val = tile[(cy - y0) * T + (cx - x0)];
             const float space2 = (float)((x - cx) * (x - cx) + (y - cy) * (y - cy));
             const float color2 = (center - val) * (center - val);
             // the argument is never positive or NaN: depths are finite
