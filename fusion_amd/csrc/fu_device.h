@@ -337,21 +337,6 @@ __device__ __forceinline__ float fu_sqrt_normal(float s) {
     return fmaf(-up, y, s) > 0.0f ? up : r;
 }
 
-// 1.0 / b in FP64 by the compiler's double-division sequence without its scaling and fixup
-// steps, for b converted from a nonzero finite float (|b| in [2^-149, 2^128], far inside the
-// range where those steps are the identity)
-__device__ __forceinline__ double fu_drcp_from_float(double b) {
-    double y = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-b, y, 1.0);
-    y = fma(y, e, y);
-    const double r = fma(-b, y, 1.0);
-    return fma(r, y, y);
-}
-// nonzero and finite (normal or denormal), one class test
-__device__ __forceinline__ bool fu_finite_nonzero(float x) { return __builtin_amdgcn_classf(x, 0x198); }
-
 // whether every component of (x, y, z) lies in the range of fu_div_refined by a denominator
 // of at most 2^31 (the 4-D norm of (x, y, z, 1)): 2^-60 <= |c| <= 2^30
 __device__ __forceinline__ bool fu_div_range3(float x, float y, float z) {

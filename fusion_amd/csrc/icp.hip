@@ -82,12 +82,16 @@ __device__ __forceinline__ void icp_row_values(const IcpArgs& a, const IcpPose& 
     }
     const f3 vcurr_g = add(mv(P.Rc, vcurr_c), P.tc);
     const f3 vcurr_p = mv(P.Rli, sub(vcurr_g, P.tl));
-    // a lane without a valid vertex divides by 1 (its result is discarded), so only a valid
-    // vertex projecting to depth 0 or infinity takes the general division
+    // Reduction.cu:237-239: `float invz = 1.0 / z`, float products and sum, + 0.5 in double,
+    // truncation.  The double quotient rounded to float is the float quotient 1.0f / z (double
+    // rounding of a quotient of floats through a format of >= 2*24+2 bits is innocuous).  A lane
+    // without a valid vertex divides by 1 (its result is discarded), so only a valid vertex
+    // with |z| outside [2^-60, 2^30] takes the general division.
     const float zp = ok ? vcurr_p.z : 1.0f;
-    const double invz = fu_finite_nonzero(zp) ? fu_drcp_from_float((double)zp) : 1.0 / (double)zp;
-    const int u = d2i((double)vcurr_p.x * invz * (double)a.fx + (double)a.cx + 0.5);
-    const int vv = d2i((double)vcurr_p.y * invz * (double)a.fy + (double)a.cy + 0.5);
+    const float azp = fabsf(zp);
+    const float invz = (azp >= 0x1p-60f && azp <= 0x1p30f) ? fu_div_refined(1.0f, zp, fu_rcp_refined(zp)) : 1.0f / zp;
+    const int u = d2i((double)(vcurr_p.x * invz * a.fx + a.cx) + 0.5);
+    const int vv = d2i((double)(vcurr_p.y * invz * a.fy + a.cy) + 0.5);
     ok = ok && !(u < 0 || vv < 0 || u >= cols || vv >= rows);
     const int j = ok ? vv * cols + u : 0;
     const f3 vlast_g = add(mv(P.Rl, xyz(vlast[j])), P.tl);

@@ -1,5 +1,5 @@
 // Self-test of the range-specialised IEEE arithmetic (fu_device.h: fu_rcp_refined,
-// fu_div_refined, fu_sqrt_normal, fu_drcp_from_float) against the compiler's IEEE division and square root on
+// fu_div_refined, fu_sqrt_normal) against the compiler's IEEE division and square root on
 // the device itself: the kernels rely on them being bit-identical inside their stated ranges.
 // It also checks the single-instruction float/double -> int conversions (fu_math.h: f2i, d2i)
 // against the rule they implement (truncation, saturation, NaN -> 0), and the bilateral
@@ -89,14 +89,13 @@ __global__ void k_selftest(uint64_t seed, unsigned n, unsigned* bad) {
             b6 += __float_as_uint(fu_expf_nonpos(u)) != __float_as_uint(fu_expf(u));
             b6 += __float_as_uint(fu_expf_nonpos(t)) != __float_as_uint(fu_expf(t));
         }
-        // 5: FP64 reciprocals of nonzero finite floats (denormal ones included)
+        // 5: the ICP projection's reciprocal: z in [2^-60, 2^30], either sign; the refined
+        // float quotient 1 / z must equal the reference's double quotient rounded to float
         {
-            float z = rnd(s, -126, 127, true);
-            if (((s >> 52) & 15) == 1) z = __uint_as_float((unsigned)(s >> 9) & 0x807fffffu);
-            if (fu_finite_nonzero(z)) {
-                const double r = fu_drcp_from_float((double)z), q = 1.0 / (double)z;
-                b4 += __double_as_longlong(r) != __double_as_longlong(q);
-            }
+            const float z = rnd(s, -60, 29, true);
+            const float r = fu_div_refined(1.0f, z, fu_rcp_refined(z));
+            b4 += __float_as_uint(r) != __float_as_uint((float)(1.0 / (double)z));
+            b4 += __float_as_uint(r) != __float_as_uint(1.0f / z);
         }
     }
     if (b6) atomicAdd(&bad[6], b6);

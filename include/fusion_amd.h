@@ -308,7 +308,8 @@ fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
  * of the compiler's IEEE sequences (no reference counterpart).  This runs n random cases of
  * each range on the device and counts the results that differ from the IEEE operation:
  * mismatches[0] quotients by a 4-D norm, [1] sdf / truncation, [2] projections (their
- * truncated pixel coordinate), [3] square roots, [4] FP64 reciprocals of floats, [5] the
+ * truncated pixel coordinate), [3] square roots, [4] the ICP projection's reciprocal 1 / z
+ * against the reference's double quotient rounded to float, [5] the
  * single-instruction float and double -> int conversions against their rule (truncation,
  * saturation, NaN -> 0), [6] the bilateral weights' exp against the general one.  All seven
  * must be 0.  Synchronises. */

@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# FU_ORACLE_LIB names another build of the same sources (the planted-mistake variants that
+# tests/test_oracle_planted.py builds)
+LIB_PATH = os.environ.get("FU_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 P = C.c_void_p
 I = C.c_int
@@ -28,6 +30,7 @@ _SIG = {
     "or_icp_sums": (None, [P, P, P, P, I, I, FP, FP, FP, FP, FP, F, F, F, F, FP, DP]),
     "or_solve6": (I, [DP, DP, DP]),
     "or_se3_exp": (None, [DP, DP]),
+    "or_euler_sine_norm": (C.c_double, [DP]),
     "or_compute_se3": (I, [C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), I, I, FP,
                            C.POINTER(I), C.POINTER(I), F, DP, DP, FP, C.POINTER(I)]),
     "or_pose_f": (None, [DP, FP, FP, FP]),
@@ -90,6 +93,12 @@ def _ptr(a):
 def _fa(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a.ctypes.data_as(FP), a
+
+
+def euler_sine_norm(p):
+    """|sin(XYZ Euler angles)| of a 4x4 motion, as the acceptance test of Tracking.cc:398-408"""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return lib().or_euler_sine_norm(p.ctypes.data_as(DP))
 
 
 def pose_f(pose):

@@ -11,6 +11,18 @@
 #include <unordered_map>
 #include <vector>
 
+// OR_PLANT selects a variant with one known mistake, for the test that shows the oracle's
+// pinning tests catch it (tests/test_oracle_planted.py); 0, the default, is the oracle itself.
+//   1 ICP projection in double        2 ICP row with the current frame's normal
+//   3 3-D dot for the integration weight   4 colour blend with old and new swapped
+//   5 z-range fragments over nx (not ny) columns   6 Euler sine of X from R(2,1), not R(1,2)
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+namespace {
+constexpr int kPlant = OR_PLANT;
+}  // namespace
+
 namespace {
 
 // ------------------------------------------------------------------ VectorMath.h semantics
@@ -233,6 +245,17 @@ extern "C" void or_resize_map(const float* vsrc, const float* nsrc, int cols, in
 }
 
 // ------------------------------------------------------------------ Reduction.cu ICPReduce
+// |sin(XYZ Euler angles)| of a motion's rotation (Tracking.cc:401: Eigen's eulerAngles(0, 1, 2)
+// then .sin(), whose norm is the same for both Euler solutions): R = Rx(a) Ry(b) Rz(c) has
+// sin b = R02, sin a = -R12 / |(R12, R22)|, sin c = -R01 / |(R01, R00)|
+extern "C" double or_euler_sine_norm(const double p[16]) {
+    const double r12 = kPlant == 6 ? p[9] : p[6];
+    const double sa = -r12 / sqrt(r12 * r12 + p[10] * p[10]);
+    const double sb = p[2];
+    const double sc = -p[1] / sqrt(p[1] * p[1] + p[0] * p[0]);
+    return sqrt(sa * sa + sb * sb + sc * sc);
+}
+
 namespace {
 // searchPoint + getRow (Reduction.cu:227-286); returns found and the 7-vector row
 inline bool icp_row(const float* vn, const float* vl, const float* nn, const float* nl, int cols, int rows, int x,
@@ -245,9 +268,16 @@ inline bool icp_row(const float* vn, const float* vl, const float* nn, const flo
     if (isn(vcurr_c.x) || (double)vcurr_c.z < 1e-3) return false;
     const V3 vcurr_g = mul(Rc, vcurr_c) + tc;
     const V3 vcurr_p = mul(Rli, vcurr_g - tl);
-    const double invz = 1.0 / (double)vcurr_p.z;
-    const int u = toid((double)vcurr_p.x * invz * (double)fx + (double)cx + 0.5);
-    const int v = toid((double)vcurr_p.y * invz * (double)fy + (double)cy + 0.5);
+    // Reduction.cu:237-239: `float invz = 1.0 / z` (a double quotient stored in a float), the
+    // products and the sum with cx in float, only the final + 0.5 in double, then truncation
+    const float invz = (float)(1.0 / (double)vcurr_p.z);
+    int u = toid((double)(vcurr_p.x * invz * fx + cx) + 0.5);
+    int v = toid((double)(vcurr_p.y * invz * fy + cy) + 0.5);
+    if constexpr (kPlant == 1) {
+        const double dz = 1.0 / (double)vcurr_p.z;
+        u = toid((double)vcurr_p.x * dz * (double)fx + (double)cx + 0.5);
+        v = toid((double)vcurr_p.y * dz * (double)fy + (double)cy + 0.5);
+    }
     if (u < 0 || v < 0 || u >= cols || v >= rows) return false;
     V4 b, c, d;
     memcpy(&b, &vl[(v * cols + u) * 4], 16);
@@ -261,7 +291,7 @@ inline bool icp_row(const float* vn, const float* vl, const float* nn, const flo
     const float dist = norm(vlast_g - vcurr_g);
     const float sine = norm(cross(ncurr_g, nlast_g));
     if (!(sine < angle_thresh && dist <= dist_thresh && !isn(ncurr_c.x) && !isn(nlast_c.x))) return false;
-    const V3 nlr = mul(Rli, nlast_g);
+    const V3 nlr = mul(Rli, kPlant == 2 ? ncurr_g : nlast_g);
     const V3 vcr = mul(Rli, vcurr_g - tl);
     const V3 vlr = mul(Rli, vlast_g - tl);
     const V3 mn = -nlr;
@@ -678,10 +708,7 @@ int compute_se3(const float* const next_vmap[3], const float* const next_nmap[3]
     double pinv[16], p[16];
     inv4(last_pose, pinv);
     mm4(pinv, next, p);
-    const double sa = -p[6] / sqrt(p[6] * p[6] + p[10] * p[10]);
-    const double sb = p[2];
-    const double sc = -p[1] / sqrt(p[1] * p[1] + p[0] * p[0]);
-    const double an = sqrt(sa * sa + sb * sb + sc * sc);
+    const double an = or_euler_sine_norm(p);
     const double tn = sqrt(p[3] * p[3] + p[7] * p[7] + p[11] * p[11]);
     if (icp_error < thresh && an <= 0.1 && tn <= 0.1) {
         memcpy(pose, next, 16 * sizeof(double));
@@ -987,7 +1014,8 @@ extern "C" void or_integrate(or_map* m, const float* depth, const uint8_t* rgb, 
                     V4 nl;
                     memcpy(&nl, &nmap[(v * cols + u) * 4], 16);
                     if (isn(nl.x)) continue;
-                    const float w = dot4(nl, normalised4(V4{pos.x, pos.y, pos.z, 1.0f}));
+                    float w = dot4(nl, normalised4(V4{pos.x, pos.y, pos.z, 1.0f}));
+                    if constexpr (kPlant == 3) w = dot(xyz(nl), normalised(pos));
                     const size_t vi = (size_t)e.ptr + loc;
                     const unsigned char* c = &rgb[(v * cols + u) * 3];
                     unsigned char* old = &m->rgb[vi * 3];
@@ -1003,7 +1031,8 @@ extern "C" void or_integrate(or_map* m, const float* depth, const uint8_t* rgb, 
                         for (int k = 0; k < 3; ++k) {
                             const float val = (float)c[k] / 255.f;
                             const float o = (float)old[k] / 255.f;
-                            const float res = (w * 0.2f * val + (1 - w * 0.2f) * o) * 255.f;
+                            const float res = kPlant == 4 ? (w * 0.2f * o + (1 - w * 0.2f) * val) * 255.f
+                                                          : (w * 0.2f * val + (1 - w * 0.2f) * o) * 255.f;
                             old[k] = tou8(res);
                         }
                         if (m->precision) {
@@ -1080,7 +1109,7 @@ extern "C" int or_create_rendering_blocks(or_map* m, float* zmin, float* zmax, i
         // fragments as createRenderingBlockList writes them (inner bound `x < ny`, lower
         // right without the fragment offset), then fillBlocks over each fragment's 16x16 tiles
         for (int fy_ = 0; fy_ < ny; ++fy_)
-            for (int fx_ = 0; fx_ < ny; ++fx_) {
+            for (int fx_ = 0; fx_ < (kPlant == 5 ? nx : ny); ++fx_) {
                 int bulx = ulx + fx_ * 16, buly = uly + fy_ * 16;
                 int blrx = ulx + 16, blry = uly + 16;
                 if (blrx > lrx) blrx = lrx;

@@ -78,6 +78,8 @@ int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap
                    const float* const last_nmap[3], int cols, int rows, const float K[4], const int iters[3],
                    const int min_count[3], float thresh, const double last_pose[16], double pose[16],
                    float* err, int* count);
+/* the acceptance test's |sin(XYZ Euler angles)| of a motion (Tracking.cc:398-408), 4x4 row-major */
+double or_euler_sine_norm(const double p[16]);
 /* photometric term (Tracking/Pyrdown.cu:127-249, Tracking/Reduction.cu:377-627) */
 void or_image_to_intensity(const uint8_t* rgb, uint8_t* img, int cols, int rows);
 void or_pyrdown_u8(const uint8_t* src, int cols, int rows, uint8_t* dst);

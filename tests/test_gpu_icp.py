@@ -43,6 +43,24 @@ def test_icp_step_sums(gpu):
     assert np.abs(x_g - x_o).max() < 1e-6
 
 
+def test_icp_step_at_a_moving_pose(gpu):
+    """the true motion from frame 0 to frame 2 as the current pose: the association's
+    projection (float invz and products, + 0.5 in double, Reduction.cu:237-239) decides pixels
+    on these maps, so the correspondence count must equal the oracle's and an independent numpy
+    restatement's exactly; A and b within FP32 summation error of the FP64 sums"""
+    from tests.icp_numpy import associate
+    K, (v0, n0), (v1, n1), gt = _pair()
+    cu = lambda a: torch.from_numpy(a).cuda()
+    A_g, b_g, r_g = A.ICPStep(cu(v1[0]), cu(v0[0]), cu(n1[0]), cu(n0[0]), gt, np.eye(4), K)
+    _, s64 = O.icp_sums(v1[0], v0[0], n1[0], n0[0], gt, np.eye(4), K)
+    ok, _, _ = associate(v1[0], n1[0], v0[0], n0[0], gt, np.eye(4), K)
+    assert r_g[1] == s64[28] == ok.sum() > 200000
+    A_o, b_o = O.unpack_sums(s64)
+    scale = np.abs(A_o).max()
+    assert np.abs(A_g - A_o).max() <= 1e-6 * scale
+    assert np.abs(b_g - b_o).max() <= 1e-6 * max(np.abs(b_o).max(), 1e-12) + 1e-9 * scale
+
+
 def _plane_maps(cols, rows, K, n=(0.1, -0.2, -1.0), c=-2.0, seed=0):
     """vertex and normal maps of the plane n.p = c seen from the origin, with a mild ripple"""
     fx, fy, cx, cy = K
