@@ -430,6 +430,24 @@ class System:
         """depth: device int16 (H, W) holding u16 values; rgb: device uint8 (H, W, 3)."""
         check(N.lib().fu_system_grab_image_from(self.handle, _p(depth), _p(rgb), _s()), "fu_system_grab_image")
 
+    def GrabImageHost(self, depth, rgb):
+        """host frame (uint16 (H, W), uint8 (H, W, 3); numpy or pinned torch CPU tensors),
+        uploaded into the system's input slots on the current stream, then the frame"""
+        for a in (depth, rgb):
+            if isinstance(a, torch.Tensor):
+                if a.is_cuda or not a.is_contiguous():
+                    raise N.FusionError("GrabImageHost: contiguous host tensors expected")
+            elif not (isinstance(a, np.ndarray) and a.flags.c_contiguous):
+                raise N.FusionError("GrabImageHost: contiguous host arrays expected")
+        ptr = lambda a: C.c_void_p(a.data_ptr() if isinstance(a, torch.Tensor) else a.ctypes.data)
+        check(N.lib().fu_system_grab_image_host(self.handle, ptr(depth), ptr(rgb), _s()), "fu_system_grab_image_host")
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) of this system"""
+        c, n = C.c_uint32(), C.c_uint32()
+        check(N.lib().fu_system_graph_counts(self.handle, C.byref(c), C.byref(n)), "fu_system_graph_counts")
+        return c.value, n.value
+
     def result(self):
         r = N.FrameResult()
         check(N.lib().fu_system_result(self.handle, C.byref(r), _s()), "fu_system_result")
@@ -625,6 +643,13 @@ class Server:
         check(N.lib().fu_server_submit(self.handle, int(sid), _p(depth), _p(rgb), int(tag), float(t_submit_ms)),
               "fu_server_submit")
 
+    def submit_host(self, sid, depth, rgb, tag=0, t_submit_ms=-1.0):
+        """queue a frame held in (pinned) host tensors: uploaded inside the step that takes it"""
+        if depth.is_cuda or rgb.is_cuda:
+            raise N.FusionError("submit_host: host tensors expected")
+        check(N.lib().fu_server_submit_host(self.handle, int(sid), _p(depth), _p(rgb), int(tag), float(t_submit_ms)),
+              "fu_server_submit_host")
+
     def submit_mesh(self, sid, vertex, normal, color, max_triangles, tag=0, t_submit_ms=-1.0):
         """queue a mesh extraction (device buffers: float (T, 9), float (T, 9), uint8 (T, 9))"""
         check(N.lib().fu_server_submit_mesh(self.handle, int(sid), _p(vertex), _p(normal), _p(color),
@@ -648,7 +673,8 @@ class Server:
         return out
 
     def run_trace(self, frames, closed_loop=False):
-        """frames: list of (session, depth, rgb, arrival_ms, tag) frame requests, or
+        """frames: list of (session, depth, rgb, arrival_ms, tag) frame requests (device tensors,
+        or pinned host tensors uploaded inside the step), or
         (session, (vertex, normal, color), None, arrival_ms, tag) mesh requests (buffers of
         shape (T, 9)); returns completions in completion order"""
         n = len(frames)
@@ -661,6 +687,7 @@ class Server:
                 tr[i].vertex, tr[i].normal, tr[i].color = _addr(v), _addr(nm), _addr(col)
             else:
                 tr[i].depth, tr[i].rgb = _addr(d), _addr(c)
+                tr[i].host = 0 if d.is_cuda else 1  # host tensors are uploaded inside the step
         out = (N.Completion * n)()
         check(N.lib().fu_server_run_trace(self.handle, tr, n, int(closed_loop), out), "fu_server_run_trace")
         return [_completion(out[i]) for i in range(n)]

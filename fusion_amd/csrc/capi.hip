@@ -1309,7 +1309,9 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     return FU_OK;
 }
 
-static fu_status capture_graph(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, hipGraphExec_t* out) {
+// The frame's work captured once per system against the system's own input slots
+// (depth_in, rgb_in); later frames copy their images into the slots and replay it.
+static fu_status capture_graph(fu_system* sys, hipGraphExec_t* out) {
     hipStream_t cs;
     HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t g;
@@ -1318,13 +1320,14 @@ static fu_status capture_graph(fu_system* sys, const uint16_t* depth, const uint
         (void)hipStreamDestroy(cs);
         return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
     }
-    queue_frame_device(sys, depth, rgb, false, cs);
+    queue_frame_device(sys, sys->b.depth_in, sys->b.rgb_in, false, cs);
     e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
     if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return set_err(FU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    sys->graph_captures++;
     return FU_OK;
 }
 
@@ -1336,16 +1339,13 @@ extern "C" fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* d
     if (sys->p.plain) {
         st = run_frame_plain(sys, depth, rgb, first, hs);
     } else if (sys->p.use_graph && !first && !sys->prof) {
-        if (sys->graph && (sys->graph_depth != depth || sys->graph_rgb != rgb)) {
-            (void)hipGraphExecDestroy(sys->graph);
-            sys->graph = nullptr;
-        }
-        if (!sys->graph) {
-            if ((st = capture_graph(sys, depth, rgb, &sys->graph))) return st;
-            sys->graph_depth = depth;
-            sys->graph_rgb = rgb;
-        }
+        const size_t n0 = (size_t)sys->p.cols * sys->p.rows;
+        if (depth != sys->b.depth_in)
+            HIPCHK(hipMemcpyAsync(sys->b.depth_in, depth, n0 * 2, hipMemcpyDeviceToDevice, hs));
+        if (rgb != sys->b.rgb_in) HIPCHK(hipMemcpyAsync(sys->b.rgb_in, rgb, n0 * 3, hipMemcpyDeviceToDevice, hs));
+        if (!sys->graph && (st = capture_graph(sys, &sys->graph))) return st;
         HIPCHK(hipGraphLaunch(sys->graph, hs));
+        sys->graph_launches++;
     } else {
         queue_frame_device(sys, depth, rgb, first, hs);
         LAUNCHCHK();
@@ -1401,6 +1401,25 @@ extern "C" fu_status fu_system_get_stage_times(fu_system* sys, fu_stage_times* o
 extern "C" fu_status fu_system_grab_image(fu_system* sys, fu_stream s) {
     if (!sys) return set_err(FU_ERR_ARG, "null system");
     return fu_system_grab_image_from(sys, sys->b.depth_in, sys->b.rgb_in, s);
+}
+
+// System::GrabImage with the frame in host memory: the upload of Frame::Frame
+// (Core/Frame.cc:61-62) on the stream, into the system's input slots, then the frame.  With
+// pinned host memory the copy is asynchronous; the host buffers must stay unchanged until it
+// has run.
+extern "C" fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s) {
+    if (!sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_system_grab_image_host: null argument");
+    const size_t n0 = (size_t)sys->p.cols * sys->p.rows;
+    HIPCHK(hipMemcpyAsync(sys->b.depth_in, depth, n0 * 2, hipMemcpyHostToDevice, S(s)));
+    HIPCHK(hipMemcpyAsync(sys->b.rgb_in, rgb, n0 * 3, hipMemcpyHostToDevice, S(s)));
+    return fu_system_grab_image_from(sys, sys->b.depth_in, sys->b.rgb_in, s);
+}
+
+extern "C" fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches) {
+    if (!sys || !captures || !launches) return set_err(FU_ERR_ARG, "null argument");
+    *captures = sys->graph_captures;
+    *launches = sys->graph_launches;
+    return FU_OK;
 }
 
 extern "C" fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s) {

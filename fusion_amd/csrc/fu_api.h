@@ -43,8 +43,8 @@ struct fu_system {
     bool prof = false;
     hipEvent_t ev[FU_NUM_STAGES + 1] = {};
     fu_stage_times times{};
-    const uint16_t* graph_depth = nullptr;
-    const uint8_t* graph_rgb = nullptr;
+    uint32_t graph_captures = 0;  // graphs captured (once per system unless profiling toggles)
+    uint32_t graph_launches = 0;
     fu_batch* batch = nullptr;  // batch context the system is attached to
     int batch_slot = -1;
     fu::f4* fuse_px = nullptr;      // packed integration inputs (depth, normal) per pixel

@@ -31,6 +31,7 @@ struct Req {
     const uint8_t* rgb;
     double t_submit;
     int kind;  // FU_REQ_FRAME or FU_REQ_MESH
+    int host;  // frame in host memory: uploaded into the session's input slot inside the step
     float* mv;
     float* mn;
     uint8_t* mc;
@@ -58,6 +59,9 @@ struct Lane {
 
 struct Session {
     fu_system* sys = nullptr;
+    uint16_t* depth_in = nullptr;  // the system's input slots (host frames are uploaded there)
+    uint8_t* rgb_in = nullptr;
+    size_t npix = 0;
     int lane = -1;
     size_t bytes = 0;
     double pixels = 0;
@@ -160,6 +164,11 @@ extern "C" fu_status fu_server_open(fu_server* sv, const fu_system_params* p, in
     Session s;
     fu_status st = fu_system_create(p, &s.sys);
     if (st) return st;
+    fu_system_buffers b;
+    fu_system_get_buffers(s.sys, &b);
+    s.depth_in = b.depth_in;
+    s.rgb_in = b.rgb_in;
+    s.npix = (size_t)p->cols * p->rows;
     // the lane with the least pixel load per step among those whose batch context takes the
     // session (a context holds one voxel format and one tracking mode)
     std::vector<int> order(sv->lanes.size());
@@ -206,8 +215,8 @@ extern "C" fu_system* fu_server_system(fu_server* sv, int32_t id) {
     return s ? s->sys : nullptr;
 }
 
-extern "C" fu_status fu_server_submit(fu_server* sv, int32_t id, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
-                                      double t_submit_ms) {
+static fu_status submit_frame(fu_server* sv, int32_t id, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
+                              double t_submit_ms, int host) {
     Session* s = get_session(sv, id);
     if (!s) return api_error(FU_ERR_ARG, "no session %d", id);
     if (!depth || !rgb) return api_error(FU_ERR_ARG, "null frame");
@@ -218,9 +227,20 @@ extern "C" fu_status fu_server_submit(fu_server* sv, int32_t id, const uint16_t*
     r.rgb = rgb;
     r.t_submit = t_submit_ms < 0 ? fu_server_now_ms(sv) : t_submit_ms;
     r.kind = FU_REQ_FRAME;
+    r.host = host;
     s->pending.push_back(r);
     sv->outstanding++;
     return FU_OK;
+}
+
+extern "C" fu_status fu_server_submit(fu_server* sv, int32_t id, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
+                                      double t_submit_ms) {
+    return submit_frame(sv, id, depth, rgb, tag, t_submit_ms, 0);
+}
+
+extern "C" fu_status fu_server_submit_host(fu_server* sv, int32_t id, const uint16_t* depth, const uint8_t* rgb,
+                                           uint64_t tag, double t_submit_ms) {
+    return submit_frame(sv, id, depth, rgb, tag, t_submit_ms, 1);
 }
 
 extern "C" fu_status fu_server_submit_mesh(fu_server* sv, int32_t id, float* vertex, float* normal, uint8_t* color,
@@ -354,6 +374,12 @@ static fu_status launch(fu_server* sv, Lane& L) {
             sys[i] = s.sys;
             dp[i] = st.req[i].depth;
             cp[i] = st.req[i].rgb;
+            if (st.req[i].host) {  // the upload of Frame::Frame (Core/Frame.cc:61-62), inside the step
+                SV_HIP(hipMemcpyAsync(s.depth_in, dp[i], s.npix * 2, hipMemcpyHostToDevice, L.stream));
+                SV_HIP(hipMemcpyAsync(s.rgb_in, cp[i], s.npix * 3, hipMemcpyHostToDevice, L.stream));
+                dp[i] = s.depth_in;
+                cp[i] = s.rgb_in;
+            }
         }
         FrameBrief* dres = L.d_res + (size_t)st.slot * kMaxBatch;
         FrameBrief* hres = L.h_res + (size_t)st.slot * kMaxBatch;
@@ -414,7 +440,7 @@ extern "C" fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* fr
         const fu_trace_frame& f = frames[i];
         if (f.kind == FU_REQ_MESH)
             return fu_server_submit_mesh(sv, f.session, f.vertex, f.normal, f.color, f.max_triangles, f.tag, t);
-        return fu_server_submit(sv, f.session, f.depth, f.rgb, f.tag, t);
+        return submit_frame(sv, f.session, f.depth, f.rgb, f.tag, t, f.host);
     };
     fu_status st = FU_OK;
     int32_t next = 0, got = 0;

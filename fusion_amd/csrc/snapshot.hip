@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -170,76 +171,154 @@ fu_status write_map(fu_map* m, Out& o) {
     return o.ok ? FU_OK : api_error(FU_ERR_ARG, "map snapshot: write failed");
 }
 
-fu_status read_map(fu_map* m, In& in, hipStream_t s) {
-    const fu_map_params& p = m->p;
-    MapView& v = m->v;
+// A map snapshot read and checked on the host before anything reaches the device: the
+// checksum over the whole map part, then the structure the kernels rely on (every chain walk
+// ends, every pool index is in range), so a damaged or inconsistent file is refused with the
+// device map untouched.
+struct MapImage {
+    unsigned ctr[C_COUNT];
+    uint32_t n_alloc = 0;
+    std::vector<HEntry> ent;
+    std::vector<unsigned long long> keys;
+    off_t pool_off = 0;  // file offset of the pool bytes
+    size_t pool_bytes = 0;
+};
+
+fu_status snapshot_error(const char* what) { return api_error(FU_ERR_ARG, "map snapshot: %s", what); }
+
+// the consistency the device's lookups and allocation depend on
+const char* check_map_image(const fu_map_params& p, const MapImage& img) {
+    const size_t nb = p.num_buckets, nx = p.num_excess, ne = nb + nx;
+    const uint32_t n_alloc = img.n_alloc;
+    if (n_alloc != (img.ctr[C_ALLOC] < p.num_blocks ? img.ctr[C_ALLOC] : p.num_blocks))
+        return "allocated-block count differs from the allocation counter";
+    const size_t n_excess = img.ctr[C_EXCESS] < nx ? img.ctr[C_EXCESS] : nx;
+    for (size_t i = 0; i < ne; ++i) {
+        const HEntry& e = img.ent[i];
+        if (e.key == kEmptyKey) {
+            if (e.ptr != -1 || e.next != 0) return "empty entry with a block or a link";
+            continue;
+        }
+        if (e.key >> 63) return "entry key out of range";
+        if (i >= nb && i - nb >= n_excess) return "excess entry beyond the excess counter";
+        if (e.ptr != -1) {
+            if (e.ptr < 0 || (e.ptr & 511) || (uint32_t)(e.ptr >> 9) >= n_alloc) return "entry block index out of range";
+            if (img.keys[e.ptr >> 9] != e.key) return "entry and block key disagree";
+        }
+        if (e.next < 0 || (size_t)e.next > n_excess) return "excess link out of range";
+        if (e.next > 0 && img.ent[nb + e.next - 1].key == kEmptyKey) return "link to an empty excess entry";
+    }
+    // every bucket's chain: acyclic, each excess entry on exactly one chain, keys in their bucket
+    std::vector<unsigned char> seen(nx, 0);
+    for (size_t b = 0; b < nb; ++b) {
+        const HEntry* e = &img.ent[b];
+        if (e->key == kEmptyKey) continue;
+        while (true) {
+            if (block_hash_raw(unpack_key(e->key)) % p.num_buckets != b) return "entry in another key's bucket";
+            if (e->next <= 0) break;
+            const size_t k = (size_t)e->next - 1;
+            if (seen[k]) return "excess chains share an entry or form a cycle";
+            seen[k] = 1;
+            e = &img.ent[nb + k];
+        }
+    }
+    for (size_t k = 0; k < nx; ++k)
+        if (img.ent[nb + k].key != kEmptyKey && !seen[k]) return "excess entry on no chain";
+    // each allocated block's key maps to that block (holes are allowed: a block whose excess
+    // slot could not be had)
+    size_t mapped = 0;
+    for (size_t i = 0; i < ne; ++i) mapped += img.ent[i].key != kEmptyKey && img.ent[i].ptr >= 0;
+    std::vector<unsigned long long> keyed;
+    for (uint32_t j = 0; j < n_alloc; ++j)
+        if (img.keys[j] != kEmptyKey) keyed.push_back(img.keys[j]);
+    if (mapped != keyed.size()) return "allocated blocks and entries disagree";
+    std::sort(keyed.begin(), keyed.end());
+    if (std::adjacent_find(keyed.begin(), keyed.end()) != keyed.end()) return "a block key allocated twice";
+    return nullptr;
+}
+
+// header, counters, entries and keys into `img`; the pool bytes go through the checksum only
+fu_status parse_map(const fu_map_params& p, In& in, MapImage& img) {
     char magic[8];
     in.get(magic, 8);
-    IOCHK(in.ok && memcmp(magic, kMapMagic, 8) == 0, "map snapshot: not a map snapshot");
+    if (!in.ok || memcmp(magic, kMapMagic, 8) != 0) return snapshot_error("not a map snapshot");
     const uint32_t ver = in.val<uint32_t>();
-    IOCHK(in.ok && ver == kVersion, "map snapshot: version %u (this library reads %u)", ver, kVersion);
+    if (!in.ok || ver != kVersion) return api_error(FU_ERR_ARG, "map snapshot: version %u (this library reads %u)", ver, kVersion);
     const uint32_t nb = in.val<uint32_t>(), nx = in.val<uint32_t>(), nk = in.val<uint32_t>();
     const uint32_t fmt = in.val<uint32_t>(), mw = in.val<uint32_t>();
     const float vs = in.val<float>(), tr = in.val<float>(), dmin = in.val<float>(), dmax = in.val<float>();
-    IOCHK(in.ok, "map snapshot: truncated header");
-    IOCHK(nb == p.num_buckets && nx == p.num_excess && nk == p.num_blocks && fmt == p.voxel_format &&
-              mw == p.max_weight && vs == p.voxel_size && tr == p.trunc_dist && dmin == p.depth_min &&
-              dmax == p.depth_max,
-          "map snapshot: written by a map of other parameters (buckets %u/%u, excess %u/%u, blocks %u/%u, format "
-          "%u/%u)", nb, p.num_buckets, nx, p.num_excess, nk, p.num_blocks, fmt, p.voxel_format);
-    // from here on a failure leaves the map empty
-    auto fail = [&](const char* what) {
-        launch_reset_map(v, s);
-        (void)hipStreamSynchronize(s);
-        return api_error(FU_ERR_ARG, "map snapshot: %s (the map was left empty)", what);
-    };
-    unsigned ctr[C_COUNT];
-    in.get(ctr, sizeof(ctr));
+    if (!in.ok) return snapshot_error("truncated header");
+    if (!(nb == p.num_buckets && nx == p.num_excess && nk == p.num_blocks && fmt == p.voxel_format &&
+          mw == p.max_weight && vs == p.voxel_size && tr == p.trunc_dist && dmin == p.depth_min && dmax == p.depth_max))
+        return api_error(FU_ERR_ARG,
+                         "map snapshot: written by a map of other parameters (buckets %u/%u, excess %u/%u, blocks "
+                         "%u/%u, format %u/%u)", nb, p.num_buckets, nx, p.num_excess, nk, p.num_blocks, fmt,
+                         p.voxel_format);
+    in.get(img.ctr, sizeof(img.ctr));
     const uint32_t bb = in.val<uint32_t>(), n_alloc = in.val<uint32_t>(), used = in.val<uint32_t>();
     const size_t ne = (size_t)p.num_buckets + p.num_excess;
-    if (!in.ok || bb != (uint32_t)vf_block_bytes(v.fmt) || n_alloc > p.num_blocks || used > ne)
-        return fail("inconsistent sizes");
-    // the entry array: all empty, then the used entries at their indices
-    std::vector<HEntry> ent(ne);
-    for (HEntry& e : ent) {
-        e.key = kEmptyKey;
-        e.ptr = -1;
-        e.next = 0;
-    }
+    if (!in.ok || bb != (uint32_t)vf_block_bytes((int)p.voxel_format) || n_alloc > p.num_blocks || used > ne)
+        return snapshot_error("inconsistent sizes");
+    img.n_alloc = n_alloc;
+    img.ent.assign(ne, HEntry{kEmptyKey, -1, 0});
+    std::vector<unsigned char> taken(ne, 0);
     for (uint32_t k = 0; k < used && in.ok; ++k) {
         const uint32_t i = in.val<uint32_t>();
         HEntry e;
         e.key = in.val<uint64_t>();
         e.ptr = in.val<int32_t>();
         e.next = in.val<int32_t>();
-        if (i >= ne) return fail("entry index out of range");
-        ent[i] = e;
+        if (!in.ok) break;
+        if (i >= ne || taken[i] || e.key == kEmptyKey) return snapshot_error("bad or repeated entry index");
+        taken[i] = 1;
+        img.ent[i] = e;
     }
-    std::vector<unsigned long long> keys(n_alloc);
-    in.get(keys.data(), n_alloc * 8ull);
-    if (!in.ok) return fail("truncated");
-    // the occupancy bitmap is a function of the allocated keys
+    img.keys.resize(n_alloc);
+    in.get(img.keys.data(), n_alloc * 8ull);
+    if (!in.ok) return snapshot_error("truncated");
+    img.pool_off = ftello(in.f);
+    img.pool_bytes = (size_t)n_alloc * bb;
+    std::vector<unsigned char> buf(img.pool_bytes < kChunk ? img.pool_bytes : kChunk);
+    for (size_t off = 0; off < img.pool_bytes && in.ok; off += buf.size()) {
+        const size_t c = img.pool_bytes - off < buf.size() ? img.pool_bytes - off : buf.size();
+        in.get(buf.data(), c);
+    }
+    if (!in.ok || !in.checksum()) return snapshot_error("truncated or corrupt (checksum)");
+    if (const char* why = check_map_image(p, img)) return snapshot_error(why);
+    return FU_OK;
+}
+
+// the checked image onto the device: entries, keys, the occupancy bitmap (a function of the
+// keys), the pool (read again from the file) and the map-wide counters; the per-frame
+// counters start at zero as after a reset
+fu_status commit_map(fu_map* m, FILE* f, const MapImage& img, hipStream_t s) {
+    MapView& v = m->v;
     std::vector<unsigned> occ((v.occ_mask + 1) / 32, 0u);
-    for (unsigned long long key : keys)
+    for (unsigned long long key : img.keys)
         if (key != kEmptyKey) {
             const unsigned hb = occ_bit(v, block_hash_raw(unpack_key(key)));
             occ[hb >> 5] |= 1u << (hb & 31);
         }
+    Pinned buf;
+    HIPOK(hipHostMalloc(&buf.p, kChunk, hipHostMallocDefault));
+    if (fseeko(f, img.pool_off, SEEK_SET) != 0) return snapshot_error("cannot re-read the pool");
     HIPOK(hipStreamSynchronize(s));
     launch_reset_map(v, s);
     HIPOK(hipStreamSynchronize(s));
-    HIPOK(hipMemcpy(v.entries, ent.data(), ne * sizeof(HEntry), hipMemcpyHostToDevice));
-    if (n_alloc) HIPOK(hipMemcpy(v.block_key, keys.data(), n_alloc * 8ull, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(v.entries, img.ent.data(), img.ent.size() * sizeof(HEntry), hipMemcpyHostToDevice));
+    if (img.n_alloc) HIPOK(hipMemcpy(v.block_key, img.keys.data(), img.n_alloc * 8ull, hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(v.occ, occ.data(), occ.size() * 4, hipMemcpyHostToDevice));
-    Pinned buf;
-    HIPOK(hipHostMalloc(&buf.p, kChunk, hipHostMallocDefault));
-    const bool pool_ok = file_to_dev(in, v.pool, (size_t)n_alloc * bb, buf);
-    if (!pool_ok || !in.checksum()) return fail("truncated or corrupt");
-    // counters: the map-wide ones; the per-frame ones start at zero as after a reset
+    In in;
+    in.f = f;
+    if (!file_to_dev(in, v.pool, img.pool_bytes, buf)) {
+        launch_reset_map(v, s);
+        (void)hipStreamSynchronize(s);
+        return snapshot_error("the file changed while it was loaded (the map was left empty)");
+    }
     unsigned c[C_COUNT] = {};
-    c[C_ALLOC] = ctr[C_ALLOC];
-    c[C_EXCESS] = ctr[C_EXCESS];
-    c[C_FLAGS] = ctr[C_FLAGS];
+    c[C_ALLOC] = img.ctr[C_ALLOC];
+    c[C_EXCESS] = img.ctr[C_EXCESS];
+    c[C_FLAGS] = img.ctr[C_FLAGS];
     HIPOK(hipMemcpy(v.ctr, c, sizeof(c), hipMemcpyHostToDevice));
     return FU_OK;
 }
@@ -264,7 +343,20 @@ extern "C" fu_status fu_map_load(fu_map* m, const char* path, fu_stream s) {
     if (!f.f) return api_error(FU_ERR_ARG, "fu_map_load: cannot open %s", path);
     In in;
     in.f = f.f;
-    return read_map(m, in, (hipStream_t)s);
+    MapImage img;
+    fu_status st = parse_map(m->p, in, img);
+    return st ? st : commit_map(m, f.f, img, (hipStream_t)s);
+}
+
+// the checks of fu_map_load on the host alone
+extern "C" fu_status fu_map_check_file(const fu_map_params* p, const char* path) {
+    if (!p || !path) return api_error(FU_ERR_ARG, "fu_map_check_file: null argument");
+    File f(open_file(path, "rb"));
+    if (!f.f) return api_error(FU_ERR_ARG, "fu_map_check_file: cannot open %s", path);
+    In in;
+    in.f = f.f;
+    MapImage img;
+    return parse_map(*p, in, img);
 }
 
 // ---- session --------------------------------------------------------------------------------
@@ -324,18 +416,29 @@ extern "C" fu_status fu_system_load(fu_system* sys, const char* path, fu_stream 
     IOCHK(cols == p.cols && rows == p.rows && rgb == p.use_rgb && fx == p.fx && fy == p.fy && cx == p.cx &&
               cy == p.cy && ds == p.depth_scale && dc == p.depth_cutoff,
           "fu_system_load: written by a session of other parameters (%dx%d, use_rgb %d)", cols, rows, rgb);
+    // the session part staged on the host and the map part checked before any device write,
+    // so a refused file leaves the session as it was
     TrackState st;
     in.get(&st, sizeof(st));
-    HIPOK(hipStreamSynchronize((hipStream_t)s));
-    Pinned buf;
-    HIPOK(hipHostMalloc(&buf.p, kChunk, hipHostMallocDefault));
     const size_t n0 = (size_t)p.cols * p.rows;
-    bool ok = file_to_dev(in, sys->b.model_vmap[0], n0 * 16, buf) && file_to_dev(in, sys->b.model_nmap[0], n0 * 16, buf);
+    std::vector<unsigned char> vm(n0 * 16), nm(n0 * 16), img[3];
+    in.get(vm.data(), vm.size());
+    in.get(nm.data(), nm.size());
     if (p.use_rgb)
-        for (int l = 0; l < 3 && ok; ++l) ok = file_to_dev(in, sys->b.last_image[l], (size_t)sys->lc[l] * sys->lr[l], buf);
-    IOCHK(ok && in.checksum(), "fu_system_load: truncated or corrupt session part");
-    fu_status r = read_map(sys->map, in, (hipStream_t)s);
+        for (int l = 0; l < 3; ++l) {
+            img[l].resize((size_t)sys->lc[l] * sys->lr[l]);
+            in.get(img[l].data(), img[l].size());
+        }
+    IOCHK(in.ok && in.checksum(), "fu_system_load: truncated or corrupt session part");
+    MapImage mi;
+    fu_status r = parse_map(sys->map->p, in, mi);
     if (r) return r;
+    hipStream_t hs = (hipStream_t)s;
+    if ((r = commit_map(sys->map, f.f, mi, hs))) return r;
+    HIPOK(hipMemcpy(sys->b.model_vmap[0], vm.data(), vm.size(), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(sys->b.model_nmap[0], nm.data(), nm.size(), hipMemcpyHostToDevice));
+    if (p.use_rgb)
+        for (int l = 0; l < 3; ++l) HIPOK(hipMemcpy(sys->b.last_image[l], img[l].data(), img[l].size(), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(sys->trk->st, &st, sizeof(st), hipMemcpyHostToDevice));
     sys->frame = frame;
     return FU_OK;

@@ -107,7 +107,7 @@ class Completion(C.Structure):
 class TraceFrame(C.Structure):
     _fields_ = [("session", C.c_int32), ("depth", C.c_void_p), ("rgb", C.c_void_p), ("arrival_ms", C.c_double),
                 ("tag", C.c_uint64), ("kind", C.c_int32), ("max_triangles", C.c_uint32), ("vertex", C.c_void_p),
-                ("normal", C.c_void_p), ("color", C.c_void_p)]
+                ("normal", C.c_void_p), ("color", C.c_void_p), ("host", C.c_int32)]
 
 
 SIGNATURES = {
@@ -144,6 +144,7 @@ SIGNATURES = {
     "fu_selftest_arith": (I, [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32), P]),
     "fu_map_save": (I, [P, C.c_char_p, P]),
     "fu_map_load": (I, [P, C.c_char_p, P]),
+    "fu_map_check_file": (I, [C.POINTER(MapParams), C.c_char_p]),
     "fu_system_save": (I, [P, C.c_char_p, P]),
     "fu_system_load": (I, [P, C.c_char_p, P]),
     "fu_mesh_write_ply": (I, [C.c_char_p, P, P, P, C.c_uint32, I, P]),
@@ -160,6 +161,8 @@ SIGNATURES = {
     "fu_system_map": (P, [P]),
     "fu_system_grab_image": (I, [P, P]),
     "fu_system_grab_image_from": (I, [P, P, P, P]),
+    "fu_system_grab_image_host": (I, [P, P, P, P]),
+    "fu_system_graph_counts": (I, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
     "fu_system_set_profiling": (I, [P, I]),
     "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
@@ -187,6 +190,7 @@ SIGNATURES = {
     "fu_server_system": (P, [P, I]),
     "fu_server_now_ms": (C.c_double, [P]),
     "fu_server_submit": (I, [P, I, P, P, C.c_uint64, C.c_double]),
+    "fu_server_submit_host": (I, [P, I, P, P, C.c_uint64, C.c_double]),
     "fu_server_submit_mesh": (I, [P, I, P, P, P, C.c_uint32, C.c_uint64, C.c_double]),
     "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),
     "fu_server_pending": (I, [P, C.POINTER(I)]),

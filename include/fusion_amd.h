@@ -298,8 +298,16 @@ fu_map* fu_system_map(fu_system* sys);
  * (copy the frame into depth_in/rgb_in first, on the same stream): pyramid, tracking
  * against the model raycast, fusion, raycast. Asynchronous. */
 fu_status fu_system_grab_image(fu_system* sys, fu_stream s);
-/* the same on caller-provided device images */
+/* the same on caller-provided device images.  With use_graph, frames after the first run as
+ * one hipGraph, captured once per system against the input slots: the images are copied into
+ * depth_in/rgb_in on `s` (unless they are the slots) and the graph is replayed. */
 fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s);
+/* the same on host images, uploaded into the input slots on `s` as Frame::Frame does
+ * (Core/Frame.cc:61-62); asynchronous for pinned (hipHostMalloc) memory, which must then stay
+ * unchanged until the copy has run */
+fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s);
+/* graphs captured and replayed by this system (no reference counterpart) */
+fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches);
 /* outcome of the last frame; synchronises */
 fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
 
@@ -321,17 +329,24 @@ fu_status fu_selftest_arith(uint64_t seed, uint32_t n, uint32_t mismatches[7], f
  * their indices), the allocated blocks' keys and voxels and the map counters, in a versioned
  * little-endian file with an FNV-1a checksum.  Loading needs a map of the same parameters
  * (buckets, excess, blocks, voxel size/format, truncation, depth range, weight cap) and
- * rebuilds the same entries and pool slots, so the map continues exactly as before.  A
- * file of other parameters is refused with the map untouched; a truncated or corrupt one
- * is refused and leaves the map empty.  Both synchronise `s`. */
+ * rebuilds the same entries and pool slots, so the map continues exactly as before.  The
+ * whole file is read and checked on the host before the device map is touched: the checksum,
+ * then the structure the kernels rely on (entry keys in their buckets, excess links in range,
+ * chains acyclic with each excess entry on one chain, block indices below the allocated count
+ * and agreeing with the block keys, no key allocated twice).  A file of other parameters, or
+ * one that fails a check, is refused with the map unchanged.  Both synchronise `s`. */
 fu_status fu_map_save(fu_map* m, const char* path, fu_stream s);
 fu_status fu_map_load(fu_map* m, const char* path, fu_stream s);
+/* the checks of fu_map_load on the host alone, against a map of parameters `p` (no device
+ * access; no reference counterpart) */
+fu_status fu_map_check_file(const fu_map_params* p, const char* path);
 /* Session snapshot (no reference counterpart; eviction and resume of a served session): the
  * map snapshot plus the tracking state, the frame count, the level-0 model maps the next frame
  * is tracked against and, with use_rgb, the last frame's intensity pyramid.  Loading needs a
  * system of the same frame size, intrinsics, depth scale/cut-off, tracking mode and map
  * parameters; the next frame then tracks and fuses exactly as it would have without the
- * round trip.  After a failed load, reset the session (fu_system_reset) before using it. */
+ * round trip.  The session part is read into host memory and the map part checked as by
+ * fu_map_load before anything is written, so a refused file leaves the session unchanged. */
 fu_status fu_system_save(fu_system* sys, const char* path, fu_stream s);
 fu_status fu_system_load(fu_system* sys, const char* path, fu_stream s);
 /* System::WriteMeshToDisk (Core/System.cc:140-193): a triangle soup of 3 vertices per
@@ -468,7 +483,7 @@ typedef struct {
 #define FU_REQ_MESH 1
 typedef struct {
     int32_t session;
-    const uint16_t* depth; /* device-resident frame of the session's size */
+    const uint16_t* depth; /* frame of the session's size: device-resident, or host memory with `host` */
     const uint8_t* rgb;
     double arrival_ms;     /* offset from the start of the trace */
     uint64_t tag;
@@ -477,6 +492,7 @@ typedef struct {
     float* vertex;
     float* normal;
     uint8_t* color;
+    int32_t host;          /* nonzero: depth and rgb are host memory, as fu_server_submit_host */
 } fu_trace_frame;
 void fu_server_params_default(fu_server_params* p);
 fu_status fu_server_create(const fu_server_params* p, fu_server** out);
@@ -488,6 +504,12 @@ double fu_server_now_ms(fu_server* sv);
 /* queue one frame; t_submit_ms < 0 means now */
 fu_status fu_server_submit(fu_server* sv, int32_t session, const uint16_t* depth, const uint8_t* rgb, uint64_t tag,
                            double t_submit_ms);
+/* the same for a frame in host memory (pinned, hipHostMalloc, for an asynchronous copy): the
+ * step that takes it uploads it into the session's input slots on its lane's stream, as
+ * Frame::Frame uploads each frame (Core/Frame.cc:61-62); the host buffers must stay unchanged
+ * until the frame's completion is returned */
+fu_status fu_server_submit_host(fu_server* sv, int32_t session, const uint16_t* depth, const uint8_t* rgb,
+                                uint64_t tag, double t_submit_ms);
 /* queue a mesh extraction of the session's map (fu_mesh_scene), ordered after the session's
  * frames submitted before it; the output buffers (9 floats, 9 floats and 9 bytes per
  * triangle) must stay untouched until its completion is returned */

@@ -33,6 +33,8 @@ _SIG = {
     "or_euler_sine_norm": (C.c_double, [DP]),
     "or_compute_se3": (I, [C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), I, I, FP,
                            C.POINTER(I), C.POINTER(I), F, DP, DP, FP, C.POINTER(I)]),
+    "or_compute_se3_trace": (I, [C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), I, I, FP,
+                                 C.POINTER(I), C.POINTER(I), F, DP, DP, DP, C.POINTER(I)]),
     "or_pose_f": (None, [DP, FP, FP, FP]),
     "or_image_to_intensity": (None, [P, P, I, I]),
     "or_pyrdown_u8": (None, [P, I, I, P]),
@@ -211,6 +213,24 @@ def compute_se3(next_v, next_n, last_v, last_n, K, last_pose, iters=(10, 5, 4), 
                               (C.c_int * 3)(*iters), (C.c_int * 3)(*min_count), thresh, lp.ctypes.data_as(DP),
                               out.ctypes.data_as(DP), C.byref(err), C.byref(cnt))
     return st, out.reshape(4, 4), err.value, cnt.value
+
+
+def compute_se3_trace(next_v, next_n, last_v, last_n, K, last_pose, iters=(10, 5, 4), min_count=(2000, 1000, 100),
+                      thresh=1e-4):
+    """compute_se3 with every iteration recorded: (status, pose, [(level, pose_curr 4x4, x 6)])"""
+    keep = [np.ascontiguousarray(a, dtype=np.float32) for a in list(next_v) + list(next_n) + list(last_v) + list(last_n)]
+    arr = lambda k: (C.c_void_p * 3)(*[keep[k * 3 + i].ctypes.data for i in range(3)])
+    rows, cols, _ = keep[0].shape
+    lp = np.ascontiguousarray(last_pose, dtype=np.float64).ravel()
+    out = np.zeros(16)
+    tr = np.zeros((sum(iters), 22))
+    n = C.c_int()
+    st = lib().or_compute_se3_trace(arr(0), arr(1), arr(2), arr(3), cols, rows, (C.c_float * 4)(*K),
+                                    (C.c_int * 3)(*iters), (C.c_int * 3)(*min_count), thresh, lp.ctypes.data_as(DP),
+                                    out.ctypes.data_as(DP), tr.ctypes.data_as(DP), C.byref(n))
+    levels = [l for l in (2, 1, 0) for _ in range(iters[l])]
+    steps = [(levels[k], tr[k, :16].reshape(4, 4), tr[k, 16:]) for k in range(n.value)]
+    return st, out.reshape(4, 4), steps
 
 
 def image_to_intensity(rgb):

@@ -652,8 +652,9 @@ struct RgbImages {
 int compute_se3(const float* const next_vmap[3], const float* const next_nmap[3], const float* const last_vmap[3],
                 const float* const last_nmap[3], const RgbImages* rgb, int cols, int rows, const float K[4],
                 const int iters[3], const int min_count[3], float thresh, const double last_pose[16], double pose[16],
-                float* err_out, int* count_out, int* rgb_count_out) {
+                float* err_out, int* count_out, int* rgb_count_out, double* trace = nullptr, int* n_trace = nullptr) {
     double next[16], delta[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    if (n_trace) *n_trace = 0;
     memcpy(next, last_pose, sizeof(next));
     float Rl[9], Rli[9], tl[3];
     or_pose_f(last_pose, Rl, Rli, tl);
@@ -698,6 +699,12 @@ int compute_se3(const float* const next_vmap[3], const float* const next_nmap[3]
             }
             double x[6], dT[16], dinv[16];
             if (or_solve6(A, b, x)) return fail();
+            if (trace) {  // the pose this iteration's sums were taken at, and its update x
+                memcpy(trace, next, 16 * sizeof(double));
+                memcpy(trace + 16, x, 6 * sizeof(double));
+                trace += 22;
+                ++*n_trace;
+            }
             or_se3_exp(x, dT);
             mm4(dT, delta, delta);
             inv4(delta, dinv);
@@ -726,6 +733,16 @@ extern "C" int or_compute_se3(const float* const next_vmap[3], const float* cons
                               const double last_pose[16], double pose[16], float* err_out, int* count_out) {
     return compute_se3(next_vmap, next_nmap, last_vmap, last_nmap, nullptr, cols, rows, K, iters, min_count, thresh,
                        last_pose, pose, err_out, count_out, nullptr);
+}
+
+// the same, recording every iteration: the current pose its sums were taken at (16 doubles)
+// and its solution x = (upsilon, omega) (6 doubles); *n_trace iterations
+extern "C" int or_compute_se3_trace(const float* const next_vmap[3], const float* const next_nmap[3],
+                                    const float* const last_vmap[3], const float* const last_nmap[3], int cols, int rows,
+                                    const float K[4], const int iters[3], const int min_count[3], float thresh,
+                                    const double last_pose[16], double pose[16], double* trace, int* n_trace) {
+    return compute_se3(next_vmap, next_nmap, last_vmap, last_nmap, nullptr, cols, rows, K, iters, min_count, thresh,
+                       last_pose, pose, nullptr, nullptr, nullptr, trace, n_trace);
 }
 
 // Tracker::ComputeSE3(icpOnly = false): RGBStep blended into every ICP iteration

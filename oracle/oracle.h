@@ -78,6 +78,12 @@ int or_compute_se3(const float* const next_vmap[3], const float* const next_nmap
                    const float* const last_nmap[3], int cols, int rows, const float K[4], const int iters[3],
                    const int min_count[3], float thresh, const double last_pose[16], double pose[16],
                    float* err, int* count);
+/* the same with every iteration recorded: 22 doubles each (the 4x4 current pose its sums were
+ * taken at, then its solution x = (upsilon, omega)); trace holds sum(iters) * 22 doubles */
+int or_compute_se3_trace(const float* const next_vmap[3], const float* const next_nmap[3],
+                         const float* const last_vmap[3], const float* const last_nmap[3], int cols, int rows,
+                         const float K[4], const int iters[3], const int min_count[3], float thresh,
+                         const double last_pose[16], double pose[16], double* trace, int* n_trace);
 /* the acceptance test's |sin(XYZ Euler angles)| of a motion (Tracking.cc:398-408), 4x4 row-major */
 double or_euler_sine_norm(const double p[16]);
 /* photometric term (Tracking/Pyrdown.cu:127-249, Tracking/Reduction.cu:377-627) */

@@ -93,24 +93,61 @@ def test_map_snapshot_round_trip_and_refusals(gpu, tmp_path):
     m3 = A.Mapping(**dict(MAP, num_buckets=1 << 16))
     with pytest.raises(A.N.FusionError):
         m3.load(path)
-    # damaged files: refused, and a map that was being loaded is left empty
+    # damaged files: refused, and the map they were loaded into is left as it was
     raw = open(path, "rb").read()
     bad = str(tmp_path / "bad.fusn")
-    open(bad, "wb").write(raw[: len(raw) // 2])
-    with pytest.raises(A.N.FusionError):
-        m2.load(bad)
-    assert m2.stats()["n_allocated"] == 0
+
+    def refused(data):
+        open(bad, "wb").write(bytes(data))
+        with pytest.raises(A.N.FusionError):
+            m2.load(bad)
+        now = _blocks(m2)
+        assert np.array_equal(ref[0], now[0]) and same_bits(ref[1], now[1]) and np.array_equal(ref[2], now[2])
+
+    refused(raw[: len(raw) // 2])
     flip = bytearray(raw)
     flip[len(raw) - 100] ^= 0x40
-    open(bad, "wb").write(bytes(flip))
-    with pytest.raises(A.N.FusionError):
-        m2.load(bad)
-    open(bad, "wb").write(b"not a snapshot at all")
-    with pytest.raises(A.N.FusionError):
-        m2.load(bad)
+    refused(flip)
+    refused(b"not a snapshot at all")
+    # a structural error under a correct checksum: the first entry's block index past the
+    # allocated blocks (entries start after the 124-byte header; 20 bytes each)
+    struct_bad = bytearray(raw[:-8])
+    n_alloc = int.from_bytes(struct_bad[116:120], "little")
+    struct_bad[124 + 12:124 + 16] = (n_alloc * 512).to_bytes(4, "little")
+    h = 1469598103934665603
+    for b in struct_bad:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    refused(struct_bad + h.to_bytes(8, "little"))
     with pytest.raises(A.N.FusionError):
         s.load(path)  # a map snapshot is not a session snapshot
     s.close()
+
+
+def test_refused_session_load_leaves_the_session_unchanged(gpu, tmp_path):
+    _, _, frames = scene_frames(frames=range(7))
+    dev = [A.to_device(*f) for f in frames]
+    a, b = _system(), _system()
+    for f in dev[:4]:
+        a.GrabImage(*f)
+        b.GrabImage(*f)
+    path = str(tmp_path / "s.fusn")
+    a.save(path)
+    raw = open(path, "rb").read()
+    bad = str(tmp_path / "bad.fusn")
+    for data in (raw[: len(raw) - 4096], raw[: len(raw) // 3], raw[:-1] + bytes([raw[-1] ^ 1])):
+        open(bad, "wb").write(data)
+        with pytest.raises(A.N.FusionError):
+            b.load(bad)
+    for f in dev[4:]:
+        a.GrabImage(*f)
+        b.GrabImage(*f)
+        ra, rb = a.result(), b.result()
+        assert ra["track_status"] == rb["track_status"] == 1 and np.array_equal(ra["pose"], rb["pose"])
+    xa, sa, wa, ca = _blocks(a.mapping())
+    xb, sb, wb, cb = _blocks(b.mapping())
+    assert np.array_equal(xa, xb) and same_bits(sa, sb) and np.array_equal(wa, wb)
+    a.close()
+    b.close()
 
 
 def _ply_header(n_vert, n_face, fmt):

@@ -32,8 +32,10 @@ def _params(i, K, use_rgb=0):
     return A.system_params(cols=cols, rows=rows, K=K, map=MAP, use_rgb=use_rgb)
 
 
-@pytest.mark.parametrize("closed_loop,use_rgb", [(False, 0), (True, 0), (False, 1)])
-def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb):
+@pytest.mark.parametrize("closed_loop,use_rgb,host", [(False, 0, False), (True, 0, False), (False, 1, False),
+                                                      (False, 0, True)])
+def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb, host):
+    """host: frames in pinned host memory, uploaded inside the steps that take them"""
     frames = _frames()
     alone_res, alone_state = _alone(frames, use_rgb)
     sv = A.Server(max_batch=3, max_inflight=2, lanes=2)
@@ -43,6 +45,8 @@ def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb):
     for i, (K, fr) in enumerate(frames):
         t = float(rng.uniform(0, 5))
         for f, (d, c) in enumerate(fr):
+            if host:
+                d, c = d.cpu().pin_memory(), c.cpu().pin_memory()
             trace.append((sids[i], d, c, t, i * 1000 + f))
             t += float(rng.exponential(4.0))  # about 250 frames/s per session, jittered
     torch.cuda.synchronize()

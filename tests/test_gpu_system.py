@@ -84,6 +84,37 @@ def test_modes_agree(gpu):
         assert np.abs(a["pose"] - c["pose"]).max() < 1e-9
 
 
+def test_graph_captured_once_and_replayed(gpu):
+    """use_graph: one capture for the whole run, every later frame a replay, with frames from
+    new device tensors each time and from pinned host memory; the map and poses equal the
+    stream-ordered run's bit for bit"""
+    sc, K, frames = scene_frames(frames=range(8))
+    s_dev, dev = _run_gpu(frames)
+    s_gra, gra = _run_gpu(frames, use_graph=True)
+    assert s_gra.graph_counts() == (1, 7)
+    s_host = A.System(A.system_params(use_graph=True, map=MAP))
+    host = []
+    for d16, rgb in frames:
+        hd = torch.from_numpy(d16.view(np.int16)).pin_memory()
+        hc = torch.from_numpy(rgb).pin_memory()
+        s_host.GrabImageHost(hd, hc)
+        host.append(s_host.result())
+    assert s_host.graph_counts() == (1, 7)
+    for a, b, c in zip(dev, gra, host):
+        assert a["track_status"] == b["track_status"] == c["track_status"]
+        assert np.array_equal(a["pose"], b["pose"]) and np.array_equal(a["pose"], c["pose"])
+    blocks = s_dev.mapping().export_blocks()
+    ref = s_dev.mapping().read_blocks(blocks)
+    for s in (s_gra, s_host):
+        assert rows_set(s.mapping().export_blocks()) == rows_set(blocks)
+        got = s.mapping().read_blocks(blocks)
+        for x, y in zip(ref, got):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+        for name in ("model_vmap", "model_nmap"):
+            assert np.array_equal(s.view(name, 0).cpu().numpy().view(np.uint32),
+                                  s_dev.view(name, 0).cpu().numpy().view(np.uint32))
+
+
 def test_failed_frame_keeps_model(gpu):
     """an empty frame fails tracking: pose rolled back, nothing fused (System.cc:108)"""
     sc, K, frames = scene_frames(frames=(0, 1))
