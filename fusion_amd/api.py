@@ -442,6 +442,12 @@ class System:
         ptr = lambda a: C.c_void_p(a.data_ptr() if isinstance(a, torch.Tensor) else a.ctypes.data)
         check(N.lib().fu_system_grab_image_host(self.handle, ptr(depth), ptr(rgb), _s()), "fu_system_grab_image_host")
 
+    def relocalise(self, pose):
+        """a lost system's relocalisation at `pose` (found by the caller): the next frame is
+        raycast there, tracking resumes after it and fusion 10 frames later"""
+        arr = (C.c_double * 16)(*np.asarray(pose, dtype=np.float64).ravel())
+        check(N.lib().fu_system_relocalise(self.handle, arr, _s()), "fu_system_relocalise")
+
     def graph_counts(self):
         """(graphs captured, graph launches) of this system"""
         c, n = C.c_uint32(), C.c_uint32()

@@ -611,7 +611,7 @@ extern "C" fu_status fu_resize_map(const float* vsrc, const float* nsrc, int col
 // pixels per lane, a kIcpPixels chunk each.  Every
 // pixel's association is a dependent gather, so latency is hidden across waves rather than
 // by a per-lane loop.  Larger frames loop over chunks.
-static constexpr int kIcpMaxBlocks = 4096;
+static_assert(kIcpMaxBlocks == 4096, "the partial-row buffers hold kIcpMaxBlocks rows");
 static int icp_blocks(int npix, int per_group = kIcpThreads) {
     int nb = (npix + per_group - 1) / per_group;
     if (nb < 1) nb = 1;
@@ -627,6 +627,9 @@ static void init_state(TrackState* h) {
         h->last_pose[i] = h->next_pose[i] = h->init_pose[i] = h->delta[i] = I;
     }
     h->status = TS_FIRST;
+    h->sm_state = 1;  // Tracker::state before the first frame
+    h->sm_prev = 1;
+    h->after_reloc = 10;  // System::num_frames_after_reloc (System.cc:72)
     refresh_float_pose(h);
 }
 
@@ -821,20 +824,13 @@ extern "C" fu_status fu_tracker_result(fu_tracker* t, fu_track_result* out, fu_s
     return FU_OK;
 }
 
-extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const float* next_nmap,
-                                 const float* last_nmap, int cols, int rows, const float Rcurr[9], const float tcurr[3],
-                                 const float Rlast[9], const float RlastInv[9], const float tlast[3], float fx,
-                                 float fy, float cx, float cy, float residual[2], double A[36], double b[6],
-                                 fu_stream s) {
-    if (!next_vmap || !last_vmap || !next_nmap || !last_nmap || !Rcurr || !tcurr || !Rlast || !RlastInv || !tlast)
-        return set_err(FU_ERR_ARG, "fu_icp_step: null argument");
-    if (cols <= 0 || rows <= 0) return set_err(FU_ERR_ARG, "fu_icp_step: bad size");
-    // scratch state per call: the reference's Tracker owns sumSE3/outSE3 (Tracking.cc:29-30)
-    static thread_local fu_tracker* scratch = nullptr;
-    if (!scratch) {
-        fu_status st = fu_tracker_create(8, 8, &scratch);
-        if (st) return st;
-    }
+// ICPStep's body on a caller-given state and partial-row buffer (kIcpMaxBlocks rows of 64
+// floats): the pose into `st`, the reduction, the sums back to the host.  Synchronises.
+fu_status fu::icp_step_on(TrackState* st, float* partials, const float* next_vmap, const float* last_vmap,
+                          const float* next_nmap, const float* last_nmap, int cols, int rows, const float Rcurr[9],
+                          const float tcurr[3], const float Rlast[9], const float RlastInv[9], const float tlast[3],
+                          float fx, float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                          hipStream_t s) {
     TrackState h;
     memset(&h, 0, sizeof(h));
     memcpy(h.Rcurr, Rcurr, 36);
@@ -843,7 +839,7 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
     memcpy(h.RlastInv, RlastInv, 36);
     memcpy(h.tlast, tlast, 12);
     h.status = TS_RUNNING;
-    HIPCHK(hipMemcpyAsync(scratch->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
     IcpArgs a;
     memset(&a, 0, sizeof(a));
     a.vcurr = (const f4*)next_vmap;
@@ -856,13 +852,13 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
     a.fy = fy;
     a.cx = cx;
     a.cy = cy;
-    a.st = scratch->st;
-    a.partials = scratch->partials;
+    a.st = st;
+    a.partials = partials;
     a.mode = 0;
-    launch_icp(a, icp_groups(cols * rows), S(s));
+    launch_icp(a, icp_groups(cols * rows), s);
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(&h, scratch->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
-    HIPCHK(hipStreamSynchronize(S(s)));
+    HIPCHK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     // CreateMatrix<6,7> (Reduction.cu:6-16)
     int shift = 0;
     for (int i = 0; i < 6; ++i)
@@ -879,6 +875,24 @@ extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap,
         residual[1] = h.sums[28];
     }
     return FU_OK;
+}
+
+extern "C" fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const float* next_nmap,
+                                 const float* last_nmap, int cols, int rows, const float Rcurr[9], const float tcurr[3],
+                                 const float Rlast[9], const float RlastInv[9], const float tlast[3], float fx,
+                                 float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                                 fu_stream s) {
+    if (!next_vmap || !last_vmap || !next_nmap || !last_nmap || !Rcurr || !tcurr || !Rlast || !RlastInv || !tlast)
+        return set_err(FU_ERR_ARG, "fu_icp_step: null argument");
+    if (cols <= 0 || rows <= 0) return set_err(FU_ERR_ARG, "fu_icp_step: bad size");
+    // scratch state per call: the reference's Tracker owns sumSE3/outSE3 (Tracking.cc:29-30)
+    static thread_local fu_tracker* scratch = nullptr;
+    if (!scratch) {
+        fu_status st = fu_tracker_create(8, 8, &scratch);
+        if (st) return st;
+    }
+    return icp_step_on(scratch->st, scratch->partials, next_vmap, last_vmap, next_nmap, last_nmap, cols, rows, Rcurr,
+                       tcurr, Rlast, RlastInv, tlast, fx, fy, cx, cy, residual, A, b, S(s));
 }
 
 // RGBStep (Reduction.h:74-84, Reduction.cu:537-627): the photometric reduction of one level.
@@ -1219,14 +1233,8 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     HIPCHK(hipStreamSynchronize(s));
     TrackState h;
     HIPCHK(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
-    if (!first) {
-        for (int i = 0; i < 16; ++i) {
-            h.next_pose[i] = h.init_pose[i] = h.last_pose[i];
-            h.delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
-        }
-        h.status = TS_RUNNING;
-        h.iters_done = 0;
-        refresh_float_pose(&h);
+    if (!first) frame_begin(&h);  // a lost tracker does not track
+    if (!first && h.status == TS_RUNNING) {
         const float K[4] = {p.fx, p.fy, p.cx, p.cy};
         int last_level = -1;
         for (int l = 2; l >= 0; --l)
@@ -1256,18 +1264,18 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
                 icp_update(&h, sums, p.use_rgb ? rsums : nullptr, p.min_count[l], fin, p.icp_thresh);
             }
     }
-    if (h.status == TS_OK || h.status == TS_FIRST)
-        for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i];
-    refresh_float_pose(&h);
+    frame_commit(&h);  // what the frame does, and SwapFrame of its pose
     HIPCHK(hipMemcpyAsync(t->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    if (p.use_rgb && (h.status == TS_OK || h.status == TS_FIRST))  // Tracker::SwapFrame
+    if (p.use_rgb && (h.gates & GATE_SWAP))  // Tracker::SwapFrame
         for (int l = 0; l < 3; ++l)
             HIPCHK(hipMemcpyAsync(b.last_image[l], b.image[l], (size_t)sys->lc[l] * sys->lr[l], hipMemcpyDeviceToDevice, s));
-    if (h.status != TS_OK && h.status != TS_FIRST) {
+    if (!(h.gates & GATE_RENDER)) {
         HIPCHK(hipStreamSynchronize(s));
         return FU_OK;
     }
-    // FuseMapColor with its synchronisations (FuseMap.cu:294-317)
+    // FuseMapColor with its synchronisations (FuseMap.cu:294-317), or UpdateVisibility alone
+    // when the frame does not fuse
+    const bool fuse = (h.gates & GATE_FUSE) != 0;
     fu_map* m = sys->map;
     PoseF dummy{};
     FuseArgs a = make_fuse(m, b.raw, rgb, b.nmap[0], p.cols, p.rows, dummy, t->st, p.fx, p.fy, p.cx, p.cy);
@@ -1275,14 +1283,16 @@ static fu_status run_frame_plain(fu_system* sys, const uint16_t* depth, const ui
     a.prgb = sys->fuse_rgb;
     const VisArgs v = make_vis(m, dummy, t->st, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
     launch_begin_pass(m->v, 7, s);
-    launch_fuse_pack(a, s);
-    launch_alloc(m->v, a, s);
-    HIPCHK(hipStreamSynchronize(s));
+    if (fuse) {
+        launch_fuse_pack(a, s);
+        launch_alloc(m->v, a, s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
     launch_visible(m->v, v, s);
     fu_status st = FU_OK;
     const unsigned nvis = read_ctr(m, C_VISIBLE, s, &st);
     if (st) return st;
-    if (nvis > 0) {
+    if (nvis > 0 && fuse) {
         launch_integrate(m->v, a, nvis < m->p.num_blocks ? nvis : m->p.num_blocks, s);
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -1415,6 +1425,20 @@ extern "C" fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* d
     return fu_system_grab_image_from(sys, sys->b.depth_in, sys->b.rgb_in, s);
 }
 
+extern "C" fu_status fu_system_relocalise(fu_system* sys, const double pose[16], fu_stream s) {
+    if (!sys || !pose) return set_err(FU_ERR_ARG, "fu_system_relocalise: null argument");
+    TrackState h;
+    HIPCHK(hipMemcpyAsync(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    if (h.sm_state != -1) return set_err(FU_ERR_STATE, "fu_system_relocalise: the system is not lost");
+    for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i] = pose[i];
+    h.reloc = 1;
+    refresh_float_pose(&h);
+    HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    return FU_OK;
+}
+
 extern "C" fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches) {
     if (!sys || !captures || !launches) return set_err(FU_ERR_ARG, "null argument");
     *captures = sys->graph_captures;
@@ -1431,7 +1455,7 @@ extern "C" fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_s
     HIPCHK(hipStreamSynchronize(S(s)));
     memcpy(out->pose, h.next_pose, sizeof(out->pose));
     out->track_status = h.status;
-    out->fused = (h.status == TS_OK || h.status == TS_FIRST) ? 1 : 0;
+    out->fused = (h.gates & GATE_FUSE) ? 1 : 0;
     out->icp_error = h.icp_error;
     out->icp_count = h.count;
     out->n_visible = c[C_VISIBLE];

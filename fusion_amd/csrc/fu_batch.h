@@ -190,5 +190,11 @@ fu_status api_error(fu_status code, const char* fmt, ...);
 fu_status mesh_async(fu_system* sys, float* vertex, float* normal, uint8_t* color, uint32_t max_triangles,
                      hipStream_t s);
 const unsigned* system_counters(fu_system* sys);
+// ICPStep on a caller-given tracking state and kIcpMaxBlocks x 64-float partial-row buffer
+fu_status icp_step_on(TrackState* st, float* partials, const float* next_vmap, const float* last_vmap,
+                      const float* next_nmap, const float* last_nmap, int cols, int rows, const float Rcurr[9],
+                      const float tcurr[3], const float Rlast[9], const float RlastInv[9], const float tlast[3],
+                      float fx, float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                      hipStream_t s);
 
 }  // namespace fu

@@ -15,6 +15,15 @@ enum TrackStatus : int {
     TS_FAILED = 2,    // NaN error or too few correspondences (:371-375)
     TS_REJECTED = 3,  // final error / motion test failed (:404-407)
     TS_FIRST = 4,     // first frame: identity pose, no tracking (Tracking.cc:66-71)
+    TS_LOST = 5,      // lost (more than 9 missed frames, Tracking.cc:92-96): not tracked, nothing fused
+    TS_RELOC = 6,     // lost, relocalised at a pose given from outside: no tracking, raycast there
+};
+
+// what a frame does after tracking (System::GrabImage, Core/System.cc:92-138)
+enum FrameGate : int {
+    GATE_FUSE = 1,    // allocation and integration (Mapping::FuseColor)
+    GATE_RENDER = 2,  // visible list, z-range tiles and raycast (UpdateVisibility / RayTrace)
+    GATE_SWAP = 4,    // the frame becomes the last frame (Tracker::SwapFrame)
 };
 
 // Per-session tracking state, resident in device memory.  Poses are 4x4 row-major FP64
@@ -40,6 +49,13 @@ struct TrackState {
     float sigma;
     float rgb_error;
     int rgb_count;
+    // the tracker's state machine (Tracking.cc:76-125, System.cc:92-138)
+    int sm_state;     // Tracker::state of the next frame: 1 before the first, 0 tracking, -1 lost
+    int sm_prev;      // the state of the previous frame (Tracker::lastState after its swap)
+    int missed;       // Tracker::noMissedFrames
+    int after_reloc;  // System::num_frames_after_reloc (10 at start)
+    int reloc;        // a relocalisation pose is in last_pose for the next (lost) frame
+    int gates;        // FrameGate bits of the current frame
     int pad_;
 };
 
@@ -50,7 +66,16 @@ struct PoseF {
 };
 
 // When `st` is set the pose comes from the tracking state and the kernel does nothing unless
-// the frame was tracked (or is the first frame).
+// the frame renders (a tracked or first frame, or one relocalised from outside); fusion
+// kernels check GATE_FUSE as well (frame_fuses).
+__device__ __forceinline__ bool frame_fuses(const TrackState* st_) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const auto st = (const __attribute__((address_space(1))) TrackState*)st_;
+#else
+    const TrackState* st = st_;
+#endif
+    return (st->gates & GATE_FUSE) != 0;
+}
 __device__ __forceinline__ bool pose_from_state(const TrackState* st_, PoseF& p) {
     // global view: the state pointer may come from a descriptor in memory (scalar loads)
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -58,8 +83,7 @@ __device__ __forceinline__ bool pose_from_state(const TrackState* st_, PoseF& p)
 #else
     const TrackState* st = st_;
 #endif
-    const int s = st->status;
-    if (s != TS_OK && s != TS_FIRST) return false;
+    if (!(st->gates & GATE_RENDER)) return false;
     const auto r = st->Rcurr;
     const auto ri = st->RcurrInv;
     p.R = m3{mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), mk3(r[6], r[7], r[8])};
@@ -129,6 +153,8 @@ struct MeshOut {
 };
 
 // work-group sizes of the ICP reduction (one pixel per lane) and of its one-group final sum
+// most work-groups of an ICP reduction (larger frames loop over pixel chunks per lane)
+constexpr int kIcpMaxBlocks = 4096;
 constexpr int kIcpThreads = 1024;
 // pixels per lane and per work-group chunk of the ICP-only reduction, and of the combined
 // ICP + photometric pass (two per lane)

@@ -248,4 +248,71 @@ FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int m
     icp_update(st, P, sums, rgb, min_count, final_iter, thresh);
 }
 
+// The state machine of Tracker::GrabFrame (Tracking.cc:76-125) and System::GrabImage
+// (System.cc:92-138) around one frame, shared by the device (one lane) and the host.
+//
+// frame_begin, before tracking: a lost tracker (state -1) does not track.  The reference
+// then calls Relocalise, which needs the keypoint map (not built here); a relocalisation pose
+// given from outside takes its place (status TS_RELOC, the frame at that pose), otherwise the
+// frame is TS_LOST.  A tracker that is not lost starts ComputeSE3 (Tracking.cc:305-314): last
+// pose = tracked pose of the previous frame, initial guess = last pose, delta = identity.
+FU_HD void frame_begin(TrackState* st) {
+    if (st->sm_state == -1) {
+        st->status = st->reloc ? TS_RELOC : TS_LOST;
+        FU_UNROLL for (int i = 0; i < 16; ++i) st->next_pose[i] = st->last_pose[i];
+        st->iters_done = 0;
+        refresh_float_pose(st);
+        return;
+    }
+    FU_UNROLL for (int i = 0; i < 16; ++i) {
+        st->next_pose[i] = st->last_pose[i];
+        st->init_pose[i] = st->last_pose[i];
+        st->delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    st->status = TS_RUNNING;
+    st->iters_done = 0;
+    refresh_float_pose(st);
+}
+
+// frame_commit, after tracking: the next state, the counters, what the frame does (gates)
+// and, when the frame becomes the last frame, its pose as the last pose.
+//   state 1 (first frame): InitTracking + SwapFrame; fused and raycast.
+//   state 0: num_frames_after_reloc + 1; a tracked frame resets the missed count and, unless
+//     the previous frame was lost (then GrabFrame returns false and nothing happens), is
+//     swapped and raycast, and fused once 10 frames have passed since a relocalisation; a
+//     frame not tracked counts as missed, and more than 9 missed frames make the tracker lost.
+//   state -1: num_frames_after_reloc = 0; relocalised: swapped and raycast (UpdateVisibility
+//     + RayTrace, no fusion), tracking resumes; otherwise still lost, nothing happens.
+FU_HD void frame_commit(TrackState* st) {
+    const int cur = st->sm_state;
+    int next = 0, gates = 0;
+    if (cur == 1) {
+        gates = GATE_FUSE | GATE_RENDER | GATE_SWAP;
+        if (st->after_reloc < 10) gates &= ~GATE_FUSE;
+    } else if (cur == 0) {
+        if (st->after_reloc < (1 << 30)) st->after_reloc += 1;
+        if (st->status == TS_OK) {
+            st->missed = 0;
+            if (st->sm_prev != -1) gates = GATE_RENDER | GATE_SWAP | (st->after_reloc >= 10 ? GATE_FUSE : 0);
+        } else {
+            st->missed += 1;
+            if (st->missed > 9) next = -1;
+        }
+    } else {
+        st->after_reloc = 0;
+        if (st->reloc) {
+            st->reloc = 0;
+            gates = GATE_RENDER | GATE_SWAP;
+        } else {
+            next = -1;
+        }
+    }
+    st->sm_prev = cur;
+    st->sm_state = next;
+    st->gates = gates;
+    if (gates & GATE_SWAP)
+        FU_UNROLL for (int i = 0; i < 16; ++i) st->last_pose[i] = st->next_pose[i];
+    refresh_float_pose(st);
+}
+
 }  // namespace fu

@@ -656,26 +656,18 @@ void launch_icp(const IcpArgs& a, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_icp_final, dim3(1), dim3(kIcpFinalThreads), 0, s, a, nblocks);
 }
 
-// start of ComputeSE3 (Tracking.cc:305-314): last pose = tracked pose of the previous
-// frame, initial guess = last pose (constant-position model), delta = identity
+// before tracking: a lost tracker does not track; otherwise the start of ComputeSE3
+// (frame_begin)
 __device__ __forceinline__ void track_begin_body(TrackState* st) {
     if (threadIdx.x != 0) return;
-    for (int i = 0; i < 16; ++i) {
-        st->next_pose[i] = st->last_pose[i];
-        st->init_pose[i] = st->last_pose[i];
-        st->delta[i] = (i % 5 == 0) ? 1.0 : 0.0;
-    }
-    st->status = TS_RUNNING;
-    st->iters_done = 0;
-    refresh_float_pose(st);
+    frame_begin(st);
 }
 __global__ void k_track_begin(TrackState* st) { track_begin_body(st); }
-// after fusion: the tracked frame becomes the last frame (Tracker::SwapFrame)
+// after tracking: the state machine decides what the frame does, and a swapped frame's pose
+// becomes the last pose (frame_commit)
 __device__ __forceinline__ void track_commit_body(TrackState* st) {
     if (threadIdx.x != 0) return;
-    if (st->status == TS_OK || st->status == TS_FIRST)
-        for (int i = 0; i < 16; ++i) st->last_pose[i] = st->next_pose[i];
-    refresh_float_pose(st);
+    frame_commit(st);
 }
 __global__ void k_track_commit(TrackState* st) { track_commit_body(st); }
 void launch_track_begin(TrackState* st, hipStream_t s) { hipLaunchKernelGGL(k_track_begin, dim3(1), dim3(64), 0, s, st); }

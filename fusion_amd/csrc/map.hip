@@ -115,7 +115,7 @@ static_assert(kAllocSlots % kAllocThreads == 0, "whole slot rows per lane");
 __device__ __forceinline__ void alloc_a_body(WG g, const MapView& m, const FuseArgs& p) {
     __shared__ unsigned long long set[kAllocSlots];
     PoseF pose = p.pose;
-    if (p.st && !pose_from_state(p.st, pose)) return;  // uniform: the frame was not tracked
+    if (p.st && !(frame_fuses(p.st) && pose_from_state(p.st, pose))) return;  // uniform: the frame does not fuse
     for (int i = threadIdx.x; i < kAllocSlots; i += blockDim.x) set[i] = kEmptyKey;
     __syncthreads();
     const auto depth = FU_G(const float, p.depth);
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kAllocThreads) void k_alloc_pass_a(MapView m, FuseA
 // one wave per new block: lane 0 links the key, all lanes initialise the block's voxels
 // (sdf NaN, weight 0, colour 0: Voxel::release, DeviceMap.h:75-79)
 __device__ __forceinline__ void alloc_b_body(WG g, const MapView& m, const TrackState* st) {
-    if (st && st->status != TS_OK && st->status != TS_FIRST) return;
+    if (st && !frame_fuses(st)) return;
     const unsigned nnew = min(m.ctr[C_NEW], m.max_new);
     const int wave = (g.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (g.nx * blockDim.x) >> 6;
@@ -360,7 +360,7 @@ __device__ __forceinline__ void integrate_body(WG g, const MapView& m, const Fus
     using IO = VoxelIO<F>;
     constexpr int H = 4;  // voxels per lane per iteration
     PoseF pose = a.pose;
-    if (a.st && !pose_from_state(a.st, pose)) return;  // uniform: the frame was not tracked
+    if (a.st && !(frame_fuses(a.st) && pose_from_state(a.st, pose))) return;  // uniform: the frame does not fuse
     // the byte / 255 quotients in LDS: a gather there instead of six global loads per voxel
     __shared__ float div255[256];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) div255[i] = u8div255(i);

@@ -310,13 +310,12 @@ __global__ __launch_bounds__(256) void k_derivative(const unsigned char* __restr
     derivative_body(hw_wg(), img, dx, dy, cols, rows);
 }
 
-// Tracker::SwapFrame for the intensity pyramid: after a tracked (or first) frame its images
+// Tracker::SwapFrame for the intensity pyramid: after a swapped frame (GATE_SWAP) its images
 // become the last frame's.  16 bytes per lane where both ends are 16-byte aligned.
 __device__ __forceinline__ void image_commit_body(int lane, const TrackState* st_, const unsigned char* src_,
                                                   unsigned char* dst_, int n) {
     const auto st = FU_G(const TrackState, st_);
-    const int s = st->status;
-    if (s != TS_OK && s != TS_FIRST) return;
+    if (!(st->gates & GATE_SWAP)) return;
     const auto src = FU_G(const unsigned char, src_);
     const auto dst = FU_G(unsigned char, dst_);
     const int i = lane * 16;

@@ -19,7 +19,7 @@
 namespace fu {
 
 __device__ __forceinline__ void zrange_init_body(WG g, const TrackState* st, unsigned* zmin, unsigned* zmax, int n) {
-    if (st && st->status != TS_OK && st->status != TS_FIRST) return;
+    if (st && !(FU_G(const TrackState, st)->gates & GATE_RENDER)) return;
     const unsigned hundred = __float_as_uint(100.0f);
     for (int i = g.x * blockDim.x + threadIdx.x; i < n; i += g.nx * blockDim.x) {
         zmin[i] = hundred;

@@ -14,6 +14,7 @@ SCENE_PATH = os.path.join(LIB_DIR, "libfu_scene.so")
 FU_OK = 0
 FU_ERR_BUDGET = -5
 FU_TRACK_RUNNING, FU_TRACK_OK, FU_TRACK_FAILED, FU_TRACK_REJECTED, FU_TRACK_FIRST = 0, 1, 2, 3, 4
+FU_TRACK_LOST, FU_TRACK_RELOC = 5, 6
 FU_VOXEL_F32_W8, FU_VOXEL_F32_W16, FU_VOXEL_F16_W8 = 0, 1, 2
 FLAGS = {1: "heap_full", 2: "excess_full", 4: "newset_full", 8: "coord_range", 16: "render_full", 32: "tri_full"}
 
@@ -163,6 +164,7 @@ SIGNATURES = {
     "fu_system_grab_image_from": (I, [P, P, P, P]),
     "fu_system_grab_image_host": (I, [P, P, P, P]),
     "fu_system_graph_counts": (I, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "fu_system_relocalise": (I, [P, C.POINTER(C.c_double), P]),
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
     "fu_system_set_profiling": (I, [P, I]),
     "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),

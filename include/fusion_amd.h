@@ -208,6 +208,12 @@ typedef struct fu_track_result {
 #define FU_TRACK_FAILED 2   /* NaN error or too few correspondences: pose rolled back (:371-375) */
 #define FU_TRACK_REJECTED 3 /* final error or motion test failed: pose rolled back (:404-407) */
 #define FU_TRACK_FIRST 4    /* first frame: identity pose (Tracking.cc:66-71) */
+/* a system's tracker is lost after more than 9 frames in a row that were not tracked
+ * (Tracking.cc:92-96); it then tracks nothing: the reference relocalises from its keypoint
+ * map, which this library does not build, so it stays lost until fu_system_relocalise or a
+ * reset */
+#define FU_TRACK_LOST 5     /* lost: not tracked, nothing fused or raycast */
+#define FU_TRACK_RELOC 6    /* relocalised at the pose given to fu_system_relocalise: raycast there */
 
 /* Tracker state and ICP scratch (Tracker::Tracker, Tracking.cc:19-43) */
 fu_status fu_tracker_create(int cols, int rows, fu_tracker** out);
@@ -310,6 +316,13 @@ fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* depth, const
 fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches);
 /* outcome of the last frame; synchronises */
 fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
+/* A lost system's relocalisation, with the pose found by the caller (the reference's
+ * Relocalise, Tracking.cc:116-125, matches keypoints against its keyframe map, which is not
+ * built here).  The next frame takes `pose` (camera to world, 4x4 row-major) as its pose:
+ * status FU_TRACK_RELOC, the model raycast there, nothing fused; tracking resumes with the
+ * frame after it, and fusion once 10 frames have passed (System.cc:97-111).  FU_ERR_STATE
+ * unless the system is lost.  Synchronises. */
+fu_status fu_system_relocalise(fu_system* sys, const double pose[16], fu_stream s);
 
 /* ---------------------------------------------------------------- self-test */
 /* The integration and ICP kernels divide and take square roots with range-specialised forms

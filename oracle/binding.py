@@ -63,6 +63,7 @@ _SIG = {
     "or_system_create": (P, [I, I, F, F, F, F, F, F, C.POINTER(I), C.POINTER(I), F, U, U, U, F, F, U, I]),
     "or_system_destroy": (None, [P]),
     "or_system_grab": (I, [P, P, P, DP]),
+    "or_system_relocalise": (I, [P, DP]),
     "or_system_map": (P, [P]),
     "or_system_buffer": (FP, [P, I, I]),
     "or_system_set_rgb": (None, [P, I]),
@@ -455,6 +456,10 @@ class System:
         pose = np.zeros(16)
         st = lib().or_system_grab(self.h, _ptr(d), _ptr(c), pose.ctypes.data_as(DP))
         return st, pose.reshape(4, 4)
+
+    def relocalise(self, pose):
+        p = np.ascontiguousarray(pose, dtype=np.float64)
+        return lib().or_system_relocalise(self.h, p.ctypes.data_as(DP))
 
     def buffer(self, which, level=0):
         rows, cols = self.rows >> level, self.cols >> level

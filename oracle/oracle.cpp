@@ -1514,6 +1514,12 @@ struct or_system {
     std::vector<uint8_t> img[3], last_img[3];
     std::vector<int16_t> dx[3], dy[3];
     int last_rgb_count = 0;
+    // Tracker::state / lastState / noMissedFrames (Tracking.cc:21) and
+    // System::num_frames_after_reloc (System.cc:72); a relocalisation pose given from outside
+    // stands in for Relocalise (Tracking.cc:411-), which needs the keypoint map
+    int state = 1, last_state = 1, missed = 0, after_reloc = 10;
+    int reloc = 0;
+    double reloc_pose[16];
 };
 
 extern "C" or_system* or_system_create(int cols, int rows, float fx, float fy, float cx, float cy, float depth_scale,
@@ -1613,8 +1619,25 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
     s->frame++;
     double pose[16];
     int status = 4;
-    if (first) {
-        memcpy(pose, s->last_pose, sizeof(pose));
+    bool swap = false;  // GrabFrame's return value: the frame becomes LastFrame
+    // Tracker::GrabFrame (Tracking.cc:59-112)
+    std::swap(s->state, s->last_state);
+    if (s->state == 1) {
+        memcpy(pose, s->last_pose, sizeof(pose));  // InitTracking: identity
+        s->last_state = 0;
+        swap = true;
+    } else if (s->state == -1) {
+        if (s->reloc) {
+            s->reloc = 0;
+            memcpy(pose, s->reloc_pose, sizeof(pose));
+            status = 6;
+            s->last_state = 0;
+            swap = true;
+        } else {
+            memcpy(pose, s->last_pose, sizeof(pose));
+            status = 5;
+            s->last_state = -1;
+        }
     } else {
         const float* nv[3] = {s->vmap[0].data(), s->vmap[1].data(), s->vmap[2].data()};
         const float* nn[3] = {s->nmap[0].data(), s->nmap[1].data(), s->nmap[2].data()};
@@ -1633,19 +1656,37 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
             status = or_compute_se3(nv, nn, lv, ln, c0, r0, s->K, s->iters, s->minc, s->thresh, s->last_pose, pose,
                                     &err, &cnt);
         }
+        if (status == 1) {
+            s->missed = 0;
+            swap = s->last_state != -1;  // the frame after a relocalisation is tracked, not swapped
+            s->last_state = 0;
+        } else {
+            s->last_state = 0;
+            if (++s->missed > 9) s->last_state = -1;
+        }
     }
+    // System::GrabImage (System.cc:92-111)
+    if (s->state == 0)
+        s->after_reloc++;
+    else if (s->state == -1)
+        s->after_reloc = 0;
     if (pose_out) memcpy(pose_out, pose, sizeof(pose));
-    if (status != 1 && status != 4) return status;
+    if (!swap) return status;
+    const bool fuse = s->state != -1 && s->after_reloc >= 10;
     memcpy(s->last_pose, pose, sizeof(pose));
-    // Tracker::SwapFrame (Tracking.cc:245-247) after a tracked (or first) frame: its images
+    // Tracker::SwapFrame (Tracking.cc:245-247) after a swapped frame: its images
     // become LastFrame's
     if (s->use_rgb)
         for (int l = 0; l < 3; ++l) s->last_img[l] = s->img[l];
     float R[9], Rinv[9], t[3];
     or_pose_f(pose, R, Rinv, t);
-    // Mapping::FuseColor + Mapping::RayTrace (Mapping.cc:76-109)
-    const uint32_t nvis = or_fuse_map_color(s->map, s->raw.data(), rgb, s->nmap[0].data(), c0, r0, R, Rinv, t, s->K[0],
-                                            s->K[1], s->K[2], s->K[3]);
+    // Mapping::FuseColor + Mapping::RayTrace (Mapping.cc:76-109); without fusion the visible
+    // blocks come from UpdateVisibility (the reference's count is left unset after a
+    // relocalisation when it neither fuses nor updates the visibility)
+    const uint32_t nvis =
+        fuse ? or_fuse_map_color(s->map, s->raw.data(), rgb, s->nmap[0].data(), c0, r0, R, Rinv, t, s->K[0], s->K[1],
+                                 s->K[2], s->K[3])
+             : or_check_visibility(s->map, Rinv, t, c0, r0, s->K[0], s->K[1], s->K[2], s->K[3], s->map->dmax, s->map->dmin);
     uint32_t nfrag = 0;
     const int rendered =
         or_create_rendering_blocks(s->map, s->zmin.data(), s->zmax.data(), s->zcols, s->zrows, s->map->dmax,
@@ -1659,6 +1700,14 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
         s->mnmap[0] = s->nmap[0];
     }
     return status;
+}
+
+// a relocalisation pose for a lost system's next frame; -1 unless the system is lost
+extern "C" int or_system_relocalise(or_system* s, const double pose[16]) {
+    if (s->last_state != -1) return -1;
+    memcpy(s->reloc_pose, pose, sizeof(s->reloc_pose));
+    s->reloc = 1;
+    return 0;
 }
 
 extern "C" void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

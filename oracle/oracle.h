@@ -151,6 +151,8 @@ or_system* or_system_create(int cols, int rows, float fx, float fy, float cx, fl
 void or_system_destroy(or_system* s);
 /* returns the track status (1 ok, 2 failed, 3 rejected, 4 first frame) */
 int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t* rgb, double pose_out[16]);
+/* a lost system's relocalisation pose for its next frame (status 6); -1 unless it is lost */
+int or_system_relocalise(or_system* s, const double pose[16]);
 or_map* or_system_map(or_system* s);
 /* 1: track with the photometric term (ComputeSE3 icpOnly = false), building the intensity
  * pyramid and derivatives each frame */

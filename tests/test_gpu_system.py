@@ -123,3 +123,80 @@ def test_failed_frame_keeps_model(gpu):
     assert res[1]["track_status"] == 2 and res[1]["fused"] == 0
     assert np.array_equal(res[1]["pose"], res[0]["pose"])
     assert res[2]["track_status"] == 1
+
+
+def _lost_sequence():
+    """frame 0; ten empty frames (missed); three good frames while lost; then, after a
+    relocalisation at frame 0's pose, frames 1..13 (relocalised frame, then tracking; fusion
+    again from the tenth frame after the relocalisation)"""
+    sc, K, fr = scene_frames(frames=range(0, 14), cols=320, rows=240)
+    empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
+    return K, [fr[0]] + [empty] * 10 + [fr[1]] * 3 + [None] + list(fr[1:14])
+
+
+def test_lost_state_and_relocalisation_match_the_oracle(gpu):
+    """Tracking.cc:59-112 / System.cc:92-111: statuses and fused flags of every frame equal the
+    oracle's; poses within 1e-5 per frame of tracking; device, graph and plain control and a
+    batch of two agree"""
+    K, seq = _lost_sequence()
+    mp = dict(MAP, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15)
+    o = O.System(cols=320, rows=240, K=K, num_buckets=mp["num_buckets"], num_excess=mp["num_excess"],
+                 num_blocks=mp["num_blocks"], voxel_size=VOX)
+    runs = {}
+    for mode in ("device", "graph", "plain"):
+        s = A.System(A.system_params(cols=320, rows=240, K=K, plain=mode == "plain", use_graph=mode == "graph", map=mp))
+        out = []
+        for f in seq:
+            if f is None:
+                s.relocalise(out[0]["pose"])
+                continue
+            s.GrabImage(*A.to_device(*f))
+            out.append(s.result())
+        runs[mode] = out
+        s.close()
+    # the batched launches (two sessions, the same frames)
+    pair = [A.System(A.system_params(cols=320, rows=240, K=K, map=mp)) for _ in range(2)]
+    bt = A.Batch(2)
+    for x in pair:
+        bt.attach(x)
+    bat = []
+    for f in seq:
+        if f is None:
+            for x in pair:
+                x.relocalise(bat[0]["pose"])
+            continue
+        d, c = A.to_device(*f)
+        bt.grab(pair, [d, d], [c, c])
+        r = [x.result() for x in pair]
+        assert np.array_equal(r[0]["pose"], r[1]["pose"])
+        bat.append(r[0])
+    runs["batch"] = bat
+    ores = []
+    for f in seq:
+        if f is None:
+            assert o.relocalise(ores[0][1]) == 0
+            continue
+        ores.append(o.grab(*f))
+    statuses = [st for st, _ in ores]
+    assert statuses[:15] == [4] + [2] * 10 + [5] * 3 + [6], statuses
+    for mode, out in runs.items():
+        assert [r["track_status"] for r in out] == statuses, mode
+        for i, (r, (st, pose)) in enumerate(zip(out, ores)):
+            assert np.abs(r["pose"] - pose).max() < 1e-5 * (i + 1), (mode, i)
+        for r in out[:15]:
+            assert r["fused"] == (r["track_status"] == 4), mode
+    for a, b in zip(runs["device"], runs["graph"]):
+        assert np.array_equal(a["pose"], b["pose"]) and a["fused"] == b["fused"]
+    for a, b in zip(runs["device"], runs["batch"]):
+        assert np.array_equal(a["pose"], b["pose"])
+    # fusion resumes at the tenth frame after the relocalisation
+    fused_after = [r["fused"] for r in runs["device"][15:]]
+    assert fused_after[:9] == [0] * 9 and fused_after[9] == (runs["device"][24]["track_status"] == 1)
+
+
+def test_relocalise_refused_unless_lost(gpu):
+    s = A.System(A.system_params(cols=320, rows=240, K=(262.5, 262.5, 159.5, 119.5),
+                                 map=dict(MAP, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15)))
+    with pytest.raises(A.N.FusionError):
+        s.relocalise(np.eye(4))
+    s.close()
