@@ -136,6 +136,7 @@ SIGNATURES = {
     "fu_compute_nmap": (I, [P, P, I, I, P]),
     "fu_resize_map": (I, [P, P, I, I, P, P, P]),
     "fu_icp_step": (I, [P, P, P, P, I, I, FP, FP, FP, FP, FP, F, F, F, F, FP, DP, DP, P]),
+    "fu_icp_step_noalloc": (I, [P, P, P, P, I, I, FP, FP, FP, FP, FP, F, F, F, F, FP, DP, DP, P]),
     "fu_tracker_create": (I, [I, I, C.POINTER(P)]),
     "fu_tracker_destroy": (I, [P]),
     "fu_tracker_set_last_pose": (I, [P, DP, P]),

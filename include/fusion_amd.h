@@ -169,6 +169,18 @@ fu_status fu_icp_step(const float* next_vmap, const float* last_vmap, const floa
                       int cols, int rows, const float Rcurr[9], const float tcurr[3], const float Rlast[9],
                       const float RlastInv[9], const float tlast[3], float fx, float fy, float cx, float cy,
                       float residual[2], double A[36], double b[6], fu_stream s);
+/* fu_icp_step without allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results; its tracking state and partial rows live
+ * in a fixed device array of the library (1 MiB + 1.5 KB, held by each GPU it runs on until
+ * the process ends) instead of memory allocated on first use.  Frames up to 2^30 pixels
+ * (FU_ERR_ARG beyond).  All calls on one GPU share that array: calls that overlap on one GPU
+ * race and must be ordered by the caller.  Synchronises. */
+fu_status fu_icp_step_noalloc(const float* next_vmap, const float* last_vmap, const float* next_nmap,
+                              const float* last_nmap, int cols, int rows, const float Rcurr[9], const float tcurr[3],
+                              const float Rlast[9], const float RlastInv[9], const float tlast[3], float fx,
+                              float fy, float cx, float cy, float residual[2], double A[36], double b[6],
+                              fu_stream s);
 
 /* ---------------------------------------------------------------- photometric term */
 /* ImageToIntensity (Reduction.h:34-35, Pyrdown.cu:196-206): (int)(0.2125 c1 + 0.7154 c0 +
