@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--device-steps", type=int, default=30,
                     help="frames per session of the capacity run with frames already in HBM (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=8, help="steps of the per-stage event profile")
+    ap.add_argument("--graph-steps", type=int, default=30,
+                    help="steps of the served capacity run repeated with captured steps (0: skip)")
     ap.add_argument("--plain-sessions", type=int, default=8)
     ap.add_argument("--plain-steps", type=int, default=12)
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed steps of the mixed-resolution run (0: skip)")
@@ -225,12 +227,14 @@ def realtime_point(A, torch, sv, sids, frames, base, fps, rng, meshes=False):
     return out, lat, done
 
 
-def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, device_steps=0):
+def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, device_steps=0, graph_steps=0):
     """open the sessions, warm up (and with `tune`, autotune the launch geometry on the
     warm-up frames), then: the capacity run (every session's frames queued at t = 0, frames
     in pinned host memory uploaded inside the steps), the same with frames already in HBM
-    (`device_steps`), one real-time run per per-session frame rate in `sweep` (the first with
-    mesh requests) and the stage profile.  Every phase takes the sessions' next frames."""
+    (`device_steps`), the capacity run and the first real-time point again with every lane's
+    steps captured as hipGraphs (`graph_steps`), one real-time run per per-session frame rate
+    in `sweep` (the first with mesh requests) and the stage profile.  Every phase takes the
+    sessions' next frames."""
     src = Frames(A, sizes)
     sv = A.Server(max_batch=args.max_batch, max_inflight=args.inflight, lanes=args.lanes, block_budget=BUDGET)
     sids = []
@@ -283,6 +287,27 @@ def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, 
         del frames, tr
         torch.cuda.empty_cache()
         lo += device_steps
+    if graph_steps > 0:
+        sv.set_graph(True)
+        frames = src.host(lo, lo + graph_steps)
+        tr = trace(sids, frames, lo)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dg = sv.run_trace(tr)
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        lo += graph_steps
+        g = dict(value=round(int(np.isin([c["track_status"] for c in dg], TRACKED).sum()) / w, 2), frames=len(dg),
+                 note="capacity run with every lane's steps replayed as captured hipGraphs")
+        if args.realtime_frames > 0 and sweep:
+            frames = src.host(lo, lo + args.realtime_frames)
+            point, _, _ = realtime_point(A, torch, sv, sids, frames, lo, sweep[0], np.random.default_rng(args.seed + 9))
+            g["rt_point"] = point
+            lo += args.realtime_frames
+        g["graph_captures"], g["graph_launches"] = sv.graph_counts()
+        res["graph"] = g
+        sv.set_graph(False)
+        del frames, tr
     rng = np.random.default_rng(args.seed + 7)
     curve, lat_rt = [], None
     for k, fps in enumerate(sweep if args.realtime_frames > 0 else ()):
@@ -550,10 +575,12 @@ def main():
             A.set_tuning(**cached)
     t = time.time()
     r = served(A, torch, args, sizes, args.steps, SWEEP, args.profile_steps, tune=args.autotune == "run",
-               device_steps=args.device_steps)
+               device_steps=args.device_steps, graph_steps=args.graph_steps)
     if r["tuned"] and args.tuning_cache:
         T.save(key, r["tuned"]["tuning"], r["tuned"]["report"], path=args.tuning_cache)
     log("served: %.0f frames/s (uploads included; %.1f s with rendering)" % (r["tracked"] / r["wall"], time.time() - t))
+    if r.get("graph"):
+        log("served, captured steps: %s" % r["graph"])
     p = plain(A, torch, args)
     log("plain: %.0f frames/s" % p["value"])
     rgbd = None
@@ -625,6 +652,7 @@ def main():
                        "mesh_requests": r.get("rt_point", {}).get("mesh_requests")},
         "latency_curve": r["latency_curve"],
         "device_resident": r.get("device_resident"),
+        "captured_steps": r.get("graph"),
         "latency_backlog_ms": {"p50": pct(r["lat_backlog"], 50), "p99": pct(r["lat_backlog"], 99)},
         "plain": p,
         "served_over_plain": round(fps / p["value"], 3),

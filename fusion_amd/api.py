@@ -572,6 +572,16 @@ class Batch:
         check(N.lib().fu_batch_get_stage_times(self.handle, C.byref(t)), "fu_batch_get_stage_times")
         return _times(t)
 
+    def set_graph(self, enable=True):
+        """run steps as captured hipGraphs (captured on a step membership's second use)"""
+        check(N.lib().fu_batch_set_graph(self.handle, int(enable)), "fu_batch_set_graph")
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) of this context"""
+        c, r = C.c_uint32(), C.c_uint32()
+        check(N.lib().fu_batch_graph_counts(self.handle, C.byref(c), C.byref(r)), "fu_batch_graph_counts")
+        return c.value, r.value
+
     def grab(self, systems, depths, rgbs):
         """one frame per system: device depth (H, W) u16-in-int16 and rgb (H, W, 3) u8 tensors"""
         n = len(systems)
@@ -589,7 +599,7 @@ class BudgetExceeded(N.FusionError):
 class Server:
     """Continuous-batching server of many reconstruction sessions (fu_server_*)."""
 
-    def __init__(self, max_batch=32, max_inflight=2, lanes=2, block_budget=0):
+    def __init__(self, max_batch=32, max_inflight=2, lanes=2, block_budget=0, use_graph=False):
         _dev()
         p = N.ServerParams()
         N.lib().fu_server_params_default(C.byref(p))
@@ -598,6 +608,18 @@ class Server:
         check(N.lib().fu_server_create(C.byref(p), C.byref(h)), "fu_server_create")
         self.handle = h
         self.params = {}
+        if use_graph:
+            self.set_graph(True)
+
+    def set_graph(self, enable=True):
+        """run every lane's steps as captured hipGraphs (fu_server_set_graph)"""
+        check(N.lib().fu_server_set_graph(self.handle, int(enable)), "fu_server_set_graph")
+
+    def graph_counts(self):
+        """(graphs captured, graph launches) summed over the lanes"""
+        c, r = C.c_uint32(), C.c_uint32()
+        check(N.lib().fu_server_graph_counts(self.handle, C.byref(c), C.byref(r)), "fu_server_graph_counts")
+        return c.value, r.value
 
     def close(self):
         if self.handle:
@@ -721,6 +743,77 @@ class _DevBuf:
 def _wrap(addr, n, shape):
     del n
     return torch.as_tensor(_DevBuf(addr, shape, "<f4"), device="cuda")
+
+
+# --------------------------------------------------------------------- keypoint map
+# fu_surf records (SURF, Mapping/DeviceMap.h:99-108): 288 bytes each; on the device a uint8
+# tensor of shape (n, 288)
+SURF_DTYPE = np.dtype([("valid", np.uint8), ("pad", np.uint8, 3), ("pos", np.float32, 3), ("normal", np.float32, 4),
+                       ("descriptor", np.float32, 64)])
+
+
+def surf_to_device(records):
+    a = np.ascontiguousarray(records, dtype=SURF_DTYPE)
+    return torch.from_numpy(a.view(np.uint8).reshape(len(a), SURF_DTYPE.itemsize).copy()).to("cuda")
+
+
+def surf_from_device(t):
+    return t.cpu().numpy().reshape(-1).copy().view(SURF_DTYPE)
+
+
+class KeyMap:
+    """KeyMap (Mapping/DeviceMap.h:163-185) with ResetKeyPoints, InsertKeyPoints and
+    CollectKeyPoints (Mapping/RenderScene.h:5-11) on the device"""
+
+    def __init__(self, grid_size=0.01, max_keys=100000, n_buckets=5):
+        _dev()
+        p = N.KeyMapParams()
+        N.lib().fu_keymap_params_default(C.byref(p))
+        p.grid_size, p.max_keys, p.n_buckets = grid_size, max_keys, n_buckets
+        self.n = max_keys * n_buckets
+        h = C.c_void_p()
+        check(N.lib().fu_keymap_create(C.byref(p), C.byref(h)), "fu_keymap_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            N.lib().fu_keymap_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ResetKeyPoints(self):
+        check(N.lib().fu_reset_key_points(self.handle, _s()), "fu_reset_key_points")
+
+    def InsertKeyPoints(self, keys, index):
+        """keys: device uint8 (n, 288) records (positions may be rewritten), index: device int32 (n,)"""
+        check(N.lib().fu_insert_key_points(self.handle, _p(keys), _p(index), int(keys.shape[0]), _s()),
+              "fu_insert_key_points")
+
+    def CollectKeyPoints(self, cap):
+        out = torch.zeros((max(cap, 1), SURF_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        n = C.c_uint32()
+        check(N.lib().fu_collect_key_points(self.handle, _p(out), int(cap), C.byref(n), _s()), "fu_collect_key_points")
+        return out[:cap], n.value
+
+    def read(self):
+        out = np.zeros(self.n, SURF_DTYPE)
+        check(N.lib().fu_keymap_read(self.handle, out.ctypes.data_as(C.c_void_p), _s()), "fu_keymap_read")
+        return out
+
+
+def BuildAdjacencyMatrix(frame_keys, map_keys, dist):
+    """BuildAdjacencyMatrix (Tracking/Reduction.h:85-88): the (n, n) consistency scores of n
+    matched pairs (device records and float32 distances)"""
+    n = int(frame_keys.shape[0])
+    out = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    check(N.lib().fu_build_adjacency_matrix(_p(frame_keys), _p(map_keys), _p(dist), n, _p(out), _s()),
+          "fu_build_adjacency_matrix")
+    return out
 
 
 # --------------------------------------------------------------------- synthetic scenes

@@ -2,7 +2,9 @@
 
 ``build()`` compiles every time it is called (hipcc cross-compiles without a GPU):
   fusion_amd/lib/libfusion_amd.so   HIP kernels + C ABI (include/fusion_amd.h)
+  fusion_amd/lib/libfusion_amd_cxx.so  the C++ drop-in layer (include/fusion_amd_compat.hpp)
   fusion_amd/lib/libfu_scene.so     synthetic scenes (include/fusion_scene.h), g++ + OpenMP
+  tests/cxx/_build/test_compat      the C++ layer's test driver (run by tests/test_gpu_compat.py)
   oracle/_build/liboracle.so        CPU oracle (test infrastructure; oracle/Makefile)
 """
 import os
@@ -22,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SOURCES = ["pyramid.hip", "map.hip", "raycast.hip", "mesh.hip", "icp.hip", "capi.hip", "server.hip",
-               "snapshot.hip", "selftest.hip", "noalloc.hip"]
+               "snapshot.hip", "selftest.hip", "noalloc.hip", "keymap.hip"]
 
 
 def _run(cmd, verbose):
@@ -52,6 +54,23 @@ def build_hip(verbose=False):
     return out
 
 
+def build_cxx(verbose=False):
+    """the C++ drop-in layer over the C ABI, and its test driver"""
+    out = os.path.join(LIB, "libfusion_amd_cxx.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-std=c++17", "-O2", "-fPIC", "-shared", os.path.join(CSRC, "compat.cpp"), "-L" + LIB,
+          "-lfusion_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, out)
+    tdir = os.path.join(ROOT, "tests", "cxx", "_build")
+    os.makedirs(tdir, exist_ok=True)
+    exe = os.path.join(tdir, "test_compat")
+    _run([HIPCC, "-std=c++17", "-O2", os.path.join(ROOT, "tests", "cxx", "test_compat.cpp"), "-L" + LIB,
+          "-lfusion_amd_cxx", "-lfusion_amd", "-lfu_scene", "-Wl,-rpath,$ORIGIN/../../../fusion_amd/lib", "-o",
+          exe + ".tmp"], verbose)
+    os.replace(exe + ".tmp", exe)
+    return out
+
+
 def build_scene(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libfu_scene.so")
@@ -70,6 +89,7 @@ def build(verbose=False):
     build_scene(verbose)
     build_oracle(verbose)
     build_hip(verbose)
+    build_cxx(verbose)
 
 
 if __name__ == "__main__":

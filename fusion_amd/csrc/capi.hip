@@ -1512,7 +1512,30 @@ struct fu_batch {
     bool prof = false;
     hipEvent_t ev[FU_NUM_STAGES + 1] = {};
     fu_stage_times times{};
+    // captured steps (fu_batch_set_graph): keyed by the step's sessions, inputs and result
+    // slot, the context's membership generation and the launch geometry's generation
+    struct Graph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    static constexpr size_t kGraphs = 32, kSeen = 64;
+    int use_graph = 0;
+    uint64_t gen = 1, tuning_gen = 0, clock = 0;
+    std::vector<Graph> graphs;
+    std::vector<std::vector<uint64_t>> seen;  // keys met once, not captured
+    hipStream_t cap_stream = nullptr;
+    uint32_t captures = 0, replays = 0;
+    void drop_graphs() {
+        for (Graph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+        seen.clear();
+        ++gen;
+    }
 };
+
+// generation of the process-wide launch geometry (captured steps depend on it)
+static uint64_t g_tuning_gen = 1;
 
 extern "C" fu_status fu_batch_create(int capacity, fu_batch** out) {
     if (!out) return set_err(FU_ERR_ARG, "null output");
@@ -1540,8 +1563,24 @@ extern "C" fu_status fu_batch_destroy(fu_batch* bt) {
         }
     for (int k = 0; k <= FU_NUM_STAGES; ++k)
         if (bt->ev[k]) (void)hipEventDestroy(bt->ev[k]);
+    bt->drop_graphs();
+    if (bt->cap_stream) (void)hipStreamDestroy(bt->cap_stream);
     (void)hipFree(bt->d_desc);
     delete bt;
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_set_graph(fu_batch* bt, int enable) {
+    if (!bt) return set_err(FU_ERR_ARG, "null batch");
+    bt->use_graph = enable != 0;
+    if (!enable) bt->drop_graphs();
+    return FU_OK;
+}
+
+extern "C" fu_status fu_batch_graph_counts(fu_batch* bt, uint32_t* captures, uint32_t* replays) {
+    if (!bt || !captures || !replays) return set_err(FU_ERR_ARG, "null argument");
+    *captures = bt->captures;
+    *replays = bt->replays;
     return FU_OK;
 }
 
@@ -1577,6 +1616,7 @@ extern "C" fu_status fu_set_tuning(const fu_tuning* t) {
     for (int i = 0; i < 3; ++i)
         if (g[i] < 256 || g[i] > 262144) return set_err(FU_ERR_ARG, "fu_set_tuning: work-group count %d (256..262144)", g[i]);
     g_tuning = *t;
+    ++g_tuning_gen;
     return FU_OK;
 }
 extern "C" fu_status fu_get_tuning(fu_tuning* t) {
@@ -1661,6 +1701,7 @@ extern "C" fu_status fu_batch_attach(fu_batch* bt, fu_system* sys) {
     fill_desc(sys, bt->h_desc[slot]);
     HIPCHK(hipMemcpy(bt->d_desc + slot, &bt->h_desc[slot], sizeof(SessDesc), hipMemcpyHostToDevice));
     bt->owner[slot] = sys;
+    bt->drop_graphs();  // a slot's session (and its launch shares) changed
     bt->fmt = sys->p.map.voxel_format;
     bt->rgb = sys->p.use_rgb ? 1 : 0;
     bt->attached++;
@@ -1673,9 +1714,117 @@ extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
     if (!bt || !sys) return set_err(FU_ERR_ARG, "null argument");
     if (sys->batch != bt) return set_err(FU_ERR_STATE, "system is not attached to this batch context");
     bt->owner[sys->batch_slot] = nullptr;
+    bt->drop_graphs();
     if (--bt->attached == 0) bt->fmt = bt->rgb = -1;
     sys->batch = nullptr;
     sys->batch_slot = -1;
+    return FU_OK;
+}
+
+// The launches of one batched step on `hs`.  b.in is set per stage from `depth` and `rgb`;
+// with `prof`, the stage events ev[] are recorded between the stages.
+static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], bool track, int use_rgb, int fmt,
+                        const void* const* depth, const void* const* rgb, FrameBrief* brief, bool prof,
+                        hipEvent_t* ev, hipStream_t hs) {
+    const int n = b.n;
+    auto mark = [&](int k) {
+        if (prof) (void)hipEventRecord(ev[k], hs);
+    };
+    mark(0);
+    for (int i = 0; i < n; ++i) b.in[i] = depth[i];
+    if (track) {
+        launch_b_resize(b, h, hs);
+    }
+    mark(FU_STAGE_RESIZE + 1);
+    launch_b_filter(b, h, hs);
+    launch_b_pyrdown(b, h, 1, hs);
+    launch_b_pyrdown(b, h, 2, hs);
+    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];  // the normal pass packs the integration inputs
+    launch_b_nmap(b, h, hs);
+    if (use_rgb) {
+        for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
+        launch_b_images(b, h, hs);
+    }
+    mark(FU_STAGE_PYRAMID + 1);
+    if (track) {
+        launch_b_track_begin(b, h, hs);
+        if (use_rgb) launch_b_rgb_lists(b, h, hs);
+        for (int l = 2; l >= 0; --l)
+            for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
+    }
+    launch_b_track_commit(b, h, hs);
+    if (use_rgb) launch_b_image_commit(b, h, hs);
+    mark(FU_STAGE_ICP + 1);
+    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
+    launch_b_alloc(b, h, hs);
+    mark(FU_STAGE_ALLOC + 1);
+    launch_b_visible(b, h, hs);
+    mark(FU_STAGE_VISIBLE + 1);
+    launch_b_integrate(b, h, fmt, hs);
+    mark(FU_STAGE_INTEGRATE + 1);
+    launch_b_zrange(b, h, hs);
+    mark(FU_STAGE_ZRANGE + 1);
+    launch_b_raycast(b, h, fmt, hs);
+    mark(FU_STAGE_RAYCAST + 1);
+    if (brief) launch_b_results(b, brief, hs);
+}
+
+// The step's captured graph: looked up by the step's key (sessions in slot order, result
+// slot, launch-geometry generation); a key seen for the second time is captured, so one-off
+// memberships of a served lane run as plain launches.  Null when the step is to be launched
+// directly.
+static fu_status step_graph(fu_batch* bt, BatchLaunch& b, const SessDesc* h, const int maxit[3],
+                            const void* const* depth, const void* const* rgb, FrameBrief* brief,
+                            hipGraphExec_t* out) {
+    *out = nullptr;
+    if (bt->tuning_gen != g_tuning_gen) {
+        bt->drop_graphs();
+        bt->tuning_gen = g_tuning_gen;
+    }
+    std::vector<uint64_t> key;
+    key.reserve(3 + (b.n + 3) / 4);
+    key.push_back((uint64_t)b.n);
+    key.push_back((uint64_t)(uintptr_t)brief);
+    key.push_back(bt->gen);
+    for (int i = 0; i < b.n; i += 4) {
+        uint64_t w = 0;
+        for (int k = 0; k < 4 && i + k < b.n; ++k) w |= (uint64_t)b.sess[i + k] << (16 * k);
+        key.push_back(w);
+    }
+    ++bt->clock;
+    for (fu_batch::Graph& g : bt->graphs)
+        if (g.key == key) {
+            g.used = bt->clock;
+            *out = g.exec;
+            return FU_OK;
+        }
+    auto seen = std::find(bt->seen.begin(), bt->seen.end(), key);
+    if (seen == bt->seen.end()) {
+        if (bt->seen.size() >= fu_batch::kSeen) bt->seen.erase(bt->seen.begin());
+        bt->seen.push_back(std::move(key));
+        return FU_OK;
+    }
+    bt->seen.erase(seen);
+    if (!bt->cap_stream) HIPCHK(hipStreamCreateWithFlags(&bt->cap_stream, hipStreamNonBlocking));
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(bt->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return set_err(FU_ERR_HIP, "batched step capture: %s", hipGetErrorString(e));
+    launch_step(b, h, maxit, true, bt->rgb == 1, bt->fmt, depth, rgb, brief, false, nullptr, bt->cap_stream);
+    e = hipStreamEndCapture(bt->cap_stream, &g);
+    if (e != hipSuccess) return set_err(FU_ERR_HIP, "batched step capture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return set_err(FU_ERR_HIP, "batched step instantiate: %s", hipGetErrorString(e));
+    if (bt->graphs.size() >= fu_batch::kGraphs) {  // evict the least recently used
+        auto lru = std::min_element(bt->graphs.begin(), bt->graphs.end(),
+                                    [](const fu_batch::Graph& x, const fu_batch::Graph& y) { return x.used < y.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        bt->graphs.erase(lru);
+    }
+    bt->graphs.push_back(fu_batch::Graph{key, exec, bt->clock});
+    bt->captures++;
+    *out = exec;
     return FU_OK;
 }
 
@@ -1708,46 +1857,44 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     const SessDesc* h = bt->h_desc.data();
     hipStream_t hs = s;
     const bool prof = bt->prof;
-    auto mark = [&](int k) {
-        if (prof) (void)hipEventRecord(bt->ev[k], hs);
-    };
-    mark(0);
-    for (int i = 0; i < n; ++i) b.in[i] = depth[i];
-    if (track) {
-        launch_b_resize(b, h, hs);
+    const void* dp[kMaxBatch];
+    const void* cp[kMaxBatch];
+    for (int i = 0; i < n; ++i) {
+        dp[i] = depth[i];
+        cp[i] = rgb[i];
     }
-    mark(FU_STAGE_RESIZE + 1);
-    launch_b_filter(b, h, hs);
-    launch_b_pyrdown(b, h, 1, hs);
-    launch_b_pyrdown(b, h, 2, hs);
-    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];  // the normal pass packs the integration inputs
-    launch_b_nmap(b, h, hs);
-    if (bt->rgb == 1) {
-        for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
-        launch_b_images(b, h, hs);
+    hipGraphExec_t exec = nullptr;
+    if (bt->use_graph && !prof && b.first == 0) {
+        // a captured step reads every frame from its system's input slots: frames elsewhere
+        // are gathered there first by one batched copy
+        BatchCopy cd, cc;
+        cd.n = cc.n = 0;
+        for (int i = 0; i < n; ++i) {
+            fu_system* x = sys[i];
+            const size_t n0 = (size_t)x->p.cols * x->p.rows;
+            if (depth[i] != x->b.depth_in) cd.add(x->b.depth_in, depth[i], n0 * 2);
+            if (rgb[i] != x->b.rgb_in) cc.add(x->b.rgb_in, rgb[i], n0 * 3);
+            dp[i] = x->b.depth_in;
+            cp[i] = x->b.rgb_in;
+        }
+        fu_status st = step_graph(bt, b, h, maxit, dp, cp, brief, &exec);
+        if (st) return st;
+        if (exec) {
+            launch_batch_copy(cd, hs);
+            launch_batch_copy(cc, hs);
+        }
+        if (!exec)  // not captured (yet): the direct launches read the frames where they are
+            for (int i = 0; i < n; ++i) {
+                dp[i] = depth[i];
+                cp[i] = rgb[i];
+            }
     }
-    mark(FU_STAGE_PYRAMID + 1);
-    if (track) {
-        launch_b_track_begin(b, h, hs);
-        if (bt->rgb == 1) launch_b_rgb_lists(b, h, hs);
-        for (int l = 2; l >= 0; --l)
-            for (int j = 0; j < maxit[l]; ++j) launch_b_icp(b, h, l, j, hs);
+    if (exec) {
+        HIPCHK(hipGraphLaunch(exec, hs));
+        bt->replays++;
+    } else {
+        launch_step(b, h, maxit, track, bt->rgb == 1, bt->fmt, dp, cp, brief, prof, bt->ev, hs);
     }
-    launch_b_track_commit(b, h, hs);
-    if (bt->rgb == 1) launch_b_image_commit(b, h, hs);
-    mark(FU_STAGE_ICP + 1);
-    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
-    launch_b_alloc(b, h, hs);
-    mark(FU_STAGE_ALLOC + 1);
-    launch_b_visible(b, h, hs);
-    mark(FU_STAGE_VISIBLE + 1);
-    launch_b_integrate(b, h, bt->fmt, hs);
-    mark(FU_STAGE_INTEGRATE + 1);
-    launch_b_zrange(b, h, hs);
-    mark(FU_STAGE_ZRANGE + 1);
-    launch_b_raycast(b, h, bt->fmt, hs);
-    mark(FU_STAGE_RAYCAST + 1);
-    if (brief) launch_b_results(b, brief, hs);
     LAUNCHCHK();
     for (int i = 0; i < n; ++i) sys[i]->frame++;
     if (prof) {

@@ -51,8 +51,8 @@ struct Lane {
     std::vector<int> sessions;
     std::deque<Step> inflight;
     std::vector<hipEvent_t> ev;  // one per ring slot
-    FrameBrief* d_res = nullptr;  // ring of max_inflight x kMaxBatch
-    FrameBrief* h_res = nullptr;  // pinned mirror
+    FrameBrief* d_res = nullptr;  // kMaxBatch results of the step being computed
+    FrameBrief* h_res = nullptr;  // pinned, max_inflight x kMaxBatch: one slot per queued step
     uint64_t launched = 0;
     double pixels = 0;  // load measure for placing sessions
 };
@@ -133,7 +133,7 @@ extern "C" fu_status fu_server_create(const fu_server_params* p, fu_server** out
         }
         const size_t nb = sizeof(FrameBrief) * kMaxBatch * p->max_inflight;
         hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipMalloc(&L.d_res, nb);
+        if (e == hipSuccess) e = hipMalloc(&L.d_res, sizeof(FrameBrief) * kMaxBatch);
         if (e == hipSuccess) e = hipHostMalloc(&L.h_res, nb, hipHostMallocDefault);
         for (int k = 0; k < p->max_inflight && e == hipSuccess; ++k) {
             hipEvent_t ev;
@@ -319,6 +319,16 @@ static fu_status retire(fu_server* sv, Lane& L) {
     return FU_OK;
 }
 
+// a step that could not be launched: its requests go back to the front of their sessions'
+// queues, so the server stays consistent (the frames can be retried or the sessions closed)
+static void unlaunch(fu_server* sv, const Step& st) {
+    for (int i = st.n - 1; i >= 0; --i) {
+        Session& s = sv->sess[st.req[i].session];
+        s.pending.push_front(st.req[i]);
+        s.in_steps--;
+    }
+}
+
 // next steps of a lane while it has room: the oldest pending frame of each session, oldest
 // sessions first
 static fu_status launch(fu_server* sv, Lane& L) {
@@ -342,10 +352,14 @@ static fu_status launch(fu_server* sv, Lane& L) {
             s.pending.pop_front();
             s.in_steps++;
             fu_status e = mesh_async(s.sys, st.req[0].mv, st.req[0].mn, st.req[0].mc, st.req[0].max_tri, L.stream);
-            if (e) return e;
-            SV_HIP(hipMemcpyAsync(L.h_res + (size_t)st.slot * kMaxBatch, system_counters(s.sys), sizeof(unsigned) * 16,
-                                  hipMemcpyDeviceToHost, L.stream));
-            SV_HIP(hipEventRecord(L.ev[st.slot], L.stream));
+            hipError_t he = hipSuccess;
+            if (!e) he = hipMemcpyAsync(L.h_res + (size_t)st.slot * kMaxBatch, system_counters(s.sys),
+                                        sizeof(unsigned) * 16, hipMemcpyDeviceToHost, L.stream);
+            if (!e && he == hipSuccess) he = hipEventRecord(L.ev[st.slot], L.stream);
+            if (e || he != hipSuccess) {
+                unlaunch(sv, st);
+                return e ? e : api_error(FU_ERR_HIP, "mesh step: %s", hipGetErrorString(he));
+            }
             L.launched++;
             L.inflight.push_back(st);
             continue;
@@ -360,12 +374,16 @@ static fu_status launch(fu_server* sv, Lane& L) {
             });
             cand.resize(sv->p.max_batch);
         }
+        // slots in session order: a lane whose pending sessions repeat launches the same step
+        // (and replays its captured graph, fu_server_set_graph)
+        std::sort(cand.begin(), cand.end());
         Step st;
         st.n = (int)cand.size();
         st.slot = (int)(L.launched % (uint64_t)sv->p.max_inflight);
         fu_system* sys[kMaxBatch];
         const uint16_t* dp[kMaxBatch];
         const uint8_t* cp[kMaxBatch];
+        hipError_t he = hipSuccess;
         for (int i = 0; i < st.n; ++i) {
             Session& s = sv->sess[cand[i]];
             st.req[i] = s.pending.front();
@@ -375,18 +393,30 @@ static fu_status launch(fu_server* sv, Lane& L) {
             dp[i] = st.req[i].depth;
             cp[i] = st.req[i].rgb;
             if (st.req[i].host) {  // the upload of Frame::Frame (Core/Frame.cc:61-62), inside the step
-                SV_HIP(hipMemcpyAsync(s.depth_in, dp[i], s.npix * 2, hipMemcpyHostToDevice, L.stream));
-                SV_HIP(hipMemcpyAsync(s.rgb_in, cp[i], s.npix * 3, hipMemcpyHostToDevice, L.stream));
+                if (he == hipSuccess) he = hipMemcpyAsync(s.depth_in, dp[i], s.npix * 2, hipMemcpyHostToDevice, L.stream);
+                if (he == hipSuccess) he = hipMemcpyAsync(s.rgb_in, cp[i], s.npix * 3, hipMemcpyHostToDevice, L.stream);
                 dp[i] = s.depth_in;
                 cp[i] = s.rgb_in;
             }
         }
-        FrameBrief* dres = L.d_res + (size_t)st.slot * kMaxBatch;
+        // one device result buffer per lane: the lane's steps run in stream order, and each
+        // step's results are copied to its host slot before the next step writes them
+        FrameBrief* dres = L.d_res;
         FrameBrief* hres = L.h_res + (size_t)st.slot * kMaxBatch;
-        fu_status e = batch_grab(L.batch, st.n, sys, dp, cp, dres, L.stream);
-        if (e) return e;
-        SV_HIP(hipMemcpyAsync(hres, dres, sizeof(FrameBrief) * st.n, hipMemcpyDeviceToHost, L.stream));
-        SV_HIP(hipEventRecord(L.ev[st.slot], L.stream));
+        fu_status e = FU_OK;
+        if (he != hipSuccess) e = api_error(FU_ERR_HIP, "frame upload: %s", hipGetErrorString(he));
+        if (!e) e = batch_grab(L.batch, st.n, sys, dp, cp, dres, L.stream);
+        if (!e) {
+            he = hipMemcpyAsync(hres, dres, sizeof(FrameBrief) * st.n, hipMemcpyDeviceToHost, L.stream);
+            if (he == hipSuccess) he = hipEventRecord(L.ev[st.slot], L.stream);
+            if (he != hipSuccess) e = api_error(FU_ERR_HIP, "frame step: %s", hipGetErrorString(he));
+        }
+        if (e) {
+            // batch_grab checks its arguments before any launch; after a HIP error the frames'
+            // device work may be partly queued, and the requests are returned all the same
+            unlaunch(sv, st);
+            return e;
+        }
         L.launched++;
         L.inflight.push_back(st);
     }
@@ -480,6 +510,28 @@ extern "C" fu_status fu_server_set_profiling(fu_server* sv, int enable) {
     for (Lane& L : sv->lanes) {
         fu_status st = fu_batch_set_profiling(L.batch, enable);
         if (st) return st;
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_set_graph(fu_server* sv, int enable) {
+    if (!sv) return api_error(FU_ERR_ARG, "null server");
+    for (Lane& L : sv->lanes) {
+        fu_status st = fu_batch_set_graph(L.batch, enable);
+        if (st) return st;
+    }
+    return FU_OK;
+}
+
+extern "C" fu_status fu_server_graph_counts(fu_server* sv, uint32_t* captures, uint32_t* replays) {
+    if (!sv || !captures || !replays) return api_error(FU_ERR_ARG, "null argument");
+    *captures = *replays = 0;
+    for (Lane& L : sv->lanes) {
+        uint32_t c = 0, r = 0;
+        fu_status st = fu_batch_graph_counts(L.batch, &c, &r);
+        if (st) return st;
+        *captures += c;
+        *replays += r;
     }
     return FU_OK;
 }

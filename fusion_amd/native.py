@@ -54,6 +54,10 @@ class SystemParams(C.Structure):
                 ("plain", C.c_int32), ("use_graph", C.c_int32), ("use_rgb", C.c_int32), ("map", MapParams)]
 
 
+class KeyMapParams(C.Structure):
+    _fields_ = [("grid_size", C.c_float), ("max_keys", C.c_uint32), ("n_buckets", C.c_uint32)]
+
+
 class FrameResult(C.Structure):
     _fields_ = [("pose", C.c_double * 16), ("track_status", C.c_int32), ("fused", C.c_int32),
                 ("icp_error", C.c_float), ("icp_count", C.c_int32), ("n_visible", C.c_uint32),
@@ -180,11 +184,15 @@ SIGNATURES = {
     "fu_batch_grab": (I, [P, I, C.POINTER(P), C.POINTER(P), C.POINTER(P), P]),
     "fu_batch_set_profiling": (I, [P, I]),
     "fu_batch_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_batch_set_graph": (I, [P, I]),
+    "fu_batch_graph_counts": (I, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fu_tuning_default": (None, [C.POINTER(Tuning)]),
     "fu_set_tuning": (I, [C.POINTER(Tuning)]),
     "fu_get_tuning": (I, [C.POINTER(Tuning)]),
     "fu_server_set_profiling": (I, [P, I]),
     "fu_server_get_stage_times": (I, [P, C.POINTER(StageTimes)]),
+    "fu_server_set_graph": (I, [P, I]),
+    "fu_server_graph_counts": (I, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fu_server_params_default": (None, [C.POINTER(ServerParams)]),
     "fu_server_create": (I, [C.POINTER(ServerParams), C.POINTER(P)]),
     "fu_server_destroy": (I, [P]),
@@ -193,6 +201,14 @@ SIGNATURES = {
     "fu_server_system": (P, [P, I]),
     "fu_server_now_ms": (C.c_double, [P]),
     "fu_server_submit": (I, [P, I, P, P, C.c_uint64, C.c_double]),
+    "fu_keymap_params_default": (None, [C.POINTER(KeyMapParams)]),
+    "fu_keymap_create": (I, [C.POINTER(KeyMapParams), C.POINTER(P)]),
+    "fu_keymap_destroy": (I, [P]),
+    "fu_reset_key_points": (I, [P, P]),
+    "fu_insert_key_points": (I, [P, P, P, C.c_uint32, P]),
+    "fu_collect_key_points": (I, [P, P, C.c_uint32, C.POINTER(C.c_uint32), P]),
+    "fu_keymap_read": (I, [P, P, P]),
+    "fu_build_adjacency_matrix": (I, [P, P, P, I, P, P]),
     "fu_server_submit_host": (I, [P, I, P, P, C.c_uint64, C.c_double]),
     "fu_server_submit_mesh": (I, [P, I, P, P, P, C.c_uint32, C.c_uint64, C.c_double]),
     "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),

@@ -336,6 +336,49 @@ fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
  * unless the system is lost.  Synchronises. */
 fu_status fu_system_relocalise(fu_system* sys, const double pose[16], fu_stream s);
 
+/* ---------------------------------------------------------------- keypoint map */
+/* SURF record of the keypoint map, in the reference's layout (Mapping/DeviceMap.h:99-108:
+ * bool valid, float3 pos, float4 normal, float descriptor[64]; 288 bytes) */
+typedef struct fu_surf {
+    uint8_t valid;
+    uint8_t pad_[3];
+    float pos[3];
+    float normal[4];
+    float descriptor[64];
+} fu_surf;
+typedef struct fu_keymap fu_keymap;
+/* KeyMap's constants (DeviceMap.h:163-170) as parameters: grid cell 0.01 m, 100000 buckets
+ * of 5 slots */
+typedef struct fu_keymap_params {
+    float grid_size;
+    uint32_t max_keys;
+    uint32_t n_buckets;
+} fu_keymap_params;
+void fu_keymap_params_default(fu_keymap_params* p);
+fu_status fu_keymap_create(const fu_keymap_params* p, fu_keymap** out); /* reset on creation */
+fu_status fu_keymap_destroy(fu_keymap* k);
+/* ResetKeyPoints (RenderScene.h:5, FuseMap.cu:366-380): every slot invalid.  Asynchronous. */
+fu_status fu_reset_key_points(fu_keymap* k, fu_stream s);
+/* InsertKeyPoints (RenderScene.h:7-8, FuseMap.cu:455-472; KeyMap::InsertKey,
+ * DeviceMap.cu:258-287) of n device records: a key whose 1 cm cell already has a valid key in
+ * its bucket takes that key's position and slot (index[i]); otherwise the bucket's first free
+ * slot takes the key; a key meeting a full bucket is left out with index[i] unchanged.
+ * Applied as one key at a time in index order (the reference's try-lock may drop a key that
+ * meets another in its bucket in the same launch).  At most 2^24 keys.  Synchronises. */
+fu_status fu_insert_key_points(fu_keymap* k, fu_surf* keys, int32_t* index, uint32_t n, fu_stream s);
+/* CollectKeyPoints (RenderScene.h:10-11, FuseMap.cu:382-453): *n_keys = the number of valid
+ * keys; valid key j (in slot order) is copied to out[j] for 0 < j < cap (the reference copies
+ * only at offsets > 0, so out[0] is left as it was).  Synchronises. */
+fu_status fu_collect_key_points(fu_keymap* k, fu_surf* out, uint32_t cap, uint32_t* n_keys, fu_stream s);
+/* every slot of the map into host memory (max_keys * n_buckets records); synchronises */
+fu_status fu_keymap_read(fu_keymap* k, fu_surf* host, fu_stream s);
+/* BuildAdjacencyMatrix (Reduction.h:85-88, KeyPoints.cu:12-77): the n x n consistency scores
+ * of n matched pairs (frame key i <-> map key i, descriptor distance dist[i]) into a dense
+ * row-major float matrix (the reference's unused column sums are not computed).
+ * Asynchronous. */
+fu_status fu_build_adjacency_matrix(const fu_surf* frame_keys, const fu_surf* map_keys, const float* dist, int32_t n,
+                                    float* matrix, fu_stream s);
+
 /* ---------------------------------------------------------------- self-test */
 /* The integration and ICP kernels divide and take square roots with range-specialised forms
  * of the compiler's IEEE sequences (no reference counterpart).  This runs n random cases of
@@ -443,6 +486,16 @@ fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_
  * synchronises at its end.  ms[] are per step, summed over steps; the counts sum over frames. */
 fu_status fu_batch_set_profiling(fu_batch* b, int enable);
 fu_status fu_batch_get_stage_times(fu_batch* b, fu_stage_times* out);
+/* Captured steps (the north-star's per-frame graph for batches): with `enable`, a step whose
+ * systems all have tracked a frame before runs as one hipGraph launch.  A graph is keyed by
+ * the step's systems in slot order and the result buffer; it is captured the second time its
+ * key is met (so one-off memberships run as direct launches) and replayed afterwards; the
+ * context keeps the 32 most recently used.  A captured step reads each frame from its
+ * system's input slots: frames passed elsewhere are first gathered there by one batched copy
+ * per kind.  Attaching or detaching a system, fu_set_tuning and disabling drop the graphs.
+ * Results are bit-identical to direct launches.  Profiled steps are never captured. */
+fu_status fu_batch_set_graph(fu_batch* b, int enable);
+fu_status fu_batch_graph_counts(fu_batch* b, uint32_t* captures, uint32_t* replays);
 
 /* ---- launch geometry of the batched stages ------------------------------------------------
  * Work-group counts, the pixel layout of a raycast wave and the placement of work-groups on
@@ -551,6 +604,9 @@ fu_status fu_server_run_trace(fu_server* sv, const fu_trace_frame* frames, int32
 /* stage profiling of every lane's steps (fu_batch_set_profiling); times summed over lanes */
 fu_status fu_server_set_profiling(fu_server* sv, int enable);
 fu_status fu_server_get_stage_times(fu_server* sv, fu_stage_times* out);
+/* captured steps on every lane (fu_batch_set_graph); counts summed over lanes */
+fu_status fu_server_set_graph(fu_server* sv, int enable);
+fu_status fu_server_graph_counts(fu_server* sv, uint32_t* captures, uint32_t* replays);
 
 #ifdef __cplusplus
 }

@@ -69,7 +69,15 @@ _SIG = {
     "or_system_set_rgb": (None, [P, I]),
     "or_system_image": (P, [P, I, I]),
     "or_system_rgb_count": (I, [P]),
+    "or_system_icp_error": (C.c_float, [P, C.POINTER(C.c_int)]),
     "or_set_threads": (None, [I]),
+    "or_keymap_create": (P, [F, I, I]),
+    "or_keymap_destroy": (None, [P]),
+    "or_keymap_reset": (None, [P]),
+    "or_keymap_insert": (None, [P, P, P, I]),
+    "or_keymap_collect": (U, [P, P, U]),
+    "or_keymap_read": (None, [P, P]),
+    "or_adjacency": (None, [P, P, P, I, P]),
 }
 
 _lib = None
@@ -478,6 +486,63 @@ class System:
     def rgb_count(self):
         return lib().or_system_rgb_count(self.h)
 
+    def icp_error(self):
+        """final ICP error and correspondence count of the last tracked frame"""
+        n = C.c_int(0)
+        e = lib().or_system_icp_error(self.h, C.byref(n))
+        return float(e), n.value
+
 
 def set_threads(n):
     lib().or_set_threads(n)
+
+
+# ---------------------------------------------------------------------- keypoint map
+# SURF records (Mapping/DeviceMap.h:99-108) as a numpy structured array of 288 bytes
+SURF = np.dtype([("valid", np.uint8), ("pad", np.uint8, 3), ("pos", np.float32, 3), ("normal", np.float32, 4),
+                 ("descriptor", np.float32, 64)])
+assert SURF.itemsize == 288
+
+
+class KeyMap:
+    """KeyMap (DeviceMap.h:163-185) with ResetKeyPoints / InsertKeyPoints / CollectKeyPoints"""
+
+    def __init__(self, grid_size=0.01, max_keys=100000, n_buckets=5):
+        self.n = max_keys * n_buckets
+        self.h = lib().or_keymap_create(grid_size, max_keys, n_buckets)
+        lib().or_keymap_reset(self.h)
+
+    def __del__(self):
+        try:
+            lib().or_keymap_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().or_keymap_reset(self.h)
+
+    def insert(self, keys, index):
+        """keys: SURF array (positions rewritten for keys that met a key of their cell), index:
+        int32 array (slot of each key; unchanged for a key that found its bucket full)"""
+        assert keys.dtype == SURF and index.dtype == np.int32 and keys.flags.c_contiguous
+        lib().or_keymap_insert(self.h, _ptr(keys), _ptr(index), len(keys))
+
+    def collect(self, cap):
+        out = np.zeros(cap, SURF)
+        n = lib().or_keymap_collect(self.h, _ptr(out), cap)
+        return out, n
+
+    def read(self):
+        out = np.zeros(self.n, SURF)
+        lib().or_keymap_read(self.h, _ptr(out))
+        return out
+
+
+def adjacency(frame_keys, map_keys, dist):
+    n = len(frame_keys)
+    out = np.zeros((n, n), np.float32)
+    fk = np.ascontiguousarray(frame_keys)
+    mk = np.ascontiguousarray(map_keys)
+    d = np.ascontiguousarray(dist, dtype=np.float32)
+    lib().or_adjacency(_ptr(fk), _ptr(mk), _ptr(d), n, _ptr(out))
+    return out

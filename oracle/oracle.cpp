@@ -1514,6 +1514,8 @@ struct or_system {
     std::vector<uint8_t> img[3], last_img[3];
     std::vector<int16_t> dx[3], dy[3];
     int last_rgb_count = 0;
+    float last_err = 0;  // final ICP error and correspondence count of the last tracked frame
+    int last_cnt = 0;
     // Tracker::state / lastState / noMissedFrames (Tracking.cc:21) and
     // System::num_frames_after_reloc (System.cc:72); a relocalisation pose given from outside
     // stands in for Relocalise (Tracking.cc:411-), which needs the keypoint map
@@ -1592,6 +1594,10 @@ extern "C" const void* or_system_image(or_system* s, int which, int level) {
     }
 }
 extern "C" int or_system_rgb_count(const or_system* s) { return s->last_rgb_count; }
+extern "C" float or_system_icp_error(const or_system* s, int* count) {
+    if (count) *count = s->last_cnt;
+    return s->last_err;
+}
 
 extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t* rgb, double pose_out[16]) {
     const int c0 = s->cols, r0 = s->rows;
@@ -1643,8 +1649,8 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
         const float* nn[3] = {s->nmap[0].data(), s->nmap[1].data(), s->nmap[2].data()};
         const float* lv[3] = {s->mvmap[0].data(), s->mvmap[1].data(), s->mvmap[2].data()};
         const float* ln[3] = {s->mnmap[0].data(), s->mnmap[1].data(), s->mnmap[2].data()};
-        float err;
-        int cnt;
+        float err = 0;
+        int cnt = 0;
         if (s->use_rgb) {
             const uint8_t* ni[3] = {s->img[0].data(), s->img[1].data(), s->img[2].data()};
             const uint8_t* li[3] = {s->last_img[0].data(), s->last_img[1].data(), s->last_img[2].data()};
@@ -1656,6 +1662,8 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
             status = or_compute_se3(nv, nn, lv, ln, c0, r0, s->K, s->iters, s->minc, s->thresh, s->last_pose, pose,
                                     &err, &cnt);
         }
+        s->last_err = err;
+        s->last_cnt = cnt;
         if (status == 1) {
             s->missed = 0;
             swap = s->last_state != -1;  // the frame after a relocalisation is tracked, not swapped
@@ -1711,3 +1719,143 @@ extern "C" int or_system_relocalise(or_system* s, const double pose[16]) {
 }
 
 extern "C" void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+// ------------------------------------------------------------------ keypoint map
+// KeyMap (Mapping/DeviceMap.h:163-185, DeviceMap.cu:203-296) and its kernels
+// ResetKeyPoints / CollectKeyPoints / InsertKeyPoints (Mapping/FuseMap.cu:366-472), with
+// inserts applied one key at a time in index order (the reference's try-lock may drop a
+// key that meets another in its bucket in the same launch).
+struct OrSurf {  // SURF (DeviceMap.h:99-108): bool valid, float3 pos, float4 normal, float[64]
+    uint8_t valid;
+    uint8_t pad[3];
+    float pos[3];
+    float normal[4];
+    float desc[64];
+};
+static_assert(sizeof(OrSurf) == 288, "SURF layout");
+
+struct or_keymap {
+    float grid;
+    int max_keys, nbuckets;
+    std::vector<OrSurf> keys;
+    std::vector<int> mutex;
+};
+
+extern "C" or_keymap* or_keymap_create(float grid, int max_keys, int nbuckets) {
+    or_keymap* k = new or_keymap();
+    k->grid = grid;
+    k->max_keys = max_keys;
+    k->nbuckets = nbuckets;
+    k->keys.assign((size_t)max_keys * nbuckets, OrSurf{});
+    k->mutex.assign((size_t)max_keys * nbuckets, 0);
+    return k;
+}
+extern "C" void or_keymap_destroy(or_keymap* k) { delete k; }
+
+// ResetKeys (DeviceMap.cu:289-296): every mutex -1, every key invalid (the rest is kept)
+extern "C" void or_keymap_reset(or_keymap* k) {
+    for (int& m : k->mutex) m = -1;
+    for (OrSurf& s : k->keys) s.valid = 0;
+}
+
+namespace {
+// KeyMap::Hash (DeviceMap.cu:206-214): MaxKeys is an int, so the xor of the wrapped products
+// is reduced with the signed remainder and made non-negative
+int key_hash(I3 p, int max_keys) {
+    const int h = (int)(((uint32_t)p.x * 73856093u) ^ ((uint32_t)p.y * 19349669u) ^ ((uint32_t)p.z * 83492791u));
+    int r = h % max_keys;
+    if (r < 0) r += max_keys;
+    return r;
+}
+// make_int3(pos / GridSize): the float quotients truncated
+I3 key_cell(const float* pos, float grid) { return I3{toi(pos[0] / grid), toi(pos[1] / grid), toi(pos[2] / grid)}; }
+}  // namespace
+
+// InsertKey (DeviceMap.cu:258-287) for keys[0..n) in order: a valid key of the same grid cell
+// in the bucket gives the inserted key its position and its slot index; otherwise the first
+// free slot of the bucket takes the key; a full bucket leaves the key out and its index as it was
+extern "C" void or_keymap_insert(or_keymap* k, OrSurf* keys, int* index, int n) {
+    for (int i = 0; i < n; ++i) {
+        OrSurf& key = keys[i];
+        const I3 p = key_cell(key.pos, k->grid);
+        const int buck = key_hash(p, k->max_keys);
+        int first = -1, found = -1;
+        for (int j = 0; j < k->nbuckets; ++j) {
+            const int b = buck * k->nbuckets + j;
+            const OrSurf& s = k->keys[b];
+            if (!s.valid && first == -1) first = b;
+            if (s.valid && eq(key_cell(s.pos, k->grid), p)) {
+                found = b;
+                break;
+            }
+        }
+        if (found >= 0) {
+            index[i] = found;
+            memcpy(key.pos, k->keys[found].pos, sizeof(key.pos));
+        } else if (first != -1) {
+            index[i] = first;
+            k->keys[first] = key;
+        }
+    }
+}
+
+// CollectKeys (FuseMap.cu:384-406) in slot order: *n = the number of valid keys; valid key j
+// (0-based) goes to out[j] for j >= 1 (the reference copies only at offsets > 0) and j < cap
+extern "C" uint32_t or_keymap_collect(const or_keymap* k, OrSurf* out, uint32_t cap) {
+    uint32_t n = 0;
+    for (const OrSurf& s : k->keys)
+        if (s.valid) {
+            if (n > 0 && n < cap && out) out[n] = s;
+            ++n;
+        }
+    return n;
+}
+
+extern "C" void or_keymap_read(const or_keymap* k, OrSurf* out) {
+    memcpy(out, k->keys.data(), k->keys.size() * sizeof(OrSurf));
+}
+
+namespace {
+inline float clamp1(float a) {  // KeyPoints.cu:6-10
+    a = a > -1.f ? a : -1.f;
+    a = a < 1.f ? a : 1.f;
+    return a;
+}
+inline V4 s_normal(const OrSurf& s) { return V4{s.normal[0], s.normal[1], s.normal[2], s.normal[3]}; }
+inline V3 s_pos(const OrSurf& s) { return V3{s.pos[0], s.pos[1], s.pos[2]}; }
+}  // namespace
+
+// BuildAdjecencyMatrixKernel (Tracking/KeyPoints.cu:12-57): n x n scores of the matched pairs
+// (frame key i <-> map key i); a 4-D dot of the normals (float4 . float4) and of
+// make_float4(direction) (w = 1) with a normal
+extern "C" void or_adjacency(const OrSurf* frame, const OrSurf* map, const float* dist, int n, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            float score = 0;
+            if (x == y) {
+                score = expf(-dist[x]);
+            } else {
+                const OrSurf &m0 = map[x], &m1 = map[y], &f0 = frame[x], &f1 = frame[y];
+                const float d00 = norm(s_pos(f0) - s_pos(f1));
+                const float d01 = norm(s_pos(m0) - s_pos(m1));
+                const V3 e10 = normalised(s_pos(f0) - s_pos(f1));
+                const V3 e11 = normalised(s_pos(m0) - s_pos(m1));
+                const V4 d10{e10.x, e10.y, e10.z, 1.0f}, d11{e11.x, e11.y, e11.z, 1.0f};
+                if (d00 <= 1e-2 || d01 <= 1e-2) {
+                    score = 0;
+                } else {
+                    const float alpha00 = acosf(clamp1(dot4(s_normal(f0), s_normal(f1))));
+                    const float beta00 = acosf(clamp1(dot4(d10, s_normal(f0))));
+                    const float gamma00 = acosf(clamp1(dot4(d10, s_normal(f1))));
+                    const float alpha01 = acosf(clamp1(dot4(s_normal(m0), s_normal(m1))));
+                    const float beta01 = acosf(clamp1(dot4(d11, s_normal(m0))));
+                    const float gamma01 = acosf(clamp1(dot4(d11, s_normal(m1))));
+                    score = expf(-(fabsf(d00 - d01) + fabsf(alpha00 - alpha01) + fabsf(beta00 - beta01) +
+                                   fabsf(gamma00 - gamma01)));
+                }
+            }
+            if (isn(score)) score = 0;
+            out[(size_t)y * n + x] = score;
+        }
+}

@@ -153,6 +153,79 @@ def test_launch_geometry_does_not_change_results(gpu, use_rgb):
             assert same_bits(va, vb) and same_bits(na, nb), (k, i)
 
 
+# step memberships of the captured-step test (indices into SESSIONS): a step's graph is
+# captured the second time its membership is met and replayed afterwards
+GRAPH_SESSIONS = (0, 1, 4)
+GRAPH_STEPS = [(0, 1, 4), (0, 1, 4), (0, 1, 4), (0, 1, 4), (0, 4), (0, 1, 4), (0, 4), (0, 4)]
+
+
+def _run_schedule(frames, use_rgb, use_graph, retune=None):
+    b = A.Batch(8)
+    if use_graph:
+        b.set_graph(True)
+    systems = {i: _system(i, frames[i][0], use_rgb) for i in GRAPH_SESSIONS}
+    for i in GRAPH_SESSIONS:
+        b.attach(systems[i])
+    nxt = dict.fromkeys(GRAPH_SESSIONS, 0)
+    res = {i: [] for i in GRAPH_SESSIONS}
+    counts = []
+    for step, members in enumerate(GRAPH_STEPS):
+        if retune is not None and step == retune[0]:
+            A.set_tuning(**retune[1])
+        b.grab([systems[i] for i in members], [frames[i][1][nxt[i]][0] for i in members],
+               [frames[i][1][nxt[i]][1] for i in members])
+        for i in members:
+            res[i].append(systems[i].result())
+            nxt[i] += 1
+        counts.append(b.graph_counts())
+    state = {i: _state(systems[i]) for i in GRAPH_SESSIONS}
+    for x in systems.values():
+        x.close()
+    b.close()
+    return res, state, counts
+
+
+@pytest.mark.parametrize("use_rgb", [0, 1])
+def test_captured_steps_equal_direct_launches(gpu, use_rgb):
+    """fu_batch_set_graph: every step's results and the final maps bit-identical to direct
+    launches; the first step of a membership runs directly, the second captures, later ones
+    replay; frames passed at fresh device addresses every step (gathered into the input
+    slots); a change of launch geometry drops the graphs"""
+    frames = {}
+    for i in GRAPH_SESSIONS:
+        kind, size, seed, cols, rows, _, _ = SESSIONS[i]
+        n = sum(i in m for m in GRAPH_STEPS)
+        _, K, fr = scene_frames(kind, size, seed, frames=range(n), cols=cols, rows=rows)
+        frames[i] = (K, [A.to_device(*f) for f in fr])
+    print("direct launches, use_rgb", use_rgb, flush=True)
+    r0, s0, c0 = _run_schedule(frames, use_rgb, False)
+    assert c0[-1] == (0, 0)
+    print("captured steps", flush=True)
+    r1, s1, c1 = _run_schedule(frames, use_rgb, True)
+    # (captures, graph launches) after each step: step 0 holds first frames (direct), (0,1,4)
+    # is captured at step 2, (0,4) met at step 4 and captured at step 6
+    assert c1 == [(0, 0), (0, 0), (1, 1), (1, 2), (1, 2), (1, 3), (2, 4), (2, 5)], c1
+    base = A.default_tuning()
+    try:
+        print("captured steps, launch geometry changed at step 3", flush=True)
+        r2, s2, c2 = _run_schedule(frames, use_rgb, True, retune=(3, TUNINGS[0]))
+    finally:
+        A.set_tuning(**base)
+    # the change drops the graph of (0,1,4) and the memberships met once: (0,1,4) runs directly
+    # at step 3 and is captured again at step 5, (0,4) is captured at step 6
+    assert c2 == [(0, 0), (0, 0), (1, 1), (1, 1), (1, 1), (2, 2), (3, 3), (3, 4)], c2
+    for r, st in ((r1, s1), (r2, s2)):
+        for i in GRAPH_SESSIONS:
+            for f, (ra, rb) in enumerate(zip(r0[i], r[i])):
+                assert ra["track_status"] == rb["track_status"], (i, f)
+                assert np.array_equal(ra["pose"], rb["pose"]), (i, f)
+                assert ra["n_visible"] == rb["n_visible"] and ra["n_allocated"] == rb["n_allocated"], (i, f)
+                assert ra["rgb_count"] == rb["rgb_count"], (i, f)
+            for a, b in zip(s0[i], st[i]):
+                assert same_bits(a, b) if a.dtype.kind == "f" else np.array_equal(a, b), i
+    assert r0[0][-1]["track_status"] == 1
+
+
 def test_batch_argument_errors(gpu):
     K = (525.0, 525.0, 319.5, 239.5)
     b = A.Batch(2)

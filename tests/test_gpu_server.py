@@ -32,13 +32,15 @@ def _params(i, K, use_rgb=0):
     return A.system_params(cols=cols, rows=rows, K=K, map=MAP, use_rgb=use_rgb)
 
 
-@pytest.mark.parametrize("closed_loop,use_rgb,host", [(False, 0, False), (True, 0, False), (False, 1, False),
-                                                      (False, 0, True)])
-def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb, host):
-    """host: frames in pinned host memory, uploaded inside the steps that take them"""
+@pytest.mark.parametrize("closed_loop,use_rgb,host,graph", [(False, 0, False, False), (True, 0, False, False),
+                                                            (False, 1, False, False), (False, 0, True, False),
+                                                            (True, 0, False, True), (False, 1, True, True)])
+def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb, host, graph):
+    """host: frames in pinned host memory, uploaded inside the steps that take them; graph:
+    steps whose membership repeats run as captured graphs"""
     frames = _frames()
     alone_res, alone_state = _alone(frames, use_rgb)
-    sv = A.Server(max_batch=3, max_inflight=2, lanes=2)
+    sv = A.Server(max_batch=3, max_inflight=2, lanes=2, use_graph=graph)
     sids = [sv.open(_params(i, K, use_rgb)) for i, (K, _) in enumerate(frames)]
     trace = []
     rng = np.random.default_rng(5)
@@ -53,6 +55,12 @@ def test_server_same_alone_and_in_mix(gpu, closed_loop, use_rgb, host):
     print("trace", len(trace), "frames, closed loop", closed_loop, flush=True)
     done = sv.run_trace(trace, closed_loop=closed_loop)
     assert len(done) == len(trace)
+    captures, replays = sv.graph_counts()
+    print("graphs captured", captures, "replayed", replays, flush=True)
+    if not graph:
+        assert captures == replays == 0
+    elif closed_loop:  # a lane's sessions step together: its membership repeats
+        assert replays > 0
     seen = {}
     for c in done:
         i, f = divmod(c["tag"], 1000)

@@ -125,19 +125,24 @@ def test_failed_frame_keeps_model(gpu):
     assert res[2]["track_status"] == 1
 
 
+RELOC_AT = 19  # index of the relocalised frame in the results
+
+
 def _lost_sequence():
-    """frame 0; ten empty frames (missed); three good frames while lost; then, after a
-    relocalisation at frame 0's pose, frames 1..13 (relocalised frame, then tracking; fusion
-    again from the tenth frame after the relocalisation)"""
-    sc, K, fr = scene_frames(frames=range(0, 14), cols=320, rows=240)
+    """frames 0..5 tracked and fused; ten empty frames (missed); three good frames while lost;
+    then, after a relocalisation at frame 5's pose, frames 6..18 (relocalised frame, then
+    tracking; fusion again from the tenth frame after the relocalisation).  Six fused frames
+    give a model that constrains every direction, so no tracked frame is near the acceptance
+    test's limits (final error <= 1.4e-5 against 1e-4, motion <= 0.036 against 0.1)."""
+    sc, K, fr = scene_frames(frames=range(0, 19), cols=320, rows=240)
     empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
-    return K, [fr[0]] + [empty] * 10 + [fr[1]] * 3 + [None] + list(fr[1:14])
+    return K, list(fr[:6]) + [empty] * 10 + [fr[6]] * 3 + [None] + list(fr[6:19])
 
 
 def test_lost_state_and_relocalisation_match_the_oracle(gpu):
     """Tracking.cc:59-112 / System.cc:92-111: statuses and fused flags of every frame equal the
-    oracle's; poses within 1e-5 per frame of tracking; device, graph and plain control and a
-    batch of two agree"""
+    oracle's; poses within 0.5 mm / 5e-4 of the oracle's independent run; device, graph and
+    plain control and a batch of two agree"""
     K, seq = _lost_sequence()
     mp = dict(MAP, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15)
     o = O.System(cols=320, rows=240, K=K, num_buckets=mp["num_buckets"], num_excess=mp["num_excess"],
@@ -148,7 +153,7 @@ def test_lost_state_and_relocalisation_match_the_oracle(gpu):
         out = []
         for f in seq:
             if f is None:
-                s.relocalise(out[0]["pose"])
+                s.relocalise(out[5]["pose"])
                 continue
             s.GrabImage(*A.to_device(*f))
             out.append(s.result())
@@ -163,7 +168,7 @@ def test_lost_state_and_relocalisation_match_the_oracle(gpu):
     for f in seq:
         if f is None:
             for x in pair:
-                x.relocalise(bat[0]["pose"])
+                x.relocalise(bat[5]["pose"])
             continue
         d, c = A.to_device(*f)
         bt.grab(pair, [d, d], [c, c])
@@ -174,24 +179,28 @@ def test_lost_state_and_relocalisation_match_the_oracle(gpu):
     ores = []
     for f in seq:
         if f is None:
-            assert o.relocalise(ores[0][1]) == 0
+            assert o.relocalise(ores[5][1]) == 0
             continue
         ores.append(o.grab(*f))
     statuses = [st for st, _ in ores]
-    assert statuses[:15] == [4] + [2] * 10 + [5] * 3 + [6], statuses
+    assert statuses[: RELOC_AT + 1] == [4] + [1] * 5 + [2] * 10 + [5] * 3 + [6], statuses
     for mode, out in runs.items():
         assert [r["track_status"] for r in out] == statuses, mode
+        # the GPU and the oracle run independently here (each tracks against its own model),
+        # so their poses drift apart by the FP32 sums' rounding, frame after frame (2e-5 after
+        # one frame); tracking from a wrong pose moves them by centimetres, and the statuses
+        # and fused flags are compared exactly
         for i, (r, (st, pose)) in enumerate(zip(out, ores)):
-            assert np.abs(r["pose"] - pose).max() < 1e-5 * (i + 1), (mode, i)
-        for r in out[:15]:
-            assert r["fused"] == (r["track_status"] == 4), mode
+            assert np.abs(r["pose"] - pose).max() < 5e-4, (mode, i)
+        for r in out[: RELOC_AT + 1]:
+            assert r["fused"] == (r["track_status"] in (1, 4)), mode
     for a, b in zip(runs["device"], runs["graph"]):
         assert np.array_equal(a["pose"], b["pose"]) and a["fused"] == b["fused"]
     for a, b in zip(runs["device"], runs["batch"]):
         assert np.array_equal(a["pose"], b["pose"])
     # fusion resumes at the tenth frame after the relocalisation
-    fused_after = [r["fused"] for r in runs["device"][15:]]
-    assert fused_after[:9] == [0] * 9 and fused_after[9] == (runs["device"][24]["track_status"] == 1)
+    fused_after = [r["fused"] for r in runs["device"][RELOC_AT + 1 :]]
+    assert fused_after[:9] == [0] * 9 and fused_after[9] == (runs["device"][RELOC_AT + 10]["track_status"] == 1)
 
 
 def test_relocalise_refused_unless_lost(gpu):
