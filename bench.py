@@ -82,6 +82,9 @@ def parse():
                     help="launch geometry of the batched stages: tune on the served workload after its warm-up "
                          "(untimed), take the cached tuning of this GPU and workload, or keep the defaults")
     ap.add_argument("--tuning-cache", default=None, help="JSON cache to store the tuning in (with --autotune run)")
+    ap.add_argument("--graph-ab", type=int, default=0,
+                    help="only compare direct launches with captured steps: alternating capacity runs of this "
+                         "many steps on one server, all frames counted")
     ap.add_argument("--probe-traffic", action="store_true",
                     help="only the program the HBM-traffic counter runs profile: a 512 MiB copy (calibration) "
                          "and 2 + 4 served steps of 64 sessions; prints no JSON")
@@ -116,6 +119,36 @@ def probe_traffic(args):
     pa = types.SimpleNamespace(max_batch=32, inflight=2, lanes=2, warmup=2, seed=args.seed, realtime_frames=0)
     r = served(A, torch, pa, sizes, 4, (), 0)
     log("traffic probe: %d frames served" % r["frames"])
+
+
+def graph_ab(A, torch, args):
+    """direct launches against captured steps on one server: alternating capacity runs of
+    `args.graph_ab` steps of every session (frames uploaded from pinned memory), all frames
+    counted; prints one JSON line"""
+    sizes = [(640, 480)] * args.sessions
+    src = Frames(A, sizes)
+    sv = A.Server(max_batch=args.max_batch, max_inflight=args.inflight, lanes=args.lanes, block_budget=BUDGET)
+    sids = [sv.open(A.system_params(cols=c, rows=r, K=kmat(c, r), map=MAP_SESSION)) for c, r in sizes]
+    warm = src.host(0, args.warmup)
+    sv.run_trace(trace(sids, warm, 0), closed_loop=True)
+    torch.cuda.synchronize()
+    lo, runs = args.warmup, []
+    for mode in ("direct", "captured", "direct", "captured"):
+        sv.set_graph(mode == "captured")
+        frames = src.host(lo, lo + args.graph_ab)
+        tr = trace(sids, frames, lo)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = sv.run_trace(tr)
+        torch.cuda.synchronize()
+        w = time.perf_counter() - t0
+        lo += args.graph_ab
+        runs.append(dict(mode=mode, frames=len(done), frames_per_s=round(len(done) / w, 1)))
+        log("%s: %.0f frames/s" % (mode, len(done) / w))
+    caps, launches = sv.graph_counts()
+    sv.close()
+    print(json.dumps(dict(workload="served %d x 640x480, %d lanes x %d" % (len(sizes), args.lanes, args.max_batch),
+                          runs=runs, graph_captures=caps, graph_launches=launches)), flush=True)
 
 
 def cpu_cores():
@@ -283,7 +316,7 @@ def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, 
         w = time.perf_counter() - t0
         res["device_resident"] = dict(
             value=round(int(np.isin([c["track_status"] for c in dd], TRACKED).sum()) / w, 2), frames=len(dd),
-            note="frames resident in HBM before the timed region (no upload)")
+            frames_per_s_all=round(len(dd) / w, 2), note="frames resident in HBM before the timed region (no upload)")
         del frames, tr
         torch.cuda.empty_cache()
         lo += device_steps
@@ -298,7 +331,9 @@ def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, 
         w = time.perf_counter() - t0
         lo += graph_steps
         g = dict(value=round(int(np.isin([c["track_status"] for c in dg], TRACKED).sum()) / w, 2), frames=len(dg),
-                 note="capacity run with every lane's steps replayed as captured hipGraphs")
+                 frames_per_s_all=round(len(dg) / w, 2),
+                 note="capacity run with every lane's steps replayed as captured hipGraphs (later frames of the "
+                      "sessions than the direct-launch runs, so compare frames_per_s_all)")
         if args.realtime_frames > 0 and sweep:
             frames = src.host(lo, lo + args.realtime_frames)
             point, _, _ = realtime_point(A, torch, sv, sids, frames, lo, sweep[0], np.random.default_rng(args.seed + 9))
@@ -562,6 +597,10 @@ def main():
         return
     import torch
     from fusion_amd import api as A
+
+    if args.graph_ab > 0:
+        graph_ab(A, torch, args)
+        return
 
     ns = args.sessions
     sizes = [(640, 480)] * ns

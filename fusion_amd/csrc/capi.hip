@@ -1725,7 +1725,7 @@ extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
 // with `prof`, the stage events ev[] are recorded between the stages.
 static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], bool track, int use_rgb, int fmt,
                         const void* const* depth, const void* const* rgb, FrameBrief* brief, bool prof,
-                        hipEvent_t* ev, hipStream_t hs) {
+                        hipEvent_t* ev, hipStream_t hs, hipEvent_t inputs_read = nullptr) {
     const int n = b.n;
     auto mark = [&](int k) {
         if (prof) (void)hipEventRecord(ev[k], hs);
@@ -1745,6 +1745,8 @@ static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], b
         for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
         launch_b_images(b, h, hs);
     }
+    // the frames are read: the later stages use the pyramid and the packed integration inputs
+    if (inputs_read) (void)hipEventRecord(inputs_read, hs);
     mark(FU_STAGE_PYRAMID + 1);
     if (track) {
         launch_b_track_begin(b, h, hs);
@@ -1832,7 +1834,7 @@ static fu_status step_graph(fu_batch* bt, BatchLaunch& b, const SessDesc* h, con
 // queue_frame_device, each launched once for all sessions.  With `brief` set, the frames'
 // results are gathered into brief[0..n).
 fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
-                         const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s) {
+                         const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s, hipEvent_t inputs_read) {
     if (!bt || !sys || !depth || !rgb) return set_err(FU_ERR_ARG, "fu_batch_grab: null argument");
     if (n < 1 || n > kMaxBatch) return set_err(FU_ERR_ARG, "fu_batch_grab: %d frames (1..%d)", n, kMaxBatch);
     BatchLaunch b;
@@ -1892,8 +1894,9 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     if (exec) {
         HIPCHK(hipGraphLaunch(exec, hs));
         bt->replays++;
+        if (inputs_read) HIPCHK(hipEventRecord(inputs_read, hs));
     } else {
-        launch_step(b, h, maxit, track, bt->rgb == 1, bt->fmt, dp, cp, brief, prof, bt->ev, hs);
+        launch_step(b, h, maxit, track, bt->rgb == 1, bt->fmt, dp, cp, brief, prof, bt->ev, hs, inputs_read);
     }
     LAUNCHCHK();
     for (int i = 0; i < n; ++i) sys[i]->frame++;

@@ -195,10 +195,12 @@ void launch_b_integrate(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t 
 void launch_b_zrange(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_raycast(BatchLaunch& b, const SessDesc* h, int fmt, hipStream_t s);
 
-// capi.hip: batch_grab with an optional device result gather (brief[0..n)), and the error
-// slot behind fu_last_error for the other translation units of the library
+// capi.hip: batch_grab with an optional device result gather (brief[0..n)) and an optional
+// event recorded on `s` once the step's kernels have read its frames (after the pyramid stage;
+// after the whole step when it runs as a captured graph), and the error slot behind
+// fu_last_error for the other translation units of the library
 fu_status batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint16_t* const* depth,
-                     const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s);
+                     const uint8_t* const* rgb, FrameBrief* brief, hipStream_t s, hipEvent_t inputs_read = nullptr);
 fu_status api_error(fu_status code, const char* fmt, ...);
 // mesh extraction of a system's map queued on `s` without synchronising; the triangle and
 // occupied-block counts land in the map's counters, whose device address `counters` returns

@@ -47,6 +47,12 @@ struct Step {
 
 struct Lane {
     hipStream_t stream = nullptr;
+    // uploads of host frames run on the server's copy stream, so a step's frames are copied
+    // while the lane's previous step computes: each upload waits until the lane's previous
+    // step has read its frames (read), and the step waits for its uploads (up[slot])
+    hipEvent_t read = nullptr;
+    bool read_recorded = false;
+    std::vector<hipEvent_t> up;
     fu_batch* batch = nullptr;
     std::vector<int> sessions;
     std::deque<Step> inflight;
@@ -74,6 +80,7 @@ struct Session {
 struct fu_server {
     fu_server_params p;
     std::vector<Lane> lanes;
+    hipStream_t copy = nullptr;  // uploads of host frames, for every lane (one DMA queue)
     std::vector<Session> sess;
     uint64_t used = 0;
     std::deque<fu_completion> done;
@@ -102,15 +109,19 @@ extern "C" fu_status fu_server_destroy(fu_server* sv) {
     if (!sv) return FU_OK;
     for (Lane& L : sv->lanes)
         if (L.stream) (void)hipStreamSynchronize(L.stream);
+    if (sv->copy) (void)hipStreamSynchronize(sv->copy);
     for (Session& s : sv->sess)
         if (s.sys) fu_system_destroy(s.sys);
     for (Lane& L : sv->lanes) {
         fu_batch_destroy(L.batch);
         for (hipEvent_t e : L.ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : L.up) (void)hipEventDestroy(e);
+        if (L.read) (void)hipEventDestroy(L.read);
         (void)hipFree(L.d_res);
         (void)hipHostFree(L.h_res);
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
+    if (sv->copy) (void)hipStreamDestroy(sv->copy);
     delete sv;
     return FU_OK;
 }
@@ -133,12 +144,16 @@ extern "C" fu_status fu_server_create(const fu_server_params* p, fu_server** out
         }
         const size_t nb = sizeof(FrameBrief) * kMaxBatch * p->max_inflight;
         hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+        if (e == hipSuccess && !sv->copy) e = hipStreamCreateWithFlags(&sv->copy, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&L.read, hipEventDisableTiming);
         if (e == hipSuccess) e = hipMalloc(&L.d_res, sizeof(FrameBrief) * kMaxBatch);
         if (e == hipSuccess) e = hipHostMalloc(&L.h_res, nb, hipHostMallocDefault);
         for (int k = 0; k < p->max_inflight && e == hipSuccess; ++k) {
-            hipEvent_t ev;
+            hipEvent_t ev, up;
             e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
             if (e == hipSuccess) L.ev.push_back(ev);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&up, hipEventDisableTiming);
+            if (e == hipSuccess) L.up.push_back(up);
         }
         if (e != hipSuccess) {
             fu_server_destroy(sv);
@@ -384,6 +399,7 @@ static fu_status launch(fu_server* sv, Lane& L) {
         const uint16_t* dp[kMaxBatch];
         const uint8_t* cp[kMaxBatch];
         hipError_t he = hipSuccess;
+        bool uploads = false;
         for (int i = 0; i < st.n; ++i) {
             Session& s = sv->sess[cand[i]];
             st.req[i] = s.pending.front();
@@ -392,20 +408,25 @@ static fu_status launch(fu_server* sv, Lane& L) {
             sys[i] = s.sys;
             dp[i] = st.req[i].depth;
             cp[i] = st.req[i].rgb;
-            if (st.req[i].host) {  // the upload of Frame::Frame (Core/Frame.cc:61-62), inside the step
-                if (he == hipSuccess) he = hipMemcpyAsync(s.depth_in, dp[i], s.npix * 2, hipMemcpyHostToDevice, L.stream);
-                if (he == hipSuccess) he = hipMemcpyAsync(s.rgb_in, cp[i], s.npix * 3, hipMemcpyHostToDevice, L.stream);
+            if (st.req[i].host) {  // the upload of Frame::Frame (Core/Frame.cc:61-62), for this step
+                if (!uploads && L.read_recorded && he == hipSuccess) he = hipStreamWaitEvent(sv->copy, L.read, 0);
+                uploads = true;
+                if (he == hipSuccess) he = hipMemcpyAsync(s.depth_in, dp[i], s.npix * 2, hipMemcpyHostToDevice, sv->copy);
+                if (he == hipSuccess) he = hipMemcpyAsync(s.rgb_in, cp[i], s.npix * 3, hipMemcpyHostToDevice, sv->copy);
                 dp[i] = s.depth_in;
                 cp[i] = s.rgb_in;
             }
         }
+        if (uploads && he == hipSuccess) he = hipEventRecord(L.up[st.slot], sv->copy);
+        if (uploads && he == hipSuccess) he = hipStreamWaitEvent(L.stream, L.up[st.slot], 0);
         // one device result buffer per lane: the lane's steps run in stream order, and each
         // step's results are copied to its host slot before the next step writes them
         FrameBrief* dres = L.d_res;
         FrameBrief* hres = L.h_res + (size_t)st.slot * kMaxBatch;
         fu_status e = FU_OK;
         if (he != hipSuccess) e = api_error(FU_ERR_HIP, "frame upload: %s", hipGetErrorString(he));
-        if (!e) e = batch_grab(L.batch, st.n, sys, dp, cp, dres, L.stream);
+        if (!e) e = batch_grab(L.batch, st.n, sys, dp, cp, dres, L.stream, L.read);
+        if (!e) L.read_recorded = true;
         if (!e) {
             he = hipMemcpyAsync(hres, dres, sizeof(FrameBrief) * st.n, hipMemcpyDeviceToHost, L.stream);
             if (he == hipSuccess) he = hipEventRecord(L.ev[st.slot], L.stream);
