@@ -100,7 +100,7 @@ def measured_traffic(kernel):
     try:
         text = open(TRAFFIC_FILE).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return int(json.loads(block)[kernel])
+        return float(json.loads(block)[kernel])
     except Exception:
         return None
 
@@ -306,6 +306,17 @@ def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, 
     nalloc = [c["n_allocated"] for c in done]
     res = dict(wall=wall, frames=len(done), tracked=int(np.isin(st, TRACKED).sum()), lat_backlog=lat_b, flags=flags,
                n_allocated_max=int(max(nalloc)), n_allocated_mean=float(np.mean(nalloc)), tuned=tuned)
+    # per-stage HIP events (each step synchronises) on the frames right after the capacity run,
+    # where the sessions track as they did in it (later in the camera paths more of them are
+    # lost in the large rooms and skip tracking)
+    if profile > 0:
+        frames = src.host(lo, lo + profile)
+        sv.set_profiling(True)
+        sv.run_trace(trace(sids, frames, lo))
+        res["stages"] = sv.stage_times()
+        sv.set_profiling(False)
+        del frames
+        lo += profile
     if device_steps > 0:
         frames = src.device(lo, lo + device_steps)
         tr = trace(sids, frames, lo)
@@ -356,14 +367,6 @@ def served(A, torch, args, sizes, steps, sweep, profile, use_rgb=0, tune=False, 
         lo += args.realtime_frames
     res["latency_curve"] = curve
     res["lat_rt"] = lat_rt or []
-    # per-stage HIP events (each step synchronises), same sessions, next frames
-    if profile > 0:
-        frames = src.host(lo, lo + profile)
-        sv.set_profiling(True)
-        sv.run_trace(trace(sids, frames, lo))
-        res["stages"] = sv.stage_times()
-        sv.set_profiling(False)
-        del frames
     res["rgb_count_mean"] = float(np.mean([c.get("rgb_count", 0) for c in done if c.get("kind", 0) == 0]))
     sv.close()
     torch.cuda.synchronize()
@@ -537,12 +540,19 @@ def rooflines(st, sizes, nsess):
         ms = st["ms"][name] / steps
         ach = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         # measured HBM bytes of a 32-frame launch (counter runs), scaled to this step size
-        tr = measured_traffic(tkey) if tkey else None
-        traffic = int(tr * per_step / 32.0) if tr else None
+        if name == "icp":  # measured per pixel-iteration: steps track different iteration counts
+            tr = measured_traffic("icp_bytes_per_pixel_iteration")
+            traffic = int(tr * st["icp_pixels"] / steps) if tr else None
+        else:
+            tr = measured_traffic(tkey) if tkey else None
+            traffic = int(tr * per_step / 32.0) if tr else None
         out[name] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                          frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, kernel=kern, bytes_per_launch=int(b),
                          avg_launch_us=round(ms * 1e3, 2), frames_per_launch=round(per_step, 2),
                          n_visible_per_frame=round(nvis, 1))
+        if name == "icp":  # a stage: the figures are per step, over the step's launch pairs
+            out[name]["note"] = ("per step: %d kb_icp + kb_icp_final launch pairs (ICP iterations), bytes and "
+                                 "time summed over them" % round(st["icp_iterations"] / frames))
     return out
 
 
@@ -662,7 +672,9 @@ def main():
     st = r.get("stages")
     rl = rooflines(st, sizes, ns) if st else None
     # the single kernel with the largest time per step (the ICP entry is a stage of many launches)
-    dominant = max(("integrate", "raycast"), key=lambda k: rl[k]["avg_launch_us"]) if rl else None
+    # the kernel with the most GPU time per step: the ICP reduction (kb_icp with its final sum,
+    # one launch pair per iteration) ahead of the raycast and the integration
+    dominant = max(("integrate", "raycast", "icp"), key=lambda k: rl[k]["avg_launch_us"]) if rl else None
     steps_p = max(1.0, st["steps"]) if st else 1.0
     out = {
         "metric": "tracked+fused 640x480 frames/s @5mm; p50/p99 frame latency, served vs plain",

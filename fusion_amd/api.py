@@ -805,6 +805,19 @@ class KeyMap:
         check(N.lib().fu_keymap_read(self.handle, out.ctypes.data_as(C.c_void_p), _s()), "fu_keymap_read")
         return out
 
+    def relocalise(self, frame_desc, frame_pos, iterations=100, seed=0):
+        """Tracker::Relocalise (Tracking.cc:411-467, graph matching off): frame_desc a device
+        float32 (n, 64) tensor, frame_pos the lifted keypoints (n, 3); returns (pose or None,
+        number of pairs that pass the ratio test)"""
+        pos = np.ascontiguousarray(frame_pos, dtype=np.float64)
+        n = int(frame_desc.shape[0])
+        pose = np.zeros(16)
+        npairs, ok = C.c_int(), C.c_int()
+        check(N.lib().fu_relocalise_keys(self.handle, _p(frame_desc), pos.ctypes.data_as(N.DP), n, int(iterations),
+                                         int(seed), pose.ctypes.data_as(N.DP), C.byref(npairs), C.byref(ok), _s()),
+              "fu_relocalise_keys")
+        return (pose.reshape(4, 4) if ok.value else None), npairs.value
+
 
 def BuildAdjacencyMatrix(frame_keys, map_keys, dist):
     """BuildAdjacencyMatrix (Tracking/Reduction.h:85-88): the (n, n) consistency scores of n
@@ -814,6 +827,49 @@ def BuildAdjacencyMatrix(frame_keys, map_keys, dist):
     check(N.lib().fu_build_adjacency_matrix(_p(frame_keys), _p(map_keys), _p(dist), n, _p(out), _s()),
           "fu_build_adjacency_matrix")
     return out
+
+
+def lift_keypoints(xy, depth, nmap, K, depth_cutoff=3.0):
+    """Frame::ExtractKeyPoints' test and back-projection (Core/Frame.cc:184-200) of keypoints
+    xy (device float32 (n, 2)) on a level-0 depth (H, W) and normal map (H, W, 4): (pos (n, 3),
+    normal (n, 4), ok (n,) uint8) device tensors"""
+    n = int(xy.shape[0])
+    rows, cols = int(depth.shape[0]), int(depth.shape[1])
+    pos = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    ok = torch.empty((n,), dtype=torch.uint8, device="cuda")
+    if n:
+        check(N.lib().fu_lift_keypoints(_p(xy), n, _p(depth), _p(nmap), cols, rows, float(K[0]), float(K[1]),
+                                        float(K[2]), float(K[3]), float(depth_cutoff), _p(pos), _p(nrm), _p(ok),
+                                        _s()), "fu_lift_keypoints")
+    return pos, nrm, ok
+
+
+def match_knn2(query, train):
+    """the two nearest train descriptors of every query descriptor (device float32 (n, 64)):
+    (idx (n, 2) int32, dist (n, 2) float32) device tensors"""
+    nq, nt = int(query.shape[0]), (0 if train is None else int(train.shape[0]))
+    idx = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+    dist = torch.empty((nq, 2), dtype=torch.float32, device="cuda")
+    if nq:
+        check(N.lib().fu_match_knn2(_p(query), nq, _p(train) if nt else None, nt, _p(idx), _p(dist), _s()),
+              "fu_match_knn2")
+    return idx, dist
+
+
+def pose_estimate(src, ref, iterations=100, seed=0):
+    """Solver::PoseEstimate (Optimization/Solver.cc): (T 4x4 mapping ref to src or None, inlier
+    mask) for (n, 3) point arrays"""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    n = int(src.shape[0])
+    T = np.zeros(16)
+    inl = np.zeros(max(n, 1), np.uint8)
+    ok = C.c_int()
+    check(N.lib().fu_pose_estimate(src.ctypes.data_as(N.DP), ref.ctypes.data_as(N.DP), n, int(iterations), int(seed),
+                                   T.ctypes.data_as(N.DP), inl.ctypes.data_as(C.c_void_p), C.byref(ok)),
+          "fu_pose_estimate")
+    return (T.reshape(4, 4) if ok.value else None), inl[:n].astype(bool)
 
 
 # --------------------------------------------------------------------- synthetic scenes

@@ -24,7 +24,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SOURCES = ["pyramid.hip", "map.hip", "raycast.hip", "mesh.hip", "icp.hip", "capi.hip", "server.hip",
-               "snapshot.hip", "selftest.hip", "noalloc.hip", "keymap.hip"]
+               "snapshot.hip", "selftest.hip", "noalloc.hip", "keymap.hip",
+               "reloc.hip"]
 
 
 def _run(cmd, verbose):

@@ -1721,33 +1721,33 @@ extern "C" fu_status fu_batch_detach(fu_batch* bt, fu_system* sys) {
     return FU_OK;
 }
 
-// The launches of one batched step on `hs`.  b.in is set per stage from `depth` and `rgb`;
-// with `prof`, the stage events ev[] are recorded between the stages.
-static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], bool track, int use_rgb, int fmt,
-                        const void* const* depth, const void* const* rgb, FrameBrief* brief, bool prof,
-                        hipEvent_t* ev, hipStream_t hs, hipEvent_t inputs_read = nullptr) {
+// A batched step on `hs`, in two parts.  The input part (model-map resize, filter, pyramid,
+// normals with the packed integration inputs, intensities) reads the frames; the rest
+// (tracking, fusion, raycast, the result gather) reads only what the input part made, so a
+// captured step is the rest alone and the frames can sit anywhere.  With `prof`, the stage
+// events ev[] are recorded between the stages.
+static void launch_inputs(BatchLaunch& b, const SessDesc* h, bool track, int use_rgb, const void* const* depth,
+                          const void* const* rgb, bool prof, hipEvent_t* ev, hipStream_t hs) {
     const int n = b.n;
-    auto mark = [&](int k) {
-        if (prof) (void)hipEventRecord(ev[k], hs);
-    };
-    mark(0);
+    if (prof) (void)hipEventRecord(ev[0], hs);
     for (int i = 0; i < n; ++i) b.in[i] = depth[i];
-    if (track) {
-        launch_b_resize(b, h, hs);
-    }
-    mark(FU_STAGE_RESIZE + 1);
+    if (track) launch_b_resize(b, h, hs);
+    if (prof) (void)hipEventRecord(ev[FU_STAGE_RESIZE + 1], hs);
     launch_b_filter(b, h, hs);
     launch_b_pyrdown(b, h, 1, hs);
     launch_b_pyrdown(b, h, 2, hs);
     for (int i = 0; i < n; ++i) b.in[i] = rgb[i];  // the normal pass packs the integration inputs
     launch_b_nmap(b, h, hs);
-    if (use_rgb) {
-        for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
-        launch_b_images(b, h, hs);
-    }
-    // the frames are read: the later stages use the pyramid and the packed integration inputs
-    if (inputs_read) (void)hipEventRecord(inputs_read, hs);
-    mark(FU_STAGE_PYRAMID + 1);
+    if (use_rgb) launch_b_images(b, h, hs);
+    if (prof) (void)hipEventRecord(ev[FU_STAGE_PYRAMID + 1], hs);
+    for (int i = 0; i < n; ++i) b.in[i] = nullptr;  // the frames are not read after this part
+}
+static void launch_rest(BatchLaunch& b, const SessDesc* h, const int maxit[3], bool track, int use_rgb, int fmt,
+                        FrameBrief* brief, bool prof, hipEvent_t* ev, hipStream_t hs) {
+    auto mark = [&](int k) {
+        if (prof) (void)hipEventRecord(ev[k], hs);
+    };
+    for (int i = 0; i < b.n; ++i) b.in[i] = nullptr;
     if (track) {
         launch_b_track_begin(b, h, hs);
         if (use_rgb) launch_b_rgb_lists(b, h, hs);
@@ -1757,7 +1757,6 @@ static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], b
     launch_b_track_commit(b, h, hs);
     if (use_rgb) launch_b_image_commit(b, h, hs);
     mark(FU_STAGE_ICP + 1);
-    for (int i = 0; i < n; ++i) b.in[i] = rgb[i];
     launch_b_alloc(b, h, hs);
     mark(FU_STAGE_ALLOC + 1);
     launch_b_visible(b, h, hs);
@@ -1771,12 +1770,11 @@ static void launch_step(BatchLaunch& b, const SessDesc* h, const int maxit[3], b
     if (brief) launch_b_results(b, brief, hs);
 }
 
-// The step's captured graph: looked up by the step's key (sessions in slot order, result
-// slot, launch-geometry generation); a key seen for the second time is captured, so one-off
-// memberships of a served lane run as plain launches.  Null when the step is to be launched
+// The captured rest of a step: looked up by the step's key (sessions in slot order, result
+// buffer, membership generation); a key met for the second time is captured, so one-off
+// memberships of a served lane run as plain launches.  Null when the rest is to be launched
 // directly.
-static fu_status step_graph(fu_batch* bt, BatchLaunch& b, const SessDesc* h, const int maxit[3],
-                            const void* const* depth, const void* const* rgb, FrameBrief* brief,
+static fu_status step_graph(fu_batch* bt, BatchLaunch& b, const SessDesc* h, const int maxit[3], FrameBrief* brief,
                             hipGraphExec_t* out) {
     *out = nullptr;
     if (bt->tuning_gen != g_tuning_gen) {
@@ -1811,7 +1809,7 @@ static fu_status step_graph(fu_batch* bt, BatchLaunch& b, const SessDesc* h, con
     hipGraph_t g;
     hipError_t e = hipStreamBeginCapture(bt->cap_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return set_err(FU_ERR_HIP, "batched step capture: %s", hipGetErrorString(e));
-    launch_step(b, h, maxit, true, bt->rgb == 1, bt->fmt, depth, rgb, brief, false, nullptr, bt->cap_stream);
+    launch_rest(b, h, maxit, true, bt->rgb == 1, bt->fmt, brief, false, nullptr, bt->cap_stream);
     e = hipStreamEndCapture(bt->cap_stream, &g);
     if (e != hipSuccess) return set_err(FU_ERR_HIP, "batched step capture: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
@@ -1867,36 +1865,17 @@ fu_status fu::batch_grab(fu_batch* bt, int n, fu_system* const* sys, const uint1
     }
     hipGraphExec_t exec = nullptr;
     if (bt->use_graph && !prof && b.first == 0) {
-        // a captured step reads every frame from its system's input slots: frames elsewhere
-        // are gathered there first by one batched copy
-        BatchCopy cd, cc;
-        cd.n = cc.n = 0;
-        for (int i = 0; i < n; ++i) {
-            fu_system* x = sys[i];
-            const size_t n0 = (size_t)x->p.cols * x->p.rows;
-            if (depth[i] != x->b.depth_in) cd.add(x->b.depth_in, depth[i], n0 * 2);
-            if (rgb[i] != x->b.rgb_in) cc.add(x->b.rgb_in, rgb[i], n0 * 3);
-            dp[i] = x->b.depth_in;
-            cp[i] = x->b.rgb_in;
-        }
-        fu_status st = step_graph(bt, b, h, maxit, dp, cp, brief, &exec);
+        fu_status st = step_graph(bt, b, h, maxit, brief, &exec);
         if (st) return st;
-        if (exec) {
-            launch_batch_copy(cd, hs);
-            launch_batch_copy(cc, hs);
-        }
-        if (!exec)  // not captured (yet): the direct launches read the frames where they are
-            for (int i = 0; i < n; ++i) {
-                dp[i] = depth[i];
-                cp[i] = rgb[i];
-            }
     }
+    launch_inputs(b, h, track, bt->rgb == 1, dp, cp, prof, bt->ev, hs);
+    // the frames are read: the rest uses the pyramid and the packed integration inputs
+    if (inputs_read) HIPCHK(hipEventRecord(inputs_read, hs));
     if (exec) {
         HIPCHK(hipGraphLaunch(exec, hs));
         bt->replays++;
-        if (inputs_read) HIPCHK(hipEventRecord(inputs_read, hs));
     } else {
-        launch_step(b, h, maxit, track, bt->rgb == 1, bt->fmt, dp, cp, brief, prof, bt->ev, hs, inputs_read);
+        launch_rest(b, h, maxit, track, bt->rgb == 1, bt->fmt, brief, prof, bt->ev, hs);
     }
     LAUNCHCHK();
     for (int i = 0; i < n; ++i) sys[i]->frame++;

@@ -157,22 +157,7 @@ struct FrameBrief {
     unsigned pad;
 };
 
-// frames of a batched step to copy into their systems' input slots (one launch per kind)
-struct BatchCopy {
-    int n;
-    void* dst[kMaxBatch];
-    const void* src[kMaxBatch];
-    unsigned bytes[kMaxBatch];
-    void add(void* d, const void* s, size_t nb) {
-        dst[n] = d;
-        src[n] = s;
-        bytes[n] = (unsigned)nb;
-        ++n;
-    }
-};
-
 // pyramid.hip
-void launch_batch_copy(const BatchCopy& c, hipStream_t s);
 void launch_b_resize(BatchLaunch& b, const SessDesc* h, hipStream_t s);  // levels 1 and 2
 void launch_b_filter(BatchLaunch& b, const SessDesc* h, hipStream_t s);
 void launch_b_pyrdown(BatchLaunch& b, const SessDesc* h, int level, hipStream_t s);

@@ -545,25 +545,4 @@ void launch_b_image_commit(BatchLaunch& b, const SessDesc* h, hipStream_t s) {
     blaunch(kb_image_commit, batch_layout(b, h, BT_PYR3, 0, 0), b, s);
 }
 
-// Copies of one batched step's frames into their systems' input slots: blockIdx.y is the
-// copy, the x groups stride over it in 16-byte words (bytes, where either end is unaligned).
-constexpr int kCopyGroups = 32;
-__global__ __launch_bounds__(256) void kb_copy(BatchCopy c) {
-    const int k = (int)blockIdx.y;
-    unsigned char* dst = (unsigned char*)c.dst[k];
-    const unsigned char* src = (const unsigned char*)c.src[k];
-    const size_t bytes = c.bytes[k];
-    const size_t lane = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
-    size_t done = 0;
-    if ((((uintptr_t)dst | (uintptr_t)src) & 15) == 0) {
-        const size_t words = bytes >> 4;
-        for (size_t i = lane; i < words; i += stride) ((uint4*)dst)[i] = ((const uint4*)src)[i];
-        done = words << 4;
-    }
-    for (size_t i = done + lane; i < bytes; i += stride) dst[i] = src[i];
-}
-void launch_batch_copy(const BatchCopy& c, hipStream_t s) {
-    if (c.n > 0) hipLaunchKernelGGL(kb_copy, dim3(kCopyGroups, c.n), dim3(256), 0, s, c);
-}
-
 }  // namespace fu

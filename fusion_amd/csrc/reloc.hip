@@ -1,0 +1,521 @@
+// Keypoint relocalisation without the detector: the parts of Frame::ExtractKeyPoints
+// (Core/Frame.cc:93-212) and Tracker::Relocalise (Tracking/Tracking.cc:411-467) that do not
+// depend on OpenCV's SURF.  The caller supplies the keypoints (pixel positions) and their
+// descriptors; the library lifts them to 3-D with the frame's depth and normal maps, matches
+// them against the key map's records (the role of cv::cuda::BFMatcher's knnMatch, k = 2, L2)
+// and estimates the pose with the reference's RANSAC / Kabsch loop (Optimization/Solver.cc).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fu_api.h"
+
+using namespace fu;
+
+namespace {
+
+// ---------------------------------------------------------------------------- lifting
+// The bounds test of InterpDepth / InterpNormal (Frame.cc:96, :133) compares y with the width
+// and x with the height, as written; where it would let the four neighbours fall outside the
+// image (y >= rows - 1 on a landscape frame: the reference reads past the image) or the
+// position is NaN, the keypoint is refused.
+__device__ __forceinline__ bool in_frame(float x, float y, int cols, int rows) {
+    if (x <= 1 || y <= 1 || y >= cols - 1 || x >= rows - 1) return false;
+    return !(isnan(x) || isnan(y) || x >= cols - 1 || y >= rows - 1);
+}
+
+// Frame::InterpDepth (Core/Frame.cc:93-129): bilinear depth at (x, y) from the four
+// neighbours, NaN when any is invalid or outside [0.3, cutoff] or the result strays more than
+// 5 mm from the upper-left one.
+__device__ __forceinline__ float interp_depth(const float* d, int cols, int rows, float x, float y, float cutoff) {
+    const float nan = __int_as_float(0x7fffffff);
+    if (!in_frame(x, y, cols, rows)) return nan;
+    const float cx = x - floorf(x), cy = y - floorf(y);
+    const int x0 = (int)floorf(x), x1 = (int)ceilf(x), y0 = (int)floorf(y), y1 = (int)ceilf(y);
+    const float d00 = d[y0 * cols + x0];
+    if (isnan(d00) || (double)d00 < 0.3 || d00 > cutoff) return nan;
+    const float d10 = d[y1 * cols + x0];
+    if (isnan(d10) || (double)d10 < 0.3 || d10 > cutoff) return nan;
+    const float d01 = d[y0 * cols + x1];
+    if (isnan(d01) || (double)d01 < 0.3 || d01 > cutoff) return nan;
+    const float d11 = d[y1 * cols + x1];
+    if (isnan(d11) || (double)d11 < 0.3 || d11 > cutoff) return nan;
+    const float e0 = d01 * cx + d00 * (1 - cx);
+    const float e1 = d11 * cx + d10 * (1 - cx);
+    const float fin = (1 - cy) * e0 + cy * e1;
+    return (double)fabsf(fin - d00) <= 0.005 ? fin : nan;
+}
+
+// Frame::InterpNormal (Frame.cc:131-165): the same blend of the four normals (4-D), NaN when
+// it differs from the upper-left one by more than 0.1 (4-D norm)
+__device__ __forceinline__ f4 interp_normal(const f4* n, int cols, int rows, float x, float y) {
+    const float nan = __int_as_float(0x7fffffff);
+    const f4 bad = mk4(nan, nan, nan, nan);
+    if (!in_frame(x, y, cols, rows)) return bad;
+    const float cx = x - floorf(x), cy = y - floorf(y);
+    const int x0 = (int)floorf(x), x1 = (int)ceilf(x), y0 = (int)floorf(y), y1 = (int)ceilf(y);
+    const f4 d00 = n[y0 * cols + x0], d10 = n[y1 * cols + x0], d01 = n[y0 * cols + x1], d11 = n[y1 * cols + x1];
+    const float ax = 1 - cx, ay = 1 - cy;
+    const f4 e0 = mk4(d01.x * cx + d00.x * ax, d01.y * cx + d00.y * ax, d01.z * cx + d00.z * ax, d01.w * cx + d00.w * ax);
+    const f4 e1 = mk4(d11.x * cx + d10.x * ax, d11.y * cx + d10.y * ax, d11.z * cx + d10.z * ax, d11.w * cx + d10.w * ax);
+    const f4 fin = mk4(e0.x * ay + e1.x * cy, e0.y * ay + e1.y * cy, e0.z * ay + e1.z * cy, e0.w * ay + e1.w * cy);
+    const f4 df = mk4(fin.x - d00.x, fin.y - d00.y, fin.z - d00.z, fin.w - d00.w);
+    const float nm = sqrtf(df.x * df.x + df.y * df.y + df.z * df.z + df.w * df.w);
+    return (double)nm <= 0.1 ? fin : bad;
+}
+
+struct LiftArgs {
+    const float* xy;     // n keypoints (x, y) in pixels
+    const float* depth;  // level-0 filtered depth
+    const f4* nmap;      // level-0 normals
+    int cols, rows, n;
+    float fx, fy, cx, cy, cutoff;
+    float* pos;          // n x 3
+    float* normal;       // n x 4
+    unsigned char* ok;   // n
+};
+
+// the per-keypoint test and back-projection of ExtractKeyPoints (Frame.cc:184-200)
+__global__ void k_lift_keys(LiftArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float x = a.xy[2 * i], y = a.xy[2 * i + 1];
+    const float nan = __int_as_float(0x7fffffff);
+    float p[3] = {nan, nan, nan};
+    f4 nr = mk4(nan, nan, nan, nan);
+    bool ok = false;
+    const float dp = interp_depth(a.depth, a.cols, a.rows, x, y, a.cutoff);
+    if (!isnan(dp) && (double)dp > 0.3 && dp < a.cutoff) {
+        const f4 n = interp_normal(a.nmap, a.cols, a.rows, x, y);
+        if (!isnan(n.x)) {
+            p[0] = dp * (x - a.cx) / a.fx;
+            p[1] = dp * (y - a.cy) / a.fy;
+            p[2] = dp;
+            nr = n;
+            ok = true;
+        }
+    }
+    a.pos[3 * i + 0] = p[0];
+    a.pos[3 * i + 1] = p[1];
+    a.pos[3 * i + 2] = p[2];
+    a.normal[4 * i + 0] = nr.x;
+    a.normal[4 * i + 1] = nr.y;
+    a.normal[4 * i + 2] = nr.z;
+    a.normal[4 * i + 3] = nr.w;
+    a.ok[i] = ok;
+}
+
+// ---------------------------------------------------------------------------- matching
+// The two nearest train descriptors of every query descriptor by L2 distance (sqrt of the sum
+// of the 64 squared differences, in dimension order); ties go to the lower train index.
+// Work-group: 256 queries, one per lane (its descriptor in registers), against a chunk of
+// kChunk train descriptors staged through LDS kTile at a time (every lane reads the same
+// record: LDS broadcast).  Each (query, chunk) pair leaves its best two; a second pass merges
+// the chunks in order.
+constexpr int kDesc = 64;
+constexpr int kTile = 128;
+constexpr int kChunk = 8192;
+
+struct Best2 {
+    float d0, d1;
+    int i0, i1;
+};
+
+__device__ __forceinline__ void push2(Best2& b, float d, int i) {
+    if (d < b.d0) {
+        b.d1 = b.d0;
+        b.i1 = b.i0;
+        b.d0 = d;
+        b.i0 = i;
+    } else if (d < b.d1) {
+        b.d1 = d;
+        b.i1 = i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_match_chunk(const float* q, int nq, const float* t, int nt, Best2* part) {
+    __shared__ float tile[kTile * kDesc];
+    const int qi = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * kChunk, c1 = min(nt, c0 + kChunk);
+    float mine[kDesc];
+#pragma unroll
+    for (int j = 0; j < kDesc; ++j) mine[j] = qi < nq ? q[(size_t)qi * kDesc + j] : 0.f;
+    Best2 b{INFINITY, INFINITY, -1, -1};
+    for (int base = c0; base < c1; base += kTile) {
+        const int cnt = min(kTile, c1 - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt * kDesc; k += 256) tile[k] = t[(size_t)base * kDesc + k];
+        __syncthreads();
+        for (int r = 0; r < cnt; ++r) {
+            float s = 0.f;
+#pragma unroll
+            for (int j = 0; j < kDesc; ++j) {
+                const float dj = mine[j] - tile[r * kDesc + j];
+                s = s + dj * dj;
+            }
+            push2(b, sqrtf(s), base + r);
+        }
+    }
+    if (qi < nq) part[(size_t)blockIdx.y * nq + qi] = b;
+}
+
+__global__ void k_match_merge(const Best2* part, int nq, int nchunks, int* idx, float* dist) {
+    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    Best2 b{INFINITY, INFINITY, -1, -1};
+    for (int c = 0; c < nchunks; ++c) {  // chunks in train order: an earlier index wins a tie
+        const Best2 p = part[(size_t)c * nq + qi];
+        if (p.i0 >= 0) push2(b, p.d0, p.i0);
+        if (p.i1 >= 0) push2(b, p.d1, p.i1);
+    }
+    idx[2 * qi] = b.i0;
+    idx[2 * qi + 1] = b.i1;
+    dist[2 * qi] = b.d0;
+    dist[2 * qi + 1] = b.d1;
+}
+
+// ---------------------------------------------------------------------------- pose
+// 3x3 helpers in double, row-major
+struct M3d {
+    double a[9];
+};
+M3d mul(const M3d& x, const M3d& y) {
+    M3d r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = x.a[i * 3] * y.a[j] + x.a[i * 3 + 1] * y.a[3 + j] + x.a[i * 3 + 2] * y.a[6 + j];
+    return r;
+}
+M3d transpose(const M3d& x) {
+    M3d r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.a[i * 3 + j] = x.a[j * 3 + i];
+    return r;
+}
+double det(const M3d& m) {
+    const double* a = m.a;
+    return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// R = U V^T for Ab = U S V^T (the Kabsch rotation of Solver.cc, R = (V U^T)^T).  One-sided
+// Jacobi on the columns of Ab gives Ab V = U S; a singular vector of a zero singular value
+// (three points of a sample span a plane) is completed as the cross product of the other two,
+// on both sides, so such a sample yields the proper rotation.
+M3d kabsch(const M3d& Ab) {
+    double A[9], V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(A, Ab.a, sizeof(A));
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        double off = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                double al = 0, be = 0, ga = 0;
+                for (int i = 0; i < 3; ++i) {
+                    al += A[i * 3 + p] * A[i * 3 + p];
+                    be += A[i * 3 + q] * A[i * 3 + q];
+                    ga += A[i * 3 + p] * A[i * 3 + q];
+                }
+                if (ga == 0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+                off = fmax(off, fabs(ga) / sqrt(al * be));
+                const double z = (be - al) / (2 * ga);
+                const double t = (z >= 0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1 + z * z));
+                const double c = 1 / sqrt(1 + t * t), s = c * t;
+                for (int i = 0; i < 3; ++i) {
+                    const double ap = A[i * 3 + p], aq = A[i * 3 + q];
+                    A[i * 3 + p] = c * ap - s * aq;
+                    A[i * 3 + q] = s * ap + c * aq;
+                    const double vp = V[i * 3 + p], vq = V[i * 3 + q];
+                    V[i * 3 + p] = c * vp - s * vq;
+                    V[i * 3 + q] = s * vp + c * vq;
+                }
+            }
+        if (off < 1e-15) break;
+    }
+    // singular values (column norms), largest first
+    double sv[3];
+    int ord[3] = {0, 1, 2};
+    for (int k = 0; k < 3; ++k) sv[k] = sqrt(A[k] * A[k] + A[3 + k] * A[3 + k] + A[6 + k] * A[6 + k]);
+    std::sort(ord, ord + 3, [&](int x, int y) { return sv[x] > sv[y]; });
+    double U[9], W[9];
+    for (int c = 0; c < 3; ++c) {
+        const int k = ord[c];
+        for (int i = 0; i < 3; ++i) W[i * 3 + c] = V[i * 3 + k];
+        const double inv = sv[k] > 0 ? 1 / sv[k] : 0;
+        for (int i = 0; i < 3; ++i) U[i * 3 + c] = A[i * 3 + k] * inv;
+    }
+    if (sv[ord[2]] <= 1e-12 * fmax(sv[ord[0]], 1e-300)) {
+        // rank <= 2: third columns as cross products of the first two
+        U[2] = U[3] * U[7] - U[6] * U[4];
+        U[5] = U[6] * U[1] - U[0] * U[7];
+        U[8] = U[0] * U[4] - U[3] * U[1];
+        W[2] = W[3] * W[7] - W[6] * W[4];
+        W[5] = W[6] * W[1] - W[0] * W[7];
+        W[8] = W[0] * W[4] - W[3] * W[1];
+    }
+    M3d u, w;
+    memcpy(u.a, U, sizeof(U));
+    memcpy(w.a, W, sizeof(W));
+    return mul(u, transpose(w));
+}
+
+struct V3d {
+    double x, y, z;
+};
+V3d operator-(V3d a, V3d b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3d operator+(V3d a, V3d b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3d operator/(V3d a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+V3d rot(const M3d& R, V3d v) {
+    return {R.a[0] * v.x + R.a[1] * v.y + R.a[2] * v.z, R.a[3] * v.x + R.a[4] * v.y + R.a[5] * v.z,
+            R.a[6] * v.x + R.a[7] * v.y + R.a[8] * v.z};
+}
+double vnorm(V3d v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+V3d vcross(V3d a, V3d b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+void outer_add(M3d& m, V3d a, V3d b) {
+    const double av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) m.a[i * 3 + j] += av[i] * bv[j];
+}
+
+// the sample generator: splitmix64 in place of the reference's srand(clock) / rand()
+struct Rng {
+    uint64_t s;
+    uint32_t next(uint32_t n) {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (uint32_t)((z >> 32) % n);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- C ABI
+extern "C" fu_status fu_lift_keypoints(const float* xy, int32_t n, const float* depth, const float* nmap, int32_t cols,
+                                       int32_t rows, float fx, float fy, float cx, float cy, float depth_cutoff,
+                                       float* pos, float* normal, uint8_t* ok, fu_stream s) {
+    if (n < 0 || cols < 1 || rows < 1) return api_error(FU_ERR_ARG, "fu_lift_keypoints: bad sizes");
+    if (n == 0) return FU_OK;
+    if (!xy || !depth || !nmap || !pos || !normal || !ok) return api_error(FU_ERR_ARG, "fu_lift_keypoints: null argument");
+    LiftArgs a{xy, depth, (const f4*)nmap, cols, rows, n, fx, fy, cx, cy, depth_cutoff, pos, normal, ok};
+    hipLaunchKernelGGL(k_lift_keys, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_lift_keypoints: %s", hipGetErrorString(e));
+    return FU_OK;
+}
+
+extern "C" fu_status fu_match_knn2(const float* query, int32_t nq, const float* train, int32_t nt, int32_t* idx,
+                                   float* dist, fu_stream s) {
+    if (nq < 0 || nt < 0) return api_error(FU_ERR_ARG, "fu_match_knn2: bad sizes");
+    if (nq == 0) return FU_OK;
+    if (!query || !idx || !dist || (nt > 0 && !train)) return api_error(FU_ERR_ARG, "fu_match_knn2: null argument");
+    if ((long long)nq * kDesc > (1ll << 31) || (long long)nt * kDesc > (1ll << 31))
+        return api_error(FU_ERR_ARG, "fu_match_knn2: %d x %d descriptors (at most 2^25 each)", nq, nt);
+    hipStream_t hs = (hipStream_t)s;
+    const int nchunks = nt > 0 ? (nt + kChunk - 1) / kChunk : 1;
+    Best2* part = nullptr;
+    if (hipMallocAsync((void**)&part, sizeof(Best2) * (size_t)nchunks * nq, hs) != hipSuccess) {
+        (void)hipGetLastError();
+        return api_error(FU_ERR_NOMEM, "fu_match_knn2: %d partial lists", nchunks);
+    }
+    if (nt > 0)
+        hipLaunchKernelGGL(k_match_chunk, dim3((nq + 255) / 256, nchunks), dim3(256), 0, hs, query, nq, train, nt, part);
+    hipLaunchKernelGGL(k_match_merge, dim3((nq + 255) / 256), dim3(256), 0, hs, part, nq, nt > 0 ? nchunks : 0, idx,
+                       dist);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipFreeAsync(part, hs);
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_match_knn2: %s", hipGetErrorString(e));
+    return FU_OK;
+}
+
+// Solver::PoseEstimate (Optimization/Solver.cc:6-183), checkAngle = false
+extern "C" fu_status fu_pose_estimate(const double* src, const double* ref, int32_t n, int32_t iterations,
+                                      uint64_t seed, double T[16], uint8_t* inlier, int32_t* ok) {
+    if (!src || !ref || !T || !ok || n < 0 || iterations < 0) return api_error(FU_ERR_ARG, "fu_pose_estimate: bad argument");
+    *ok = 0;
+    for (int k = 0; k < 16; ++k) T[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    if (n < 3) return FU_OK;
+    auto S = [&](int i) { return V3d{src[3 * i], src[3 * i + 1], src[3 * i + 2]}; };
+    auto Rf = [&](int i) { return V3d{ref[3 * i], ref[3 * i + 1], ref[3 * i + 2]}; };
+    M3d R_best{{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    V3d t_best{0, 0, 0};
+    int inliers_best = 0;
+    const float thresh_inlier = 0.05f;
+    const float thresh_confidence = 0.95f;
+    const int min_iter = 20;
+    float ratio = 0.f, confidence = 0.f;
+    std::vector<char> outlier;  // the last evaluated hypothesis' outliers (Solver.cc:84-93)
+    Rng rng{seed};
+    int it = 0;
+    while (it < iterations) {
+        int smp[3];
+        for (int k = 0; k < 3; ++k) smp[k] = (int)rng.next((uint32_t)n);
+        const bool bad = smp[0] == smp[1] || smp[1] == smp[2] || smp[2] == smp[0];
+        V3d sa = S(smp[0]), sb = S(smp[1]), sc = S(smp[2]);
+        V3d ra = Rf(smp[0]), rb = Rf(smp[1]), rc = Rf(smp[2]);
+        const float sd = (float)vnorm(vcross(sb - sa, sa - sc));  // stored in floats (Solver.cc:52-53)
+        const float rd = (float)vnorm(vcross(rb - ra, ra - rc));
+        if (bad || (double)sd < 1e-6 || (double)rd < 1e-6) {
+            ++it;
+            continue;
+        }
+        const V3d sm = (sa + sb + sc) / 3, rm = (ra + rb + rc) / 3;
+        M3d Ab{{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+        outer_add(Ab, sa - sm, ra - rm);
+        outer_add(Ab, sb - sm, rb - rm);
+        outer_add(Ab, sc - sm, rc - rm);
+        M3d R = kabsch(Ab);
+        if (det(R) < 0) continue;  // as Solver.cc:82: a new sample, the iteration not counted
+        const V3d t = sm - rot(R, rm);
+        int nin = 0;
+        outlier.assign(n, 1);
+        for (int i = 0; i < n; ++i)
+            if (vnorm(S(i) - (rot(R, Rf(i)) + t)) <= thresh_inlier) {
+                ++nin;
+                outlier[i] = 0;
+            }
+        if (nin > inliers_best) {
+            V3d smn{0, 0, 0}, rmn{0, 0, 0};
+            for (int i = 0; i < n; ++i)
+                if (!outlier[i]) {
+                    smn = smn + S(i);
+                    rmn = rmn + Rf(i);
+                }
+            smn = smn / nin;
+            rmn = rmn / nin;
+            M3d Ab2{{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+            for (int i = 0; i < n; ++i)
+                if (!outlier[i]) outer_add(Ab2, S(i) - smn, Rf(i) - rmn);
+            R_best = kabsch(Ab2);
+            t_best = smn - rot(R_best, rmn);
+            inliers_best = nin;
+            ratio = (float)nin / (float)n;
+            // pow of a float and an int is the double pow (Solver.cc:124)
+            confidence = (float)(1 - pow(1 - pow((double)ratio, 3), (double)(it + 1)));
+            if (it >= min_iter && confidence >= thresh_confidence) break;
+        }
+        ++it;
+    }
+    if (outlier.empty()) return FU_OK;  // no hypothesis was evaluated
+    // the final set: the best model's inliers added to the last evaluated hypothesis' (the
+    // reference does not reset the flags first, Solver.cc:138-143)
+    for (int i = 0; i < n; ++i)
+        if (vnorm(S(i) - (rot(R_best, Rf(i)) + t_best)) <= thresh_inlier) outlier[i] = 0;
+    int N = 0;
+    V3d smn{0, 0, 0}, rmn{0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        if (!outlier[i]) {
+            smn = smn + S(i);
+            rmn = rmn + Rf(i);
+            ++N;
+        }
+    if (N == 0) return FU_OK;
+    smn = smn / N;
+    rmn = rmn / N;
+    M3d Ab{{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    for (int i = 0; i < n; ++i)
+        if (!outlier[i]) outer_add(Ab, S(i) - smn, Rf(i) - rmn);
+    R_best = kabsch(Ab);
+    if (det(R_best) < 0) return FU_OK;
+    t_best = smn - rot(R_best, rmn);
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) T[i * 4 + j] = R_best.a[i * 3 + j];
+    }
+    T[3] = t_best.x;
+    T[7] = t_best.y;
+    T[11] = t_best.z;
+    if (inlier)
+        for (int i = 0; i < n; ++i) inlier[i] = !outlier[i];
+    (void)inliers_best;
+    *ok = 1;
+    return FU_OK;
+}
+
+// Tracker::Relocalise with useGraphMatching off, its default (Tracking.cc:411-467): the map's
+// keys (UpdateMapKeys, Mapping.cc:142-151), the two nearest map descriptors of every frame
+// descriptor, the ratio test d0 < 0.9 d1, PoseEstimate over the surviving pairs (frame point
+// <- map point) and the pose as the inverse of its transform.  frame_desc: nq x 64 floats in
+// device memory; frame_pos: nq x 3 doubles on the host (the lifted keypoints).
+extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, const double* frame_pos, int32_t nq,
+                                        int32_t iterations, uint64_t seed, double pose[16], int32_t* n_pairs,
+                                        int32_t* ok, fu_stream s) {
+    if (!km || !pose || !ok || nq < 0 || (nq > 0 && (!frame_desc || !frame_pos)))
+        return api_error(FU_ERR_ARG, "fu_relocalise_keys: bad argument");
+    *ok = 0;
+    if (n_pairs) *n_pairs = 0;
+    for (int k = 0; k < 16; ++k) pose[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    hipStream_t hs = (hipStream_t)s;
+    uint32_t nk = 0;
+    fu_status st = fu_collect_key_points(km, nullptr, 0, &nk, s);
+    if (st) return st;
+    if (nk == 0 || nq == 0) return FU_OK;  // no map keys (Tracking.cc:417-418)
+    // the records; the first is never written by CollectKeyPoints and reads as zeros here
+    fu_surf* keys = nullptr;
+    float* train = nullptr;
+    int32_t* idx = nullptr;
+    float* dist = nullptr;
+    auto release = [&]() {
+        (void)hipFree(keys);
+        (void)hipFree(train);
+        (void)hipFree(idx);
+        (void)hipFree(dist);
+    };
+    if (hipMalloc((void**)&keys, sizeof(fu_surf) * nk) != hipSuccess ||
+        hipMalloc((void**)&train, sizeof(float) * 64 * nk) != hipSuccess ||
+        hipMalloc((void**)&idx, sizeof(int32_t) * 2 * nq) != hipSuccess ||
+        hipMalloc((void**)&dist, sizeof(float) * 2 * nq) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return api_error(FU_ERR_NOMEM, "fu_relocalise_keys: %u keys", nk);
+    }
+    std::vector<fu_surf> host(nk);
+    std::vector<int32_t> hidx(2 * (size_t)nq);
+    std::vector<float> hdist(2 * (size_t)nq);
+    hipError_t e = hipMemsetAsync(keys, 0, sizeof(fu_surf) * nk, hs);
+    if (e == hipSuccess) {
+        uint32_t again = 0;
+        st = fu_collect_key_points(km, keys, nk, &again, s);
+    }
+    // descriptors packed for the matcher: 256 bytes of each 288-byte record
+    if (!st && e == hipSuccess)
+        e = hipMemcpy2DAsync(train, 64 * sizeof(float), (const char*)keys + offsetof(fu_surf, descriptor), sizeof(fu_surf),
+                             64 * sizeof(float), nk, hipMemcpyDeviceToDevice, hs);
+    if (!st && e == hipSuccess) st = fu_match_knn2(frame_desc, nq, train, (int32_t)nk, idx, dist, s);
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(host.data(), keys, sizeof(fu_surf) * nk, hipMemcpyDeviceToHost, hs);
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(hidx.data(), idx, sizeof(int32_t) * 2 * nq, hipMemcpyDeviceToHost, hs);
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(hdist.data(), dist, sizeof(float) * 2 * nq, hipMemcpyDeviceToHost, hs);
+    if (!st && e == hipSuccess) e = hipStreamSynchronize(hs);
+    release();
+    if (st) return st;
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_relocalise_keys: %s", hipGetErrorString(e));
+    // the ratio test (Tracking.cc:437-444); with one map key the second distance is infinite
+    std::vector<double> src, ref;
+    for (int i = 0; i < nq; ++i) {
+        if (hidx[2 * i] < 0) continue;
+        if ((double)hdist[2 * i] < 0.90 * (double)hdist[2 * i + 1]) {
+            const fu_surf& k = host[hidx[2 * i]];
+            src.insert(src.end(), {frame_pos[3 * i], frame_pos[3 * i + 1], frame_pos[3 * i + 2]});
+            ref.insert(ref.end(), {(double)k.pos[0], (double)k.pos[1], (double)k.pos[2]});
+        }
+    }
+    const int np = (int)(src.size() / 3);
+    if (n_pairs) *n_pairs = np;
+    if (np < 3) return FU_OK;
+    double T[16];
+    int32_t good = 0;
+    if ((st = fu_pose_estimate(src.data(), ref.data(), np, iterations, seed, T, nullptr, &good))) return st;
+    if (!good) return FU_OK;
+    // pose = T^-1 for the rigid T (frame point = R * map point + t)
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) pose[i * 4 + j] = T[j * 4 + i];
+        pose[i * 4 + 3] = -(T[0 * 4 + i] * T[3] + T[1 * 4 + i] * T[7] + T[2 * 4 + i] * T[11]);
+    }
+    pose[12] = pose[13] = pose[14] = 0;
+    pose[15] = 1;
+    *ok = 1;
+    return FU_OK;
+}

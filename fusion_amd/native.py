@@ -209,6 +209,10 @@ SIGNATURES = {
     "fu_collect_key_points": (I, [P, P, C.c_uint32, C.POINTER(C.c_uint32), P]),
     "fu_keymap_read": (I, [P, P, P]),
     "fu_build_adjacency_matrix": (I, [P, P, P, I, P, P]),
+    "fu_lift_keypoints": (I, [P, I, P, P, I, I, F, F, F, F, F, P, P, P, P]),
+    "fu_match_knn2": (I, [P, I, P, I, P, P, P]),
+    "fu_pose_estimate": (I, [DP, DP, I, I, C.c_uint64, DP, P, C.POINTER(I)]),
+    "fu_relocalise_keys": (I, [P, P, DP, I, I, C.c_uint64, DP, C.POINTER(I), C.POINTER(I), P]),
     "fu_server_submit_host": (I, [P, I, P, P, C.c_uint64, C.c_double]),
     "fu_server_submit_mesh": (I, [P, I, P, P, P, C.c_uint32, C.c_uint64, C.c_double]),
     "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),

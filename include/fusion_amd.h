@@ -379,6 +379,45 @@ fu_status fu_keymap_read(fu_keymap* k, fu_surf* host, fu_stream s);
 fu_status fu_build_adjacency_matrix(const fu_surf* frame_keys, const fu_surf* map_keys, const float* dist, int32_t n,
                                     float* matrix, fu_stream s);
 
+/* ---------------------------------------------------------------- relocalisation
+ * The detector-free parts of keypoint relocalisation.  The reference extracts SURF keypoints
+ * with OpenCV (Frame::ExtractKeyPoints, Core/Frame.cc:168-212), which this library does not
+ * have: the caller brings each keypoint's pixel position and 64-float descriptor. */
+/* the keypoint test and back-projection of Frame::ExtractKeyPoints (Frame.cc:184-200, with
+ * InterpDepth / InterpNormal, :93-165, and their bounds test as written: y against the width,
+ * x against the height): for keypoint i at xy[2i], xy[2i+1] (device), the bilinear depth of
+ * the level-0 filtered depth and normal of the level-0 normal map; ok[i] = 1 with pos (x, y, z
+ * camera coordinates, 3 floats) and normal (4 floats) where both pass, else 0 and NaNs.
+ * Asynchronous. */
+fu_status fu_lift_keypoints(const float* xy, int32_t n, const float* depth, const float* nmap, int32_t cols,
+                            int32_t rows, float fx, float fy, float cx, float cy, float depth_cutoff, float* pos,
+                            float* normal, uint8_t* ok, fu_stream s);
+/* the two nearest train descriptors of each query descriptor (knnMatch with k = 2 of
+ * cv::cuda::BFMatcher, NORM_L2; Tracking.cc:436): idx[2i], idx[2i+1] and their distances
+ * (sqrt of the sum of squared differences) in dist, nearest first; ties go to the lower train
+ * index; -1 and +inf where there are fewer than two.  64-float descriptors, row-major, device
+ * memory.  Asynchronous. */
+fu_status fu_match_knn2(const float* query, int32_t nq, const float* train, int32_t nt, int32_t* idx, float* dist,
+                        fu_stream s);
+/* Solver::PoseEstimate (Optimization/Solver.cc:6-183) on the host: RANSAC over samples of
+ * three pairs, Kabsch on each (rejecting reflections), the best model refitted on its
+ * inliers (0.05 m), stopping after 20 iterations once the confidence reaches 0.95; then the
+ * final refit on the inlier set as the reference forms it.  T (row-major 4x4) maps ref
+ * points to src points; inlier[i] (optional) marks the final set; *ok = 0 when n < 3, no
+ * sample was usable or the final rotation is a reflection.  The samples come from a
+ * generator seeded by `seed` (the reference seeds rand() from the clock). */
+fu_status fu_pose_estimate(const double* src, const double* ref, int32_t n, int32_t iterations, uint64_t seed,
+                           double T[16], uint8_t* inlier, int32_t* ok);
+/* Tracker::Relocalise with graph matching off, its default (Tracking.cc:411-467): the map's
+ * keys (CollectKeyPoints, whose first record is never written and reads as zeros), the two
+ * nearest map descriptors of each frame descriptor (device, nq x 64), the ratio test
+ * d0 < 0.9 d1, then fu_pose_estimate over the pairs (frame_pos: host, nq x 3, the lifted
+ * keypoints) with `iterations` (the reference's maxIterReloc is 100).  On success *ok = 1 and
+ * pose is the frame's camera-to-world pose, to hand to fu_system_relocalise.  Synchronises. */
+fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, const double* frame_pos, int32_t nq,
+                             int32_t iterations, uint64_t seed, double pose[16], int32_t* n_pairs, int32_t* ok,
+                             fu_stream s);
+
 /* ---------------------------------------------------------------- self-test */
 /* The integration and ICP kernels divide and take square roots with range-specialised forms
  * of the compiler's IEEE sequences (no reference counterpart).  This runs n random cases of
@@ -487,13 +526,14 @@ fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_
 fu_status fu_batch_set_profiling(fu_batch* b, int enable);
 fu_status fu_batch_get_stage_times(fu_batch* b, fu_stage_times* out);
 /* Captured steps (the north-star's per-frame graph for batches): with `enable`, a step whose
- * systems all have tracked a frame before runs as one hipGraph launch.  A graph is keyed by
- * the step's systems in slot order and the result buffer; it is captured the second time its
- * key is met (so one-off memberships run as direct launches) and replayed afterwards; the
- * context keeps the 32 most recently used.  A captured step reads each frame from its
- * system's input slots: frames passed elsewhere are first gathered there by one batched copy
- * per kind.  Attaching or detaching a system, fu_set_tuning and disabling drop the graphs.
- * Results are bit-identical to direct launches.  Profiled steps are never captured. */
+ * systems all have tracked a frame before runs its input stage (resize, filter, pyramid,
+ * normals: the launches that read the frames) directly and everything after it -- tracking,
+ * fusion, raycast, results -- as one hipGraph launch.  A graph is keyed by the step's systems
+ * in slot order and the result buffer; it is captured the second time its key is met (so
+ * one-off memberships run as direct launches) and replayed afterwards; the context keeps the
+ * 32 most recently used.  Attaching or detaching a system, fu_set_tuning and disabling drop
+ * the graphs.  Results are bit-identical to direct launches.  Profiled steps are never
+ * captured. */
 fu_status fu_batch_set_graph(fu_batch* b, int enable);
 fu_status fu_batch_graph_counts(fu_batch* b, uint32_t* captures, uint32_t* replays);
 

@@ -78,6 +78,8 @@ _SIG = {
     "or_keymap_collect": (U, [P, P, U]),
     "or_keymap_read": (None, [P, P]),
     "or_adjacency": (None, [P, P, P, I, P]),
+    "or_lift_keypoints": (None, [P, I, P, P, I, I, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, P, P, P]),
+    "or_match_knn2": (None, [P, I, P, I, P, P]),
 }
 
 _lib = None
@@ -546,3 +548,28 @@ def adjacency(frame_keys, map_keys, dist):
     d = np.ascontiguousarray(dist, dtype=np.float32)
     lib().or_adjacency(_ptr(fk), _ptr(mk), _ptr(d), n, _ptr(out))
     return out
+
+
+def lift_keypoints(xy, depth, nmap, K, cutoff=3.0):
+    """Frame::ExtractKeyPoints' test and back-projection: (pos (n, 3), normal (n, 4), ok (n,))"""
+    xy = np.ascontiguousarray(xy, dtype=np.float32)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    nm = np.ascontiguousarray(nmap, dtype=np.float32)
+    n = len(xy)
+    rows, cols = d.shape
+    pos = np.zeros((n, 3), np.float32)
+    nrm = np.zeros((n, 4), np.float32)
+    ok = np.zeros(n, np.uint8)
+    lib().or_lift_keypoints(_ptr(xy), n, _ptr(d), _ptr(nm), cols, rows, K[0], K[1], K[2], K[3], cutoff, _ptr(pos),
+                            _ptr(nrm), _ptr(ok))
+    return pos, nrm, ok
+
+
+def match_knn2(query, train):
+    """knnMatch, k = 2, L2: (idx (n, 2), dist (n, 2))"""
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    t = np.ascontiguousarray(train, dtype=np.float32)
+    idx = np.zeros((len(q), 2), np.int32)
+    dist = np.zeros((len(q), 2), np.float32)
+    lib().or_match_knn2(_ptr(q), len(q), _ptr(t), len(t), _ptr(idx), _ptr(dist))
+    return idx, dist

@@ -1859,3 +1859,122 @@ extern "C" void or_adjacency(const OrSurf* frame, const OrSurf* map, const float
             out[(size_t)y * n + x] = score;
         }
 }
+
+// ------------------------------------------------------------------ keypoint relocalisation
+// Frame::InterpDepth / InterpNormal / ExtractKeyPoints' per-keypoint test and back-projection
+// (Core/Frame.cc:93-200).  The bounds test compares y with the width and x with the height,
+// as written (:96, :133).  Parity pinned by an independent numpy statement of the same rules
+// and the closed form of a plane (tests/test_oracle_reloc.py).
+namespace {
+// the test as written, and (where it would let the neighbours leave the image, which the
+// reference then reads past) positions outside the image or NaN refused
+bool or_in_frame(float x, float y, int cols, int rows) {
+    if (x <= 1 || y <= 1 || y >= cols - 1 || x >= rows - 1) return false;
+    return !(isn(x) || isn(y) || x >= cols - 1 || y >= rows - 1);
+}
+float or_interp_depth(const float* d, int cols, int rows, float x, float y, float cutoff) {
+    const float nan = std::nanf("");
+    if (!or_in_frame(x, y, cols, rows)) return nan;
+    const float cx = x - std::floor(x), cy = y - std::floor(y);
+    const int ulx = (int)std::floor(x), uly = (int)std::floor(y);
+    const int llx = (int)std::floor(x), lly = (int)std::ceil(y);
+    const int urx = (int)std::ceil(x), ury = (int)std::floor(y);
+    const int lrx = (int)std::ceil(x), lry = (int)std::ceil(y);
+    const float d00 = d[uly * cols + ulx];
+    if (isn(d00) || d00 < 0.3 || d00 > cutoff) return nan;
+    const float d10 = d[lly * cols + llx];
+    if (isn(d10) || d10 < 0.3 || d10 > cutoff) return nan;
+    const float d01 = d[ury * cols + urx];
+    if (isn(d01) || d01 < 0.3 || d01 > cutoff) return nan;
+    const float d11 = d[lry * cols + lrx];
+    if (isn(d11) || d11 < 0.3 || d11 > cutoff) return nan;
+    const float d0 = d01 * cx + d00 * (1 - cx);
+    const float d1 = d11 * cx + d10 * (1 - cx);
+    const float fin = (1 - cy) * d0 + cy * d1;
+    return std::abs(fin - d00) <= 0.005 ? fin : nan;
+}
+V4 or_interp_normal(const float* nm, int cols, int rows, float x, float y) {
+    const float nan = std::nanf("");
+    const V4 bad{nan, nan, nan, nan};
+    if (!or_in_frame(x, y, cols, rows)) return bad;
+    const float cx = x - std::floor(x), cy = y - std::floor(y);
+    auto at = [&](int xx, int yy) {
+        const float* p = nm + 4 * ((size_t)yy * cols + xx);
+        return V4{p[0], p[1], p[2], p[3]};
+    };
+    const V4 d00 = at((int)std::floor(x), (int)std::floor(y));
+    const V4 d10 = at((int)std::floor(x), (int)std::ceil(y));
+    const V4 d01 = at((int)std::ceil(x), (int)std::floor(y));
+    const V4 d11 = at((int)std::ceil(x), (int)std::ceil(y));
+    // float4 * float and float4 + float4 per component (VectorMath.h)
+    auto sc = [](V4 a, float s) { return V4{a.x * s, a.y * s, a.z * s, a.w * s}; };
+    auto ad = [](V4 a, V4 b) { return V4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; };
+    const V4 d0 = ad(sc(d01, cx), sc(d00, 1 - cx));
+    const V4 d1 = ad(sc(d11, cx), sc(d10, 1 - cx));
+    const V4 fin = ad(sc(d0, 1 - cy), sc(d1, cy));
+    const V4 df{fin.x - d00.x, fin.y - d00.y, fin.z - d00.z, fin.w - d00.w};
+    const float n = std::sqrt(dot4(df, df));
+    return n <= 0.1 ? fin : bad;
+}
+}  // namespace
+
+extern "C" void or_lift_keypoints(const float* xy, int n, const float* depth, const float* nmap, int cols, int rows,
+                                  float fx, float fy, float cx, float cy, float cutoff, float* pos, float* normal,
+                                  uint8_t* ok) {
+    for (int i = 0; i < n; ++i) {
+        const float x = xy[2 * i], y = xy[2 * i + 1];
+        const float nan = std::nanf("");
+        float p[3] = {nan, nan, nan};
+        V4 nr{nan, nan, nan, nan};
+        bool good = false;
+        const float dp = or_interp_depth(depth, cols, rows, x, y, cutoff);
+        if (!isn(dp) && dp > 0.3 && dp < cutoff) {
+            const V4 nn = or_interp_normal(nmap, cols, rows, x, y);
+            if (!isn(nn.x)) {
+                p[0] = dp * (x - cx) / fx;
+                p[1] = dp * (y - cy) / fy;
+                p[2] = dp;
+                nr = nn;
+                good = true;
+            }
+        }
+        memcpy(pos + 3 * i, p, sizeof(p));
+        normal[4 * i + 0] = nr.x;
+        normal[4 * i + 1] = nr.y;
+        normal[4 * i + 2] = nr.z;
+        normal[4 * i + 3] = nr.w;
+        ok[i] = good;
+    }
+}
+
+// knnMatch, k = 2, L2 (the BFMatcher of Tracking.cc:436): every train descriptor in index
+// order, the distance the square root of the sum of squared differences in dimension order;
+// a later descriptor replaces one only when strictly nearer
+extern "C" void or_match_knn2(const float* q, int nq, const float* t, int nt, int32_t* idx, float* dist) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nq; ++i) {
+        float d0 = INFINITY, d1 = INFINITY;
+        int i0 = -1, i1 = -1;
+        for (int j = 0; j < nt; ++j) {
+            float s = 0;
+            for (int k = 0; k < 64; ++k) {
+                const float e = q[(size_t)i * 64 + k] - t[(size_t)j * 64 + k];
+                s = s + e * e;
+            }
+            const float d = std::sqrt(s);
+            if (d < d0) {
+                d1 = d0;
+                i1 = i0;
+                d0 = d;
+                i0 = j;
+            } else if (d < d1) {
+                d1 = d;
+                i1 = j;
+            }
+        }
+        idx[2 * i] = i0;
+        idx[2 * i + 1] = i1;
+        dist[2 * i] = d0;
+        dist[2 * i + 1] = d1;
+    }
+}

@@ -189,8 +189,8 @@ def _run_schedule(frames, use_rgb, use_graph, retune=None):
 def test_captured_steps_equal_direct_launches(gpu, use_rgb):
     """fu_batch_set_graph: every step's results and the final maps bit-identical to direct
     launches; the first step of a membership runs directly, the second captures, later ones
-    replay; frames passed at fresh device addresses every step (gathered into the input
-    slots); a change of launch geometry drops the graphs"""
+    replay; frames passed at fresh device addresses every step (the captured part does not read
+    them); a change of launch geometry drops the graphs"""
     frames = {}
     for i in GRAPH_SESSIONS:
         kind, size, seed, cols, rows, _, _ = SESSIONS[i]
