@@ -333,14 +333,32 @@ struct Sampler {
         const int x0 = f2i(pt.x), x1 = f2i(pt.x + 1.0f);
         const int y0 = f2i(pt.y), y1 = f2i(pt.y + 1.0f);
         const int z0 = f2i(pt.z), z1 = f2i(pt.z + 1.0f);
-        const int ix0 = (x0 >> 3) - lo.x, ix1 = (x1 >> 3) - lo.x;
-        const int iy0 = ((y0 >> 3) - lo.y) << 1, iy1 = ((y1 >> 3) - lo.y) << 1;
-        const int iz0 = ((z0 >> 3) - lo.z) << 2, iz1 = ((z1 >> 3) - lo.z) << 2;
+        // corner k's block is h[x | y << 1 | z << 2], each bit saying whether that axis of the
+        // corner lies in lo + 1: chosen by a tree of selects (z, then y, then x: 24 selects)
+        // rather than indexing h per corner, which compiles to a compare-and-select chain
+        const bool bx0 = (x0 >> 3) != lo.x, bx1 = (x1 >> 3) != lo.x;
+        const bool by0 = (y0 >> 3) != lo.y, by1 = (y1 >> 3) != lo.y;
+        const bool bz0 = (z0 >> 3) != lo.z, bz1 = (z1 >> 3) != lo.z;
+        int pz[2][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            pz[0][e] = bz0 ? h[4 + e] : h[e];
+            pz[1][e] = bz1 ? h[4 + e] : h[e];
+        }
+        int row[2][2][2];  // [z][y][x]
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                row[c][0][x] = by0 ? pz[c][2 + x] : pz[c][x];
+                row[c][1][x] = by1 ? pz[c][2 + x] : pz[c][x];
+            }
         float sv[8];
         int hl = -1;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int hk = h[((k & 1) ? ix1 : ix0) | ((k & 2) ? iy1 : iy0) | ((k & 4) ? iz1 : iz0)];
+            const int* r = row[k >> 2][(k >> 1) & 1];
+            const int hk = ((k & 1) ? bx1 : bx0) ? r[1] : r[0];
             const int li = (((k & 4) ? z1 : z0) & 7) << 6 | (((k & 2) ? y1 : y0) & 7) << 3 | (((k & 1) ? x1 : x0) & 7);
             const float raw = IO::sdf(IO::block(m, hk >= 0 ? hk : 0), li);
             sv[k] = hk >= 0 ? raw : fnan();
@@ -491,7 +509,10 @@ __device__ __forceinline__ void raycast_body(WG g, const MapView& m, const Rayca
     FU_G(f4, a.nmap)[y * a.cols + x] = no;
 }
 template <int F>
-__global__ __launch_bounds__(256) void k_raycast(MapView m, RaycastArgs a, int wave) {
+// eight waves per SIMD: the march is latency-bound, and the select trees of the normal samples
+// would otherwise take the kernel past 64 registers per lane
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_raycast(MapView m, RaycastArgs a,
+                                                                                            int wave) {
     raycast_body<F>(hw_wg(), m, a, wave);
 }
 
@@ -524,7 +545,7 @@ __global__ __launch_bounds__(256) void kb_project(BatchLaunch b) {
     project_body<kLds>(g, cload(d.m), cload(d.zr));
 }
 template <int F>
-__global__ __launch_bounds__(256) void kb_raycast(BatchLaunch b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void kb_raycast(BatchLaunch b) {
     WG g;
     const int s = batch_slot(b, batch_wg(b), g);
     const SessDesc& d = b.desc[b.sess[s]];
