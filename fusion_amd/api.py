@@ -443,10 +443,37 @@ class System:
         check(N.lib().fu_system_grab_image_host(self.handle, ptr(depth), ptr(rgb), _s()), "fu_system_grab_image_host")
 
     def relocalise(self, pose):
-        """a lost system's relocalisation at `pose` (found by the caller): the next frame is
-        raycast there, tracking resumes after it and fusion 10 frames later"""
+        """a lost system's relocalisation at `pose` (found by the caller, e.g. KeyMap.relocalise
+        on the lost frame's keypoints): the next frame is raycast there, tracking resumes after
+        it and fusion 10 frames later"""
         arr = (C.c_double * 16)(*np.asarray(pose, dtype=np.float64).ravel())
         check(N.lib().fu_system_relocalise(self.handle, arr, _s()), "fu_system_relocalise")
+
+    def validate_poses(self, poses, iters_reloc=(3, 3, 2), thresh_reloc=1e-3):
+        """Tracker::ValidatePose (Tracking.cc:636-682) on a lost system's current frame:
+        (ok, accepted per candidate: -1 skipped / 0 / 1, the candidate refined or -1)"""
+        P = np.ascontiguousarray(np.asarray(poses, dtype=np.float64).reshape(-1, 16))
+        n = len(P)
+        acc = np.zeros(max(n, 1), np.int32)
+        chosen = np.zeros(1, np.int32)
+        ok = C.c_int()
+        check(N.lib().fu_system_validate_poses(self.handle, P.ctypes.data_as(N.DP), n, (C.c_int * 3)(*iters_reloc),
+                                               float(thresh_reloc), acc.ctypes.data_as(C.c_void_p),
+                                               chosen.ctypes.data_as(C.c_void_p), C.byref(ok), _s()),
+              "fu_system_validate_poses")
+        return bool(ok.value), acc[:n].copy(), int(chosen[0])
+
+    def relocalise_graph(self, keymap, frame_desc, frame_pos, frame_normal, iterations=100, seed=0):
+        """Tracker::Relocalise with graph matching on: (ok, number of candidate poses)"""
+        pos = np.ascontiguousarray(frame_pos, dtype=np.float32)
+        nrm = np.ascontiguousarray(frame_normal, dtype=np.float32)
+        n = int(frame_desc.shape[0])
+        nc, ok = C.c_int(), C.c_int()
+        check(N.lib().fu_system_relocalise_graph(self.handle, keymap.handle, _p(frame_desc),
+                                                 pos.ctypes.data_as(C.c_void_p), nrm.ctypes.data_as(C.c_void_p), n,
+                                                 int(iterations), int(seed), C.byref(nc), C.byref(ok), _s()),
+              "fu_system_relocalise_graph")
+        return bool(ok.value), nc.value
 
     def graph_counts(self):
         """(graphs captured, graph launches) of this system"""
@@ -817,6 +844,38 @@ class KeyMap:
                                          int(seed), pose.ctypes.data_as(N.DP), C.byref(npairs), C.byref(ok), _s()),
               "fu_relocalise_keys")
         return (pose.reshape(4, 4) if ok.value else None), npairs.value
+
+
+    def graph_candidates(self, frame_desc, frame_pos, frame_normal, iterations=100, seed=0):
+        """Tracker::Relocalise with graph matching on, through GenerateGraph's PoseEstimate
+        calls (Tracking.cc:432-453, 469-629): (candidate camera poses (k, 4, 4), size of the
+        match list)"""
+        pos = np.ascontiguousarray(frame_pos, dtype=np.float32)
+        nrm = np.ascontiguousarray(frame_normal, dtype=np.float32)
+        n = int(frame_desc.shape[0])
+        poses = np.zeros(5 * 16)
+        npose, npairs = C.c_int(), C.c_int()
+        check(N.lib().fu_relocalise_graph_candidates(self.handle, _p(frame_desc), pos.ctypes.data_as(C.c_void_p),
+                                                     nrm.ctypes.data_as(C.c_void_p), n, int(iterations), int(seed),
+                                                     poses.ctypes.data_as(N.DP), C.byref(npose), C.byref(npairs),
+                                                     _s()), "fu_relocalise_graph_candidates")
+        return poses.reshape(5, 4, 4)[: npose.value].copy(), npairs.value
+
+
+def select_subgraphs(matrix, n_lists=10, max_select=400, min_score=0.1):
+    """GenerateGraph's sub-graph selection (Tracking.cc:515-596) over a device float32 (n, n)
+    consistency matrix: (n_selected (n_lists,), refined lists, rank sums (n,), rank order (n,)),
+    host arrays"""
+    n = int(matrix.shape[0])
+    rank = torch.empty((max(n, 1),), dtype=torch.float32, device="cuda")
+    order = torch.empty((max(n, 1),), dtype=torch.int32, device="cuda")
+    nsel = torch.empty((n_lists,), dtype=torch.int32, device="cuda")
+    nref = torch.empty((n_lists,), dtype=torch.int32, device="cuda")
+    ref = torch.empty((n_lists, max_select), dtype=torch.int32, device="cuda")
+    check(N.lib().fu_select_subgraphs(_p(matrix) if n else None, n, int(n_lists), int(max_select), float(min_score),
+                                      _p(rank), _p(order), _p(nsel), _p(nref), _p(ref), _s()), "fu_select_subgraphs")
+    nsel, nref, ref = nsel.cpu().numpy(), nref.cpu().numpy(), ref.cpu().numpy()
+    return nsel, [ref[i, : nref[i]].copy() for i in range(n_lists)], rank.cpu().numpy()[:n], order.cpu().numpy()[:n]
 
 
 def BuildAdjacencyMatrix(frame_keys, map_keys, dist):

@@ -1439,6 +1439,162 @@ extern "C" fu_status fu_system_relocalise(fu_system* sys, const double pose[16],
     return FU_OK;
 }
 
+// ------------------------------------------------------- relocalisation by graph matching
+static fu_pose_f pose_f_of(const double P[16]) {  // Frame::GpuRotation / GpuInvRotation / GpuTranslation
+    fu_pose_f f;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            f.R[i * 3 + j] = (float)P[i * 4 + j];
+            f.Rinv[i * 3 + j] = (float)P[j * 4 + i];
+        }
+        f.t[i] = (float)P[i * 4 + 3];
+    }
+    return f;
+}
+
+// Mapping::UpdateVisibility at P (Mapping.cc:60-66); unless fewer than min_vis blocks are
+// visible, Mapping::RayTrace into the model maps (Mapping.cc:92-109: left as they are when no
+// block renders, or, with keep_own, the frame's own maps as after a swap) and, with resize,
+// LastFrame->ResizeImages.  *rendered tells whether the visibility test passed.
+static fu_status render_at(fu_system* sys, const double P[16], unsigned min_vis, bool keep_own, bool resize,
+                           bool* rendered, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    fu_map* m = sys->map;
+    const fu_pose_f pf = pose_f_of(P);
+    const PoseF pose = posef_from(&pf);
+    *rendered = false;
+    const VisArgs v = make_vis(m, pose, nullptr, p.cols, p.rows, p.fx, p.fy, p.cx, p.cy, m->p.depth_min, m->p.depth_max);
+    launch_begin_pass(m->v, 2, s);
+    launch_visible(m->v, v, s);
+    LAUNCHCHK();
+    fu_status st = FU_OK;
+    const unsigned nvis = read_ctr(m, C_VISIBLE, s, &st);
+    if (st) return st;
+    if (nvis < min_vis) return FU_OK;
+    *rendered = true;
+    unsigned nfrag = 0;
+    if (nvis > 0) {
+        const ZRangeArgs z = make_zr(m, b.zrange_min, b.zrange_max, b.zcols, b.zrows, m->p.depth_max, m->p.depth_min,
+                                     pose, nullptr, -1, p.fx, p.fy, p.cx, p.cy);
+        launch_begin_pass(m->v, 4, s);
+        launch_zrange(m->v, z, s);
+        LAUNCHCHK();
+        nfrag = read_ctr(m, C_RENDER, s, &st);
+        if (st) return st;
+    }
+    if (nfrag > 0) {
+        const RaycastArgs r = make_rc(m, b.model_vmap[0], b.model_nmap[0], p.cols, p.rows, b.zrange_min, b.zrange_max,
+                                      b.zcols, b.zrows, pose, nullptr, (float)(1.0 / (double)p.fx),
+                                      (float)(1.0 / (double)p.fy), p.cx, p.cy);
+        launch_raycast(m->v, r, s);
+    } else if (keep_own) {
+        HIPCHK(hipMemcpyAsync(b.model_vmap[0], b.vmap[0], (size_t)p.cols * p.rows * 16, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(b.model_nmap[0], b.nmap[0], (size_t)p.cols * p.rows * 16, hipMemcpyDeviceToDevice, s));
+    }
+    if (resize) queue_resize_model(sys, s);
+    LAUNCHCHK();
+    return FU_OK;
+}
+
+// ComputeSE3(icpOnly = true, iters, thresh) of the system's current frame against the model
+// maps, from P, on the scratch tracker t
+static fu_status icp_from(fu_system* sys, fu_tracker* t, const double P[16], const int32_t iters[3], float thresh,
+                          fu_track_result* out, hipStream_t s) {
+    const fu_system_params& p = sys->p;
+    fu_system_buffers& b = sys->b;
+    fu_status st = fu_tracker_set_last_pose(t, P, s);
+    if (st) return st;
+    const float* nv[3] = {b.vmap[0], b.vmap[1], b.vmap[2]};
+    const float* nn[3] = {b.nmap[0], b.nmap[1], b.nmap[2]};
+    const float* lv[3] = {b.model_vmap[0], b.model_vmap[1], b.model_vmap[2]};
+    const float* ln[3] = {b.model_nmap[0], b.model_nmap[1], b.model_nmap[2]};
+    const float K[4] = {p.fx, p.fy, p.cx, p.cy};
+    queue_se3(t, nv, nn, lv, ln, nullptr, K, iters, p.min_count, thresh, s);
+    LAUNCHCHK();
+    return fu_tracker_result(t, out, s);
+}
+
+// Tracker::ValidatePose (Tracking.cc:636-682) on a lost system's current frame
+extern "C" fu_status fu_system_validate_poses(fu_system* sys, const double* poses, int32_t n,
+                                              const int32_t iters_reloc[3], float thresh_reloc, int32_t* accepted,
+                                              int32_t* chosen, int32_t* ok, fu_stream s) {
+    if (!sys || !ok || !iters_reloc || n < 0 || (n > 0 && (!poses || !accepted)))
+        return set_err(FU_ERR_ARG, "fu_system_validate_poses: bad argument");
+    for (int l = 0; l < 3; ++l)
+        if (iters_reloc[l] < 0) return set_err(FU_ERR_ARG, "fu_system_validate_poses: negative iterations");
+    *ok = 0;
+    if (chosen) *chosen = -1;
+    hipStream_t hs = S(s);
+    TrackState h;
+    HIPCHK(hipMemcpyAsync(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost, hs));
+    HIPCHK(hipStreamSynchronize(hs));
+    if (h.sm_state != -1) return set_err(FU_ERR_STATE, "fu_system_validate_poses: the system is not lost");
+    fu_tracker* t = nullptr;
+    fu_status st = fu_tracker_create(sys->p.cols, sys->p.rows, &t);
+    if (st) return st;
+    int first = -1;
+    fu_track_result res;
+    for (int i = 0; i < n && !st; ++i) {
+        const double* P = poses + 16 * i;
+        bool rendered = false;
+        accepted[i] = -1;
+        // below 512 visible blocks the candidate is skipped (:642-645)
+        if ((st = render_at(sys, P, 512, false, true, &rendered, hs)) || !rendered) continue;
+        if ((st = icp_from(sys, t, P, iters_reloc, thresh_reloc, &res, hs))) continue;
+        accepted[i] = res.status == FU_TRACK_OK;
+        if (accepted[i] && first < 0) first = i;
+    }
+    // the first candidate that passed (the reference sorts the passing errors with
+    // cv::sortIdx on a one-column matrix, which sorts each one-element row and leaves index 0
+    // first), rendered again and refined with the tracking iterations and threshold
+    bool rendered = false;
+    if (!st && first >= 0) st = render_at(sys, poses + 16 * first, 0, false, true, &rendered, hs);
+    if (!st && first >= 0) st = icp_from(sys, t, poses + 16 * first, sys->p.iters, sys->p.icp_thresh, &res, hs);
+    fu_tracker_destroy(t);
+    if (st) return st;
+    if (chosen) *chosen = first;
+    if (first < 0 || res.status != FU_TRACK_OK) return FU_OK;
+    // relocalised (Tracking.cc:101-105): the frame takes the refined pose and is swapped in;
+    // GrabImage in state -1 renders it without fusion (System.cc:99-120); tracking resumes
+    // with the next frame, which is not swapped (Tracking.cc:76-86)
+    for (int i = 0; i < 16; ++i) h.last_pose[i] = h.next_pose[i] = res.pose[i];
+    h.sm_state = 0;
+    h.sm_prev = -1;
+    h.status = TS_RELOC;
+    h.gates = GATE_RENDER | GATE_SWAP;
+    h.reloc = 0;
+    h.icp_error = res.icp_error;
+    h.count = res.count;
+    refresh_float_pose(&h);
+    HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, hs));
+    if (sys->p.use_rgb)  // Tracker::SwapFrame of the intensity pyramid
+        for (int l = 0; l < 3; ++l)
+            HIPCHK(hipMemcpyAsync(sys->b.last_image[l], sys->b.image[l], (size_t)sys->lc[l] * sys->lr[l],
+                                  hipMemcpyDeviceToDevice, hs));
+    if ((st = render_at(sys, h.next_pose, 0, true, false, &rendered, hs))) return st;
+    HIPCHK(hipStreamSynchronize(hs));
+    *ok = 1;
+    return FU_OK;
+}
+
+// Tracker::Relocalise with graph matching on (Tracking.cc:411-453, 469-682)
+extern "C" fu_status fu_system_relocalise_graph(fu_system* sys, fu_keymap* km, const float* frame_desc,
+                                                const float* frame_pos, const float* frame_normal, int32_t nq,
+                                                int32_t iterations, uint64_t seed, int32_t* n_candidates,
+                                                int32_t* ok, fu_stream s) {
+    if (!sys || !km || !ok) return set_err(FU_ERR_ARG, "fu_system_relocalise_graph: null argument");
+    *ok = 0;
+    double poses[5 * 16];
+    int32_t np = 0, accepted[5], chosen = -1;
+    fu_status st = fu_relocalise_graph_candidates(km, frame_desc, frame_pos, frame_normal, nq, iterations, seed, poses,
+                                                  &np, nullptr, s);
+    if (st) return st;
+    if (n_candidates) *n_candidates = np;
+    const int32_t iters_reloc[3] = {3, 3, 2};  // Tracker::ITERATIONS_RELOC (Tracking.h:101)
+    return fu_system_validate_poses(sys, poses, np, iters_reloc, 1e-3f, accepted, &chosen, ok, s);
+}
+
 extern "C" fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches) {
     if (!sys || !captures || !launches) return set_err(FU_ERR_ARG, "null argument");
     *captures = sys->graph_captures;

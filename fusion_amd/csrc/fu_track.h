@@ -252,9 +252,10 @@ FU_HD void icp_update(TrackState* st, const float* sums, const float* rgb, int m
 // (System.cc:92-138) around one frame, shared by the device (one lane) and the host.
 //
 // frame_begin, before tracking: a lost tracker (state -1) does not track.  The reference
-// then calls Relocalise, which needs the keypoint map (not built here); a relocalisation pose
-// given from outside takes its place (status TS_RELOC, the frame at that pose), otherwise the
-// frame is TS_LOST.  A tracker that is not lost starts ComputeSE3 (Tracking.cc:305-314): last
+// then calls Relocalise on the frame; here a relocalisation pose given before the frame
+// (fu_system_relocalise) places it (status TS_RELOC, the frame at that pose), otherwise the
+// frame is TS_LOST (fu_system_validate_poses may relocalise that frame afterwards, as the
+// reference's graph-matching Relocalise does).  A tracker that is not lost starts ComputeSE3 (Tracking.cc:305-314): last
 // pose = tracked pose of the previous frame, initial guess = last pose, delta = identity.
 FU_HD void frame_begin(TrackState* st) {
     if (st->sm_state == -1) {

@@ -180,6 +180,143 @@ __global__ void k_match_merge(const Best2* part, int nq, int nchunks, int* idx, 
     dist[2 * qi + 1] = b.d1;
 }
 
+// ---------------------------------------------------------------------------- graph selection
+// Tracker::GenerateGraph's choice of mutually consistent pairs (Tracking.cc:515-596) over the
+// consistency matrix C (row-major n x n, C[a * n + b] = conMatrix(a, b)), without reading the
+// matrix back to the host as the reference does:
+//   k_rank_sums   the column sums (cv::cuda::reduce over rows): one lane per column, the rows
+//                 in order, summed in double and rounded to float (OpenCV's order is its own);
+//   k_rank_order  the pairs by rank, largest first (cv::sortIdx descending): each lane counts
+//                 the ranks above its own, ties going to the lower index (std::sort leaves
+//                 them in no fixed order), against LDS tiles of the ranks; a NaN rank counts
+//                 as -inf, so the positions are a permutation;
+//   k_subgraphs   one 1024-lane work-group per list: the head and the pairs ranked after it
+//                 with C[head][pair] > min_score, kept in rank order by a block-wide ballot
+//                 prefix until max_sel are selected; then, one lane per selected pair, the
+//                 refinement test against every later pair of the list.
+constexpr int kSelLanes = 1024;
+constexpr int kMaxSelect = 1024;
+
+__global__ void k_rank_sums(const float* C, int n, float* rank) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    double s = 0;
+    for (int y = 0; y < n; ++y) s += (double)C[(size_t)y * n + x];
+    rank[x] = (float)s;
+}
+
+__device__ __forceinline__ float rank_key(float v) { return isnan(v) ? -INFINITY : v; }
+
+__global__ __launch_bounds__(256) void k_rank_order(const float* rank, int n, int* order) {
+    __shared__ float tile[256];
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const float kx = x < n ? rank_key(rank[x]) : 0.f;
+    int pos = 0;
+    for (int base = 0; base < n; base += 256) {
+        __syncthreads();
+        const int y = base + threadIdx.x;
+        tile[threadIdx.x] = y < n ? rank_key(rank[y]) : 0.f;
+        __syncthreads();
+        const int m = min(256, n - base);
+        for (int k = 0; k < m; ++k) {
+            const float ky = tile[k];
+            pos += (ky > kx) || (ky == kx && base + k < x);
+        }
+    }
+    if (x < n) order[pos] = x;
+}
+
+struct SelArgs {
+    const float* C;
+    const int* order;
+    int n, max_sel;
+    float min_score;
+    int* n_selected;  // per list
+    int* n_refined;   // per list
+    int* refined;     // max_sel per list
+};
+
+// the exclusive prefix of `flag` over the work-group (lanes in order) and the total
+__device__ __forceinline__ int block_prefix(bool flag, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(flag);
+    const int in_wave = __popcll(bal & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (lane == 0) wsum[wid] = __popcll(bal);
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < kSelLanes / 64; ++w) {
+        const int c = wsum[w];
+        before += w < wid ? c : 0;
+        all += c;
+    }
+    total = all;
+    return before + in_wave;
+}
+
+__global__ __launch_bounds__(kSelLanes) void k_subgraphs(SelArgs a) {
+    __shared__ int sel[kMaxSelect];
+    __shared__ float hs[kMaxSelect];  // C[head][sel[k]]
+    __shared__ int wsum[kSelLanes / 64];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    if (i >= a.n) {  // no pairs from this rank on (Tracking.cc:538)
+        if (tid == 0) a.n_selected[i] = a.n_refined[i] = 0;
+        return;
+    }
+    const int head = a.order[i];
+    const float* Ch = a.C + (size_t)head * a.n;
+    int count = 1;  // the head is always taken (:544-548)
+    if (tid == 0) {
+        sel[0] = head;
+        hs[0] = Ch[head];
+    }
+    for (int base = i + 1; base < a.n && count < a.max_sel; base += kSelLanes) {
+        const int j = base + tid;
+        int idx = 0;
+        float sc = 0.f;
+        bool f = false;
+        if (j < a.n) {
+            idx = a.order[j];
+            sc = Ch[idx];
+            f = sc > a.min_score;  // :551-555
+        }
+        int total;
+        const int slot = count + block_prefix(f, wsum, total);
+        if (f && slot < a.max_sel) {
+            sel[slot] = idx;
+            hs[slot] = sc;
+        }
+        count = min(count + total, a.max_sel);  // the same in every lane
+    }
+    __syncthreads();
+    if (tid == 0) a.n_selected[i] = count;
+    if (count < 3) {  // no PoseEstimate from fewer than three (:565)
+        if (tid == 0) a.n_refined[i] = 0;
+        return;
+    }
+    // lane k keeps the k-th pair unless a later one is nearly exclusive with it and scores
+    // higher with the head (:567-582)
+    bool keep = false;
+    if (tid >= 1 && tid < count) {
+        const int av = sel[tid];
+        const float ha = hs[tid];
+        const float* Ca = a.C + (size_t)av * a.n;
+        keep = true;
+        for (int l = tid + 1; l < count; ++l) {
+            if (!(hs[l] > ha)) continue;
+            const int b = sel[l];
+            if (Ca[b] < 5e-3f || a.C[(size_t)b * a.n + av] < 5e-3f) {
+                keep = false;
+                break;
+            }
+        }
+    }
+    int total;
+    const int slot = block_prefix(keep, wsum, total);
+    if (keep) a.refined[(size_t)i * a.max_sel + slot] = sel[tid];
+    if (tid == 0) a.n_refined[i] = total;
+}
+
 // ---------------------------------------------------------------------------- pose
 // 3x3 helpers in double, row-major
 struct M3d {
@@ -435,25 +572,18 @@ extern "C" fu_status fu_pose_estimate(const double* src, const double* ref, int3
     return FU_OK;
 }
 
-// Tracker::Relocalise with useGraphMatching off, its default (Tracking.cc:411-467): the map's
-// keys (UpdateMapKeys, Mapping.cc:142-151), the two nearest map descriptors of every frame
-// descriptor, the ratio test d0 < 0.9 d1, PoseEstimate over the surviving pairs (frame point
-// <- map point) and the pose as the inverse of its transform.  frame_desc: nq x 64 floats in
-// device memory; frame_pos: nq x 3 doubles on the host (the lifted keypoints).
-extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, const double* frame_pos, int32_t nq,
-                                        int32_t iterations, uint64_t seed, double pose[16], int32_t* n_pairs,
-                                        int32_t* ok, fu_stream s) {
-    if (!km || !pose || !ok || nq < 0 || (nq > 0 && (!frame_desc || !frame_pos)))
-        return api_error(FU_ERR_ARG, "fu_relocalise_keys: bad argument");
-    *ok = 0;
-    if (n_pairs) *n_pairs = 0;
-    for (int k = 0; k < 16; ++k) pose[k] = (k % 5 == 0) ? 1.0 : 0.0;
+namespace {
+// the map's keys (UpdateMapKeys, Mapping.cc:142-151; the first record is never written by
+// CollectKeyPoints and reads as zeros here) and the two nearest of them to each frame
+// descriptor, on the host
+fu_status collect_and_match(fu_keymap* km, const float* frame_desc, int32_t nq, std::vector<fu_surf>& host,
+                            std::vector<int32_t>& hidx, std::vector<float>& hdist, fu_stream s, const char* who) {
     hipStream_t hs = (hipStream_t)s;
     uint32_t nk = 0;
     fu_status st = fu_collect_key_points(km, nullptr, 0, &nk, s);
     if (st) return st;
+    host.clear();
     if (nk == 0 || nq == 0) return FU_OK;  // no map keys (Tracking.cc:417-418)
-    // the records; the first is never written by CollectKeyPoints and reads as zeros here
     fu_surf* keys = nullptr;
     float* train = nullptr;
     int32_t* idx = nullptr;
@@ -470,11 +600,11 @@ extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, 
         hipMalloc((void**)&dist, sizeof(float) * 2 * nq) != hipSuccess) {
         (void)hipGetLastError();
         release();
-        return api_error(FU_ERR_NOMEM, "fu_relocalise_keys: %u keys", nk);
+        return api_error(FU_ERR_NOMEM, "%s: %u keys", who, nk);
     }
-    std::vector<fu_surf> host(nk);
-    std::vector<int32_t> hidx(2 * (size_t)nq);
-    std::vector<float> hdist(2 * (size_t)nq);
+    host.resize(nk);
+    hidx.resize(2 * (size_t)nq);
+    hdist.resize(2 * (size_t)nq);
     hipError_t e = hipMemsetAsync(keys, 0, sizeof(fu_surf) * nk, hs);
     if (e == hipSuccess) {
         uint32_t again = 0;
@@ -491,7 +621,40 @@ extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, 
     if (!st && e == hipSuccess) e = hipStreamSynchronize(hs);
     release();
     if (st) return st;
-    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_relocalise_keys: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
+    return FU_OK;
+}
+
+// the camera pose of a PoseEstimate transform T (frame point = R * map point + t): T^-1
+void rigid_inverse(const double T[16], double pose[16]) {
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) pose[i * 4 + j] = T[j * 4 + i];
+        pose[i * 4 + 3] = -(T[0 * 4 + i] * T[3] + T[1 * 4 + i] * T[7] + T[2 * 4 + i] * T[11]);
+    }
+    pose[12] = pose[13] = pose[14] = 0;
+    pose[15] = 1;
+}
+}  // namespace
+
+// Tracker::Relocalise with useGraphMatching off, its default (Tracking.cc:411-467): the map's
+// keys, the two nearest map descriptors of every frame descriptor, the ratio test
+// d0 < 0.9 d1, PoseEstimate over the surviving pairs (frame point <- map point) and the pose
+// as the inverse of its transform.  frame_desc: nq x 64 floats in device memory; frame_pos:
+// nq x 3 doubles on the host (the lifted keypoints).
+extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, const double* frame_pos, int32_t nq,
+                                        int32_t iterations, uint64_t seed, double pose[16], int32_t* n_pairs,
+                                        int32_t* ok, fu_stream s) {
+    if (!km || !pose || !ok || nq < 0 || (nq > 0 && (!frame_desc || !frame_pos)))
+        return api_error(FU_ERR_ARG, "fu_relocalise_keys: bad argument");
+    *ok = 0;
+    if (n_pairs) *n_pairs = 0;
+    for (int k = 0; k < 16; ++k) pose[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    std::vector<fu_surf> host;
+    std::vector<int32_t> hidx;
+    std::vector<float> hdist;
+    fu_status st = collect_and_match(km, frame_desc, nq, host, hidx, hdist, s, "fu_relocalise_keys");
+    if (st) return st;
+    if (host.empty()) return FU_OK;
     // the ratio test (Tracking.cc:437-444); with one map key the second distance is infinite
     std::vector<double> src, ref;
     for (int i = 0; i < nq; ++i) {
@@ -509,13 +672,155 @@ extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, 
     int32_t good = 0;
     if ((st = fu_pose_estimate(src.data(), ref.data(), np, iterations, seed, T, nullptr, &good))) return st;
     if (!good) return FU_OK;
-    // pose = T^-1 for the rigid T (frame point = R * map point + t)
-    for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) pose[i * 4 + j] = T[j * 4 + i];
-        pose[i * 4 + 3] = -(T[0 * 4 + i] * T[3] + T[1 * 4 + i] * T[7] + T[2 * 4 + i] * T[11]);
-    }
-    pose[12] = pose[13] = pose[14] = 0;
-    pose[15] = 1;
+    rigid_inverse(T, pose);
     *ok = 1;
+    return FU_OK;
+}
+
+// GenerateGraph's selection on the device (Tracking.cc:515-596)
+extern "C" fu_status fu_select_subgraphs(const float* matrix, int32_t n, int32_t n_lists, int32_t max_select,
+                                         float min_score, float* rank_sum, int32_t* order, int32_t* n_selected,
+                                         int32_t* n_refined, int32_t* refined, fu_stream s) {
+    if (n < 0 || n > 46340 || n_lists < 1 || n_lists > 65535 || max_select < 1 || max_select > kMaxSelect)
+        return api_error(FU_ERR_ARG, "fu_select_subgraphs: n %d (at most 46340), %d lists, %d per list (at most %d)", n,
+                         n_lists, max_select, kMaxSelect);
+    if (!n_selected || !n_refined || !refined || (n > 0 && (!matrix || !rank_sum || !order)))
+        return api_error(FU_ERR_ARG, "fu_select_subgraphs: null argument");
+    hipStream_t hs = (hipStream_t)s;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_rank_sums, dim3((n + 255) / 256), dim3(256), 0, hs, matrix, n, rank_sum);
+        hipLaunchKernelGGL(k_rank_order, dim3((n + 255) / 256), dim3(256), 0, hs, rank_sum, n, order);
+    }
+    const SelArgs a{matrix, order, n, max_select, min_score, n_selected, n_refined, refined};
+    hipLaunchKernelGGL(k_subgraphs, dim3(n_lists), dim3(kSelLanes), 0, hs, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_select_subgraphs: %s", hipGetErrorString(e));
+    return FU_OK;
+}
+
+// Tracker::Relocalise with useGraphMatching on (Tracking.cc:432-453) through GenerateGraph's
+// PoseEstimate calls (:469-629): the match list keeps both nearest map keys of a query that
+// fails the ratio test; pairs whose map key is not valid are dropped; their consistency
+// matrix (BuildAdjacencyMatrix) and sub-graph selection run on the device; then PoseEstimate
+// on the pairs of the first list of three or more selected pairs, once per kept list up to
+// five (the reference's cv::sortIdx over its one-column list of counts leaves every index at
+// 0, so each call takes that first list), when it holds four or more refined pairs.  Call c
+// draws its samples from seed + c.  frame_pos (nq x 3) and frame_normal (nq x 4): host
+// floats, the lifted keypoints.
+extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* frame_desc, const float* frame_pos,
+                                                    const float* frame_normal, int32_t nq, int32_t iterations,
+                                                    uint64_t seed, double poses[5 * 16], int32_t* n_poses,
+                                                    int32_t* n_pairs, fu_stream s) {
+    if (!km || !poses || !n_poses || nq < 0 || iterations < 0 ||
+        (nq > 0 && (!frame_desc || !frame_pos || !frame_normal)))
+        return api_error(FU_ERR_ARG, "fu_relocalise_graph_candidates: bad argument");
+    constexpr int kLists = 10, kSelect = 400, kPoseLists = 5;  // Tracking.h:129-132
+    constexpr float kMinScore = 0.1f;
+    *n_poses = 0;
+    if (n_pairs) *n_pairs = 0;
+    std::vector<fu_surf> host;
+    std::vector<int32_t> hidx;
+    std::vector<float> hdist;
+    fu_status st = collect_and_match(km, frame_desc, nq, host, hidx, hdist, s, "fu_relocalise_graph_candidates");
+    if (st) return st;
+    if (host.empty()) return FU_OK;
+    // the match list (Tracking.cc:437-444), then the valid map keys (:485-505)
+    std::vector<std::pair<int, int>> refd;  // (query, train)
+    std::vector<float> rdist;
+    for (int i = 0; i < nq; ++i) {
+        if (hidx[2 * i] < 0) continue;
+        refd.push_back({i, hidx[2 * i]});
+        rdist.push_back(hdist[2 * i]);
+        if (!((double)hdist[2 * i] < 0.90 * (double)hdist[2 * i + 1]) && hidx[2 * i + 1] >= 0) {
+            refd.push_back({i, hidx[2 * i + 1]});
+            rdist.push_back(hdist[2 * i + 1]);
+        }
+    }
+    if (n_pairs) *n_pairs = (int32_t)refd.size();
+    if (refd.size() < 3) return FU_OK;
+    std::vector<fu_surf> fk, mk;
+    std::vector<float> dd;
+    for (size_t r = 0; r < refd.size(); ++r) {
+        const fu_surf& key = host[refd[r].second];
+        if (!key.valid) continue;
+        fu_surf f;
+        memset(&f, 0, sizeof(f));
+        f.valid = 1;  // the reference leaves this record's flag unset
+        memcpy(f.pos, frame_pos + 3 * refd[r].first, sizeof(f.pos));
+        memcpy(f.normal, frame_normal + 4 * refd[r].first, sizeof(f.normal));
+        fk.push_back(f);
+        mk.push_back(key);
+        dd.push_back(rdist[r]);
+    }
+    const int n = (int)fk.size();
+    if (n == 0) return FU_OK;  // rank.cols == 0 (:523-524)
+    if (n > 46340) return api_error(FU_ERR_ARG, "fu_relocalise_graph_candidates: %d pairs (at most 46340)", n);
+    hipStream_t hs = (hipStream_t)s;
+    fu_surf *dfk = nullptr, *dmk = nullptr;
+    float *ddist = nullptr, *mat = nullptr, *rank = nullptr;
+    int32_t *order = nullptr, *nsel = nullptr;
+    auto release = [&]() {
+        (void)hipFree(dfk);
+        (void)hipFree(dmk);
+        (void)hipFree(ddist);
+        (void)hipFree(mat);
+        (void)hipFree(rank);
+        (void)hipFree(order);
+        (void)hipFree(nsel);
+    };
+    const size_t nsel_words = 2 * kLists + (size_t)kLists * kSelect;
+    if (hipMalloc((void**)&dfk, sizeof(fu_surf) * n) != hipSuccess ||
+        hipMalloc((void**)&dmk, sizeof(fu_surf) * n) != hipSuccess ||
+        hipMalloc((void**)&ddist, sizeof(float) * n) != hipSuccess ||
+        hipMalloc((void**)&mat, sizeof(float) * (size_t)n * n) != hipSuccess ||
+        hipMalloc((void**)&rank, sizeof(float) * n) != hipSuccess ||
+        hipMalloc((void**)&order, sizeof(int32_t) * n) != hipSuccess ||
+        hipMalloc((void**)&nsel, sizeof(int32_t) * nsel_words) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return api_error(FU_ERR_NOMEM, "fu_relocalise_graph_candidates: %d pairs", n);
+    }
+    std::vector<int32_t> hsel(nsel_words);
+    hipError_t e = hipMemcpyAsync(dfk, fk.data(), sizeof(fu_surf) * n, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(dmk, mk.data(), sizeof(fu_surf) * n, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(ddist, dd.data(), sizeof(float) * n, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) st = fu_build_adjacency_matrix(dfk, dmk, ddist, n, mat, s);
+    if (e == hipSuccess && !st)
+        st = fu_select_subgraphs(mat, n, kLists, kSelect, kMinScore, rank, order, nsel, nsel + kLists,
+                                 nsel + 2 * kLists, s);
+    if (e == hipSuccess && !st)
+        e = hipMemcpyAsync(hsel.data(), nsel, sizeof(int32_t) * nsel_words, hipMemcpyDeviceToHost, hs);
+    if (e == hipSuccess && !st) e = hipStreamSynchronize(hs);
+    release();
+    if (st) return st;
+    if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_relocalise_graph_candidates: %s", hipGetErrorString(e));
+    const int32_t* n_selected = hsel.data();
+    const int32_t* n_refined = hsel.data() + kLists;
+    const int32_t* lists = hsel.data() + 2 * kLists;
+    int kept = 0, first = -1;
+    for (int i = 0; i < kLists; ++i)
+        if (n_selected[i] >= 3) {
+            if (first < 0) first = i;
+            ++kept;
+        }
+    if (kept == 0) return FU_OK;  // "not enough points" (:591-594)
+    const int M = std::min(kept, kPoseLists);
+    const int R = n_refined[first];
+    if (R <= 3) return FU_OK;  // every call skipped (:601-602)
+    std::vector<double> src(3 * (size_t)R), ref(3 * (size_t)R);
+    for (int k = 0; k < R; ++k) {
+        const int p = lists[(size_t)first * kSelect + k];
+        for (int c = 0; c < 3; ++c) {
+            src[3 * k + c] = (double)fk[p].pos[c];
+            ref[3 * k + c] = (double)mk[p].pos[c];
+        }
+    }
+    for (int c = 0; c < M; ++c) {
+        double T[16];
+        int32_t good = 0;
+        if ((st = fu_pose_estimate(src.data(), ref.data(), R, iterations, seed + (uint64_t)c, T, nullptr, &good)))
+            return st;
+        if (good) rigid_inverse(T, poses + 16 * (*n_poses)++);
+    }
     return FU_OK;
 }

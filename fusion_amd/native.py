@@ -213,6 +213,10 @@ SIGNATURES = {
     "fu_match_knn2": (I, [P, I, P, I, P, P, P]),
     "fu_pose_estimate": (I, [DP, DP, I, I, C.c_uint64, DP, P, C.POINTER(I)]),
     "fu_relocalise_keys": (I, [P, P, DP, I, I, C.c_uint64, DP, C.POINTER(I), C.POINTER(I), P]),
+    "fu_select_subgraphs": (I, [P, I, I, I, F, P, P, P, P, P, P]),
+    "fu_relocalise_graph_candidates": (I, [P, P, P, P, I, I, C.c_uint64, DP, C.POINTER(I), C.POINTER(I), P]),
+    "fu_system_validate_poses": (I, [P, DP, I, C.POINTER(I), F, P, P, C.POINTER(I), P]),
+    "fu_system_relocalise_graph": (I, [P, P, P, P, P, I, I, C.c_uint64, C.POINTER(I), C.POINTER(I), P]),
     "fu_server_submit_host": (I, [P, I, P, P, C.c_uint64, C.c_double]),
     "fu_server_submit_mesh": (I, [P, I, P, P, P, C.c_uint32, C.c_uint64, C.c_double]),
     "fu_server_poll": (I, [P, C.POINTER(Completion), I, C.POINTER(I)]),

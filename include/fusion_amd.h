@@ -221,11 +221,12 @@ typedef struct fu_track_result {
 #define FU_TRACK_REJECTED 3 /* final error or motion test failed: pose rolled back (:404-407) */
 #define FU_TRACK_FIRST 4    /* first frame: identity pose (Tracking.cc:66-71) */
 /* a system's tracker is lost after more than 9 frames in a row that were not tracked
- * (Tracking.cc:92-96); it then tracks nothing: the reference relocalises from its keypoint
- * map, which this library does not build, so it stays lost until fu_system_relocalise or a
- * reset */
+ * (Tracking.cc:92-96); it then tracks nothing until it is relocalised: by graph matching
+ * against a key map (fu_system_relocalise_graph / fu_system_validate_poses, on the lost frame
+ * itself, as the reference's Relocalise), by a pose given for the next frame
+ * (fu_system_relocalise, e.g. from fu_relocalise_keys), or by a reset */
 #define FU_TRACK_LOST 5     /* lost: not tracked, nothing fused or raycast */
-#define FU_TRACK_RELOC 6    /* relocalised at the pose given to fu_system_relocalise: raycast there */
+#define FU_TRACK_RELOC 6    /* relocalised (fu_system_relocalise's pose, or fu_system_validate_poses): raycast there */
 
 /* Tracker state and ICP scratch (Tracker::Tracker, Tracking.cc:19-43) */
 fu_status fu_tracker_create(int cols, int rows, fu_tracker** out);
@@ -328,9 +329,10 @@ fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* depth, const
 fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches);
 /* outcome of the last frame; synchronises */
 fu_status fu_system_result(fu_system* sys, fu_frame_result* out, fu_stream s);
-/* A lost system's relocalisation, with the pose found by the caller (the reference's
- * Relocalise, Tracking.cc:116-125, matches keypoints against its keyframe map, which is not
- * built here).  The next frame takes `pose` (camera to world, 4x4 row-major) as its pose:
+/* A lost system's relocalisation with a pose the caller found (for example with
+ * fu_relocalise_keys, the reference's default Relocalise, Tracking.cc:411-467, from the lost
+ * frame's keypoints; fu_system_relocalise_graph relocalises the lost frame itself).  The next
+ * frame takes `pose` (camera to world, 4x4 row-major) as its pose:
  * status FU_TRACK_RELOC, the model raycast there, nothing fused; tracking resumes with the
  * frame after it, and fusion once 10 frames have passed (System.cc:97-111).  FU_ERR_STATE
  * unless the system is lost.  Synchronises. */
@@ -417,6 +419,55 @@ fu_status fu_pose_estimate(const double* src, const double* ref, int32_t n, int3
 fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, const double* frame_pos, int32_t nq,
                              int32_t iterations, uint64_t seed, double pose[16], int32_t* n_pairs, int32_t* ok,
                              fu_stream s);
+
+/* Graph matching (Tracker::GenerateGraph, Tracking.cc:469-629; the reference switches it on
+ * from its viewer).  fu_select_subgraphs is its choice of mutually consistent pairs over the
+ * n x n consistency matrix of fu_build_adjacency_matrix (device, row-major): rank_sum[x] =
+ * the sum of column x (cv::cuda::reduce over rows; here in double in row order, rounded to
+ * float), order = the pairs by rank, largest first (cv::sortIdx descending; ties to the lower
+ * index, NaN ranks last); for each list i < n_lists, the i-th ranked pair heads it and the
+ * pairs ranked after it join while matrix[head][pair] > min_score, up to max_select pairs
+ * (head included; n_selected[i]); a list of three or more is refined: its k-th pair (k >= 1;
+ * the head is not kept, as written) stays unless a later pair of the list has a score below
+ * 5e-3 with it either way and a higher score with the head.  n_refined[i] pairs in
+ * refined[i * max_select ...], in list order.  The reference's constants: 10 lists, 400
+ * pairs, 0.1 (Tracking.h:129-132).  n <= 46340, max_select <= 1024; all buffers device
+ * memory (rank_sum and order: n each).  Asynchronous. */
+fu_status fu_select_subgraphs(const float* matrix, int32_t n, int32_t n_lists, int32_t max_select, float min_score,
+                              float* rank_sum, int32_t* order, int32_t* n_selected, int32_t* n_refined,
+                              int32_t* refined, fu_stream s);
+/* Tracker::Relocalise with graph matching on, through GenerateGraph's PoseEstimate calls
+ * (Tracking.cc:432-453, 469-629): the match list keeps both nearest map keys of a query that
+ * fails the ratio test, pairs whose map key is not valid are dropped, the consistency matrix
+ * and sub-graphs are built on the device, and PoseEstimate runs on the refined pairs of the
+ * first list of three or more selected pairs, once for each such list up to five (the
+ * reference orders the lists with cv::sortIdx on a one-column matrix, which leaves every
+ * index at 0, so each call takes the first list), when it holds four or more pairs.  Call c
+ * draws from seed + c.  poses: up to 5 camera-to-world candidates (row-major 4x4) in
+ * *n_poses; *n_pairs the match list's size.  frame_desc: device, nq x 64; frame_pos (nq x 3)
+ * and frame_normal (nq x 4): host floats, the lifted keypoints.  Synchronises. */
+fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* frame_desc, const float* frame_pos,
+                                         const float* frame_normal, int32_t nq, int32_t iterations, uint64_t seed,
+                                         double poses[5 * 16], int32_t* n_poses, int32_t* n_pairs, fu_stream s);
+/* Tracker::ValidatePose (Tracking.cc:636-682) on a lost system's current frame (the frame
+ * last given to it, which came back FU_TRACK_LOST): for each candidate pose, the visible
+ * blocks there (skipped below 512: accepted[i] = -1), the model raycast there and ComputeSE3
+ * (ICP only) from it with iters_reloc and thresh_reloc (the reference's {3, 3, 2} and 1e-3;
+ * accepted[i] = 1 if it passes, else 0); the first that passes (*chosen) is rendered again and
+ * refined with the system's tracking iterations and threshold.  If that passes, *ok = 1 and
+ * the frame is relocalised: its pose the refined one (fu_system_result: FU_TRACK_RELOC), the
+ * model raycast there, nothing fused; tracking resumes with the next frame and fusion once 10
+ * frames have passed, as after fu_system_relocalise.  A failed attempt leaves the model maps
+ * rendered at the last candidate tried (the reference's LastFrame maps, likewise) and the
+ * system lost.  FU_ERR_STATE unless the system is lost.  Synchronises. */
+fu_status fu_system_validate_poses(fu_system* sys, const double* poses, int32_t n, const int32_t iters_reloc[3],
+                                   float thresh_reloc, int32_t* accepted, int32_t* chosen, int32_t* ok, fu_stream s);
+/* Tracker::Relocalise with graph matching on, whole: fu_relocalise_graph_candidates, then
+ * fu_system_validate_poses with {3, 3, 2} iterations and 1e-3 (Tracking.h:101-104).
+ * *n_candidates (optional): the poses PoseEstimate gave.  Synchronises. */
+fu_status fu_system_relocalise_graph(fu_system* sys, fu_keymap* km, const float* frame_desc, const float* frame_pos,
+                                     const float* frame_normal, int32_t nq, int32_t iterations, uint64_t seed,
+                                     int32_t* n_candidates, int32_t* ok, fu_stream s);
 
 /* ---------------------------------------------------------------- self-test */
 /* The integration and ICP kernels divide and take square roots with range-specialised forms

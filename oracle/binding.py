@@ -80,6 +80,10 @@ _SIG = {
     "or_adjacency": (None, [P, P, P, I, P]),
     "or_lift_keypoints": (None, [P, I, P, P, I, I, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, P, P, P]),
     "or_match_knn2": (None, [P, I, P, I, P, P]),
+    "or_graph_select": (None, [P, I, I, I, F, P, P, P, P, P]),
+    "or_graph_inputs": (I, [P, P, I, P, I, P, P, P, P, P, P, I, C.POINTER(C.c_int)]),
+    "or_graph_lists": (I, [P, P, P, I, I, I, P, C.POINTER(C.c_int)]),
+    "or_system_validate_poses": (I, [P, DP, I, C.POINTER(I), F, P, P, DP]),
 }
 
 _lib = None
@@ -467,6 +471,18 @@ class System:
         st = lib().or_system_grab(self.h, _ptr(d), _ptr(c), pose.ctypes.data_as(DP))
         return st, pose.reshape(4, 4)
 
+    def validate_poses(self, poses, iters_reloc=(3, 3, 2), thresh_reloc=1e-3):
+        """Tracker::ValidatePose on the current (lost) frame: (ok, accepted per candidate,
+        chosen candidate, refined pose)"""
+        P = np.ascontiguousarray(np.asarray(poses, dtype=np.float64).reshape(-1, 4, 4))
+        n = len(P)
+        acc = np.zeros(max(n, 1), np.int32)
+        chosen = np.zeros(1, np.int32)
+        out = np.eye(4)
+        ok = lib().or_system_validate_poses(self.h, P.ctypes.data_as(DP), n, (I * 3)(*iters_reloc), thresh_reloc,
+                                            _ptr(acc), _ptr(chosen), out.ctypes.data_as(DP))
+        return ok, acc[:n], int(chosen[0]), out
+
     def relocalise(self, pose):
         p = np.ascontiguousarray(pose, dtype=np.float64)
         return lib().or_system_relocalise(self.h, p.ctypes.data_as(DP))
@@ -573,3 +589,52 @@ def match_knn2(query, train):
     dist = np.zeros((len(q), 2), np.float32)
     lib().or_match_knn2(_ptr(q), len(q), _ptr(t), len(t), _ptr(idx), _ptr(dist))
     return idx, dist
+
+
+def graph_select(matrix, n_lists=10, max_sel=400, min_score=0.1):
+    """GenerateGraph's sub-graph selection over a consistency matrix: (n_selected (n_lists,),
+    refined lists (list of arrays), rank sums (n,), rank order (n,))"""
+    C_ = np.ascontiguousarray(matrix, dtype=np.float32)
+    n = C_.shape[0]
+    nsel = np.zeros(n_lists, np.int32)
+    nref = np.zeros(n_lists, np.int32)
+    ref = np.zeros((n_lists, max_sel), np.int32)
+    rank = np.zeros(max(n, 1), np.float32)
+    order = np.zeros(max(n, 1), np.int32)
+    lib().or_graph_select(_ptr(C_), n, n_lists, max_sel, min_score, _ptr(nsel), _ptr(nref), _ptr(ref), _ptr(rank),
+                          _ptr(order))
+    return nsel, [ref[i, : nref[i]].copy() for i in range(n_lists)], rank[:n], order[:n]
+
+
+def graph_inputs(idx, dist, keys, frame_pos, frame_normal):
+    """Relocalise's match list with graph matching on, filtered by map-key validity:
+    (frame keys, map keys, distances, query of each pair, size of the list before the filter)"""
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    dist = np.ascontiguousarray(dist, dtype=np.float32)
+    keys = np.ascontiguousarray(keys)
+    fp = np.ascontiguousarray(frame_pos, dtype=np.float32)
+    fn = np.ascontiguousarray(frame_normal, dtype=np.float32)
+    nq = len(idx)
+    cap = 2 * nq
+    fk = np.zeros(max(cap, 1), SURF)
+    mk = np.zeros(max(cap, 1), SURF)
+    d = np.zeros(max(cap, 1), np.float32)
+    q = np.zeros(max(cap, 1), np.int32)
+    nm = C.c_int(0)
+    n = lib().or_graph_inputs(_ptr(idx), _ptr(dist), nq, _ptr(keys), len(keys), _ptr(fp), _ptr(fn), _ptr(fk), _ptr(mk),
+                              _ptr(d), _ptr(q), cap, C.byref(nm))
+    return fk[:n], mk[:n], d[:n], q[:n], nm.value
+
+
+def graph_lists(n_selected, refined, max_sel=400, n_select=5):
+    """the number of PoseEstimate calls GenerateGraph makes and the pairs they take"""
+    nl = len(n_selected)
+    nsel = np.ascontiguousarray(n_selected, dtype=np.int32)
+    nref = np.array([len(r) for r in refined], np.int32)
+    ref = np.zeros((nl, max_sel), np.int32)
+    for i, r in enumerate(refined):
+        ref[i, : len(r)] = r
+    out = np.zeros(max_sel, np.int32)
+    nlist = C.c_int(0)
+    M = lib().or_graph_lists(_ptr(nsel), _ptr(nref), _ptr(ref), nl, max_sel, n_select, _ptr(out), C.byref(nlist))
+    return M, out[: nlist.value].copy()

@@ -1517,8 +1517,9 @@ struct or_system {
     float last_err = 0;  // final ICP error and correspondence count of the last tracked frame
     int last_cnt = 0;
     // Tracker::state / lastState / noMissedFrames (Tracking.cc:21) and
-    // System::num_frames_after_reloc (System.cc:72); a relocalisation pose given from outside
-    // stands in for Relocalise (Tracking.cc:411-), which needs the keypoint map
+    // System::num_frames_after_reloc (System.cc:72); a relocalisation pose given before a lost
+    // frame places that frame (or_system_relocalise), or or_system_validate_poses relocalises
+    // the lost frame itself (Relocalise with graph matching, Tracking.cc:411-682)
     int state = 1, last_state = 1, missed = 0, after_reloc = 10;
     int reloc = 0;
     double reloc_pose[16];
@@ -1977,4 +1978,219 @@ extern "C" void or_match_knn2(const float* q, int nq, const float* t, int nt, in
         dist[2 * i] = d0;
         dist[2 * i + 1] = d1;
     }
+}
+
+// ------------------------------------------------------------------ graph matching
+// Tracker::GenerateGraph's selection of consistent pairs (Tracking/Tracking.cc:515-596) over
+// the consistency matrix C of n matched pairs (row-major, C[a * n + b] = conMatrix(a, b)).
+//   rank: the column sums (cv::cuda::reduce over the rows, CV_REDUCE_SUM, :520); OpenCV's
+//     summation order is its own, so here the sum is taken in double in row order and
+//     rounded to float;
+//   order: the pairs by rank, largest first (cv::sortIdx descending, :527); ties, which
+//     std::sort leaves in no fixed order, go to the lower index, and a NaN rank sorts as -inf;
+//   list i (i < n_lists, :532-589): its head is the i-th pair in rank order; the pairs ranked
+//     after it join while C[head][pair] > min_score, until max_sel pairs (the head included)
+//     are selected; a list of three or more is refined: its k-th pair a (k >= 1; the head is
+//     not kept, as written) stays unless a later pair b of the list is nearly exclusive with
+//     it (C[a][b] or C[b][a] below 5e-3) and scores higher with the head (C[head][b] >
+//     C[head][a]).
+// Outputs per list: the number selected and the refined pairs (max_sel slots per list).
+extern "C" void or_graph_select(const float* C, int n, int n_lists, int max_sel, float min_score, int32_t* n_selected,
+                                int32_t* n_refined, int32_t* refined, float* rank_out, int32_t* order_out) {
+    std::vector<float> rank(n);
+    for (int x = 0; x < n; ++x) {
+        double s = 0;
+        for (int y = 0; y < n; ++y) s += (double)C[(size_t)y * n + x];
+        rank[x] = (float)s;
+    }
+    std::vector<int> order(n);
+    for (int x = 0; x < n; ++x) order[x] = x;
+    auto key = [&](int x) { return isn(rank[x]) ? -INFINITY : rank[x]; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
+    if (rank_out) memcpy(rank_out, rank.data(), sizeof(float) * n);
+    if (order_out) memcpy(order_out, order.data(), sizeof(int32_t) * n);
+    auto c = [&](int a, int b) { return C[(size_t)a * n + b]; };
+    for (int i = 0; i < n_lists; ++i) {
+        std::vector<int> sel;
+        int head = -1;
+        for (int j = i; j < n; ++j) {
+            const int idx = order[j];
+            if (sel.empty()) {
+                sel.push_back(idx);
+                head = idx;
+            } else if (c(head, idx) > min_score) {
+                sel.push_back(idx);
+            }
+            if ((int)sel.size() >= max_sel) break;
+        }
+        const int S = (int)sel.size();
+        n_selected[i] = S;
+        int R = 0;
+        if (S >= 3)
+            for (int k = 1; k < S; ++k) {
+                const int a = sel[k];
+                int l = k + 1;
+                for (; l < S; ++l) {
+                    const int b = sel[l];
+                    if (c(a, b) < 5e-3f || c(b, a) < 5e-3f)
+                        if (c(head, b) > c(head, a)) break;
+                }
+                if (l >= S) refined[(size_t)i * max_sel + R++] = a;
+            }
+        n_refined[i] = R;
+    }
+}
+
+// Tracker::Relocalise with graph matching on (Tracking.cc:432-453) and GenerateGraph's list of
+// pairs (:485-505): every query whose two nearest map keys pass the ratio test
+// d0 < 0.9 d1 gives its nearest; every other gives both (in that order); pairs whose map key
+// is not valid are dropped.  idx / dist: the knnMatch output (2 per query, -1 / +inf where
+// missing); keys: the map's collected records.  The frame key of a pair takes the lifted
+// position and normal of its query and is valid (the reference leaves `valid` of that record
+// unset).  Returns the number of pairs (at most cap); *n_matches is the refined list's size
+// before the validity filter (Relocalise gives up below 3).
+extern "C" int or_graph_inputs(const int32_t* idx, const float* dist, int nq, const OrSurf* keys, int nk,
+                               const float* frame_pos, const float* frame_normal, OrSurf* frame_out, OrSurf* map_out,
+                               float* dist_out, int32_t* query_out, int cap, int* n_matches) {
+    std::vector<std::pair<int, int>> ref;  // (query, slot 0 or 1)
+    for (int i = 0; i < nq; ++i) {
+        if (idx[2 * i] < 0) continue;
+        if ((double)dist[2 * i] < 0.90 * (double)dist[2 * i + 1]) {
+            ref.push_back({i, 0});
+        } else {
+            ref.push_back({i, 0});
+            if (idx[2 * i + 1] >= 0) ref.push_back({i, 1});
+        }
+    }
+    *n_matches = (int)ref.size();
+    int n = 0;
+    for (auto& r : ref) {
+        const int t = idx[2 * r.first + r.second];
+        if (t >= nk || !keys[t].valid) continue;
+        if (n >= cap) break;
+        OrSurf f{};
+        f.valid = 1;
+        memcpy(f.pos, frame_pos + 3 * r.first, sizeof(f.pos));
+        memcpy(f.normal, frame_normal + 4 * r.first, sizeof(f.normal));
+        frame_out[n] = f;
+        map_out[n] = keys[t];
+        dist_out[n] = dist[2 * r.first + r.second];
+        query_out[n] = r.first;
+        ++n;
+    }
+    return n;
+}
+
+// GenerateGraph's sub-graphs to PoseEstimate inputs (Tracking.cc:591-629), from the selection:
+// the lists with three or more selected pairs are kept in list order and the number of
+// PoseEstimate calls is M = min(n_select, kept).  The reference orders the kept lists with
+// cv::sortIdx(cvNoSelected, tmp, CV_SORT_DESCENDING) on a one-column matrix, which sorts each
+// one-element row, so tmp holds zeros and all M calls take the first kept list; a list of
+// three or fewer refined pairs is skipped (:601-602).  Returns M when the first kept list has
+// four or more refined pairs (its pairs in *list / *n_list), else 0.
+extern "C" int or_graph_lists(const int32_t* n_selected, const int32_t* n_refined, const int32_t* refined, int n_lists,
+                              int max_sel, int n_select, int32_t* list, int* n_list) {
+    int kept = 0, first = -1;
+    for (int i = 0; i < n_lists; ++i)
+        if (n_selected[i] >= 3) {
+            if (first < 0) first = i;
+            ++kept;
+        }
+    *n_list = 0;
+    if (kept == 0) return 0;
+    const int M = kept < n_select ? kept : n_select;
+    const int R = n_refined[first];
+    if (R <= 3) return 0;
+    memcpy(list, refined + (size_t)first * max_sel, sizeof(int32_t) * R);
+    *n_list = R;
+    return M;
+}
+
+// Tracker::ValidatePose (Tracking.cc:636-682) on a lost system's current frame: for each
+// candidate pose, UpdateVisibility there (skipped below 512 visible blocks), RayTrace and
+// ResizeImages of the model maps, and ComputeSE3(icpOnly, iters_reloc, thresh_reloc) from it;
+// the first candidate that passes (the reference sorts the passing errors with
+// cv::sortIdx on a one-column matrix, which leaves index 0 first) is rendered again and
+// refined with the tracking iterations and threshold.  On success the frame is the
+// relocalised one (Tracking.cc:101-105, System.cc:99-120): its pose the refined pose, swapped,
+// the model raycast there, nothing fused; returns 1.  accepted[i]: -1 skipped for visibility,
+// 0 rejected, 1 passed; *chosen the candidate refined (-1 if none).
+namespace {
+// UpdateVisibility at P, then (unless fewer than min_vis blocks are visible) RayTrace into the
+// model maps (left as they are when nothing renders) and ResizeImages; false when skipped
+bool or_render_at(or_system* s, const double P[16], uint32_t min_vis) {
+    const int c0 = s->cols, r0 = s->rows;
+    float R[9], Rinv[9], t[3];
+    or_pose_f(P, R, Rinv, t);
+    const uint32_t nvis =
+        or_check_visibility(s->map, Rinv, t, c0, r0, s->K[0], s->K[1], s->K[2], s->K[3], s->map->dmax, s->map->dmin);
+    if (nvis < min_vis) return false;
+    uint32_t nfrag = 0;
+    if (or_create_rendering_blocks(s->map, s->zmin.data(), s->zmax.data(), s->zcols, s->zrows, s->map->dmax,
+                                   s->map->dmin, Rinv, t, nvis, s->K[0], s->K[1], s->K[2], s->K[3], &nfrag))
+        or_raycast(s->map, s->mvmap[0].data(), s->mnmap[0].data(), c0, r0, s->zmin.data(), s->zmax.data(), s->zcols,
+                   s->zrows, R, Rinv, t, (float)(1.0 / (double)s->K[0]), (float)(1.0 / (double)s->K[1]), s->K[2],
+                   s->K[3]);
+    for (int l = 1; l < 3; ++l)
+        or_resize_map(s->mvmap[l - 1].data(), s->mnmap[l - 1].data(), c0 >> (l - 1), r0 >> (l - 1), s->mvmap[l].data(),
+                      s->mnmap[l].data());
+    return true;
+}
+}  // namespace
+
+extern "C" int or_system_validate_poses(or_system* s, const double* poses, int n, const int iters_reloc[3],
+                                        float thresh_reloc, int32_t* accepted, int32_t* chosen, double pose_out[16]) {
+    *chosen = -1;
+    if (s->last_state != -1) return -1;
+    const float* nv[3] = {s->vmap[0].data(), s->vmap[1].data(), s->vmap[2].data()};
+    const float* nn[3] = {s->nmap[0].data(), s->nmap[1].data(), s->nmap[2].data()};
+    const float* lv[3] = {s->mvmap[0].data(), s->mvmap[1].data(), s->mvmap[2].data()};
+    const float* ln[3] = {s->mnmap[0].data(), s->mnmap[1].data(), s->mnmap[2].data()};
+    int first = -1;
+    for (int i = 0; i < n; ++i) {
+        const double* P = poses + 16 * i;
+        accepted[i] = -1;
+        if (!or_render_at(s, P, 512)) continue;
+        double pose[16];
+        float err = 0;
+        int cnt = 0;
+        const int st = or_compute_se3(nv, nn, lv, ln, s->cols, s->rows, s->K, iters_reloc, s->minc, thresh_reloc, P, pose,
+                                      &err, &cnt);
+        accepted[i] = st == 1;
+        if (st == 1 && first < 0) first = i;
+    }
+    if (first < 0) return 0;
+    *chosen = first;
+    const double* P = poses + 16 * first;
+    or_render_at(s, P, 0);
+    double pose[16];
+    float err = 0;
+    int cnt = 0;
+    const int st = or_compute_se3(nv, nn, lv, ln, s->cols, s->rows, s->K, s->iters, s->minc, s->thresh, P, pose, &err,
+                                  &cnt);
+    if (st != 1) return 0;
+    // relocalised: Track() sets lastState = 0 and swaps the frame in; GrabImage (state -1)
+    // renders it without fusion
+    s->last_state = 0;
+    s->last_err = err;
+    s->last_cnt = cnt;
+    memcpy(s->last_pose, pose, sizeof(pose));
+    if (pose_out) memcpy(pose_out, pose, sizeof(pose));
+    if (s->use_rgb)
+        for (int l = 0; l < 3; ++l) s->last_img[l] = s->img[l];
+    float R[9], Rinv[9], t[3];
+    or_pose_f(pose, R, Rinv, t);
+    const uint32_t nvis = or_check_visibility(s->map, Rinv, t, s->cols, s->rows, s->K[0], s->K[1], s->K[2], s->K[3],
+                                              s->map->dmax, s->map->dmin);
+    uint32_t nfrag = 0;
+    if (or_create_rendering_blocks(s->map, s->zmin.data(), s->zmax.data(), s->zcols, s->zrows, s->map->dmax,
+                                   s->map->dmin, Rinv, t, nvis, s->K[0], s->K[1], s->K[2], s->K[3], &nfrag)) {
+        or_raycast(s->map, s->mvmap[0].data(), s->mnmap[0].data(), s->cols, s->rows, s->zmin.data(), s->zmax.data(),
+                   s->zcols, s->zrows, R, Rinv, t, (float)(1.0 / (double)s->K[0]), (float)(1.0 / (double)s->K[1]),
+                   s->K[2], s->K[3]);
+    } else {
+        s->mvmap[0] = s->vmap[0];
+        s->mnmap[0] = s->nmap[0];
+    }
+    return 1;
 }
