@@ -706,7 +706,7 @@ def main():
         "captured_steps": r.get("graph"),
         "latency_backlog_ms": {"p50": pct(r["lat_backlog"], 50), "p99": pct(r["lat_backlog"], 99)},
         "plain": p,
-        "served_over_plain": round(fps / p["value"], 3),
+        "served_over_plain": round(fps / p["value"], 3) if p["value"] > 0 else None,
         "mixed_resolution": mixed,
         "single_stream": single,
         "rgbd_tracking": rgbd,
