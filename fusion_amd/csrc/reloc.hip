@@ -488,7 +488,12 @@ extern "C" fu_status fu_pose_estimate(const double* src, const double* ref, int3
     std::vector<char> outlier;  // the last evaluated hypothesis' outliers (Solver.cc:84-93)
     Rng rng{seed};
     int it = 0;
-    while (it < iterations) {
+    // a reflection re-draws without counting the iteration (as the reference); a three-point
+    // sample spans at most a plane and its completed bases are right-handed, so that does not
+    // repeat, but the draws are bounded all the same
+    long long draws = 0;
+    const long long max_draws = 64ll * (iterations > 0 ? iterations : 1);
+    while (it < iterations && draws++ < max_draws) {
         int smp[3];
         for (int k = 0; k < 3; ++k) smp[k] = (int)rng.next((uint32_t)n);
         const bool bad = smp[0] == smp[1] || smp[1] == smp[2] || smp[2] == smp[0];
