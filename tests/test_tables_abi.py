@@ -120,3 +120,27 @@ def test_api_rejects_non_contiguous_tensors():
         A._addr(t.transpose(0, 1))
     with pytest.raises(N.FusionError):
         A._p(t[:, ::2])
+
+
+def test_relocalisation_argument_errors_without_device():
+    """the relocalisation entry points refuse bad arguments before any device call"""
+    lib = N.lib()
+    I = C.c_int
+    # sub-graph selection: null buffers, too many pairs, lists too long
+    assert lib.fu_select_subgraphs(None, 10, 10, 400, 0.1, None, None, None, None, None, None) == -1
+    assert b"null" in lib.fu_last_error()
+    assert lib.fu_select_subgraphs(None, 46341, 10, 400, 0.1, None, None, None, None, None, None) == -1
+    assert b"46340" in lib.fu_last_error()
+    assert lib.fu_select_subgraphs(None, 10, 10, 1025, 0.1, None, None, None, None, None, None) == -1
+    assert lib.fu_select_subgraphs(None, 10, 0, 400, 0.1, None, None, None, None, None, None) == -1
+    poses = (C.c_double * 80)()
+    npose, npairs, ok, chosen = I(), I(), I(), I()
+    assert lib.fu_relocalise_graph_candidates(None, None, None, None, 5, 100, 0, poses, C.byref(npose),
+                                              C.byref(npairs), None) == -1
+    assert lib.fu_relocalise_keys(None, None, None, 5, 100, 0, poses, C.byref(npairs), C.byref(ok), None) == -1
+    it = (I * 3)(3, 3, 2)
+    assert lib.fu_system_validate_poses(None, poses, 1, it, 1e-3, None, C.byref(chosen), C.byref(ok), None) == -1
+    assert lib.fu_system_relocalise_graph(None, None, None, None, None, 5, 100, 0, None, C.byref(ok), None) == -1
+    assert lib.fu_match_knn2(None, -1, None, 0, None, None, None) == -1
+    assert lib.fu_lift_keypoints(None, -1, None, None, 640, 480, 525.0, 525.0, 319.5, 239.5, 3.0, None, None, None,
+                                 None) == -1
