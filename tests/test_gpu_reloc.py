@@ -294,3 +294,65 @@ def test_relocalise_graph_end_to_end(gpu):
         assert s.result()["track_status"] == 1
     s.close()
     km.close()
+
+
+def test_validate_poses_with_the_photometric_term_and_in_a_batch(gpu):
+    """ValidatePose runs ICP only (Tracking.cc:657, 681) also for a system that tracks with the
+    photometric term, whose intensity pyramid is swapped in with the relocalised frame: GPU =
+    oracle in choice and pose, and RGB+ICP tracking resumes.  Two systems driven through one
+    batch and relocalised alike continue bit for bit alike in the batch."""
+    from tests.test_oracle_graph import _perturb
+    from tests.helpers import scene_frames
+    sc, K, fr = scene_frames(frames=range(0, 9), cols=320, rows=240)
+    empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
+    mp = dict(num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
+              trunc_dist=float(np.float32(0.01) * np.float32(3.0)), max_new_blocks=1 << 16)
+    seq = list(fr[:6]) + [empty] * 10 + [fr[6]]
+    want = [4] + [1] * 5 + [2] * 10 + [5]
+    # photometric term
+    s = A.System(A.system_params(cols=320, rows=240, K=K, map=mp, use_rgb=1))
+    o = O.System(cols=320, rows=240, K=K, num_buckets=mp["num_buckets"], num_excess=mp["num_excess"],
+                 num_blocks=mp["num_blocks"], voxel_size=0.01, use_rgb=True)
+    st, ost, poses = [], [], []
+    for f in seq:
+        s.GrabImage(*A.to_device(*f))
+        r = s.result()
+        st.append(r["track_status"])
+        poses.append(r["pose"])
+        ost.append(o.grab(*f)[0])
+    assert st == ost == want
+    near = _perturb(poses[5], [0.012, -0.01, 0.01], 1.5, [0, 1, 1])
+    far = np.eye(4)
+    far[:3, 3] = [0, 0, 40.0]
+    ok, acc, chosen = s.validate_poses([far, near])
+    ook, oacc, ochosen, opose = o.validate_poses([far, near])
+    assert ok and ook == 1 and list(acc) == list(oacc) == [-1, 1] and chosen == ochosen == 1
+    assert np.abs(s.result()["pose"] - opose).max() < 5e-4
+    for f in fr[7:9]:
+        s.GrabImage(*A.to_device(*f))
+        g = s.result()
+        ost_, op = o.grab(*f)
+        assert g["track_status"] == ost_ == 1 and g["rgb_count"] > 0
+        assert np.abs(g["pose"] - op).max() < 2e-3
+    s.close()
+    # two systems in one batch
+    pair = [A.System(A.system_params(cols=320, rows=240, K=K, map=mp)) for _ in range(2)]
+    bt = A.Batch(2)
+    for x in pair:
+        bt.attach(x)
+    for f in seq:
+        d, c = A.to_device(*f)
+        bt.grab(pair, [d, d], [c, c])
+    assert [x.result()["track_status"] for x in pair] == [5, 5]
+    got = [x.validate_poses([far, near]) for x in pair]
+    assert all(g[0] for g in got) and all(list(g[1]) == [-1, 1] for g in got)
+    r0, r1 = pair[0].result(), pair[1].result()
+    assert r0["track_status"] == r1["track_status"] == 6 and np.array_equal(r0["pose"], r1["pose"])
+    for f in fr[7:9]:
+        d, c = A.to_device(*f)
+        bt.grab(pair, [d, d], [c, c])
+        r0, r1 = pair[0].result(), pair[1].result()
+        assert r0["track_status"] == r1["track_status"] == 1 and np.array_equal(r0["pose"], r1["pose"])
+    for x in pair:
+        bt.detach(x)
+        x.close()
