@@ -1106,6 +1106,7 @@ extern "C" fu_status fu_system_reset(fu_system* sys, fu_stream s) {
     HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
     HIPCHK(hipStreamSynchronize(S(s)));
     sys->frame = 0;
+    sys->frame_at_load = 0;
     return FU_OK;
 }
 
@@ -1530,6 +1531,8 @@ extern "C" fu_status fu_system_validate_poses(fu_system* sys, const double* pose
     HIPCHK(hipMemcpyAsync(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost, hs));
     HIPCHK(hipStreamSynchronize(hs));
     if (h.sm_state != -1) return set_err(FU_ERR_STATE, "fu_system_validate_poses: the system is not lost");
+    if (sys->frame == sys->frame_at_load)  // the pyramid to validate against is a grabbed frame's
+        return set_err(FU_ERR_STATE, "fu_system_validate_poses: no frame since the system was reset or loaded");
     fu_tracker* t = nullptr;
     fu_status st = fu_tracker_create(sys->p.cols, sys->p.rows, &t);
     if (st) return st;

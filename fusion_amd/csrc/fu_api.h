@@ -39,6 +39,7 @@ struct fu_system {
     fu_system_buffers b;
     int lc[3], lr[3];
     unsigned frame = 0;
+    unsigned frame_at_load = 0;  // frame count at creation, reset or snapshot load
     hipGraphExec_t graph = nullptr;
     bool prof = false;
     hipEvent_t ev[FU_NUM_STAGES + 1] = {};

@@ -441,6 +441,7 @@ extern "C" fu_status fu_system_load(fu_system* sys, const char* path, fu_stream 
         for (int l = 0; l < 3; ++l) HIPOK(hipMemcpy(sys->b.last_image[l], img[l].data(), img[l].size(), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(sys->trk->st, &st, sizeof(st), hipMemcpyHostToDevice));
     sys->frame = frame;
+    sys->frame_at_load = frame;
     return FU_OK;
 }
 

@@ -459,7 +459,8 @@ fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* frame_desc,
  * model raycast there, nothing fused; tracking resumes with the next frame and fusion once 10
  * frames have passed, as after fu_system_relocalise.  A failed attempt leaves the model maps
  * rendered at the last candidate tried (the reference's LastFrame maps, likewise) and the
- * system lost.  FU_ERR_STATE unless the system is lost.  Synchronises. */
+ * system lost.  FU_ERR_STATE unless the system is lost and has been given a frame since it
+ * was reset or loaded.  Synchronises. */
 fu_status fu_system_validate_poses(fu_system* sys, const double* poses, int32_t n, const int32_t iters_reloc[3],
                                    float thresh_reloc, int32_t* accepted, int32_t* chosen, int32_t* ok, fu_stream s);
 /* Tracker::Relocalise with graph matching on, whole: fu_relocalise_graph_candidates, then

@@ -356,3 +356,39 @@ def test_validate_poses_with_the_photometric_term_and_in_a_batch(gpu):
     for x in pair:
         bt.detach(x)
         x.close()
+
+
+def test_lost_system_through_a_snapshot(gpu, tmp_path):
+    """a lost session saved and loaded stays lost; validation needs a frame given after the
+    load, and then relocalises it exactly as the original session does"""
+    from tests.test_oracle_graph import _perturb
+    from tests.helpers import scene_frames
+    sc, K, fr = scene_frames(frames=range(0, 8), cols=320, rows=240)
+    empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
+    mp = dict(num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
+              trunc_dist=float(np.float32(0.01) * np.float32(3.0)), max_new_blocks=1 << 16)
+    ref = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    for f in fr[:7]:
+        ref.GrabImage(*A.to_device(*f))
+    truth = ref.result()["pose"]  # frame 6 as tracking places it
+    ref.close()
+    s = _lost_gpu(K, fr, empty, mp)  # frames 0..5, ten empty frames, frame 6 (lost)
+    path = str(tmp_path / "lost.fusess")
+    s.save(path)
+    t = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    t.load(path)
+    cand = [_perturb(truth, [0.012, -0.01, 0.01], 1.5, [0, 1, 1])]
+    with pytest.raises(A.N.FusionError):  # no frame since the load
+        t.validate_poses(cand)
+    t.GrabImage(*A.to_device(*fr[6]))
+    assert t.result()["track_status"] == 5
+    a = s.validate_poses(cand)
+    b = t.validate_poses(cand)
+    assert a[0] and b[0] and list(a[1]) == list(b[1]) and a[2] == b[2]
+    assert np.array_equal(s.result()["pose"], t.result()["pose"])
+    s.GrabImage(*A.to_device(*fr[7]))
+    t.GrabImage(*A.to_device(*fr[7]))
+    assert s.result()["track_status"] == t.result()["track_status"] == 1
+    assert np.array_equal(s.result()["pose"], t.result()["pose"])
+    s.close()
+    t.close()
