@@ -4,7 +4,7 @@
   fusion_amd/lib/libfusion_amd.so   HIP kernels + C ABI (include/fusion_amd.h)
   fusion_amd/lib/libfusion_amd_cxx.so  the C++ drop-in layer (include/fusion_amd_compat.hpp)
   fusion_amd/lib/libfu_scene.so     synthetic scenes (include/fusion_scene.h), g++ + OpenMP
-  tests/cxx/_build/test_compat      the C++ layer's test driver (run by tests/test_gpu_compat.py)
+  tests/cxx/_build/test_compat      the C++ layer's test program (run by tests/test_gpu_compat.py)
   oracle/_build/liboracle.so        CPU oracle (test infrastructure; oracle/Makefile)
 """
 import os
@@ -56,7 +56,7 @@ def build_hip(verbose=False):
 
 
 def build_cxx(verbose=False):
-    """the C++ drop-in layer over the C ABI, and its test driver"""
+    """the C++ drop-in layer over the C ABI, and its test program"""
     out = os.path.join(LIB, "libfusion_amd_cxx.so")
     tmp = out + ".tmp"
     _run([HIPCC, "-std=c++17", "-O2", "-fPIC", "-shared", os.path.join(CSRC, "compat.cpp"), "-L" + LIB,

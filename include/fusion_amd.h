@@ -577,7 +577,7 @@ fu_status fu_batch_grab(fu_batch* b, int n, fu_system* const* sys, const uint16_
  * synchronises at its end.  ms[] are per step, summed over steps; the counts sum over frames. */
 fu_status fu_batch_set_profiling(fu_batch* b, int enable);
 fu_status fu_batch_get_stage_times(fu_batch* b, fu_stage_times* out);
-/* Captured steps (the north-star's per-frame graph for batches): with `enable`, a step whose
+/* Captured steps (a batch's per-frame work as a hipGraph): with `enable`, a step whose
  * systems all have tracked a frame before runs its input stage (resize, filter, pyramid,
  * normals: the launches that read the frames) directly and everything after it -- tracking,
  * fusion, raycast, results -- as one hipGraph launch.  A graph is keyed by the step's systems

@@ -1,4 +1,4 @@
-// A System-shaped driver written against the reference's host API (Tracking/Reduction.h,
+// A System-shaped test program written against the reference's host API (Tracking/Reduction.h,
 // Mapping/RenderScene.h; include/fusion_amd_compat.hpp): per frame FillImages, the ICP loop
 // of Tracker::ComputeSE3 with its host solve (Tracking.cc:296-409), FuseMapColor,
 // CreateRenderingBlocks and Raycast (System.cc:92-138), MeshScene at the end, and the keypoint

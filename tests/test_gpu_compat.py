@@ -1,7 +1,7 @@
 """The C++ drop-in layer (include/fusion_amd_compat.hpp, libfusion_amd_cxx.so): a System-shaped
-driver written against the reference's Tracking/Reduction.h and Mapping/RenderScene.h API
-(tests/cxx/test_compat.cpp) runs six frames, meshes the map and uses the keypoint map; each of
-its calls is checked bit for bit against the same call through the C ABI inside the driver.
+test program written against the reference's Tracking/Reduction.h and Mapping/RenderScene.h
+API (tests/cxx/test_compat.cpp) runs six frames, meshes the map and uses the keypoint map; each
+of its calls is checked bit for bit against the same call through the C ABI inside the program.
 Here its tracked poses are compared with the library's reference-style system (host solve
 per ICP iteration) on the same frames."""
 import os
