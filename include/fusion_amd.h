@@ -669,6 +669,9 @@ fu_status fu_server_create(const fu_server_params* p, fu_server** out);
 fu_status fu_server_destroy(fu_server* sv);
 fu_status fu_server_open(fu_server* sv, const fu_system_params* p, int32_t* session); /* FU_ERR_BUDGET if full */
 fu_status fu_server_close(fu_server* sv, int32_t session);  /* its submitted frames must have completed */
+/* the session's system (owned by the server): for its results, map and snapshots, and to
+ * relocalise a lost session (fu_system_relocalise_graph, fu_system_validate_poses,
+ * fu_system_relocalise) once its submitted frames have completed (fu_server_poll) */
 fu_system* fu_server_system(fu_server* sv, int32_t session);
 double fu_server_now_ms(fu_server* sv);
 /* queue one frame; t_submit_ms < 0 means now */

@@ -149,3 +149,57 @@ def test_server_mesh_request(gpu):
     assert np.array_equal(key(v_a), key(v_b))
     sv.close_session(sid)
     sv.close()
+
+
+def test_served_session_relocalised_through_its_system(gpu):
+    """a served session that is lost is relocalised on its system handle once its submitted
+    frames have completed (fu_server_system + fu_system_validate_poses), then served on, with
+    the poses of a system run alone on the same frames, bit for bit"""
+    from tests.helpers import scene_frames
+    from tests.test_oracle_graph import _perturb
+    sc, K, fr = scene_frames(frames=range(0, 9), cols=320, rows=240)
+    empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
+    mp = dict(num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
+              trunc_dist=float(np.float32(0.01) * np.float32(3.0)), max_new_blocks=1 << 16)
+    params = lambda: A.system_params(cols=320, rows=240, K=K, map=mp)
+    ref = A.System(params())
+    for f in fr[:7]:
+        ref.GrabImage(*A.to_device(*f))
+    cand = [_perturb(ref.result()["pose"], [0.012, -0.01, 0.01], 1.5, [0, 1, 1])]
+    ref.close()
+    seq = list(fr[:6]) + [empty] * 10 + [fr[6]]
+    dev = [A.to_device(*f) for f in seq + list(fr[7:9])]
+    # alone
+    s = A.System(params())
+    alone = []
+    for i, (d, c) in enumerate(dev):
+        if i == len(seq):
+            assert s.validate_poses(cand)[0]
+            alone.append(s.result())
+        s.GrabImage(d, c)
+        alone.append(s.result())
+    s.close()
+    # served, next to a second session that tracks throughout
+    sv = A.Server(max_batch=4, max_inflight=2, lanes=1)
+    sid = sv.open(params())
+    other = sv.open(params())
+    served = []
+    for i, (d, c) in enumerate(dev):
+        if i == len(seq):
+            sys_view = sv.system(sid)
+            ok, acc, chosen = sys_view.validate_poses(cand)
+            assert ok and list(acc) == [1] and chosen == 0
+            served.append(sys_view.result())
+        sv.submit(sid, d, c, tag=i)
+        sv.submit(other, *dev[min(i, 5)], tag=1000 + i)
+        done = [x for x in sv.drain() if x["session"] == sid]
+        assert len(done) == 1
+        served.append(done[0])
+    assert [r["track_status"] for r in served] == [r["track_status"] for r in alone]
+    assert [r["track_status"] for r in served][-4:] == [5, 6, 1, 1]
+    for a, b in zip(alone, served):
+        assert np.array_equal(a["pose"], b["pose"])
+    sv.close_session(sid)
+    sv.close_session(other)
+    sv.close()
+    torch.cuda.synchronize()
