@@ -30,7 +30,11 @@
  *     overflowing (undefined in C++).
  *
  * Pinning (tests/test_oracle.py checks every function against closed forms, brute force or
- * numpy/scipy; the reference has no tests or golden vectors of its own).  PARITY UNPINNED:
+ * numpy/scipy; the reference has no tests or golden vectors of its own; the keypoint map,
+ * lifting, matching, graph selection and ValidatePose in tests/test_oracle_keymap.py,
+ * test_oracle_reloc.py and test_oracle_graph.py: independent numpy statements, a plane's
+ * closed form, a consistency matrix worked by hand, the selection rules checked pair by pair,
+ * and the pose tracking gives the relocalised frame).  PARITY UNPINNED:
  * the exact bits of the reference's third-party solve and exponential - Eigen's LDLT
  * (Tracking.cc:388, Eigen >= 3.2, unversioned) and Sophus::SE3d::exp (Tracking.cc:389,
  * unpinned) - which or_solve6 / or_se3_exp replace by a Cholesky solve and the closed-form
