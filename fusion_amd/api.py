@@ -475,6 +475,10 @@ class System:
               "fu_system_relocalise_graph")
         return bool(ok.value), nc.value
 
+    def set_mapping(self, enabled):
+        """Tracker::mappingDisabled: with mapping off frames are tracked and rendered, never fused"""
+        check(N.lib().fu_system_set_mapping(self.handle, int(bool(enabled)), _s()), "fu_system_set_mapping")
+
     def graph_counts(self):
         """(graphs captured, graph launches) of this system"""
         c, n = C.c_uint32(), C.c_uint32()

@@ -1103,10 +1103,25 @@ extern "C" fu_status fu_system_reset(fu_system* sys, fu_stream s) {
     if (st) return st;
     TrackState h;
     init_state(&h);
+    h.mapping_off = sys->mapping_off ? 1 : 0;  // RebootSystem keeps mappingDisabled (System.cc:272-277)
     HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
     HIPCHK(hipStreamSynchronize(S(s)));
     sys->frame = 0;
     sys->frame_at_load = 0;
+    return FU_OK;
+}
+
+// Tracker::mappingDisabled (Tracking.h:45; set by ReadMapFromDisk, System.cc:266-270, and
+// from the viewer): frames are still tracked, rendered and raycast, never fused
+extern "C" fu_status fu_system_set_mapping(fu_system* sys, int enabled, fu_stream s) {
+    if (!sys) return set_err(FU_ERR_ARG, "fu_system_set_mapping: null system");
+    TrackState h;
+    HIPCHK(hipMemcpyAsync(&h, sys->trk->st, sizeof(h), hipMemcpyDeviceToHost, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    h.mapping_off = enabled ? 0 : 1;
+    HIPCHK(hipMemcpyAsync(sys->trk->st, &h, sizeof(h), hipMemcpyHostToDevice, S(s)));
+    HIPCHK(hipStreamSynchronize(S(s)));
+    sys->mapping_off = !enabled;
     return FU_OK;
 }
 

@@ -40,6 +40,7 @@ struct fu_system {
     int lc[3], lr[3];
     unsigned frame = 0;
     unsigned frame_at_load = 0;  // frame count at creation, reset or snapshot load
+    bool mapping_off = false;    // Tracker::mappingDisabled, kept across fu_system_reset
     hipGraphExec_t graph = nullptr;
     bool prof = false;
     hipEvent_t ev[FU_NUM_STAGES + 1] = {};

@@ -56,7 +56,7 @@ struct TrackState {
     int after_reloc;  // System::num_frames_after_reloc (10 at start)
     int reloc;        // a relocalisation pose is in last_pose for the next (lost) frame
     int gates;        // FrameGate bits of the current frame
-    int pad_;
+    int mapping_off;  // Tracker::mappingDisabled: frames are tracked and rendered, never fused
 };
 
 // camera pose as the kernels use it: R (camera->world), Rinv = R^T, t

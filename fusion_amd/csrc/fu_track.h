@@ -308,6 +308,7 @@ FU_HD void frame_commit(TrackState* st) {
             next = -1;
         }
     }
+    if (st->mapping_off) gates &= ~GATE_FUSE;  // System.cc:110-112: mapping disabled, no fusion
     st->sm_prev = cur;
     st->sm_state = next;
     st->gates = gates;

@@ -442,6 +442,7 @@ extern "C" fu_status fu_system_load(fu_system* sys, const char* path, fu_stream 
     HIPOK(hipMemcpy(sys->trk->st, &st, sizeof(st), hipMemcpyHostToDevice));
     sys->frame = frame;
     sys->frame_at_load = frame;
+    sys->mapping_off = st.mapping_off != 0;
     return FU_OK;
 }
 

@@ -170,6 +170,7 @@ SIGNATURES = {
     "fu_system_grab_image_host": (I, [P, P, P, P]),
     "fu_system_graph_counts": (I, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "fu_system_relocalise": (I, [P, C.POINTER(C.c_double), P]),
+    "fu_system_set_mapping": (I, [P, I, P]),
     "fu_system_result": (I, [P, C.POINTER(FrameResult), P]),
     "fu_system_set_profiling": (I, [P, I]),
     "fu_system_get_stage_times": (I, [P, C.POINTER(StageTimes)]),

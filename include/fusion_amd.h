@@ -325,6 +325,13 @@ fu_status fu_system_grab_image_from(fu_system* sys, const uint16_t* depth, const
  * (Core/Frame.cc:61-62); asynchronous for pinned (hipHostMalloc) memory, which must then stay
  * unchanged until the copy has run */
 fu_status fu_system_grab_image_host(fu_system* sys, const uint16_t* depth, const uint8_t* rgb, fu_stream s);
+/* Tracker::mappingDisabled (Tracking.h:45): with mapping off, frames are tracked, rendered and
+ * raycast but never fused (System.cc:108-126), and fu_system_result reports fused = 0.
+ * ReadMapFromDisk (System.cc:266-270) turns it off after a map load and restarts tracking;
+ * here: fu_system_reset, fu_map_load on fu_system_map, fu_system_set_mapping(sys, 0).  The
+ * setting survives fu_system_reset (as RebootSystem, System.cc:272-277) and is part of a
+ * session snapshot.  Synchronises. */
+fu_status fu_system_set_mapping(fu_system* sys, int enabled, fu_stream s);
 /* graphs captured and replayed by this system (no reference counterpart) */
 fu_status fu_system_graph_counts(fu_system* sys, uint32_t* captures, uint32_t* launches);
 /* outcome of the last frame; synchronises */

@@ -64,6 +64,7 @@ _SIG = {
     "or_system_destroy": (None, [P]),
     "or_system_grab": (I, [P, P, P, DP]),
     "or_system_relocalise": (I, [P, DP]),
+    "or_system_set_mapping": (None, [P, I]),
     "or_system_map": (P, [P]),
     "or_system_buffer": (FP, [P, I, I]),
     "or_system_set_rgb": (None, [P, I]),
@@ -470,6 +471,9 @@ class System:
         pose = np.zeros(16)
         st = lib().or_system_grab(self.h, _ptr(d), _ptr(c), pose.ctypes.data_as(DP))
         return st, pose.reshape(4, 4)
+
+    def set_mapping(self, enabled):
+        lib().or_system_set_mapping(self.h, 1 if enabled else 0)
 
     def validate_poses(self, poses, iters_reloc=(3, 3, 2), thresh_reloc=1e-3):
         """Tracker::ValidatePose on the current (lost) frame: (ok, accepted per candidate,

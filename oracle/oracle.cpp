@@ -1523,6 +1523,7 @@ struct or_system {
     int state = 1, last_state = 1, missed = 0, after_reloc = 10;
     int reloc = 0;
     double reloc_pose[16];
+    int mapping_off = 0;  // Tracker::mappingDisabled (Tracking.h:45)
 };
 
 extern "C" or_system* or_system_create(int cols, int rows, float fx, float fy, float cx, float cy, float depth_scale,
@@ -1681,7 +1682,7 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
         s->after_reloc = 0;
     if (pose_out) memcpy(pose_out, pose, sizeof(pose));
     if (!swap) return status;
-    const bool fuse = s->state != -1 && s->after_reloc >= 10;
+    const bool fuse = !s->mapping_off && s->state != -1 && s->after_reloc >= 10;  // System.cc:110-111
     memcpy(s->last_pose, pose, sizeof(pose));
     // Tracker::SwapFrame (Tracking.cc:245-247) after a swapped frame: its images
     // become LastFrame's
@@ -1710,6 +1711,10 @@ extern "C" int or_system_grab(or_system* s, const uint16_t* depth, const uint8_t
     }
     return status;
 }
+
+// Tracker::mappingDisabled: swapped frames are rendered (UpdateVisibility + RayTrace,
+// System.cc:114-120) and never fused
+extern "C" void or_system_set_mapping(or_system* s, int enabled) { s->mapping_off = enabled ? 0 : 1; }
 
 // a relocalisation pose for a lost system's next frame; -1 unless the system is lost
 extern "C" int or_system_relocalise(or_system* s, const double pose[16]) {

@@ -213,3 +213,62 @@ def test_relocalise_refused_unless_lost(gpu):
     with pytest.raises(A.N.FusionError):
         s.relocalise(np.eye(4))
     s.close()
+
+
+def test_mapping_disabled_matches_the_oracle(gpu, tmp_path):
+    """Tracker::mappingDisabled (System.cc:108-126): frames tracked and rendered, never fused,
+    the model frozen, statuses and fused flags as the oracle's; ReadMapFromDisk's sequence
+    (System.cc:266-270: the map loaded, mapping off, tracking restarted) on a fresh system; the
+    setting kept across a reset (RebootSystem)"""
+    sc, K, fr = scene_frames(frames=range(0, 8), cols=320, rows=240)
+    mp = dict(MAP, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15)
+    s = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    o = O.System(cols=320, rows=240, K=K, num_buckets=mp["num_buckets"], num_excess=mp["num_excess"],
+                 num_blocks=mp["num_blocks"], voxel_size=VOX)
+    res, ores = [], []
+    for i, f in enumerate(fr[:7]):
+        if i == 3:
+            s.set_mapping(False)
+            o.set_mapping(False)
+            before = rows_set(s.mapping().export_blocks())
+        if i == 6:
+            assert rows_set(s.mapping().export_blocks()) == before  # frozen while mapping is off
+            s.set_mapping(True)
+            o.set_mapping(True)
+        s.GrabImage(*A.to_device(*f))
+        res.append(s.result())
+        ores.append(o.grab(*f))
+    assert [r["track_status"] for r in res] == [st for st, _ in ores] == [4, 1, 1, 1, 1, 1, 1]
+    assert [r["fused"] for r in res] == [1, 1, 1, 0, 0, 0, 1]
+    for r, (st, pose) in zip(res, ores):
+        assert np.abs(r["pose"] - pose).max() < 1e-3
+    # the GPU and the oracle track independently, so a block on the edge of the truncation
+    # band may differ once the poses differ (as in test_pipeline_matches_oracle)
+    gb, ob = rows_set(s.mapping().export_blocks()), rows_set(o.map.export_blocks())
+    assert len(gb & ob) / len(gb | ob) > 0.999
+    # the reference's map loading: the map of frames 0..2 (saved above while frozen) into a
+    # fresh system, mapping off, tracking from the next frame as a first frame
+    path = str(tmp_path / "map.fusn")
+    t0 = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    for f in fr[:3]:
+        t0.GrabImage(*A.to_device(*f))
+    t0.mapping().save(path)
+    blocks = rows_set(t0.mapping().export_blocks())
+    t0.close()
+    t = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    t.mapping().load(path)
+    t.set_mapping(False)
+    for i, f in enumerate(fr[:3]):
+        t.GrabImage(*A.to_device(*f))
+        r = t.result()
+        assert r["track_status"] == (4 if i == 0 else 1) and r["fused"] == 0
+        # tracked against the model of frames 0..2 rather than of the frames before it: the
+        # same place to within a voxel (the raycast surface sits about half a voxel out)
+        assert np.abs(r["pose"] - res[i]["pose"]).max() < 1.5e-2
+    assert rows_set(t.mapping().export_blocks()) == blocks
+    # a reset keeps the setting (RebootSystem, System.cc:272-277)
+    t.reset()
+    t.GrabImage(*A.to_device(*fr[0]))
+    assert t.result()["fused"] == 0 and len(t.mapping().export_blocks()) == 0
+    s.close()
+    t.close()

@@ -71,3 +71,26 @@ def test_missed_count_resets_on_a_tracked_frame():
         assert o.grab(*empty)[0] == 2
     assert o.grab(*fr[2])[0] == 1
     assert o.relocalise(np.eye(4)) == -1
+
+
+def test_mapping_disabled_freezes_the_model():
+    """Tracker::mappingDisabled (System.cc:108-126): with mapping off, frames are tracked and
+    rendered against the model, which does not change (the same blocks, the same voxels);
+    with mapping on again the map grows from the next frame"""
+    sc, K, fr = scene_frames(frames=range(0, 9), cols=320, rows=240, K=(525.0, 525.0, 319.5, 239.5))
+    o = O.System(cols=320, rows=240, K=K, num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15)
+    for f in fr[:3]:
+        assert o.grab(*f)[0] in (1, 4)
+    b0 = o.map.export_blocks()
+    v0 = o.map.read_blocks(b0)
+    o.set_mapping(False)
+    for f in fr[3:6]:
+        assert o.grab(*f)[0] == 1
+    b1 = o.map.export_blocks()
+    assert np.array_equal(b0, b1)
+    v1 = o.map.read_blocks(b1)
+    assert all(np.array_equal(x, y, equal_nan=True) if x.dtype.kind == "f" else np.array_equal(x, y)
+               for x, y in zip(v0, v1))
+    o.set_mapping(True)
+    assert o.grab(*fr[6])[0] == 1
+    assert len(o.map.export_blocks()) > len(b0)
