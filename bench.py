@@ -77,6 +77,7 @@ def parse():
     ap.add_argument("--max-batch", type=int, default=22)
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
+    ap.add_argument("--reloc", type=int, default=1, help="measure keypoint relocalisation on one session (0: skip)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--autotune", choices=["run", "cache", "off"], default="run",
                     help="launch geometry of the batched stages: tune on the served workload after its warm-up "
@@ -455,6 +456,107 @@ def single_stream(A, torch, args):
                          "(captured once); latency = HIP events around upload + replay")
 
 
+def relocalisation(A, torch, n_map=20000, n_lift=8000, reps=5):
+    """keypoint relocalisation on one 640x480 session at 5 mm: a key map of `n_map` keys (the
+    keypoints lifted from a tracked frame at seeded pixels, the rest seeded points), the
+    session lost after ten empty frames, and its lost frame's keypoints (the map's keypoints
+    seen from that frame, descriptors perturbed); no detector, so keypoints and descriptors are
+    synthetic.  Times (ms, median of `reps` where the call leaves no state): key insert and
+    collect, the 2-NN matcher (and its arithmetic rate), the default relocalisation, the
+    graph-matching candidates, and ValidatePose on the lost frame (once: it relocalises)."""
+    sc = A.Scene(0, (4.0, 4.0, 3.0), 7)
+    fr = [sc.render(i, 640, 480, K640, 5000.0, True) for i in range(8)]
+    params = lambda: A.system_params(map=MAP_SESSION)
+    ref = A.System(params())
+    poses = []
+    for f in fr[:7]:
+        ref.GrabImage(*A.to_device(*f))
+        poses.append(ref.result()["pose"])
+    # keypoints of frame 3 lifted on its level-0 depth and normals
+    s = A.System(params())
+    for f in fr[:4]:
+        s.GrabImage(*A.to_device(*f))
+    rng = np.random.default_rng(3)
+    xy = np.stack([rng.uniform(4, 636, n_lift), rng.uniform(4, 476, n_lift)], 1).astype(np.float32)
+    p3, n3, ok3 = A.lift_keypoints(torch.from_numpy(xy).cuda(), s.view("depth"), s.view("nmap"), K640)
+    ok3 = ok3.cpu().numpy() == 1
+    p3, n3 = p3.cpu().numpy()[ok3][:n_map].astype(np.float64), n3.cpu().numpy()[ok3][:n_map]
+    P3 = poses[3]
+    keys = np.zeros(n_map, A.SURF_DTYPE)
+    keys["valid"] = 1
+    nk = len(p3)
+    keys["pos"][:nk] = p3 @ P3[:3, :3].T + P3[:3, 3]
+    keys["normal"][:nk, :3] = n3[:, :3].astype(np.float64) @ P3[:3, :3].T
+    keys["normal"][:nk, 3] = n3[:, 3]
+    keys["pos"][nk:] = rng.uniform([-3, -3, 0.3], [3, 3, 4], (n_map - nk, 3))
+    nr = rng.normal(size=(n_map - nk, 3))
+    keys["normal"][nk:, :3] = nr / np.linalg.norm(nr, axis=1, keepdims=True)
+    keys["descriptor"] = rng.normal(size=(n_map, 64)).astype(np.float32)
+    kd = A.surf_to_device(keys)
+    km = A.KeyMap(max_keys=100003, n_buckets=5)
+
+    def timed(fn, n):
+        out, ts = None, []
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(1e3 * (time.perf_counter() - t0))
+        return out, round(float(np.median(ts)), 3)
+
+    def insert():
+        km.ResetKeyPoints()
+        km.InsertKeyPoints(kd, torch.full((n_map,), -1, dtype=torch.int32, device="cuda"))
+
+    _, t_insert = timed(insert, reps)
+    (rec_dev, n_rec), t_collect = timed(lambda: km.CollectKeyPoints(n_map + 8), reps)
+    # the session lost: frames 4, 5, ten empty frames, frame 6
+    empty = (np.zeros((480, 640), np.uint16), np.zeros((480, 640, 3), np.uint8))
+    for f in list(fr[4:6]) + [empty] * 10 + [fr[6]]:
+        s.GrabImage(*A.to_device(*f))
+    lost = s.result()["track_status"] == 5
+    # frame 6's keypoints: the frame-3 keys projected with frame 6's tracked pose, lifted there
+    P6 = poses[6]
+    c6 = (keys["pos"][:nk].astype(np.float64) - P6[:3, 3]) @ P6[:3, :3]
+    u = c6[:, 0] / c6[:, 2] * K640[0] + K640[2]
+    v = c6[:, 1] / c6[:, 2] * K640[1] + K640[3]
+    inside = np.nonzero((c6[:, 2] > 0.3) & (u > 3) & (u < 636) & (v > 3) & (v < 476))[0]
+    xy6 = torch.from_numpy(np.stack([u[inside], v[inside]], 1).astype(np.float32)).cuda()
+    p6, n6, ok6 = A.lift_keypoints(xy6, s.view("depth"), s.view("nmap"), K640)
+    sel = ok6.cpu().numpy() == 1
+    q = inside[sel]
+    desc = (keys["descriptor"][q] + rng.normal(0, 0.05, (len(q), 64))).astype(np.float32)
+    dq = torch.from_numpy(desc).cuda()
+    pos6, nrm6 = p6.cpu().numpy()[sel], n6.cpu().numpy()[sel]
+    train = torch.from_numpy(np.ascontiguousarray(A.surf_from_device(rec_dev)[:n_rec]["descriptor"])).cuda()
+    _, t_match = timed(lambda: A.match_knn2(dq, train), reps)
+    (pose_d, npairs), t_default = timed(lambda: km.relocalise(dq, pos6, iterations=100, seed=1), reps)
+    (cands, nmatch), t_cand = timed(lambda: km.graph_candidates(dq, pos6, nrm6, iterations=100, seed=1), reps)
+    okv, acc, t_validate = False, [], None
+    if len(cands):
+        (okv, acc, _), t_validate = timed(lambda: s.validate_poses(cands), 1)
+    err = None
+    if okv:
+        g = s.result()["pose"]
+        err = {"translation_m": round(float(np.abs(g[:3, 3] - P6[:3, 3]).max()), 5),
+               "rotation_max_abs": round(float(np.abs(g[:3, :3] - P6[:3, :3]).max()), 5)}
+    flops = 3.0 * 64 * len(q) * n_rec
+    out = dict(workload="640x480 at 5 mm; %d map keys, %d frame keypoints (synthetic positions and descriptors; "
+                        "no detector)" % (n_rec, len(q)),
+               lost=bool(lost), ms={"reset_and_insert_keys": t_insert, "collect_keys": t_collect, "match_knn2": t_match,
+                                    "relocalise_default": t_default, "graph_candidates": t_cand,
+                                    "validate_poses": t_validate},
+               match_tflops=round(flops / (t_match * 1e-3) / 1e12, 2),
+               default_pairs=npairs, default_found=pose_d is not None, graph_matches=nmatch,
+               candidates=len(cands), accepted=[int(a) for a in acc], relocalised=bool(okv),
+               pose_error_vs_tracked=err)
+    km.close()
+    s.close()
+    ref.close()
+    return out
+
+
 def ingest(A, torch, n=120):
     """recorded-sequence replay: 640x480 frames of room 0 written in the TUM layout (16-bit
     depth PNG + RGB PNG), then decoded on the host (8 threads ahead of the GPU), copied to
@@ -666,12 +768,14 @@ def main():
 
     single = single_stream(A, torch, args) if args.single_steps > 0 else None
     replay = ingest(A, torch) if args.single_steps > 0 else None
+    reloc = relocalisation(A, torch) if args.reloc else None
+    if reloc:
+        log("relocalisation: %s" % reloc["ms"])
     cpu = cpu_baseline(A, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     fps = r["tracked"] / r["wall"]
     st = r.get("stages")
     rl = rooflines(st, sizes, ns) if st else None
-    # the single kernel with the largest time per step (the ICP entry is a stage of many launches)
     # the kernel with the most GPU time per step: the ICP reduction (kb_icp with its final sum,
     # one launch pair per iteration) ahead of the raycast and the integration
     dominant = max(("integrate", "raycast", "icp"), key=lambda k: rl[k]["avg_launch_us"]) if rl else None
@@ -711,6 +815,7 @@ def main():
         "single_stream": single,
         "rgbd_tracking": rgbd,
         "recorded_replay": replay,
+        "relocalisation": reloc,
         "stage_ms_per_step": ({k: round(v / steps_p, 4) for k, v in st["ms"].items()} if st else None),
         "map": {"n_allocated_max": r["n_allocated_max"], "n_allocated_mean": round(r["n_allocated_mean"], 1),
                 "flags": r["flags"]},

@@ -115,34 +115,45 @@ __global__ void k_lift_keys(LiftArgs a) {
 // The two nearest train descriptors of every query descriptor by L2 distance (sqrt of the sum
 // of the 64 squared differences, in dimension order); ties go to the lower train index.
 // Work-group: 256 queries, one per lane (its descriptor in registers), against a chunk of
-// kChunk train descriptors staged through LDS kTile at a time (every lane reads the same
-// record: LDS broadcast).  Each (query, chunk) pair leaves its best two; a second pass merges
-// the chunks in order.
+// train descriptors staged through LDS kTile at a time (every lane reads the same record: LDS
+// broadcast).  Each (query, chunk) pair leaves its best two; a second pass merges the chunks.
+// The chunk length is chosen per call so that the grid holds about kTargetGroups work-groups
+// (a few hundred queries against tens of thousands of keys would otherwise fill a dozen CUs).
+// Both passes keep the two smallest (distance, index) pairs in lexicographic order, a total
+// order over the candidates (NaN and +inf distances never enter), so the result is the same for
+// every chunking and merge order: that of one lane scanning the train set in index order with
+// strict comparisons.
 constexpr int kDesc = 64;
 constexpr int kTile = 128;
 constexpr int kChunk = 8192;
+constexpr int kTargetGroups = 4096;
 
 struct Best2 {
     float d0, d1;
     int i0, i1;
 };
 
+__device__ __forceinline__ bool lex_less(float d, int i, float e, int j) { return d < e || (d == e && i < j); }
+
+// (d, i) into the best two; the empty slots are (+inf, -1), which no candidate with a finite
+// distance ties
 __device__ __forceinline__ void push2(Best2& b, float d, int i) {
-    if (d < b.d0) {
+    if (lex_less(d, i, b.d0, b.i0)) {
         b.d1 = b.d0;
         b.i1 = b.i0;
         b.d0 = d;
         b.i0 = i;
-    } else if (d < b.d1) {
+    } else if (lex_less(d, i, b.d1, b.i1)) {
         b.d1 = d;
         b.i1 = i;
     }
 }
 
-__global__ __launch_bounds__(256) void k_match_chunk(const float* q, int nq, const float* t, int nt, Best2* part) {
+__global__ __launch_bounds__(256) void k_match_chunk(const float* q, int nq, const float* t, int nt, int chunk,
+                                                     Best2* part) {
     __shared__ float tile[kTile * kDesc];
     const int qi = blockIdx.x * 256 + threadIdx.x;
-    const int c0 = blockIdx.y * kChunk, c1 = min(nt, c0 + kChunk);
+    const int c0 = blockIdx.y * chunk, c1 = min(nt, c0 + chunk);
     float mine[kDesc];
 #pragma unroll
     for (int j = 0; j < kDesc; ++j) mine[j] = qi < nq ? q[(size_t)qi * kDesc + j] : 0.f;
@@ -159,25 +170,47 @@ __global__ __launch_bounds__(256) void k_match_chunk(const float* q, int nq, con
                 const float dj = mine[j] - tile[r * kDesc + j];
                 s = s + dj * dj;
             }
-            push2(b, sqrtf(s), base + r);
+            const float d = sqrtf(s);
+            if (d < b.d1) push2(b, d, base + r);  // indices rise within the chunk: d < d1 is the test
         }
     }
     if (qi < nq) part[(size_t)blockIdx.y * nq + qi] = b;
 }
 
-__global__ void k_match_merge(const Best2* part, int nq, int nchunks, int* idx, float* dist) {
-    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+// one wave per query: lane l merges chunks l, l + 64, ..., then the lanes' best two are merged
+// across the wave
+__global__ __launch_bounds__(256) void k_match_merge(const Best2* part, int nq, int nchunks, int* idx, float* dist) {
+    const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (qi >= nq) return;
     Best2 b{INFINITY, INFINITY, -1, -1};
-    for (int c = 0; c < nchunks; ++c) {  // chunks in train order: an earlier index wins a tie
+    for (int c = lane; c < nchunks; c += 64) {
         const Best2 p = part[(size_t)c * nq + qi];
         if (p.i0 >= 0) push2(b, p.d0, p.i0);
         if (p.i1 >= 0) push2(b, p.d1, p.i1);
     }
-    idx[2 * qi] = b.i0;
-    idx[2 * qi + 1] = b.i1;
-    dist[2 * qi] = b.d0;
-    dist[2 * qi + 1] = b.d1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float d0 = __shfl_xor(b.d0, off), d1 = __shfl_xor(b.d1, off);
+        const int i0 = __shfl_xor(b.i0, off), i1 = __shfl_xor(b.i1, off);
+        if (i0 >= 0) push2(b, d0, i0);
+        if (i1 >= 0) push2(b, d1, i1);
+    }
+    if (lane == 0) {
+        idx[2 * qi] = b.i0;
+        idx[2 * qi + 1] = b.i1;
+        dist[2 * qi] = b.d0;
+        dist[2 * qi + 1] = b.d1;
+    }
+}
+
+// train records per chunk: about kTargetGroups work-groups in all, at least one LDS tile and at
+// most kChunk records each, a whole number of tiles
+static int match_chunk_len(int nq, int nt) {
+    const long long qgroups = (nq + 255) / 256;
+    long long want = ((long long)nt * qgroups + kTargetGroups - 1) / kTargetGroups;
+    want = (want + kTile - 1) / kTile * kTile;
+    return (int)std::min<long long>(kChunk, std::max<long long>(kTile, want));
 }
 
 // ---------------------------------------------------------------------------- graph selection
@@ -453,15 +486,17 @@ extern "C" fu_status fu_match_knn2(const float* query, int32_t nq, const float* 
     if ((long long)nq * kDesc > (1ll << 31) || (long long)nt * kDesc > (1ll << 31))
         return api_error(FU_ERR_ARG, "fu_match_knn2: %d x %d descriptors (at most 2^25 each)", nq, nt);
     hipStream_t hs = (hipStream_t)s;
-    const int nchunks = nt > 0 ? (nt + kChunk - 1) / kChunk : 1;
+    const int chunk = match_chunk_len(nq, nt);
+    const int nchunks = nt > 0 ? (nt + chunk - 1) / chunk : 1;
     Best2* part = nullptr;
     if (hipMallocAsync((void**)&part, sizeof(Best2) * (size_t)nchunks * nq, hs) != hipSuccess) {
         (void)hipGetLastError();
         return api_error(FU_ERR_NOMEM, "fu_match_knn2: %d partial lists", nchunks);
     }
     if (nt > 0)
-        hipLaunchKernelGGL(k_match_chunk, dim3((nq + 255) / 256, nchunks), dim3(256), 0, hs, query, nq, train, nt, part);
-    hipLaunchKernelGGL(k_match_merge, dim3((nq + 255) / 256), dim3(256), 0, hs, part, nq, nt > 0 ? nchunks : 0, idx,
+        hipLaunchKernelGGL(k_match_chunk, dim3((nq + 255) / 256, nchunks), dim3(256), 0, hs, query, nq, train, nt, chunk,
+                           part);
+    hipLaunchKernelGGL(k_match_merge, dim3((nq + 3) / 4), dim3(256), 0, hs, part, nq, nt > 0 ? nchunks : 0, idx,
                        dist);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipFreeAsync(part, hs);

@@ -36,23 +36,31 @@ def test_lift_keypoints_bit_exact(gpu):
     assert not ok[[-8, -6, -5, -3, -2, -1]].any()
 
 
-@pytest.mark.parametrize("nq,nt", [(700, 20000), (5, 1), (3, 0), (300, 8192), (1, 8193)])
+@pytest.mark.parametrize("nq,nt", [(700, 20000), (5, 1), (3, 0), (300, 8192), (1, 8193), (4000, 40000)])
 def test_match_knn2_matches_oracle(gpu, nq, nt):
+    """the train set is cut into chunks whose length depends on the sizes (128 records for
+    (700, 20000) and (1, 8193), 256 for (4000, 40000)); ties are planted across chunks, and
+    across chunks that different lanes of the merge hold (chunk c goes to lane c % 64)"""
     rng = np.random.default_rng(nq + nt)
     t = rng.normal(size=(max(nt, 1), 64)).astype(np.float32)[:nt]
-    if nt > 8200:  # a duplicate in the next chunk: the lower index wins the tie
+    if nt > 8200:  # duplicates later in the train set: the lower index wins the tie
         t[8192 + 5] = t[5]
+        t[300] = t[9001] = t[7]
     src = rng.integers(0, max(nt, 1), nq)
     q = (t[src] + rng.normal(0, 0.05, (nq, 64))).astype(np.float32) if nt else rng.normal(size=(nq, 64)).astype(
         np.float32)
     if nt > 8200:
         q[0] = t[5]
+        if nq > 1:
+            q[1] = t[7]
     idx, dist = A.match_knn2(torch.from_numpy(q).cuda(), torch.from_numpy(t).cuda() if nt else None)
     io, do = O.match_knn2(q, t)
     assert np.array_equal(idx.cpu().numpy(), io)
     assert np.array_equal(dist.cpu().numpy().view(np.uint32), do.view(np.uint32))
     if nt > 8200:
         assert io[0, 0] == 5 and io[0, 1] == 8197 and do[0, 0] == 0
+        if nq > 1:
+            assert io[1, 0] == 7 and io[1, 1] == 300 and do[1, 1] == 0
 
 
 def test_relocalise_recovers_the_pose(gpu):
