@@ -78,6 +78,9 @@ def parse():
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-oracle baseline (0: skip)")
     ap.add_argument("--reloc", type=int, default=1, help="measure keypoint relocalisation on one session (0: skip)")
+    ap.add_argument("--reloc-only", action="store_true",
+                    help="only the keypoint relocalisation measurement (one JSON object; the program its kernel "
+                         "trace profiles)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--autotune", choices=["run", "cache", "off"], default="run",
                     help="launch geometry of the batched stages: tune on the served workload after its warm-up "
@@ -712,6 +715,9 @@ def main():
 
     if args.graph_ab > 0:
         graph_ab(A, torch, args)
+        return
+    if args.reloc_only:
+        print(json.dumps({"relocalisation": relocalisation(A, torch)}), flush=True)
         return
 
     ns = args.sessions

@@ -115,16 +115,39 @@ __global__ void k_collect_count(const Surf* keys, int n, unsigned* counts) {
     __syncthreads();
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
-__global__ void k_collect_prefix(unsigned* counts, int groups, unsigned* total) {
-    // one lane walks the group counts (at most a few thousand)
-    if (threadIdx.x != 0) return;
-    unsigned s = 0;
-    for (int g = 0; g < groups; ++g) {
-        const unsigned c = counts[g];
-        counts[g] = s;
-        s += c;
+// the exclusive prefix of the group counts, in place, and their total: one work-group of
+// kPrefixLanes lanes, lane t owning a run of consecutive counts, the runs' sums scanned across
+// the work-group (integer sums: the same as one lane walking the counts in order)
+constexpr int kPrefixLanes = 1024;
+__global__ __launch_bounds__(kPrefixLanes) void k_collect_prefix(unsigned* counts, int groups, unsigned* total) {
+    __shared__ unsigned wsum[kPrefixLanes / 64];
+    const int per = (groups + kPrefixLanes - 1) / kPrefixLanes;
+    const int b = min(groups, (int)threadIdx.x * per), e = min(groups, b + per);
+    unsigned own = 0;
+    for (int g = b; g < e; ++g) own += counts[g];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = own;  // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(x, d);
+        if (lane >= d) x += y;
     }
-    *total = s;
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < kPrefixLanes / 64; ++k) {
+        const unsigned c = wsum[k];
+        before += k < w ? c : 0u;
+        all += c;
+    }
+    unsigned run = before + x - own;
+    for (int g = b; g < e; ++g) {
+        const unsigned c = counts[g];
+        counts[g] = run;
+        run += c;
+    }
+    if (threadIdx.x == 0) *total = all;
 }
 __global__ void k_collect_scatter(const Surf* keys, int n, const unsigned* base, Surf* out, unsigned cap) {
     __shared__ unsigned wave_base[kCollect / 64];
@@ -320,7 +343,7 @@ extern "C" fu_status fu_collect_key_points(fu_keymap* k, fu_surf* out, uint32_t 
     hipStream_t hs = (hipStream_t)s;
     const int n = n_slots(k), g = n_groups(k);
     hipLaunchKernelGGL(k_collect_count, dim3(g), dim3(kCollect), 0, hs, (const Surf*)k->a.keys, n, k->aux + 2);
-    hipLaunchKernelGGL(k_collect_prefix, dim3(1), dim3(64), 0, hs, k->aux + 2, g, k->aux + 1);
+    hipLaunchKernelGGL(k_collect_prefix, dim3(1), dim3(kPrefixLanes), 0, hs, k->aux + 2, g, k->aux + 1);
     if (cap) hipLaunchKernelGGL(k_collect_scatter, dim3(g), dim3(kCollect), 0, hs, (const Surf*)k->a.keys, n, k->aux + 2,
                                 (Surf*)out, cap);
     KM_HIP(hipGetLastError());

@@ -217,46 +217,71 @@ static int match_chunk_len(int nq, int nt) {
 // Tracker::GenerateGraph's choice of mutually consistent pairs (Tracking.cc:515-596) over the
 // consistency matrix C (row-major n x n, C[a * n + b] = conMatrix(a, b)), without reading the
 // matrix back to the host as the reference does:
-//   k_rank_sums   the column sums (cv::cuda::reduce over rows): one lane per column, the rows
-//                 in order, summed in double and rounded to float (OpenCV's order is its own);
-//   k_rank_order  the pairs by rank, largest first (cv::sortIdx descending): each lane counts
-//                 the ranks above its own, ties going to the lower index (std::sort leaves
-//                 them in no fixed order), against LDS tiles of the ranks; a NaN rank counts
-//                 as -inf, so the positions are a permutation;
+//   k_rank_sums   the column sums (cv::cuda::reduce over rows): each column's rows in order,
+//                 summed in double and rounded to float (OpenCV's order is its own).  A
+//                 work-group takes 64 columns: all its lanes stage tiles of kSumRows rows in
+//                 LDS, and one wave adds them, a lane per column;
+//   k_rank_order  the pairs by rank, largest first (cv::sortIdx descending): a pair's position
+//                 is the number of ranks above its own, ties going to the lower index
+//                 (std::sort leaves them in no fixed order), counted against LDS tiles of the
+//                 ranks by four waves a quarter each and added; a NaN rank counts as -inf, so
+//                 the positions are a permutation;
 //   k_subgraphs   one 1024-lane work-group per list: the head and the pairs ranked after it
 //                 with C[head][pair] > min_score, kept in rank order by a block-wide ballot
-//                 prefix until max_sel are selected; then, one lane per selected pair, the
-//                 refinement test against every later pair of the list.
+//                 prefix until max_sel are selected; then the refinement test of each selected
+//                 pair against every later pair of the list, one wave per pair, its lanes
+//                 over the later pairs (the test is an "any", so their order is free).
 constexpr int kSelLanes = 1024;
 constexpr int kMaxSelect = 1024;
 
-__global__ void k_rank_sums(const float* C, int n, float* rank) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    double s = 0;
-    for (int y = 0; y < n; ++y) s += (double)C[(size_t)y * n + x];
-    rank[x] = (float)s;
+constexpr int kSumRows = 128;  // rows per LDS tile of k_rank_sums (32 KB)
+
+// launch: ceil(n / 64) work-groups of 256 lanes
+__global__ __launch_bounds__(256) void k_rank_sums(const float* C, int n, float* rank) {
+    __shared__ float tile[kSumRows][64];
+    const int tid = threadIdx.x, col = tid & 63;
+    const int c = blockIdx.x * 64 + col;
+    double s = 0;  // wave 0: column c
+    for (int r0 = 0; r0 < n; r0 += kSumRows) {
+        const int nr = min(kSumRows, n - r0);
+        __syncthreads();
+        // lane (w, col) loads rows w, w + 4, ... of its column: each wave reads 64 consecutive
+        // floats of a row per load
+#pragma unroll 8
+        for (int r = tid >> 6; r < kSumRows; r += 4)
+            tile[r][col] = (r < nr && c < n) ? C[(size_t)(r0 + r) * n + c] : 0.f;
+        __syncthreads();
+        if (tid < 64)
+            for (int r = 0; r < nr; ++r) s += (double)tile[r][col];
+    }
+    if (tid < 64 && c < n) rank[c] = (float)s;
 }
 
 __device__ __forceinline__ float rank_key(float v) { return isnan(v) ? -INFINITY : v; }
 
+// launch: ceil(n / 64) work-groups of 256 lanes; lane (w, l) counts for pair 64 * group + l
+// against entries w * 256 .. w * 256 + 255 of every 1024-rank tile
 __global__ __launch_bounds__(256) void k_rank_order(const float* rank, int n, int* order) {
-    __shared__ float tile[256];
-    const int x = blockIdx.x * 256 + threadIdx.x;
+    __shared__ float tile[1024];
+    __shared__ int part[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
     const float kx = x < n ? rank_key(rank[x]) : 0.f;
     int pos = 0;
-    for (int base = 0; base < n; base += 256) {
+    for (int base = 0; base < n; base += 1024) {
         __syncthreads();
-        const int y = base + threadIdx.x;
-        tile[threadIdx.x] = y < n ? rank_key(rank[y]) : 0.f;
+        for (int k = threadIdx.x; k < 1024; k += 256) tile[k] = base + k < n ? rank_key(rank[base + k]) : 0.f;
         __syncthreads();
-        const int m = min(256, n - base);
-        for (int k = 0; k < m; ++k) {
+        const int lo = w * 256, hi = min(n - base, lo + 256);
+#pragma unroll 8
+        for (int k = lo; k < hi; ++k) {
             const float ky = tile[k];
             pos += (ky > kx) || (ky == kx && base + k < x);
         }
     }
-    if (x < n) order[pos] = x;
+    part[w][lane] = pos;
+    __syncthreads();
+    if (w == 0 && x < n) order[part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]] = x;
 }
 
 struct SelArgs {
@@ -327,23 +352,27 @@ __global__ __launch_bounds__(kSelLanes) void k_subgraphs(SelArgs a) {
         if (tid == 0) a.n_refined[i] = 0;
         return;
     }
-    // lane k keeps the k-th pair unless a later one is nearly exclusive with it and scores
-    // higher with the head (:567-582)
-    bool keep = false;
-    if (tid >= 1 && tid < count) {
-        const int av = sel[tid];
-        const float ha = hs[tid];
+    // the t-th pair is kept unless a later one is nearly exclusive with it and scores higher
+    // with the head (:567-582); the head (t = 0) is never kept.  Wave w tests pairs w + 1,
+    // w + 17, ..., its lanes over the later pairs 64 at a time.
+    __shared__ unsigned char kept[kMaxSelect];
+    const int lane = tid & 63, wid = tid >> 6;
+    for (int t = 1 + wid; t < count; t += kSelLanes / 64) {
+        const int av = sel[t];
+        const float ha = hs[t];
         const float* Ca = a.C + (size_t)av * a.n;
-        keep = true;
-        for (int l = tid + 1; l < count; ++l) {
-            if (!(hs[l] > ha)) continue;
+        bool drop = false;
+#pragma unroll 4
+        for (int l = t + 1 + lane; l < count; l += 64) {
             const int b = sel[l];
-            if (Ca[b] < 5e-3f || a.C[(size_t)b * a.n + av] < 5e-3f) {
-                keep = false;
-                break;
-            }
+            const bool x = Ca[b] < 5e-3f || a.C[(size_t)b * a.n + av] < 5e-3f;
+            drop = drop || (hs[l] > ha && x);
         }
+        if (lane == 0) kept[t] = 0;
+        if (__ballot(drop) == 0ull && lane == 0) kept[t] = 1;
     }
+    __syncthreads();
+    const bool keep = tid >= 1 && tid < count && kept[tid];
     int total;
     const int slot = block_prefix(keep, wsum, total);
     if (keep) a.refined[(size_t)i * a.max_sel + slot] = sel[tid];
@@ -613,38 +642,58 @@ extern "C" fu_status fu_pose_estimate(const double* src, const double* ref, int3
 }
 
 namespace {
+// record j of `out` = keys[idx[j]], or zeros where idx[j] < 0 (a query without a neighbour);
+// one lane per 16 bytes
+__global__ void k_gather_keys(const fu_surf* keys, const int32_t* idx, int n, fu_surf* out) {
+    constexpr int Q = (int)(sizeof(fu_surf) / 16);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * Q) return;
+    const int j = t / Q, q = t % Q;
+    const int k = idx[j];
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (k >= 0) v = reinterpret_cast<const uint4*>(&keys[k])[q];
+    reinterpret_cast<uint4*>(&out[j])[q] = v;
+}
+static_assert(sizeof(fu_surf) % 16 == 0, "records move as 16-byte words");
+
 // the map's keys (UpdateMapKeys, Mapping.cc:142-151; the first record is never written by
 // CollectKeyPoints and reads as zeros here) and the two nearest of them to each frame
-// descriptor, on the host
-fu_status collect_and_match(fu_keymap* km, const float* frame_desc, int32_t nq, std::vector<fu_surf>& host,
-                            std::vector<int32_t>& hidx, std::vector<float>& hdist, fu_stream s, const char* who) {
+// descriptor: `nk` map keys; on the host, the neighbours' indices and distances (2 per query,
+// -1 / +inf where the map has fewer keys) and their records in the same slots (zeros for -1).
+// Device buffers are stream-ordered allocations; only the matched records come back.
+fu_status collect_and_match(fu_keymap* km, const float* frame_desc, int32_t nq, uint32_t& nk,
+                            std::vector<fu_surf>& matched, std::vector<int32_t>& hidx, std::vector<float>& hdist,
+                            fu_stream s, const char* who) {
     hipStream_t hs = (hipStream_t)s;
-    uint32_t nk = 0;
+    nk = 0;
     fu_status st = fu_collect_key_points(km, nullptr, 0, &nk, s);
     if (st) return st;
-    host.clear();
+    matched.clear();
     if (nk == 0 || nq == 0) return FU_OK;  // no map keys (Tracking.cc:417-418)
-    fu_surf* keys = nullptr;
+    fu_surf *keys = nullptr, *mrec = nullptr;
     float* train = nullptr;
     int32_t* idx = nullptr;
     float* dist = nullptr;
     auto release = [&]() {
-        (void)hipFree(keys);
-        (void)hipFree(train);
-        (void)hipFree(idx);
-        (void)hipFree(dist);
+        if (keys) (void)hipFreeAsync(keys, hs);
+        if (mrec) (void)hipFreeAsync(mrec, hs);
+        if (train) (void)hipFreeAsync(train, hs);
+        if (idx) (void)hipFreeAsync(idx, hs);
+        if (dist) (void)hipFreeAsync(dist, hs);
     };
-    if (hipMalloc((void**)&keys, sizeof(fu_surf) * nk) != hipSuccess ||
-        hipMalloc((void**)&train, sizeof(float) * 64 * nk) != hipSuccess ||
-        hipMalloc((void**)&idx, sizeof(int32_t) * 2 * nq) != hipSuccess ||
-        hipMalloc((void**)&dist, sizeof(float) * 2 * nq) != hipSuccess) {
+    const int nm = 2 * nq;
+    if (hipMallocAsync((void**)&keys, sizeof(fu_surf) * nk, hs) != hipSuccess ||
+        hipMallocAsync((void**)&mrec, sizeof(fu_surf) * nm, hs) != hipSuccess ||
+        hipMallocAsync((void**)&train, sizeof(float) * 64 * nk, hs) != hipSuccess ||
+        hipMallocAsync((void**)&idx, sizeof(int32_t) * nm, hs) != hipSuccess ||
+        hipMallocAsync((void**)&dist, sizeof(float) * nm, hs) != hipSuccess) {
         (void)hipGetLastError();
         release();
         return api_error(FU_ERR_NOMEM, "%s: %u keys", who, nk);
     }
-    host.resize(nk);
-    hidx.resize(2 * (size_t)nq);
-    hdist.resize(2 * (size_t)nq);
+    matched.resize(nm);
+    hidx.resize(nm);
+    hdist.resize(nm);
     hipError_t e = hipMemsetAsync(keys, 0, sizeof(fu_surf) * nk, hs);
     if (e == hipSuccess) {
         uint32_t again = 0;
@@ -655,11 +704,16 @@ fu_status collect_and_match(fu_keymap* km, const float* frame_desc, int32_t nq, 
         e = hipMemcpy2DAsync(train, 64 * sizeof(float), (const char*)keys + offsetof(fu_surf, descriptor), sizeof(fu_surf),
                              64 * sizeof(float), nk, hipMemcpyDeviceToDevice, hs);
     if (!st && e == hipSuccess) st = fu_match_knn2(frame_desc, nq, train, (int32_t)nk, idx, dist, s);
-    if (!st && e == hipSuccess) e = hipMemcpyAsync(host.data(), keys, sizeof(fu_surf) * nk, hipMemcpyDeviceToHost, hs);
-    if (!st && e == hipSuccess) e = hipMemcpyAsync(hidx.data(), idx, sizeof(int32_t) * 2 * nq, hipMemcpyDeviceToHost, hs);
-    if (!st && e == hipSuccess) e = hipMemcpyAsync(hdist.data(), dist, sizeof(float) * 2 * nq, hipMemcpyDeviceToHost, hs);
-    if (!st && e == hipSuccess) e = hipStreamSynchronize(hs);
+    if (!st && e == hipSuccess) {
+        const int lanes = nm * (int)(sizeof(fu_surf) / 16);
+        hipLaunchKernelGGL(k_gather_keys, dim3((lanes + 255) / 256), dim3(256), 0, hs, keys, idx, nm, mrec);
+        e = hipGetLastError();
+    }
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(matched.data(), mrec, sizeof(fu_surf) * nm, hipMemcpyDeviceToHost, hs);
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(hidx.data(), idx, sizeof(int32_t) * nm, hipMemcpyDeviceToHost, hs);
+    if (!st && e == hipSuccess) e = hipMemcpyAsync(hdist.data(), dist, sizeof(float) * nm, hipMemcpyDeviceToHost, hs);
     release();
+    if (!st && e == hipSuccess) e = hipStreamSynchronize(hs);
     if (st) return st;
     if (e != hipSuccess) return api_error(FU_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
     return FU_OK;
@@ -689,18 +743,19 @@ extern "C" fu_status fu_relocalise_keys(fu_keymap* km, const float* frame_desc, 
     *ok = 0;
     if (n_pairs) *n_pairs = 0;
     for (int k = 0; k < 16; ++k) pose[k] = (k % 5 == 0) ? 1.0 : 0.0;
-    std::vector<fu_surf> host;
+    std::vector<fu_surf> matched;
     std::vector<int32_t> hidx;
     std::vector<float> hdist;
-    fu_status st = collect_and_match(km, frame_desc, nq, host, hidx, hdist, s, "fu_relocalise_keys");
+    uint32_t nk = 0;
+    fu_status st = collect_and_match(km, frame_desc, nq, nk, matched, hidx, hdist, s, "fu_relocalise_keys");
     if (st) return st;
-    if (host.empty()) return FU_OK;
+    if (matched.empty()) return FU_OK;
     // the ratio test (Tracking.cc:437-444); with one map key the second distance is infinite
     std::vector<double> src, ref;
     for (int i = 0; i < nq; ++i) {
         if (hidx[2 * i] < 0) continue;
         if ((double)hdist[2 * i] < 0.90 * (double)hdist[2 * i + 1]) {
-            const fu_surf& k = host[hidx[2 * i]];
+            const fu_surf& k = matched[2 * i];
             src.insert(src.end(), {frame_pos[3 * i], frame_pos[3 * i + 1], frame_pos[3 * i + 2]});
             ref.insert(ref.end(), {(double)k.pos[0], (double)k.pos[1], (double)k.pos[2]});
         }
@@ -728,8 +783,8 @@ extern "C" fu_status fu_select_subgraphs(const float* matrix, int32_t n, int32_t
         return api_error(FU_ERR_ARG, "fu_select_subgraphs: null argument");
     hipStream_t hs = (hipStream_t)s;
     if (n > 0) {
-        hipLaunchKernelGGL(k_rank_sums, dim3((n + 255) / 256), dim3(256), 0, hs, matrix, n, rank_sum);
-        hipLaunchKernelGGL(k_rank_order, dim3((n + 255) / 256), dim3(256), 0, hs, rank_sum, n, order);
+        hipLaunchKernelGGL(k_rank_sums, dim3((n + 63) / 64), dim3(256), 0, hs, matrix, n, rank_sum);
+        hipLaunchKernelGGL(k_rank_order, dim3((n + 63) / 64), dim3(256), 0, hs, rank_sum, n, order);
     }
     const SelArgs a{matrix, order, n, max_select, min_score, n_selected, n_refined, refined};
     hipLaunchKernelGGL(k_subgraphs, dim3(n_lists), dim3(kSelLanes), 0, hs, a);
@@ -758,21 +813,23 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
     constexpr float kMinScore = 0.1f;
     *n_poses = 0;
     if (n_pairs) *n_pairs = 0;
-    std::vector<fu_surf> host;
+    std::vector<fu_surf> matched;
     std::vector<int32_t> hidx;
     std::vector<float> hdist;
-    fu_status st = collect_and_match(km, frame_desc, nq, host, hidx, hdist, s, "fu_relocalise_graph_candidates");
+    uint32_t nk = 0;
+    fu_status st =
+        collect_and_match(km, frame_desc, nq, nk, matched, hidx, hdist, s, "fu_relocalise_graph_candidates");
     if (st) return st;
-    if (host.empty()) return FU_OK;
+    if (matched.empty()) return FU_OK;
     // the match list (Tracking.cc:437-444), then the valid map keys (:485-505)
-    std::vector<std::pair<int, int>> refd;  // (query, train)
+    std::vector<std::pair<int, int>> refd;  // (query, neighbour slot 2 * query + 0 or 1)
     std::vector<float> rdist;
     for (int i = 0; i < nq; ++i) {
         if (hidx[2 * i] < 0) continue;
-        refd.push_back({i, hidx[2 * i]});
+        refd.push_back({i, 2 * i});
         rdist.push_back(hdist[2 * i]);
         if (!((double)hdist[2 * i] < 0.90 * (double)hdist[2 * i + 1]) && hidx[2 * i + 1] >= 0) {
-            refd.push_back({i, hidx[2 * i + 1]});
+            refd.push_back({i, 2 * i + 1});
             rdist.push_back(hdist[2 * i + 1]);
         }
     }
@@ -781,7 +838,7 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
     std::vector<fu_surf> fk, mk;
     std::vector<float> dd;
     for (size_t r = 0; r < refd.size(); ++r) {
-        const fu_surf& key = host[refd[r].second];
+        const fu_surf& key = matched[refd[r].second];
         if (!key.valid) continue;
         fu_surf f;
         memset(&f, 0, sizeof(f));
@@ -800,22 +857,17 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
     float *ddist = nullptr, *mat = nullptr, *rank = nullptr;
     int32_t *order = nullptr, *nsel = nullptr;
     auto release = [&]() {
-        (void)hipFree(dfk);
-        (void)hipFree(dmk);
-        (void)hipFree(ddist);
-        (void)hipFree(mat);
-        (void)hipFree(rank);
-        (void)hipFree(order);
-        (void)hipFree(nsel);
+        for (void* p : {(void*)dfk, (void*)dmk, (void*)ddist, (void*)mat, (void*)rank, (void*)order, (void*)nsel})
+            if (p) (void)hipFreeAsync(p, hs);
     };
     const size_t nsel_words = 2 * kLists + (size_t)kLists * kSelect;
-    if (hipMalloc((void**)&dfk, sizeof(fu_surf) * n) != hipSuccess ||
-        hipMalloc((void**)&dmk, sizeof(fu_surf) * n) != hipSuccess ||
-        hipMalloc((void**)&ddist, sizeof(float) * n) != hipSuccess ||
-        hipMalloc((void**)&mat, sizeof(float) * (size_t)n * n) != hipSuccess ||
-        hipMalloc((void**)&rank, sizeof(float) * n) != hipSuccess ||
-        hipMalloc((void**)&order, sizeof(int32_t) * n) != hipSuccess ||
-        hipMalloc((void**)&nsel, sizeof(int32_t) * nsel_words) != hipSuccess) {
+    if (hipMallocAsync((void**)&dfk, sizeof(fu_surf) * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&dmk, sizeof(fu_surf) * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&ddist, sizeof(float) * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&mat, sizeof(float) * (size_t)n * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&rank, sizeof(float) * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&order, sizeof(int32_t) * n, hs) != hipSuccess ||
+        hipMallocAsync((void**)&nsel, sizeof(int32_t) * nsel_words, hs) != hipSuccess) {
         (void)hipGetLastError();
         release();
         return api_error(FU_ERR_NOMEM, "fu_relocalise_graph_candidates: %d pairs", n);
@@ -830,8 +882,8 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
                                  nsel + 2 * kLists, s);
     if (e == hipSuccess && !st)
         e = hipMemcpyAsync(hsel.data(), nsel, sizeof(int32_t) * nsel_words, hipMemcpyDeviceToHost, hs);
-    if (e == hipSuccess && !st) e = hipStreamSynchronize(hs);
     release();
+    if (e == hipSuccess && !st) e = hipStreamSynchronize(hs);
     if (st) return st;
     if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_relocalise_graph_candidates: %s", hipGetErrorString(e));
     const int32_t* n_selected = hsel.data();
