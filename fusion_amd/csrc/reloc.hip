@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "fu_api.h"
@@ -907,12 +909,27 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
             ref[3 * k + c] = (double)mk[p].pos[c];
         }
     }
+    // the M calls are independent (call c draws from seed + c alone), so calls 1 .. M-1 run on
+    // host threads of their own beside call 0; the poses are kept in call order
+    double T[kPoseLists][16];
+    int32_t good[kPoseLists] = {};
+    fu_status cst[kPoseLists] = {};
+    auto call = [&](int c) {
+        cst[c] = fu_pose_estimate(src.data(), ref.data(), R, iterations, seed + (uint64_t)c, T[c], nullptr, &good[c]);
+    };
+    std::vector<std::thread> workers;
+    for (int c = 1; c < M; ++c) {
+        try {
+            workers.emplace_back(call, c);
+        } catch (const std::system_error&) {
+            call(c);  // no thread to be had: the call runs here
+        }
+    }
+    call(0);
+    for (std::thread& w : workers) w.join();
     for (int c = 0; c < M; ++c) {
-        double T[16];
-        int32_t good = 0;
-        if ((st = fu_pose_estimate(src.data(), ref.data(), R, iterations, seed + (uint64_t)c, T, nullptr, &good)))
-            return st;
-        if (good) rigid_inverse(T, poses + 16 * (*n_poses)++);
+        if (cst[c]) return api_error(cst[c], "fu_relocalise_graph_candidates: PoseEstimate call %d failed", c);
+        if (good[c]) rigid_inverse(T[c], poses + 16 * (*n_poses)++);
     }
     return FU_OK;
 }
