@@ -238,25 +238,52 @@ constexpr int kMaxSelect = 1024;
 
 constexpr int kSumRows = 128;  // rows per LDS tile of k_rank_sums (32 KB)
 
-// launch: ceil(n / 64) work-groups of 256 lanes
+// launch: ceil(n / 64) work-groups of 256 lanes.  Lane t loads columns 4 (t & 15) .. +3 of
+// rows t >> 4, (t >> 4) + 16, ... of a tile (eight 16-byte loads when the row is whole and
+// aligned), the next tile's loads issued before wave 0 adds the current one from LDS.
 __global__ __launch_bounds__(256) void k_rank_sums(const float* C, int n, float* rank) {
     __shared__ float tile[kSumRows][64];
     const int tid = threadIdx.x, col = tid & 63;
-    const int c = blockIdx.x * 64 + col;
-    double s = 0;  // wave 0: column c
+    const int c0 = blockIdx.x * 64;
+    const int q = (tid & 15) * 4, r_lane = tid >> 4;
+    // whole 16-byte words: an aligned matrix and n a multiple of 4 keep every row start aligned
+    const bool vec = ((uintptr_t)C & 15) == 0 && (n & 3) == 0 && c0 + q + 3 < n;
+    float4 buf[kSumRows / 16];
+    auto fetch = [&](int r0) {
+#pragma unroll
+        for (int k = 0; k < kSumRows / 16; ++k) {
+            const int r = r0 + r_lane + 16 * k;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < n) {
+                const float* row = C + (size_t)r * n + c0 + q;
+                if (vec) {
+                    v = *reinterpret_cast<const float4*>(row);
+                } else {
+                    v.x = c0 + q < n ? row[0] : 0.f;
+                    v.y = c0 + q + 1 < n ? row[1] : 0.f;
+                    v.z = c0 + q + 2 < n ? row[2] : 0.f;
+                    v.w = c0 + q + 3 < n ? row[3] : 0.f;
+                }
+            }
+            buf[k] = v;
+        }
+    };
+    double s = 0;  // wave 0: column c0 + col
+    fetch(0);
     for (int r0 = 0; r0 < n; r0 += kSumRows) {
-        const int nr = min(kSumRows, n - r0);
+        __syncthreads();  // the previous tile is summed
+#pragma unroll
+        for (int k = 0; k < kSumRows / 16; ++k) *reinterpret_cast<float4*>(&tile[r_lane + 16 * k][q]) = buf[k];
         __syncthreads();
-        // lane (w, col) loads rows w, w + 4, ... of its column: each wave reads 64 consecutive
-        // floats of a row per load
-#pragma unroll 8
-        for (int r = tid >> 6; r < kSumRows; r += 4)
-            tile[r][col] = (r < nr && c < n) ? C[(size_t)(r0 + r) * n + c] : 0.f;
-        __syncthreads();
-        if (tid < 64)
-            for (int r = 0; r < nr; ++r) s += (double)tile[r][col];
+        if (r0 + kSumRows < n) fetch(r0 + kSumRows);
+        // rows past n hold +0, which leaves the sum as it is (it starts at +0 and so is never
+        // -0), so every tile is summed whole
+        if (tid < 64) {
+#pragma unroll 16
+            for (int r = 0; r < kSumRows; ++r) s += (double)tile[r][col];
+        }
     }
-    if (tid < 64 && c < n) rank[c] = (float)s;
+    if (tid < 64 && c0 + col < n) rank[c0 + col] = (float)s;
 }
 
 __device__ __forceinline__ float rank_key(float v) { return isnan(v) ? -INFINITY : v; }
@@ -272,11 +299,13 @@ __global__ __launch_bounds__(256) void k_rank_order(const float* rank, int n, in
     int pos = 0;
     for (int base = 0; base < n; base += 1024) {
         __syncthreads();
-        for (int k = threadIdx.x; k < 1024; k += 256) tile[k] = base + k < n ? rank_key(rank[base + k]) : 0.f;
+        // slots past n hold -inf: never above a rank, and equal to one only with an index
+        // above it, so they never count
+        for (int k = threadIdx.x; k < 1024; k += 256) tile[k] = base + k < n ? rank_key(rank[base + k]) : -INFINITY;
         __syncthreads();
-        const int lo = w * 256, hi = min(n - base, lo + 256);
-#pragma unroll 8
-        for (int k = lo; k < hi; ++k) {
+        const int lo = w * 256;
+#pragma unroll 16
+        for (int k = lo; k < lo + 256; ++k) {
             const float ky = tile[k];
             pos += (ky > kx) || (ky == kx && base + k < x);
         }

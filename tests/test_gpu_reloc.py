@@ -113,8 +113,12 @@ def _sel_matrix(rng, n, exclusive=0.05):
     return C
 
 
-def _same_selection(C, n_lists, max_sel, score):
-    g = A.select_subgraphs(torch.from_numpy(C).cuda(), n_lists, max_sel, score)
+def _same_selection(C, n_lists, max_sel, score, offset=0):
+    """offset: the matrix starts this many floats into its allocation (a view not 16-byte aligned)"""
+    n = C.shape[0]
+    dev = torch.zeros(n * n + offset, dtype=torch.float32, device="cuda")
+    dev[offset:] = torch.from_numpy(C.ravel()).cuda()
+    g = A.select_subgraphs(dev[offset:].view(n, n), n_lists, max_sel, score)
     o = O.graph_select(C, n_lists, max_sel, score)
     assert np.array_equal(g[2].view(np.uint32), o[2].view(np.uint32))  # rank sums, bit for bit
     assert np.array_equal(g[3], o[3])
@@ -133,6 +137,7 @@ def test_select_subgraphs_matches_oracle(gpu):
                                        (700, 10, 400, 0.1), (700, 12, 1024, 0.05), (700, 10, 1, 0.1)):
         print("select", n, n_lists, max_sel, score, flush=True)
         _same_selection(_sel_matrix(rng, n), n_lists, max_sel, score)
+    _same_selection(_sel_matrix(rng, 700), 10, 400, 0.1, offset=1)
     # few qualify: the scan runs over several 1024-pair chunks; duplicate columns tie; a NaN column
     C = _sel_matrix(rng, 5000)
     C[:, 17] = C[:, 3]
