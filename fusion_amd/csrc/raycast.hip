@@ -461,10 +461,15 @@ __device__ __forceinline__ bool ray_shade(const MapView& m, const RaycastArgs& a
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
+        // result + (+-1 on axis k / 2, 0 elsewhere): the other two coordinates as they are, which
+        // is what adding +0.0 gives them for the corners and fractions read here (only a -0
+        // would turn +0, and truncation, floor and x - floor(x) treat the two alike), so the
+        // samples sharing a coordinate share its corners
         const float d = (k & 1) ? -1.0f : 1.0f;
-        const f3 o = mk3(k / 2 == 0 ? d : 0.0f, k / 2 == 1 ? d : 0.0f, k / 2 == 2 ? d : 0.0f);
+        const f3 pt = mk3(k / 2 == 0 ? result.x + d : result.x, k / 2 == 1 ? result.y + d : result.y,
+                          k / 2 == 2 ? result.z + d : result.z);
         bool v2;
-        s[k] = smp.interp_nb(add(result, o), lo, hb, v2);
+        s[k] = smp.interp_nb(pt, lo, hb, v2);
         ok = ok && !(isnanf_(s[k]) || s[k] == 1.0f || !v2);
     }
     if (!ok) return false;
