@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so) - TEST INFRASTRUCTURE ONLY.
 
 Imported only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py, as
-the checker.  Arrays are numpy, host memory.
+what the GPU results are compared with.  Arrays are numpy, host memory.
 """
 import ctypes as C
 import os

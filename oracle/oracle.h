@@ -1,8 +1,9 @@
 /*
  * CPU oracle of the dense RGB-D pipeline - TEST INFRASTRUCTURE ONLY.
  *
- * A scalar C++ restatement of the reference's CUDA algorithm, used by tests/ (as the
- * checker), by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The product
+ * A scalar C++ restatement of the reference's CUDA algorithm, used by tests/ (as what the
+ * GPU results are compared with), by __graft_entry__.smoke() and by the cpu_baseline leg of
+ * bench.py.  The product
  * library never links or calls it.  Every function cites the reference file:line it follows.
  * Floating-point arithmetic is written in the reference's operation order and the library is
  * built with -ffp-contract=off, so single-precision results are those of a strict IEEE
