@@ -3,8 +3,8 @@
  *
  * A scalar C++ restatement of the reference's CUDA algorithm, used by tests/ (as what the
  * GPU results are compared with), by __graft_entry__.smoke() and by the cpu_baseline leg of
- * bench.py.  The product
- * library never links or calls it.  Every function cites the reference file:line it follows.
+ * bench.py.  The product library never links or calls it.  Every function cites the
+ * reference file:line it follows.
  * Floating-point arithmetic is written in the reference's operation order and the library is
  * built with -ffp-contract=off, so single-precision results are those of a strict IEEE
  * evaluation of the reference's expressions.
