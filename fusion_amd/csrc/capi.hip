@@ -1044,6 +1044,7 @@ extern "C" fu_status fu_system_destroy(fu_system* sys) {
     for (void* p : sys->allocs) (void)hipFree(p);
     fu_map_destroy(sys->map);
     fu_tracker_destroy(sys->trk);
+    fu_tracker_destroy(sys->reloc_trk);
     delete sys;
     return FU_OK;
 }
@@ -1548,9 +1549,10 @@ extern "C" fu_status fu_system_validate_poses(fu_system* sys, const double* pose
     if (h.sm_state != -1) return set_err(FU_ERR_STATE, "fu_system_validate_poses: the system is not lost");
     if (sys->frame == sys->frame_at_load)  // the pyramid to validate against is a grabbed frame's
         return set_err(FU_ERR_STATE, "fu_system_validate_poses: no frame since the system was reset or loaded");
-    fu_tracker* t = nullptr;
-    fu_status st = fu_tracker_create(sys->p.cols, sys->p.rows, &t);
-    if (st) return st;
+    // the candidates' ICP runs on a tracker of the system's own, kept for later attempts
+    fu_status st = FU_OK;
+    if (!sys->reloc_trk && (st = fu_tracker_create(sys->p.cols, sys->p.rows, &sys->reloc_trk))) return st;
+    fu_tracker* t = sys->reloc_trk;
     int first = -1;
     fu_track_result res;
     for (int i = 0; i < n && !st; ++i) {
@@ -1569,7 +1571,6 @@ extern "C" fu_status fu_system_validate_poses(fu_system* sys, const double* pose
     bool rendered = false;
     if (!st && first >= 0) st = render_at(sys, poses + 16 * first, 0, false, true, &rendered, hs);
     if (!st && first >= 0) st = icp_from(sys, t, poses + 16 * first, sys->p.iters, sys->p.icp_thresh, &res, hs);
-    fu_tracker_destroy(t);
     if (st) return st;
     if (chosen) *chosen = first;
     if (first < 0 || res.status != FU_TRACK_OK) return FU_OK;

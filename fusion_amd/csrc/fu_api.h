@@ -35,6 +35,7 @@ struct fu_system {
     fu_system_params p;
     fu_map* map = nullptr;
     fu_tracker* trk = nullptr;
+    fu_tracker* reloc_trk = nullptr;  // ValidatePose's ICP, created on its first use
     std::vector<void*> allocs;
     fu_system_buffers b;
     int lc[3], lr[3];
