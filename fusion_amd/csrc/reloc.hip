@@ -242,7 +242,7 @@ constexpr int kSumRows = 128;  // rows per LDS tile of k_rank_sums (32 KB)
 // rows t >> 4, (t >> 4) + 16, ... of a tile (eight 16-byte loads when the row is whole and
 // aligned), the next tile's loads issued before wave 0 adds the current one from LDS.
 __global__ __launch_bounds__(256) void k_rank_sums(const float* C, int n, float* rank) {
-    __shared__ float tile[kSumRows][64];
+    __shared__ __align__(16) float tile[kSumRows][64];  // written as 16-byte words
     const int tid = threadIdx.x, col = tid & 63;
     const int c0 = blockIdx.x * 64;
     const int q = (tid & 15) * 4, r_lane = tid >> 4;
