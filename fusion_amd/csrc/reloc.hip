@@ -744,7 +744,9 @@ fu_status collect_and_match(fu_keymap* km, const float* frame_desc, int32_t nq, 
     if (!st && e == hipSuccess) e = hipMemcpyAsync(hidx.data(), idx, sizeof(int32_t) * nm, hipMemcpyDeviceToHost, hs);
     if (!st && e == hipSuccess) e = hipMemcpyAsync(hdist.data(), dist, sizeof(float) * nm, hipMemcpyDeviceToHost, hs);
     release();
-    if (!st && e == hipSuccess) e = hipStreamSynchronize(hs);
+    // also after a failure: nothing queued may still write into the host vectors
+    const hipError_t es = hipStreamSynchronize(hs);
+    if (e == hipSuccess) e = es;
     if (st) return st;
     if (e != hipSuccess) return api_error(FU_ERR_HIP, "%s: %s", who, hipGetErrorString(e));
     return FU_OK;
@@ -914,7 +916,8 @@ extern "C" fu_status fu_relocalise_graph_candidates(fu_keymap* km, const float* 
     if (e == hipSuccess && !st)
         e = hipMemcpyAsync(hsel.data(), nsel, sizeof(int32_t) * nsel_words, hipMemcpyDeviceToHost, hs);
     release();
-    if (e == hipSuccess && !st) e = hipStreamSynchronize(hs);
+    const hipError_t es = hipStreamSynchronize(hs);  // also after a failure (host buffers)
+    if (e == hipSuccess) e = es;
     if (st) return st;
     if (e != hipSuccess) return api_error(FU_ERR_HIP, "fu_relocalise_graph_candidates: %s", hipGetErrorString(e));
     const int32_t* n_selected = hsel.data();
