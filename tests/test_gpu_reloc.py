@@ -254,6 +254,44 @@ def test_validate_poses_match_the_oracle(gpu):
     s.close()
 
 
+def test_validate_poses_twice_on_one_system(gpu):
+    """ValidatePose keeps its ICP tracker from one call to the next: an attempt whose every
+    candidate fails (one skipped, one rejected by ICP) and then one that succeeds give, on the
+    second call, the accepted flags, choice and relocalised pose of a fresh lost system, bit
+    for bit, and the next frame tracks alike"""
+    from tests.test_oracle_graph import _perturb
+    sc, K, fr = scene_frames(frames=range(0, 8), cols=320, rows=240)
+    empty = (np.zeros((240, 320), np.uint16), np.zeros((240, 320, 3), np.uint8))
+    mp = dict(num_buckets=1 << 16, num_excess=1 << 15, num_blocks=1 << 15, voxel_size=0.01,
+              trunc_dist=float(np.float32(0.01) * np.float32(3.0)), max_new_blocks=1 << 16)
+    ref = A.System(A.system_params(cols=320, rows=240, K=K, map=mp))
+    for f in fr[:7]:
+        ref.GrabImage(*A.to_device(*f))
+    truth = ref.result()["pose"]
+    ref.close()
+    far = np.eye(4)
+    far[:3, 3] = [0, 0, 40.0]
+    bad = [far, _perturb(truth, [0.25, -0.2, 0.15], 4.0, [1, 2, 0])]
+    good = [_perturb(truth, [0.012, -0.01, 0.01], 1.5, [0, 1, 1]), _perturb(truth, [-0.01, 0.0, 0.01], 1.0, [1, 0, 0])]
+    s = _lost_gpu(K, fr, empty, mp)
+    ok, acc, chosen = s.validate_poses(bad)
+    assert not ok and list(acc) == [-1, 0] and chosen == -1
+    assert s.result()["track_status"] == 5  # still lost
+    ok, acc, chosen = s.validate_poses(good)
+    fresh = _lost_gpu(K, fr, empty, mp)
+    ok2, acc2, chosen2 = fresh.validate_poses(good)
+    assert ok and ok2 and list(acc) == list(acc2) == [1, 1] and chosen == chosen2 == 0
+    r, r2 = s.result(), fresh.result()
+    assert r["track_status"] == r2["track_status"] == 6
+    assert np.array_equal(r["pose"], r2["pose"])
+    s.GrabImage(*A.to_device(*fr[7]))
+    fresh.GrabImage(*A.to_device(*fr[7]))
+    g, g2 = s.result(), fresh.result()
+    assert g["track_status"] == g2["track_status"] == 1 and np.array_equal(g["pose"], g2["pose"])
+    s.close()
+    fresh.close()
+
+
 def test_relocalise_graph_end_to_end(gpu):
     """a lost system relocalised from a key map built on an earlier frame: keypoints lifted
     from frame 3, put in the map at frame 3's pose; frame 6's keypoints observe the same points
