@@ -15,9 +15,14 @@ from tests.test_oracle_keymap import _keys
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("max_keys,nb,n", [(97, 2, 300), (100000, 5, 5000), (1, 3, 10), (1000, 5, 20000)])
+@pytest.mark.parametrize("max_keys,nb,n", [(97, 2, 300), (100000, 5, 5000), (1, 3, 10), (1000, 5, 20000),
+                                           (262147, 5, 5000)])
 def test_keymap_insert_collect_reset(gpu, max_keys, nb, n):
+    """(262147, 5): 1,310,735 slots, 1,281 groups of the collection, so its prefix pass gives
+    lanes 0-639 runs of two group counts, lane 640 one and the rest none; the whole slot array
+    (377 MB) is compared only for the smaller maps"""
     rng = np.random.default_rng(max_keys + n)
+    whole = max_keys * nb <= 600_000
     g = A.KeyMap(max_keys=max_keys, n_buckets=nb)
     o = O.KeyMap(max_keys=max_keys, n_buckets=nb)
     for rep in range(3):  # repeated inserts: new keys, keys of occupied cells, full buckets
@@ -30,7 +35,8 @@ def test_keymap_insert_collect_reset(gpu, max_keys, nb, n):
         o.insert(ko, io)
         assert np.array_equal(id_.cpu().numpy(), io), rep
         assert np.array_equal(A.surf_from_device(kd).view(np.uint8), ko.view(np.uint8)), rep
-        assert np.array_equal(g.read().view(np.uint8), o.read().view(np.uint8)), rep
+        if whole:
+            assert np.array_equal(g.read().view(np.uint8), o.read().view(np.uint8)), rep
         cap = 3 * n
         out_g, ng = g.CollectKeyPoints(cap)
         out_o, no = o.collect(cap)
@@ -39,7 +45,8 @@ def test_keymap_insert_collect_reset(gpu, max_keys, nb, n):
         assert np.array_equal(A.surf_from_device(out_g)[1:m].view(np.uint8), out_o[1:m].view(np.uint8))
     g.ResetKeyPoints()
     o.reset()
-    assert np.array_equal(g.read().view(np.uint8), o.read().view(np.uint8))
+    if whole:
+        assert np.array_equal(g.read().view(np.uint8), o.read().view(np.uint8))
     assert g.CollectKeyPoints(4)[1] == 0
 
 
