@@ -774,9 +774,14 @@ def main():
 
     single = single_stream(A, torch, args) if args.single_steps > 0 else None
     replay = ingest(A, torch) if args.single_steps > 0 else None
-    reloc = relocalisation(A, torch) if args.reloc else None
-    if reloc:
-        log("relocalisation: %s" % reloc["ms"])
+    reloc = None
+    if args.reloc:
+        try:
+            reloc = relocalisation(A, torch)
+            log("relocalisation: %s" % reloc["ms"])
+        except Exception as e:  # a side measurement: reported, never allowed to lose the line
+            reloc = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("relocalisation failed: %s" % reloc["error"])
     cpu = cpu_baseline(A, args.cpu_seconds) if args.cpu_seconds > 0 else None
 
     fps = r["tracked"] / r["wall"]
